@@ -1,0 +1,168 @@
+/*
+ * pdspy_amd.h -- C ABI of the MI355X (gfx950) continuum radiative-transfer
+ * path that pdspy's Model drives.  It replaces the process-and-files boundary
+ * pdspy uses today:
+ *
+ *   Model.write_radmc3d         pdspy/modeling/Model.py:723-778   -> rt_model_create
+ *   radmc3d mctherm (subprocess) pdspy/radmc3d/run.py:9-38
+ *     + read.dust_temperature    pdspy/radmc3d/read.pyx:433-478   -> rt_thermal,
+ *                                                                    rt_get_temperature
+ *   radmc3d image (os.system)    pdspy/radmc3d/run.py:157-289
+ *     + read.image               pdspy/radmc3d/read.pyx:25-152    -> rt_image
+ *   imtovis / interpolate_model  pdspy/imaging/imtovis.py:6-27,
+ *                                pdspy/interferometry/interpolate_model.py:11-57
+ *                                                                 -> rt_vis_dft
+ *   chisq                        pdspy/interferometry/libinterferometry.pyx:610-633,
+ *                                pdspy/utils/emcee.py:31-43      -> rt_chisq
+ *
+ * Conventions: plain C types only; every function returns 0 on success and a
+ * negative RT_E* code on failure (rt_last_error gives the message).  Pointer
+ * arguments named h_* are host memory, d_* are device memory of the context's
+ * GPU; buffers are owned by the caller.  A context is bound to one device and
+ * one HIP stream and is not thread-safe.
+ */
+#ifndef PDSPY_AMD_H
+#define PDSPY_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RT_OK = 0,
+    RT_E_ARG = -1,      /* invalid argument or input data (NaN, bad walls ...) */
+    RT_E_HIP = -2,      /* HIP runtime error */
+    RT_E_NOMEM = -3,    /* device allocation failed */
+    RT_E_WATCHDOG = -4, /* kernel time budget exceeded; packets were killed */
+    RT_E_OVERFLOW = -5, /* a fixed-point energy tally overflowed */
+    RT_E_LIMIT = -6     /* a compile-time limit (species, wavelengths) exceeded */
+};
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_model rt_model;
+
+/* Model description, as pdspy writes it for RADMC-3D (Model.py:723-778):
+ * walls in cm / rad / rad (amr_grid.inp), densities in g/cm^3 with r fastest
+ * (dust_density.inp, write.py:259-264), opacities on the wavelength grid
+ * (wavelength_micron.inp + dustkappa_*.inp), blackbody point stars
+ * (stars.inp with -T, write.py:148-172).  theta walls must start at 0 and end
+ * at pi/2 (equatorial mirror, as RADMC-3D grids from pdspy do) or pi;
+ * phi walls span [0, 2 pi].                                                  */
+typedef struct {
+    int nr, ntheta, nphi;
+    const double *h_r_walls;     /* nr+1, cm */
+    const double *h_theta_walls; /* ntheta+1, rad */
+    const double *h_phi_walls;   /* nphi+1, rad */
+    int nspec;
+    const double *h_rho; /* [nspec][nphi][ntheta][nr] g/cm^3 */
+    int nlam;
+    const double *h_lam_um; /* nlam, ascending */
+    const double *h_kabs;   /* [nspec][nlam] cm^2/g */
+    const double *h_ksca;   /* [nspec][nlam] cm^2/g */
+    const double *h_g;      /* [nspec][nlam] HG asymmetry, NULL = isotropic */
+    int nstar;
+    const double *h_stars; /* [nstar][5] = R_cm, T_K, x_cm, y_cm, z_cm */
+    int ntemp;             /* emission-table T grid (RADMC-3D ntemp), 0 = 1000 */
+    double temp0, temp1;   /* its range in K (RADMC-3D temp0/temp1), 0 = 0.01/1e5 */
+} rt_model_desc;
+
+/* Thermal Monte Carlo options (the radmc3d.inp keys pdspy passes,
+ * write.py:1-146, plus GPU launch controls).  Zero fields take defaults. */
+typedef struct {
+    uint64_t max_events;   /* per-packet interaction cap (default 1e6)         */
+    uint64_t epoch_first;  /* packets in the first epoch (default max(1024,N/4096)) */
+    double epoch_growth;   /* epoch k+1 adds max(first, N_done*(growth-1)); 1.25 */
+    int count_visits;      /* 1: count packet entries per cell                  */
+    int mrw;               /* 1: modified random walk (RADMC-3D modified_random_walk) */
+    double mrw_tauthres;   /* MRW trigger: Rosseland tau to the nearest wall (10) */
+    int mrw_count_trigger; /* MRW trigger: interactions in the cell first (100)  */
+    double mrw_gamma;      /* MRW sphere: radius = dist_to_wall - gamma/alpha_R (2) */
+    double max_seconds;    /* watchdog per launch (default 60 s)                 */
+    int block_size;        /* threads per block (0 = 256)                         */
+    int blocks_per_cu;     /* persistent blocks per CU (0 = auto)                 */
+} rt_thermal_opts;
+
+typedef struct {
+    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw;
+    double eps;       /* packet energy, erg/s */
+    double lum;       /* total stellar luminosity on the wavelength grid, erg/s */
+    int n_epochs;
+    int watchdog_fired;
+    double kernel_ms; /* device time of the MC launches (HIP events) */
+    double total_ms;  /* device time of the whole run incl. temperature updates */
+} rt_thermal_stats;
+
+/* Camera (RADMC-3D `image` options pdspy passes, Model.py:515-534):
+ * parallel rays, pixel (j,k) centred at ((k - nx/2) pix, (j - ny/2) pix) cm
+ * (integer division; pdspy's zoomau centring), observer direction
+ * (sin i sin phi, -sin i cos phi, cos i), image rotated by posang.           */
+typedef struct {
+    int nx, ny, nsub;   /* pixels and sub-rays per pixel side (>=1) */
+    double pix_cm;      /* pixel size in the model plane: pixelsize["] * dpc * AU */
+    double incl_deg, posang_deg, phi_deg;
+    double lam_um;      /* image wavelength; opacity log-log interpolated */
+    int mode;           /* 0 intensity erg/s/cm^2/Hz/sr, 1 optical depth (tracetau) */
+    int incl_star;      /* 1: add stars as point sources (pdspy nostar=False) */
+    double tau_stop;    /* stop a ray once tau exceeds this (0 = 40) */
+} rt_camera;
+
+int rt_create(int device, rt_ctx **out);
+int rt_destroy(rt_ctx *ctx);
+/* HIP stream (hipStream_t) for all later work; NULL = the null stream. */
+int rt_set_stream(rt_ctx *ctx, void *hip_stream);
+const char *rt_last_error(const rt_ctx *ctx);
+/* Number of CUs of the context's device and its wall-clock rate (kHz). */
+int rt_device_info(const rt_ctx *ctx, int *n_cu, int *wallclock_khz);
+
+/* Uploads grid, densities and opacities, builds the emission tables.
+ * Replaces write_radmc3d's input files (Model.py:723-778). */
+int rt_model_create(rt_ctx *ctx, const rt_model_desc *desc, rt_model **out);
+int rt_model_destroy(rt_model *m);
+int rt_model_ncells(const rt_model *m);
+/* Replace the model's densities (host, same layout as rt_model_desc.h_rho). */
+int rt_model_set_density(rt_model *m, const double *h_rho);
+
+/* Thermal MC (RADMC-3D `mctherm`, run.py:9-38): nphot packets, Philox key =
+ * seed.  Temperatures stay on the device; bit-reproducible for a given seed. */
+int rt_thermal(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *opts,
+               rt_thermal_stats *stats);
+/* Per-species temperatures [nspec][cells] (r fastest, dust_temperature.dat
+ * order, read.pyx:433-478) and visits[cells] (may be NULL). */
+int rt_get_temperature(rt_model *m, double *h_T, uint64_t *h_visits);
+/* Device copy of the same temperatures as float [nspec][cells]. */
+int rt_get_temperature_device(rt_model *m, float *d_T);
+
+/* Formal-solution image at one wavelength (RADMC-3D `image`, run.py:157-289,
+ * with scattering_mode_max=0).  d_T[cells]: temperature used for imaging (pdspy
+ * writes the density-weighted species mean, Model.py:759-769); d_rho
+ * [nspec][cells] densities or NULL for the model's; d_img[ny][nx] output. */
+int rt_image(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,
+             float *d_img);
+/* Adjoint of rt_image: given d_dimg = dL/dimg, writes dL/dlog(rho)
+ * [nspec][cells] and dL/dT [cells].  tau is recomputed, not stored. */
+int rt_image_adjoint(rt_model *m, const rt_camera *cam, const float *d_T,
+                     const float *d_rho, const float *d_dimg, float *d_g_logrho,
+                     float *d_g_T);
+
+/* Visibilities of an image at K (u,v) points (wavelengths):
+ * V = sum_jk I[j][k] exp(-2 pi i (u x_k + v y_j)), x_k = (k - nx/2) dxy,
+ * y_j = (j - ny/2) dxy (rad), phase centre shifted by (dra, ddec) rad.
+ * On the FFT grid this equals imtovis (imtovis.py:13-24). */
+int rt_vis_dft(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+               const double *d_u, const double *d_v, int K, double dra, double ddec,
+               float *d_re, float *d_im);
+/* Transpose: d_g_img[j][k] = dL/dI given dL/dRe, dL/dIm at each point. */
+int rt_vis_dft_adjoint(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx,
+                       int ny, double dxy, const double *d_u, const double *d_v, int K,
+                       double dra, double ddec, float *d_g_img);
+/* chi^2 = sum w ((dre-mre)^2 + (dim-mim)^2) into d_out[0] (double), and
+ * (optional, non-NULL) d_gre/d_gim = d chi^2 / d(mre, mim). */
+int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d_w,
+             const float *d_mre, const float *d_mim, int K, double *d_out, float *d_gre,
+             float *d_gim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""FP64 CPU oracle for the continuum radiative-transfer path (ctypes wrapper).
+
+TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, as the checker or
+the timed CPU baseline.  The product package ``pdspy_amd`` never imports it.
+
+The C source (``rt_oracle.c``) restates what RADMC-3D computes for pdspy's
+``Model.run_thermal`` / ``run_image`` (pdspy/modeling/Model.py:221-256,
+:515-577) plus the visibility DFT and chi^2 (imaging/imtovis.py:6-27,
+interferometry/interpolate_model.py:11-57, interferometry/libinterferometry.pyx:610-633).
+
+Model inputs are plain numpy arrays (a dict or any object with the fields
+``rw, tw, pw, rho, lam_um, kabs, ksca, gsca, stars``), never product objects.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _setup(_lib)
+    return _lib
+
+
+class _Model(C.Structure):
+    _fields_ = [("nr", C.c_int), ("nt", C.c_int), ("np", C.c_int),
+                ("rw", C.c_void_p), ("tw", C.c_void_p), ("pw", C.c_void_p),
+                ("nspec", C.c_int), ("rho", C.c_void_p), ("nlam", C.c_int),
+                ("lam_um", C.c_void_p), ("kabs", C.c_void_p), ("ksca", C.c_void_p),
+                ("gsca", C.c_void_p), ("nstar", C.c_int), ("star", C.c_void_p)]
+
+
+class _ThermOpts(C.Structure):
+    _fields_ = [("nphot", C.c_uint64), ("seed", C.c_uint64), ("ntemp", C.c_int),
+                ("temp0", C.c_double), ("temp1", C.c_double),
+                ("max_events", C.c_uint64), ("lagged", C.c_int),
+                ("epoch_first", C.c_uint64), ("epoch_growth", C.c_double),
+                ("nthreads", C.c_int)]
+
+
+class _ThermStats(C.Structure):
+    _fields_ = [("n_cross", C.c_uint64), ("n_abs", C.c_uint64), ("n_scat", C.c_uint64),
+                ("n_escape", C.c_uint64), ("n_killed", C.c_uint64),
+                ("n_events", C.c_uint64), ("eps", C.c_double), ("lum", C.c_double),
+                ("tally_tau_sum", C.c_double), ("n_epochs", C.c_int)]
+
+
+class _Camera(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsub", C.c_int),
+                ("pix_cm", C.c_double), ("incl", C.c_double), ("posang", C.c_double),
+                ("phi", C.c_double), ("lam_um", C.c_double), ("mode", C.c_int),
+                ("incl_star", C.c_int), ("tau_stop", C.c_double)]
+
+
+def _setup(L):
+    P = C.c_void_p
+    L.or_thermal.argtypes = [C.POINTER(_Model), C.POINTER(_ThermOpts), P, P,
+                             C.POINTER(_ThermStats)]
+    L.or_image.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P]
+    L.or_image_adjoint.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P, P, P]
+    L.or_trace_ray.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P]
+    L.or_vis_dft.argtypes = [P, C.c_int, C.c_int, C.c_double, P, P, C.c_int,
+                             C.c_double, C.c_double, P, P]
+    L.or_vis_dft_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, P,
+                                     C.c_int, C.c_double, C.c_double, P]
+    L.or_chisq.argtypes = [P, P, P, P, P, C.c_int]
+    L.or_chisq.restype = C.c_double
+    L.or_philox4x32_10.argtypes = [P, P, P]
+    L.or_planck.argtypes = [C.c_double, C.c_double]
+    L.or_planck.restype = C.c_double
+    L.or_dplanck_dT.argtypes = [C.c_double, C.c_double]
+    L.or_dplanck_dT.restype = C.c_double
+    L.or_emis_per_gram.argtypes = [P, P, C.c_int, C.c_double]
+    L.or_emis_per_gram.restype = C.c_double
+    L.or_freq_grid.argtypes = [P, C.c_int, P, P]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _get(m, k):
+    return m[k] if isinstance(m, dict) else getattr(m, k)
+
+
+class _Pinned:
+    """Holds numpy arrays alive while the C struct points at them."""
+
+    def __init__(self, m):
+        self.rw = _f64(_get(m, "rw"))
+        self.tw = _f64(_get(m, "tw"))
+        self.pw = _f64(_get(m, "pw"))
+        rho = _f64(_get(m, "rho"))
+        self.nspec = rho.shape[0]
+        self.rho = rho
+        self.lam = _f64(_get(m, "lam_um"))
+        self.kabs = _f64(_get(m, "kabs")).reshape(self.nspec, -1)
+        self.ksca = _f64(_get(m, "ksca")).reshape(self.nspec, -1)
+        g = _get(m, "gsca")
+        self.gsca = None if g is None else _f64(g).reshape(self.nspec, -1)
+        self.stars = _f64(_get(m, "stars")).reshape(-1, 5)
+        nr, nt, np_ = len(self.rw) - 1, len(self.tw) - 1, len(self.pw) - 1
+        assert rho.size == self.nspec * nr * nt * np_, "rho must be [nspec][np][nt][nr]"
+        self.s = _Model(nr, nt, np_, self.rw.ctypes.data, self.tw.ctypes.data,
+                        self.pw.ctypes.data, self.nspec, self.rho.ctypes.data,
+                        len(self.lam), self.lam.ctypes.data, self.kabs.ctypes.data,
+                        self.ksca.ctypes.data,
+                        None if self.gsca is None else self.gsca.ctypes.data,
+                        self.stars.shape[0], self.stars.ctypes.data)
+        self.ncell = nr * nt * np_
+
+
+def thermal(m, nphot, seed=0, ntemp=1000, temp0=0.01, temp1=1e5, max_events=0,
+            lagged=True, epoch_first=None, epoch_growth=1.25, nthreads=0):
+    """Returns (T[nspec][cells] float64, visits[cells] uint64, stats dict)."""
+    pm = _Pinned(m)
+    if epoch_first is None:
+        epoch_first = max(1024, int(nphot) // 4096)
+    o = _ThermOpts(int(nphot), int(seed), ntemp, temp0, temp1, int(max_events),
+                   1 if lagged else 0, int(epoch_first), float(epoch_growth), int(nthreads))
+    T = np.zeros((pm.nspec, pm.ncell))
+    visits = np.zeros(pm.ncell, dtype=np.uint64)
+    st = _ThermStats()
+    rc = lib().or_thermal(C.byref(pm.s), C.byref(o), T.ctypes.data, visits.ctypes.data,
+                          C.byref(st))
+    if rc != 0:
+        raise RuntimeError("or_thermal failed: %d" % rc)
+    stats = {k: getattr(st, k) for k, _ in _ThermStats._fields_}
+    return T, visits, stats
+
+
+def _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop,
+            ny=None):
+    return _Camera(int(npix), int(ny if ny is not None else npix), int(nsub), float(pix_cm),
+                   float(incl), float(posang), float(phi), float(lam_um), int(mode),
+                   int(incl_star), float(tau_stop))
+
+
+def image(m, T, kap, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0, lam_um=1300.0,
+          mode=0, incl_star=False, nsub=1, tau_stop=40.0):
+    """Returns img[ny][nx] (erg/s/cm^2/Hz/sr, or tau for mode=1)."""
+    pm = _Pinned(m)
+    T = _f64(T).reshape(-1)
+    kap = _f64(kap).reshape(-1)
+    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop)
+    img = np.zeros((cam.ny, cam.nx))
+    rc = lib().or_image(C.byref(pm.s), T.ctypes.data, kap.ctypes.data, C.byref(cam),
+                        img.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("or_image failed: %d" % rc)
+    return img
+
+
+def image_adjoint(m, T, kap, dimg, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0,
+                  lam_um=1300.0, incl_star=False, nsub=1, tau_stop=40.0):
+    pm = _Pinned(m)
+    T = _f64(T).reshape(-1)
+    kap = _f64(kap).reshape(-1)
+    dimg = _f64(dimg)
+    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, 0, incl_star, nsub, tau_stop)
+    g_lr = np.zeros((pm.nspec, pm.ncell))
+    g_T = np.zeros(pm.ncell)
+    rc = lib().or_image_adjoint(C.byref(pm.s), T.ctypes.data, kap.ctypes.data,
+                                C.byref(cam), dimg.ctypes.data, g_lr.ctypes.data,
+                                g_T.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("or_image_adjoint failed: %d" % rc)
+    return g_lr, g_T
+
+
+def trace_ray(m, o, d, maxseg=100000):
+    pm = _Pinned(m)
+    o = _f64(o)
+    d = _f64(d)
+    cell = np.zeros(maxseg, dtype=np.int32)
+    ln = np.zeros(maxseg)
+    hemi = np.zeros(maxseg, dtype=np.int32)
+    n = lib().or_trace_ray(C.byref(pm.s), o.ctypes.data, d.ctypes.data, maxseg,
+                           cell.ctypes.data, ln.ctypes.data, hemi.ctypes.data)
+    if n < 0:
+        raise RuntimeError("or_trace_ray failed: %d" % n)
+    return cell[:n], ln[:n], hemi[:n]
+
+
+def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0):
+    img = _f64(img)
+    u = _f64(u)
+    v = _f64(v)
+    re = np.zeros(len(u))
+    im = np.zeros(len(u))
+    ny, nx = img.shape
+    lib().or_vis_dft(img.ctypes.data, nx, ny, float(dxy), u.ctypes.data, v.ctypes.data,
+                     len(u), float(dra), float(ddec), re.ctypes.data, im.ctypes.data)
+    return re, im
+
+
+def vis_dft_adjoint(gre, gim, nx, ny, dxy, u, v, dra=0.0, ddec=0.0):
+    gre = _f64(gre)
+    gim = _f64(gim)
+    u = _f64(u)
+    v = _f64(v)
+    g = np.zeros((ny, nx))
+    lib().or_vis_dft_adjoint(gre.ctypes.data, gim.ctypes.data, nx, ny, float(dxy),
+                             u.ctypes.data, v.ctypes.data, len(u), float(dra),
+                             float(ddec), g.ctypes.data)
+    return g
+
+
+def chisq(dre, dim, w, mre, mim):
+    a = [_f64(x) for x in (dre, dim, w, mre, mim)]
+    return lib().or_chisq(*[x.ctypes.data for x in a], len(a[0]))
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().or_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def planck(nu, T):
+    return lib().or_planck(float(nu), float(T))
+
+
+def dplanck_dT(nu, T):
+    return lib().or_dplanck_dT(float(nu), float(T))
+
+
+def emis_per_gram(kabs, lam_um, T):
+    k = _f64(kabs)
+    lam = _f64(lam_um)
+    return lib().or_emis_per_gram(k.ctypes.data, lam.ctypes.data, len(lam), float(T))
+
+
+def freq_grid(lam_um):
+    lam = _f64(lam_um)
+    nu = np.zeros(len(lam))
+    dnu = np.zeros(len(lam))
+    lib().or_freq_grid(lam.ctypes.data, len(lam), nu.ctypes.data, dnu.ctypes.data)
+    return nu, dnu

@@ -1,0 +1,1369 @@
+/*
+ * rt_oracle.c -- FP64 CPU restatement of the continuum RT path (test
+ * infrastructure only; see oracle.h for who may call it).
+ *
+ * What each part restates:
+ *  - Planck function / derivative: pdspy/misc/B_nu.py:4-6, dB_nu.py:4-7, with
+ *    pdspy's CGS constants (pdspy/constants/physics.py).
+ *  - Grid, densities, walls and cell order: pdspy/modeling/Grid.py:71-80,
+ *    Model.py:754-758, radmc3d/write.py:198-266 (r fastest, theta in
+ *    [0, pi/2] mirrored at the equator).
+ *  - Stars as blackbody point sources (write.py:148-172, -T => blackbody,
+ *    istar_sphere unset => point source).
+ *  - Thermal MC: what RADMC-3D `mctherm` computes (run.py:9-38): packets of
+ *    equal energy L/N, tau = -ln(xi) flights, albedo-weighted scattering,
+ *    Bjorkman & Wood (2001) immediate re-emission drawn from the
+ *    kappa_nu dB_nu/dT spectrum tabulated on RADMC-3D's (ntemp, temp0, temp1)
+ *    temperature grid (write.py:47-52), path-length (Lucy 1999) absorbed-energy
+ *    estimator, and per-species temperatures written back as
+ *    dust_temperature.dat (read.pyx:433-478, Model.py:249-254).
+ *  - Camera: formal solution I = sum S (1-e^-dtau) e^-tau with S = B_nu(T) and
+ *    absorption-only opacity (RADMC-3D image with scattering_mode_max=0),
+ *    pixel centring of Model.py:522-527.
+ *  - Visibilities: DFT equal to imtovis (imaging/imtovis.py:13-24) on the FFT
+ *    grid and to an exact sampler at arbitrary (u,v) (interpolate_model.py:20-27).
+ *  - chi^2: interferometry/libinterferometry.pyx:619-633.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* pdspy CGS constants (pdspy/constants/physics.py) */
+#define C_H 6.6260755e-27
+#define C_C 2.99792458e10
+#define C_K 1.380658e-16
+#define PI_D 3.14159265358979323846
+#define TWO_PI_D 6.28318530717958647692
+
+#define TALLY_SHIFT 28 /* fixed-point tally resolution: 2^-28 of kmax*dr */
+
+/* ------------------------------------------------------------------ Philox */
+#define PH_M0 0xD2511F53u
+#define PH_M1 0xCD9E8D57u
+#define PH_W0 0x9E3779B9u
+#define PH_W1 0xBB67AE85u
+
+void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
+                      uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += PH_W0;
+            k1 += PH_W1;
+        }
+        uint64_t p0 = (uint64_t)PH_M0 * c0;
+        uint64_t p1 = (uint64_t)PH_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int idx;
+} rng_t;
+
+static void rng_init(rng_t *r, uint64_t seed, uint64_t pid, uint32_t stream) {
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->ctr[0] = 0;
+    r->ctr[1] = (uint32_t)pid;
+    r->ctr[2] = (uint32_t)(pid >> 32);
+    r->ctr[3] = stream;
+    r->idx = 4;
+}
+
+/* uniform in (0,1) from the top 24 bits: ((x>>8)+0.5)*2^-24, exact in FP32 */
+static double uni(rng_t *r) {
+    if (r->idx == 4) {
+        or_philox4x32_10(r->ctr, r->key, r->buf);
+        r->ctr[0]++;
+        r->idx = 0;
+    }
+    uint32_t x = r->buf[r->idx++];
+    return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0);
+}
+
+/* --------------------------------------------------------------- Planck */
+double or_planck(double nu, double T) {
+    if (T <= 0) return 0.0;
+    double x = C_H * nu / (C_K * T);
+    if (x > 700.0) return 2.0 * C_H * nu * nu * nu / (C_C * C_C) * exp(-x);
+    return 2.0 * C_H * nu * nu * nu / (C_C * C_C) / expm1(x);
+}
+
+double or_dplanck_dT(double nu, double T) {
+    if (T <= 0) return 0.0;
+    double x = C_H * nu / (C_K * T);
+    double pre = 2.0 * C_H * C_H * nu * nu * nu * nu / (C_C * C_C * C_K * T * T);
+    if (x > 700.0) return pre * exp(-x);
+    double s = sinh(0.5 * x);
+    return pre / (4.0 * s * s);
+}
+
+/* ln of the Planck function, safe for any x > 0 */
+static double ln_planck(double nu, double T) {
+    double x = C_H * nu / (C_K * T);
+    double lnexpm1 = (x > 30.0) ? x + log1p(-exp(-x)) : log(expm1(x));
+    return log(2.0 * C_H / (C_C * C_C)) + 3.0 * log(nu) - lnexpm1;
+}
+
+static double ln_dplanck(double nu, double T) {
+    double x = C_H * nu / (C_K * T);
+    double ln_pre = log(2.0 * C_H * C_H / (C_C * C_C * C_K)) + 4.0 * log(nu) -
+                    2.0 * log(T);
+    /* 1/(4 sinh^2(x/2)) = e^-x / (1-e^-x)^2 */
+    double ln_f = -x - 2.0 * log(-expm1(-x));
+    return ln_pre + ln_f;
+}
+
+void or_freq_grid(const double *lam_um, int nlam, double *nu, double *dnu) {
+    for (int i = 0; i < nlam; ++i) nu[i] = C_C / (lam_um[i] * 1e-4);
+    if (nlam == 1) {
+        dnu[0] = 1.0;
+        return;
+    }
+    for (int i = 0; i < nlam; ++i) {
+        if (i == 0)
+            dnu[i] = 0.5 * fabs(nu[1] - nu[0]);
+        else if (i == nlam - 1)
+            dnu[i] = 0.5 * fabs(nu[nlam - 1] - nu[nlam - 2]);
+        else
+            dnu[i] = 0.5 * fabs(nu[i + 1] - nu[i - 1]);
+    }
+}
+
+double or_emis_per_gram(const double *kabs, const double *lam_um, int nlam,
+                        double T) {
+    double *nu = malloc(sizeof(double) * nlam), *dnu = malloc(sizeof(double) * nlam);
+    or_freq_grid(lam_um, nlam, nu, dnu);
+    double s = 0;
+    for (int i = 0; i < nlam; ++i) s += kabs[i] * or_planck(nu[i], T) * dnu[i];
+    free(nu);
+    free(dnu);
+    return 4.0 * PI_D * s;
+}
+
+/* ------------------------------------------------------------- geometry */
+typedef struct {
+    int nr, nt, np, mirror, ncell;
+    const double *rw, *tw, *pw;
+    double *ct, *st; /* cos/sin of theta walls */
+    double *cp, *sp; /* cos/sin of phi walls */
+    double *vol;     /* cell volumes (mirror doubles them) */
+    double *dr;      /* radial cell width per ir */
+} geom_t;
+
+#define INF_D 1e300
+
+enum { W_NONE = 0, W_ROUT, W_RIN, W_TLO, W_THI, W_PLO, W_PHI, W_HOLE };
+
+typedef struct {
+    double x, y, z, dx, dy, dz;
+    int ir, it, ip; /* ir = -1 hole, ir = nr outside */
+    int hemi;       /* +1 / -1 (mirror mode: which copy of the half-space) */
+} pk_t;
+
+static int geom_init(geom_t *g, const or_model *m) {
+    memset(g, 0, sizeof(*g));
+    g->nr = m->nr;
+    g->nt = m->nt;
+    g->np = m->np;
+    g->rw = m->rw;
+    g->tw = m->tw;
+    g->pw = m->pw;
+    if (m->nr < 1 || m->nt < 1 || m->np < 1) return -1;
+    for (int i = 0; i < m->nr; ++i)
+        if (!(m->rw[i + 1] > m->rw[i]) || !(m->rw[i] > 0)) return -2;
+    for (int i = 0; i < m->nt; ++i)
+        if (!(m->tw[i + 1] > m->tw[i])) return -2;
+    for (int i = 0; i < m->np; ++i)
+        if (!(m->pw[i + 1] > m->pw[i])) return -2;
+    if (fabs(m->tw[0]) > 1e-12) return -3;
+    if (fabs(m->tw[m->nt] - 0.5 * PI_D) < 1e-9)
+        g->mirror = 1;
+    else if (fabs(m->tw[m->nt] - PI_D) < 1e-9)
+        g->mirror = 0;
+    else
+        return -3;
+    if (fabs(m->pw[0]) > 1e-12 || fabs(m->pw[m->np] - TWO_PI_D) > 1e-9) return -4;
+    g->ncell = m->nr * m->nt * m->np;
+    g->ct = malloc(sizeof(double) * (m->nt + 1));
+    g->st = malloc(sizeof(double) * (m->nt + 1));
+    g->cp = malloc(sizeof(double) * (m->np + 1));
+    g->sp = malloc(sizeof(double) * (m->np + 1));
+    for (int j = 0; j <= m->nt; ++j) {
+        g->ct[j] = cos(m->tw[j]);
+        g->st[j] = sin(m->tw[j]);
+    }
+    if (g->mirror) {
+        g->ct[m->nt] = 0.0;
+        g->st[m->nt] = 1.0;
+    }
+    for (int k = 0; k <= m->np; ++k) {
+        g->cp[k] = cos(m->pw[k]);
+        g->sp[k] = sin(m->pw[k]);
+    }
+    g->vol = malloc(sizeof(double) * g->ncell);
+    g->dr = malloc(sizeof(double) * m->nr);
+    for (int i = 0; i < m->nr; ++i) g->dr[i] = m->rw[i + 1] - m->rw[i];
+    for (int k = 0; k < m->np; ++k)
+        for (int j = 0; j < m->nt; ++j)
+            for (int i = 0; i < m->nr; ++i) {
+                double r1 = m->rw[i], r2 = m->rw[i + 1];
+                double dv_r = (r2 - r1) * (r2 * r2 + r1 * r2 + r1 * r1) / 3.0;
+                double dv_t = g->ct[j] - g->ct[j + 1];
+                double dv_p = m->pw[k + 1] - m->pw[k];
+                g->vol[i + m->nr * (j + m->nt * k)] =
+                    dv_r * dv_t * dv_p * (g->mirror ? 2.0 : 1.0);
+            }
+    return 0;
+}
+
+static void geom_free(geom_t *g) {
+    free(g->ct);
+    free(g->st);
+    free(g->cp);
+    free(g->sp);
+    free(g->vol);
+    free(g->dr);
+}
+
+static int bsearch_walls(const double *w, int n, double v) {
+    /* index i in [0, n-1] with w[i] <= v < w[i+1], clamped */
+    if (v <= w[0]) return 0;
+    if (v >= w[n]) return n - 1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (w[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+static void locate_angles(const geom_t *g, pk_t *p) {
+    double rho = sqrt(p->x * p->x + p->y * p->y);
+    if (g->mirror) {
+        p->hemi = (p->z >= 0) ? 1 : -1;
+        double th = atan2(rho, fabs(p->z));
+        p->it = bsearch_walls(g->tw, g->nt, th);
+    } else {
+        p->hemi = 1;
+        double th = atan2(rho, p->z);
+        p->it = bsearch_walls(g->tw, g->nt, th);
+    }
+    if (g->np > 1) {
+        double ph = atan2(p->y, p->x);
+        if (ph < 0) ph += TWO_PI_D;
+        p->ip = bsearch_walls(g->pw, g->np, ph);
+    } else {
+        p->ip = 0;
+    }
+}
+
+static void locate(const geom_t *g, pk_t *p) {
+    double r = sqrt(p->x * p->x + p->y * p->y + p->z * p->z);
+    if (r < g->rw[0]) {
+        p->ir = -1;
+        p->hemi = (g->mirror && p->z < 0) ? -1 : 1;
+        return;
+    }
+    if (r >= g->rw[g->nr]) {
+        p->ir = g->nr;
+        return;
+    }
+    p->ir = bsearch_walls(g->rw, g->nr, r);
+    locate_angles(g, p);
+}
+
+static double sph_out(double b, double rr, double R) {
+    double c = rr - R * R;
+    double D = b * b - c;
+    if (D < 0) D = 0;
+    double sq = sqrt(D);
+    double t = (b <= 0) ? (-b + sq) : (-c / (b + sq));
+    return t > 0 ? t : 0.0;
+}
+
+static double sph_in(double b, double rr, double R) {
+    if (b >= 0) return INF_D;
+    double c = rr - R * R;
+    double D = b * b - c;
+    if (D < 0) return INF_D;
+    double t = c / (-b + sqrt(D));
+    return t > 0 ? t : 0.0;
+}
+
+/* Exit through the cone at polar angle th (cos cth, sin sth) in the folded
+ * frame.  req = -1: leaving towards smaller theta, +1: towards larger theta.
+ * The cone function f = rho^2 cos^2 - z^2 sin^2 is quadratic in t; the root
+ * with f' > 0 is t+ = (-B+sqrt(D))/A and with f' < 0 is t-; on the nappe
+ * sign(z) = sign(cos th) the sign of d(theta)/dt is sign(f') * sign(cos th). */
+static double cone_hit(double x, double y, double z, double dx, double dy,
+                       double dz, double cth, double sth, int req) {
+    double c2 = cth * cth, s2 = sth * sth;
+    double A = (dx * dx + dy * dy) * c2 - dz * dz * s2;
+    double B = (x * dx + y * dy) * c2 - z * dz * s2;
+    double C = (x * x + y * y) * c2 - z * z * s2;
+    double D = B * B - A * C;
+    if (D < 0) return INF_D;
+    double sq = sqrt(D);
+    int sgn_c = cth > 0 ? 1 : -1;
+    double t;
+    if (req * sgn_c > 0) {
+        t = (B < 0) ? (-B + sq) / A : -C / (B + sq);
+    } else {
+        t = (B > 0) ? -(B + sq) / A : C / (-B + sq);
+    }
+    if (!(t >= 0) || !(t < INF_D)) return INF_D;
+    double zh = z + t * dz;
+    if (zh * cth < 0) return INF_D;
+    return t;
+}
+
+static double phi_hit(double x, double y, double dx, double dy, double cph,
+                      double sph, int req) {
+    double g0 = -x * sph + y * cph;
+    double g1 = -dx * sph + dy * cph;
+    if (req < 0 ? !(g1 < 0) : !(g1 > 0)) return INF_D;
+    double t = -g0 / g1;
+    if (!(t >= 0)) return INF_D;
+    double xh = x + t * dx, yh = y + t * dy;
+    if (xh * cph + yh * sph < 0) return INF_D;
+    return t;
+}
+
+/* distance to the next wall of the current cell and the wall's kind */
+static double next_wall(const geom_t *g, const pk_t *p, int *kind) {
+    double rr = p->x * p->x + p->y * p->y + p->z * p->z;
+    double b = p->x * p->dx + p->y * p->dy + p->z * p->dz;
+    if (p->ir < 0) {
+        *kind = W_HOLE;
+        return sph_out(b, rr, g->rw[0]);
+    }
+    double tbest = sph_out(b, rr, g->rw[p->ir + 1]);
+    *kind = W_ROUT;
+    double t = sph_in(b, rr, g->rw[p->ir]);
+    if (t < tbest) {
+        tbest = t;
+        *kind = W_RIN;
+    }
+    double zf = p->hemi * p->z, dzf = p->hemi * p->dz;
+    int j = p->it;
+    if (j > 0) { /* lower-theta wall (not the axis) */
+        if (!g->mirror && fabs(g->ct[j]) < 1e-15) {
+            /* equatorial plane seen from the lower hemisphere: g = -z, need g'<0 */
+            if (p->dz > 0) {
+                t = -p->z / p->dz;
+                if (t < 0) t = INF_D;
+                if (t < tbest) {
+                    tbest = t;
+                    *kind = W_TLO;
+                }
+            }
+        } else {
+            t = cone_hit(p->x, p->y, zf, p->dx, p->dy, dzf, g->ct[j], g->st[j], -1);
+            if (t < tbest) {
+                tbest = t;
+                *kind = W_TLO;
+            }
+        }
+    }
+    if (j + 1 < g->nt || g->mirror) {
+        if (g->mirror && j + 1 == g->nt) {
+            if (dzf < 0) {
+                t = -zf / dzf;
+                if (t < 0) t = 0;
+                if (t < tbest) {
+                    tbest = t;
+                    *kind = W_THI;
+                }
+            }
+        } else if (!g->mirror && fabs(g->ct[j + 1]) < 1e-15) {
+            if (p->dz < 0) {
+                t = -p->z / p->dz;
+                if (t < 0) t = INF_D;
+                if (t < tbest) {
+                    tbest = t;
+                    *kind = W_THI;
+                }
+            }
+        } else {
+            t = cone_hit(p->x, p->y, zf, p->dx, p->dy, dzf, g->ct[j + 1],
+                         g->st[j + 1], +1);
+            if (t < tbest) {
+                tbest = t;
+                *kind = W_THI;
+            }
+        }
+    }
+    if (g->np > 1) {
+        int k = p->ip;
+        t = phi_hit(p->x, p->y, p->dx, p->dy, g->cp[k], g->sp[k], -1);
+        if (t < tbest) {
+            tbest = t;
+            *kind = W_PLO;
+        }
+        t = phi_hit(p->x, p->y, p->dx, p->dy, g->cp[k + 1], g->sp[k + 1], +1);
+        if (t < tbest) {
+            tbest = t;
+            *kind = W_PHI;
+        }
+    }
+    return tbest;
+}
+
+static void snap_radius(pk_t *p, double R) {
+    double r = sqrt(p->x * p->x + p->y * p->y + p->z * p->z);
+    if (r > 0) {
+        double f = R / r;
+        p->x *= f;
+        p->y *= f;
+        p->z *= f;
+    }
+}
+
+/* move by t and cross the wall of kind k; returns 1 if the packet left the grid */
+static int cross_wall(const geom_t *g, pk_t *p, double t, int kind) {
+    p->x += t * p->dx;
+    p->y += t * p->dy;
+    p->z += t * p->dz;
+    switch (kind) {
+    case W_HOLE:
+        snap_radius(p, g->rw[0]);
+        p->ir = 0;
+        locate_angles(g, p);
+        return 0;
+    case W_ROUT:
+        p->ir++;
+        if (p->ir >= g->nr) return 1;
+        snap_radius(p, g->rw[p->ir]);
+        return 0;
+    case W_RIN:
+        snap_radius(p, g->rw[p->ir]);
+        p->ir--;
+        return 0;
+    case W_TLO:
+        p->it--;
+        return 0;
+    case W_THI:
+        if (g->mirror && p->it + 1 == g->nt) {
+            p->z = 0.0;
+            p->hemi = -p->hemi;
+        } else {
+            p->it++;
+        }
+        return 0;
+    case W_PLO:
+        p->ip = (p->ip - 1 + g->np) % g->np;
+        return 0;
+    case W_PHI:
+        p->ip = (p->ip + 1) % g->np;
+        return 0;
+    }
+    return 0;
+}
+
+static inline int cell_of(const geom_t *g, const pk_t *p) {
+    return p->ir + g->nr * (p->it + g->nt * p->ip);
+}
+
+int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg,
+                 int *cell, double *len, int *hemi) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    pk_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = o[0];
+    p.y = o[1];
+    p.z = o[2];
+    p.dx = d[0];
+    p.dy = d[1];
+    p.dz = d[2];
+    double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    double R = g.rw[g.nr];
+    int n = 0;
+    if (rr >= R * R) {
+        double b = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
+        double D = b * b - (rr - R * R);
+        if (D <= 0 || b >= 0) {
+            geom_free(&g);
+            return 0;
+        }
+        double t = -b - sqrt(D);
+        p.x += t * p.dx;
+        p.y += t * p.dy;
+        p.z += t * p.dz;
+        snap_radius(&p, R);
+        p.ir = g.nr - 1;
+        locate_angles(&g, &p);
+    } else {
+        locate(&g, &p);
+    }
+    while (p.ir >= -1 && p.ir < g.nr) {
+        int kind;
+        double t = next_wall(&g, &p, &kind);
+        if (n < maxseg) {
+            cell[n] = (p.ir < 0) ? -1 : cell_of(&g, &p);
+            len[n] = t;
+            if (hemi) hemi[n] = p.hemi;
+        }
+        n++;
+        if (cross_wall(&g, &p, t, kind)) break;
+        if (n > 10000000) break;
+    }
+    geom_free(&g);
+    return n;
+}
+
+/* ------------------------------------------------------------- tables */
+typedef struct {
+    int nlam, ntemp, ngroup, nspec, nstar;
+    double temp0, temp1, lnt0, dlnt;
+    double *nu, *dnu;
+    int *grp;        /* species -> group */
+    int *grep;       /* group -> representative species */
+    double *kmax;    /* [ngroup] */
+    double *lnq;     /* [ngroup][ntemp] */
+    double *cdf;     /* [ngroup][ntemp][nlam] */
+    double *scdf;    /* [nstar][nlam] */
+    double *spick;   /* [nstar] cumulative luminosity fractions */
+    double lum;
+} tab_t;
+
+static void tab_free(tab_t *t) {
+    free(t->nu);
+    free(t->dnu);
+    free(t->grp);
+    free(t->grep);
+    free(t->kmax);
+    free(t->lnq);
+    free(t->cdf);
+    free(t->scdf);
+    free(t->spick);
+}
+
+static double logsumexp(const double *v, int n) {
+    double mx = -INFINITY;
+    for (int i = 0; i < n; ++i)
+        if (v[i] > mx) mx = v[i];
+    if (mx == -INFINITY) return -INFINITY;
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += exp(v[i] - mx);
+    return mx + log(s);
+}
+
+static void cdf_from_log(const double *lw, int n, double *cdf) {
+    double mx = -INFINITY;
+    for (int i = 0; i < n; ++i)
+        if (lw[i] > mx) mx = lw[i];
+    if (mx == -INFINITY) { /* nothing emits: put everything in the last bin */
+        for (int i = 0; i < n; ++i) cdf[i] = (i == n - 1) ? 1.0 : 0.0;
+        return;
+    }
+    double s = 0;
+    for (int i = 0; i < n; ++i) {
+        s += exp(lw[i] - mx);
+        cdf[i] = s;
+    }
+    for (int i = 0; i < n; ++i) cdf[i] /= s;
+    cdf[n - 1] = 1.0;
+}
+
+static int tab_init(tab_t *t, const or_model *m, const or_thermal_opts *o) {
+    memset(t, 0, sizeof(*t));
+    int nl = m->nlam, ns = m->nspec;
+    t->nlam = nl;
+    t->nspec = ns;
+    t->nstar = m->nstar;
+    t->ntemp = o->ntemp;
+    t->temp0 = o->temp0;
+    t->temp1 = o->temp1;
+    if (t->ntemp < 2 || !(t->temp1 > t->temp0) || !(t->temp0 > 0)) return -10;
+    t->lnt0 = log(t->temp0);
+    t->dlnt = (log(t->temp1) - t->lnt0) / (t->ntemp - 1);
+    t->nu = malloc(sizeof(double) * nl);
+    t->dnu = malloc(sizeof(double) * nl);
+    or_freq_grid(m->lam_um, nl, t->nu, t->dnu);
+    /* groups: species with identical absorption opacity share temperatures */
+    t->grp = malloc(sizeof(int) * ns);
+    t->grep = malloc(sizeof(int) * ns);
+    int ng = 0;
+    for (int s = 0; s < ns; ++s) {
+        int found = -1;
+        for (int q = 0; q < ng && found < 0; ++q) {
+            const double *a = m->kabs + (size_t)t->grep[q] * nl, *b = m->kabs + (size_t)s * nl;
+            if (memcmp(a, b, sizeof(double) * nl) == 0) found = q;
+        }
+        if (found < 0) {
+            t->grep[ng] = s;
+            found = ng++;
+        }
+        t->grp[s] = found;
+    }
+    t->ngroup = ng;
+    t->kmax = malloc(sizeof(double) * ng);
+    t->lnq = malloc(sizeof(double) * ng * t->ntemp);
+    t->cdf = malloc(sizeof(double) * (size_t)ng * t->ntemp * nl);
+    double *lw = malloc(sizeof(double) * nl);
+    for (int q = 0; q < ng; ++q) {
+        const double *ka = m->kabs + (size_t)t->grep[q] * nl;
+        double km = 0;
+        for (int i = 0; i < nl; ++i)
+            if (ka[i] > km) km = ka[i];
+        if (!(km > 0)) km = 1.0;
+        t->kmax[q] = km;
+        for (int it = 0; it < t->ntemp; ++it) {
+            double T = exp(t->lnt0 + it * t->dlnt);
+            for (int i = 0; i < nl; ++i)
+                lw[i] = (ka[i] > 0) ? log(ka[i] * t->dnu[i]) + ln_planck(t->nu[i], T)
+                                    : -INFINITY;
+            t->lnq[q * t->ntemp + it] = log(4.0 * PI_D) + logsumexp(lw, nl);
+            for (int i = 0; i < nl; ++i)
+                lw[i] = (ka[i] > 0) ? log(ka[i] * t->dnu[i]) + ln_dplanck(t->nu[i], T)
+                                    : -INFINITY;
+            cdf_from_log(lw, nl, t->cdf + ((size_t)q * t->ntemp + it) * nl);
+        }
+    }
+    /* stars: blackbody point sources, L = 4 pi R^2 * pi * sum B dnu */
+    t->scdf = malloc(sizeof(double) * (m->nstar > 0 ? m->nstar : 1) * nl);
+    t->spick = malloc(sizeof(double) * (m->nstar > 0 ? m->nstar : 1));
+    double ltot = 0;
+    for (int k = 0; k < m->nstar; ++k) {
+        double R = m->star[5 * k], Ts = m->star[5 * k + 1];
+        double s = 0;
+        for (int i = 0; i < nl; ++i) {
+            double bd = or_planck(t->nu[i], Ts) * t->dnu[i];
+            s += bd;
+            lw[i] = (bd > 0) ? log(bd) : -INFINITY;
+        }
+        cdf_from_log(lw, nl, t->scdf + (size_t)k * nl);
+        double L = 4.0 * PI_D * R * R * PI_D * s;
+        ltot += L;
+        t->spick[k] = ltot;
+    }
+    for (int k = 0; k < m->nstar; ++k) t->spick[k] /= ltot;
+    if (m->nstar > 0) t->spick[m->nstar - 1] = 1.0;
+    t->lum = ltot;
+    free(lw);
+    return 0;
+}
+
+/* T from absorbed power per gram e (erg/s/g), log-log interpolation of lnq */
+static double temp_from_e(const tab_t *t, int q, double e) {
+    if (!(e > 0)) return 0.0;
+    const double *lq = t->lnq + q * t->ntemp;
+    double le = log(e);
+    int n = t->ntemp, m;
+    if (le <= lq[0])
+        m = 0;
+    else if (le >= lq[n - 1])
+        m = n - 2;
+    else {
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) / 2;
+            if (lq[mid] <= le)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        m = lo;
+    }
+    double lt = t->lnt0 + (m + (le - lq[m]) / (lq[m + 1] - lq[m])) * t->dlnt;
+    return exp(lt);
+}
+
+static int sample_cdf(const double *cdf, int n, double u) {
+    int lo = -1, hi = n - 1; /* smallest i with u <= cdf[i] */
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (u <= cdf[mid])
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return hi;
+}
+
+static void iso_dir(rng_t *r, double *d) {
+    double mu = 2.0 * uni(r) - 1.0;
+    double ph = TWO_PI_D * uni(r);
+    double s = sqrt(fmax(0.0, 1.0 - mu * mu));
+    d[0] = s * cos(ph);
+    d[1] = s * sin(ph);
+    d[2] = mu;
+}
+
+static void hg_dir(rng_t *r, double g, double *d) {
+    double u = uni(r);
+    double ct;
+    if (fabs(g) < 1e-6)
+        ct = 2.0 * u - 1.0;
+    else {
+        double f = (1.0 - g * g) / (1.0 - g + 2.0 * g * u);
+        ct = (1.0 + g * g - f * f) / (2.0 * g);
+    }
+    if (ct > 1) ct = 1;
+    if (ct < -1) ct = -1;
+    double ps = TWO_PI_D * uni(r);
+    double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+    double cps = cos(ps), sps = sin(ps);
+    double ux = d[0], uy = d[1], uz = d[2];
+    if (fabs(uz) > 0.99999) {
+        d[0] = st * cps;
+        d[1] = st * sps;
+        d[2] = (uz > 0 ? 1.0 : -1.0) * ct;
+    } else {
+        double s = sqrt(1.0 - uz * uz);
+        d[0] = st * (ux * uz * cps - uy * sps) / s + ux * ct;
+        d[1] = st * (uy * uz * cps + ux * sps) / s + uy * ct;
+        d[2] = -st * cps * s + uz * ct;
+    }
+    double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d[0] /= n;
+    d[1] /= n;
+    d[2] /= n;
+}
+
+/* ------------------------------------------------------------ thermal MC */
+typedef struct {
+    const or_model *m;
+    const geom_t *g;
+    const tab_t *t;
+    const or_thermal_opts *o;
+    uint64_t *tally;   /* [ncell][ngroup] fixed point */
+    double *W;         /* [ncell][ngroup] units per (kabs*s) */
+    double *Tcur;      /* [ncell][ngroup] lagged temperatures */
+    double *econv;     /* [ncell][ngroup] erg/s/g per tally unit */
+    uint64_t *visits;
+    int overflow;
+} mc_t;
+
+typedef struct {
+    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events;
+} cnt_t;
+
+static void deposit(mc_t *mc, int c, int ifr, double s) {
+    const tab_t *t = mc->t;
+    int ng = t->ngroup;
+    for (int q = 0; q < ng; ++q) {
+        double ka = mc->m->kabs[(size_t)t->grep[q] * t->nlam + ifr];
+        double v = ka * s * mc->W[(size_t)c * ng + q];
+        uint64_t add = (uint64_t)llround(v);
+        if (add) {
+            uint64_t old;
+#pragma omp atomic capture
+            {
+                old = mc->tally[(size_t)c * ng + q];
+                mc->tally[(size_t)c * ng + q] += add;
+            }
+            if (old > ((uint64_t)1 << 62)) mc->overflow = 1;
+        }
+    }
+}
+
+static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
+    const or_model *m = mc->m;
+    const geom_t *g = mc->g;
+    const tab_t *t = mc->t;
+    const int nl = t->nlam, ns = m->nspec, ng = t->ngroup;
+    rng_t r;
+    rng_init(&r, mc->o->seed, pid, 0);
+    double u = uni(&r);
+    int ks = 0;
+    while (ks < m->nstar - 1 && u > t->spick[ks]) ks++;
+    int ifr = sample_cdf(t->scdf + (size_t)ks * nl, nl, uni(&r));
+    double d[3];
+    iso_dir(&r, d);
+    pk_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = m->star[5 * ks + 2];
+    p.y = m->star[5 * ks + 3];
+    p.z = m->star[5 * ks + 4];
+    p.dx = d[0];
+    p.dy = d[1];
+    p.dz = d[2];
+    locate(g, &p);
+    if (p.ir >= g->nr) {
+        cnt->n_escape++;
+        return;
+    }
+    if (p.ir >= 0 && mc->visits) {
+#pragma omp atomic
+        mc->visits[cell_of(g, &p)]++;
+    }
+    double tau = -log(uni(&r));
+    uint64_t nev = 0;
+    for (;;) {
+        int kind;
+        double tw = next_wall(g, &p, &kind);
+        double aabs = 0, asca = 0;
+        int c = -1;
+        if (p.ir >= 0) {
+            c = cell_of(g, &p);
+            for (int s = 0; s < ns; ++s) {
+                double rho = m->rho[(size_t)s * g->ncell + c];
+                aabs += rho * m->kabs[(size_t)s * nl + ifr];
+                asca += rho * m->ksca[(size_t)s * nl + ifr];
+            }
+        }
+        double aext = aabs + asca;
+        if (aext * tw > tau) {
+            double s = tau / aext;
+            deposit(mc, c, ifr, s);
+            p.x += s * p.dx;
+            p.y += s * p.dy;
+            p.z += s * p.dz;
+            cnt->n_events++;
+            nev++;
+            double ue = uni(&r);
+            if (ue * aext < asca) {
+                cnt->n_scat++;
+                int hg = 0;
+                double gg = 0;
+                if (m->gsca) {
+                    double us = uni(&r) * asca, acc = 0;
+                    int sp = ns - 1;
+                    for (int s = 0; s < ns; ++s) {
+                        acc += m->rho[(size_t)s * g->ncell + c] * m->ksca[(size_t)s * nl + ifr];
+                        if (us <= acc) {
+                            sp = s;
+                            break;
+                        }
+                    }
+                    gg = m->gsca[(size_t)sp * nl + ifr];
+                    hg = 1;
+                }
+                if (hg) {
+                    double dd[3] = {p.dx, p.dy, p.dz};
+                    hg_dir(&r, gg, dd);
+                    p.dx = dd[0];
+                    p.dy = dd[1];
+                    p.dz = dd[2];
+                } else {
+                    iso_dir(&r, d);
+                    p.dx = d[0];
+                    p.dy = d[1];
+                    p.dz = d[2];
+                }
+            } else {
+                cnt->n_abs++;
+                double us = uni(&r) * aabs, acc = 0;
+                int sp = ns - 1;
+                for (int s = 0; s < ns; ++s) {
+                    acc += m->rho[(size_t)s * g->ncell + c] * m->kabs[(size_t)s * nl + ifr];
+                    if (us <= acc) {
+                        sp = s;
+                        break;
+                    }
+                }
+                int q = t->grp[sp];
+                double T;
+                if (mc->o->lagged) {
+                    T = mc->Tcur[(size_t)c * ng + q];
+                } else {
+                    uint64_t tv;
+#pragma omp atomic read
+                    tv = mc->tally[(size_t)c * ng + q];
+                    T = temp_from_e(t, q, (double)tv * mc->econv[(size_t)c * ng + q]);
+                }
+                double x = (T > 0) ? (log(T) - t->lnt0) / t->dlnt : 0.0;
+                if (x < 0) x = 0;
+                if (x > t->ntemp - 1) x = t->ntemp - 1;
+                int lo = (int)x;
+                if (lo > t->ntemp - 2) lo = t->ntemp - 2;
+                double w = x - lo;
+                int row = lo + (uni(&r) < w ? 1 : 0);
+                ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, uni(&r));
+                iso_dir(&r, d);
+                p.dx = d[0];
+                p.dy = d[1];
+                p.dz = d[2];
+            }
+            if (mc->o->max_events && nev >= mc->o->max_events) {
+                cnt->n_killed++;
+                return;
+            }
+            tau = -log(uni(&r));
+        } else {
+            if (c >= 0) deposit(mc, c, ifr, tw);
+            tau -= aext * tw;
+            cnt->n_cross++;
+            if (cross_wall(g, &p, tw, kind)) {
+                cnt->n_escape++;
+                return;
+            }
+            if (p.ir >= 0 && mc->visits) {
+#pragma omp atomic
+                mc->visits[cell_of(g, &p)]++;
+            }
+        }
+    }
+}
+
+static void snapshot(mc_t *mc) {
+    int ng = mc->t->ngroup, nc = mc->g->ncell;
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nc; ++c)
+        for (int q = 0; q < ng; ++q) {
+            size_t k = (size_t)c * ng + q;
+            mc->Tcur[k] = temp_from_e(mc->t, q, (double)mc->tally[k] * mc->econv[k]);
+        }
+}
+
+int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
+               uint64_t *visits, or_thermal_stats *st) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    if (m->nstar < 1 || m->nspec < 1 || m->nlam < 1) {
+        geom_free(&g);
+        return -5;
+    }
+    tab_t t;
+    rc = tab_init(&t, m, o);
+    if (rc) {
+        geom_free(&g);
+        tab_free(&t);
+        return rc;
+    }
+    int ng = t.ngroup, nc = g.ncell;
+    mc_t mc;
+    memset(&mc, 0, sizeof(mc));
+    mc.m = m;
+    mc.g = &g;
+    mc.t = &t;
+    mc.o = o;
+    mc.tally = calloc((size_t)nc * ng, sizeof(uint64_t));
+    mc.W = malloc(sizeof(double) * nc * ng);
+    mc.econv = malloc(sizeof(double) * nc * ng);
+    mc.Tcur = calloc((size_t)nc * ng, sizeof(double));
+    mc.visits = visits;
+    if (visits) memset(visits, 0, sizeof(uint64_t) * nc);
+    double eps = t.lum / (double)o->nphot;
+    for (int c = 0; c < nc; ++c) {
+        int ir = c % g.nr;
+        for (int q = 0; q < ng; ++q) {
+            double W = ldexp(1.0, TALLY_SHIFT) / (t.kmax[q] * g.dr[ir]);
+            mc.W[(size_t)c * ng + q] = W;
+            mc.econv[(size_t)c * ng + q] = eps / (W * g.vol[c]);
+        }
+    }
+    cnt_t tot;
+    memset(&tot, 0, sizeof(tot));
+    int nep = 0;
+    if (!o->lagged) {
+        for (uint64_t p = 0; p < o->nphot; ++p) run_packet(&mc, p, &tot);
+        nep = 1;
+    } else {
+#ifdef _OPENMP
+        if (o->nthreads > 0) omp_set_num_threads(o->nthreads);
+#endif
+        uint64_t done = 0;
+        uint64_t first = o->epoch_first ? o->epoch_first : 1;
+        while (done < o->nphot) {
+            uint64_t sz = (uint64_t)floor((double)done * (o->epoch_growth - 1.0));
+            if (sz < first) sz = first;
+            uint64_t end = done + sz;
+            if (end > o->nphot) end = o->nphot;
+            uint64_t a = 0, b = 0, c2 = 0, d = 0, e = 0, f = 0;
+            int64_t lo = (int64_t)done, hi = (int64_t)end;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e, f)
+            for (int64_t p = lo; p < hi; ++p) {
+                cnt_t cc;
+                memset(&cc, 0, sizeof(cc));
+                run_packet(&mc, (uint64_t)p, &cc);
+                a += cc.n_cross;
+                b += cc.n_abs;
+                c2 += cc.n_scat;
+                d += cc.n_escape;
+                e += cc.n_killed;
+                f += cc.n_events;
+            }
+            tot.n_cross += a;
+            tot.n_abs += b;
+            tot.n_scat += c2;
+            tot.n_escape += d;
+            tot.n_killed += e;
+            tot.n_events += f;
+            done = end;
+            nep++;
+            snapshot(&mc);
+        }
+    }
+    snapshot(&mc);
+    /* expected number of absorptions implied by the path-length tallies:
+     * sum_c sum_s rho_s * (kabs_s * path) */
+    double tsum = 0;
+    for (int c = 0; c < nc; ++c)
+        for (int s = 0; s < m->nspec; ++s) {
+            int q = t.grp[s];
+            tsum += (double)mc.tally[(size_t)c * ng + q] / mc.W[(size_t)c * ng + q] *
+                    m->rho[(size_t)s * nc + c];
+        }
+    for (int s = 0; s < m->nspec; ++s) {
+        int q = t.grp[s];
+        for (int c = 0; c < nc; ++c) {
+            double rho = m->rho[(size_t)s * nc + c];
+            T_out[(size_t)s * nc + c] = (rho > 0) ? mc.Tcur[(size_t)c * ng + q] : 0.0;
+        }
+    }
+    if (st) {
+        st->n_cross = tot.n_cross;
+        st->n_abs = tot.n_abs;
+        st->n_scat = tot.n_scat;
+        st->n_escape = tot.n_escape;
+        st->n_killed = tot.n_killed;
+        st->n_events = tot.n_events;
+        st->eps = eps;
+        st->lum = t.lum;
+        st->tally_tau_sum = tsum;
+        st->n_epochs = nep;
+    }
+    int ovf = mc.overflow;
+    free(mc.tally);
+    free(mc.W);
+    free(mc.econv);
+    free(mc.Tcur);
+    tab_free(&t);
+    geom_free(&g);
+    return ovf ? -20 : 0;
+}
+
+/* ---------------------------------------------------------------- camera */
+typedef struct {
+    double n[3], ex[3], ey[3];
+    double cpa, spa;
+} cam_frame_t;
+
+/* observer direction n = (sin i sin phi, -sin i cos phi, cos i);
+ * image axes ex = (cos phi, sin phi, 0), ey = n x ex (z projects upwards). */
+static void cam_frame(const or_camera *cam, cam_frame_t *f) {
+    double i = cam->incl * PI_D / 180.0, ph = cam->phi * PI_D / 180.0,
+           pa = cam->posang * PI_D / 180.0;
+    f->n[0] = sin(i) * sin(ph);
+    f->n[1] = -sin(i) * cos(ph);
+    f->n[2] = cos(i);
+    f->ex[0] = cos(ph);
+    f->ex[1] = sin(ph);
+    f->ex[2] = 0.0;
+    f->ey[0] = f->n[1] * f->ex[2] - f->n[2] * f->ex[1];
+    f->ey[1] = f->n[2] * f->ex[0] - f->n[0] * f->ex[2];
+    f->ey[2] = f->n[0] * f->ex[1] - f->n[1] * f->ex[0];
+    f->cpa = cos(pa);
+    f->spa = sin(pa);
+}
+
+typedef struct {
+    int cell;
+    double s, dtau, S, tau0;
+} seg_t;
+
+/* Integrate one ray entering from the observer side.  Collects segments when
+ * segs != NULL.  Returns intensity (mode 0) or total tau (mode 1).         */
+static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
+                            const double *kap, const or_camera *cam,
+                            const cam_frame_t *f, double X, double Y, seg_t *segs,
+                            int maxseg, int *nseg) {
+    /* sky (X,Y) -> model image plane (x', y') rotated by -posang */
+    double xp = X * f->cpa + Y * f->spa;
+    double yp = -X * f->spa + Y * f->cpa;
+    double R = g->rw[g->nr];
+    double o[3], d[3];
+    for (int a = 0; a < 3; ++a) {
+        o[a] = xp * f->ex[a] + yp * f->ey[a] + 2.0 * R * f->n[a];
+        d[a] = -f->n[a];
+    }
+    if (nseg) *nseg = 0;
+    double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    double b = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
+    double D = b * b - (rr - R * R);
+    if (D <= 0) return 0.0;
+    double tin = -b - sqrt(D);
+    pk_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = o[0] + tin * d[0];
+    p.y = o[1] + tin * d[1];
+    p.z = o[2] + tin * d[2];
+    p.dx = d[0];
+    p.dy = d[1];
+    p.dz = d[2];
+    snap_radius(&p, R);
+    p.ir = g->nr - 1;
+    locate_angles(g, &p);
+    double I = 0, tau = 0;
+    int ns = m->nspec, n = 0;
+    double nu = C_C / (cam->lam_um * 1e-4);
+    for (int guard = 0; guard < 100000000; ++guard) {
+        int kind;
+        double tw = next_wall(g, &p, &kind);
+        if (p.ir >= 0) {
+            int c = cell_of(g, &p);
+            double a = 0;
+            for (int s = 0; s < ns; ++s) a += m->rho[(size_t)s * g->ncell + c] * kap[s];
+            double dtau = a * tw;
+            double S = or_planck(nu, T[c]);
+            if (cam->mode == 0) I += exp(-tau) * (-expm1(-dtau)) * S;
+            if (segs && n < maxseg) {
+                segs[n].cell = c;
+                segs[n].s = tw;
+                segs[n].dtau = dtau;
+                segs[n].S = S;
+                segs[n].tau0 = tau;
+            }
+            n++;
+            tau += dtau;
+            if (cam->mode == 0 && tau > cam->tau_stop) break;
+        }
+        if (cross_wall(g, &p, tw, kind)) break;
+    }
+    if (nseg) *nseg = n;
+    return cam->mode == 0 ? I : tau;
+}
+
+/* optical depth from point q to the observer along +n */
+static double star_tau(const or_model *m, const geom_t *g, const double *kap,
+                       const double *q, const double *n, seg_t *segs, int maxseg,
+                       int *nseg) {
+    pk_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = q[0];
+    p.y = q[1];
+    p.z = q[2];
+    p.dx = n[0];
+    p.dy = n[1];
+    p.dz = n[2];
+    locate(g, &p);
+    double tau = 0;
+    int k = 0;
+    while (p.ir >= -1 && p.ir < g->nr) {
+        int kind;
+        double tw = next_wall(g, &p, &kind);
+        if (p.ir >= 0) {
+            int c = cell_of(g, &p);
+            double a = 0;
+            for (int s = 0; s < m->nspec; ++s) a += m->rho[(size_t)s * g->ncell + c] * kap[s];
+            if (segs && k < maxseg) {
+                segs[k].cell = c;
+                segs[k].s = tw;
+                segs[k].dtau = a * tw;
+            }
+            k++;
+            tau += a * tw;
+        }
+        if (cross_wall(g, &p, tw, kind)) break;
+    }
+    if (nseg) *nseg = k;
+    return tau;
+}
+
+static int star_pixel(const or_camera *cam, const cam_frame_t *f, const double *q,
+                      int *jj, int *kk) {
+    double xp = q[0] * f->ex[0] + q[1] * f->ex[1] + q[2] * f->ex[2];
+    double yp = q[0] * f->ey[0] + q[1] * f->ey[1] + q[2] * f->ey[2];
+    /* model plane -> sky: rotate by +posang */
+    double X = xp * f->cpa - yp * f->spa;
+    double Y = xp * f->spa + yp * f->cpa;
+    int k = (int)floor(X / cam->pix_cm + 0.5) + cam->nx / 2;
+    int j = (int)floor(Y / cam->pix_cm + 0.5) + cam->ny / 2;
+    if (k < 0 || k >= cam->nx || j < 0 || j >= cam->ny) return 0;
+    *jj = j;
+    *kk = k;
+    return 1;
+}
+
+int or_image(const or_model *m, const double *T, const double *kap,
+             const or_camera *cam, double *img) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    cam_frame_t f;
+    cam_frame(cam, &f);
+    int nsub = cam->nsub > 0 ? cam->nsub : 1;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int j = 0; j < cam->ny; ++j)
+        for (int k = 0; k < cam->nx; ++k) {
+            double acc = 0;
+            for (int a = 0; a < nsub; ++a)
+                for (int b = 0; b < nsub; ++b) {
+                    double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
+                    double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
+                    acc += trace_cam_ray(m, &g, T, kap, cam, &f, X, Y, NULL, 0, NULL);
+                }
+            img[(size_t)j * cam->nx + k] = acc / (nsub * nsub);
+        }
+    if (cam->incl_star && cam->mode == 0) {
+        double nu = C_C / (cam->lam_um * 1e-4);
+        for (int s = 0; s < m->nstar; ++s) {
+            const double *q = m->star + 5 * s + 2;
+            int jj, kk;
+            if (!star_pixel(cam, &f, q, &jj, &kk)) continue;
+            double R = m->star[5 * s], Ts = m->star[5 * s + 1];
+            double F = PI_D * R * R * or_planck(nu, Ts) / (cam->pix_cm * cam->pix_cm);
+            img[(size_t)jj * cam->nx + kk] += F * exp(-star_tau(m, &g, kap, q, f.n, NULL, 0, NULL));
+        }
+    }
+    geom_free(&g);
+    return 0;
+}
+
+int or_image_adjoint(const or_model *m, const double *T, const double *kap,
+                     const or_camera *cam, const double *dimg, double *g_logrho,
+                     double *g_T) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    cam_frame_t f;
+    cam_frame(cam, &f);
+    int nc = g.ncell, ns = m->nspec;
+    memset(g_logrho, 0, sizeof(double) * ns * nc);
+    memset(g_T, 0, sizeof(double) * nc);
+    int nsub = cam->nsub > 0 ? cam->nsub : 1;
+    int maxseg = 4 * (g.nr + g.nt + g.np) + 64;
+    seg_t *segs = malloc(sizeof(seg_t) * maxseg);
+    double nu = C_C / (cam->lam_um * 1e-4);
+    for (int j = 0; j < cam->ny; ++j)
+        for (int k = 0; k < cam->nx; ++k) {
+            double gI = dimg[(size_t)j * cam->nx + k] / (nsub * nsub);
+            if (gI == 0) continue;
+            for (int a = 0; a < nsub; ++a)
+                for (int b = 0; b < nsub; ++b) {
+                    double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
+                    double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
+                    int n;
+                    double I = trace_cam_ray(m, &g, T, kap, cam, &f, X, Y, segs, maxseg, &n);
+                    if (n > maxseg) {
+                        free(segs);
+                        geom_free(&g);
+                        return -30;
+                    }
+                    double cum = 0;
+                    for (int q = 0; q < n; ++q) {
+                        seg_t *sg = &segs[q];
+                        double att = exp(-sg->tau0);
+                        double contrib = att * (-expm1(-sg->dtau)) * sg->S;
+                        cum += contrib;
+                        double Rk = I - cum;
+                        double dI_ddtau = att * exp(-sg->dtau) * sg->S - Rk;
+                        for (int s = 0; s < ns; ++s) {
+                            double rho = m->rho[(size_t)s * nc + sg->cell];
+                            g_logrho[(size_t)s * nc + sg->cell] +=
+                                gI * dI_ddtau * sg->s * rho * kap[s];
+                        }
+                        g_T[sg->cell] += gI * att * (-expm1(-sg->dtau)) *
+                                         or_dplanck_dT(nu, T[sg->cell]);
+                    }
+                }
+        }
+    if (cam->incl_star && cam->mode == 0) {
+        for (int s = 0; s < m->nstar; ++s) {
+            const double *q = m->star + 5 * s + 2;
+            int jj, kk;
+            if (!star_pixel(cam, &f, q, &jj, &kk)) continue;
+            double gI = dimg[(size_t)jj * cam->nx + kk];
+            double R = m->star[5 * s], Ts = m->star[5 * s + 1];
+            double F = PI_D * R * R * or_planck(nu, Ts) / (cam->pix_cm * cam->pix_cm);
+            int n;
+            double tau = star_tau(m, &g, kap, q, f.n, segs, maxseg, &n);
+            double Is = F * exp(-tau);
+            for (int q2 = 0; q2 < n && q2 < maxseg; ++q2)
+                for (int sp = 0; sp < ns; ++sp) {
+                    double rho = m->rho[(size_t)sp * nc + segs[q2].cell];
+                    g_logrho[(size_t)sp * nc + segs[q2].cell] +=
+                        gI * (-Is) * segs[q2].s * rho * kap[sp];
+                }
+        }
+    }
+    free(segs);
+    geom_free(&g);
+    return 0;
+}
+
+/* ---------------------------------------------------------- visibilities */
+void or_vis_dft(const double *img, int nx, int ny, double dxy, const double *u,
+                const double *v, int K, double dra, double ddec, double *re,
+                double *im) {
+#pragma omp parallel
+    {
+        double *ax = malloc(sizeof(double) * 2 * nx), *by = malloc(sizeof(double) * 2 * ny);
+#pragma omp for schedule(static)
+        for (int q = 0; q < K; ++q) {
+            for (int k = 0; k < nx; ++k) {
+                double ph = -TWO_PI_D * u[q] * (k - nx / 2) * dxy;
+                ax[2 * k] = cos(ph);
+                ax[2 * k + 1] = sin(ph);
+            }
+            for (int j = 0; j < ny; ++j) {
+                double ph = -TWO_PI_D * v[q] * (j - ny / 2) * dxy;
+                by[2 * j] = cos(ph);
+                by[2 * j + 1] = sin(ph);
+            }
+            double sr = 0, si = 0;
+            for (int j = 0; j < ny; ++j) {
+                double rr = 0, ri = 0;
+                const double *row = img + (size_t)j * nx;
+                for (int k = 0; k < nx; ++k) {
+                    rr += row[k] * ax[2 * k];
+                    ri += row[k] * ax[2 * k + 1];
+                }
+                sr += rr * by[2 * j] - ri * by[2 * j + 1];
+                si += rr * by[2 * j + 1] + ri * by[2 * j];
+            }
+            double ph = -TWO_PI_D * (u[q] * dra + v[q] * ddec);
+            double c = cos(ph), s = sin(ph);
+            re[q] = sr * c - si * s;
+            im[q] = sr * s + si * c;
+        }
+        free(ax);
+        free(by);
+    }
+}
+
+void or_vis_dft_adjoint(const double *gre, const double *gim, int nx, int ny,
+                        double dxy, const double *u, const double *v, int K,
+                        double dra, double ddec, double *g_img) {
+    memset(g_img, 0, sizeof(double) * nx * ny);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nx; ++k) {
+            double acc = 0;
+            for (int q = 0; q < K; ++q) {
+                double ph = -TWO_PI_D * (u[q] * ((k - nx / 2) * dxy + dra) +
+                                         v[q] * ((j - ny / 2) * dxy + ddec));
+                acc += gre[q] * cos(ph) + gim[q] * sin(ph);
+            }
+            g_img[(size_t)j * nx + k] = acc;
+        }
+}
+
+double or_chisq(const double *dre, const double *dim, const double *w,
+                const double *mre, const double *mim, int K) {
+    double s = 0;
+    for (int q = 0; q < K; ++q) {
+        double a = dre[q] - mre[q], b = dim[q] - mim[q];
+        s += (a * a + b * b) * w[q];
+    }
+    return s;
+}

@@ -1,0 +1,15 @@
+"""pdspy_amd -- MI355X (gfx950) continuum radiative transfer for pdspy models.
+
+A drop-in beside pdspy's RADMC-3D path: ``run_thermal``, ``run_image``,
+``run_visibilities`` and ``interpolate_model`` take pdspy ``Model`` objects and
+fill the same outputs; the work runs in hand-written HIP kernels reached
+through a C ABI (``include/pdspy_amd.h``).  ``GPUModelMixin`` adds
+``code="mi355x"`` to a pdspy Model subclass.
+"""
+from .rt import (DeviceModel, GPUModelMixin, chisq, fft_grid_uv, imaging_temperature,
+                 interpolate_model, model_arrays, run_image, run_thermal, run_visibilities,
+                 vis_dft, vis_dft_adjoint)
+
+__all__ = ["DeviceModel", "GPUModelMixin", "chisq", "fft_grid_uv", "imaging_temperature",
+           "interpolate_model", "model_arrays", "run_image", "run_thermal", "run_visibilities",
+           "vis_dft", "vis_dft_adjoint"]

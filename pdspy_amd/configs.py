@@ -1,0 +1,98 @@
+"""Synthetic inputs for the benchmark workloads, built with the pdspy-style
+builders in :mod:`pdspy_amd.inputs` exactly as a pdspy user would
+(YSOModel.set_spherical_grid takes WALL counts, so n cells = n+1 walls).
+
+* ``disk_c1``      -- flared disk, 64x64x1 cells, 0.1-100 au, 1e-4 Msun, Sigma ~ r^-1.
+* ``disk_envelope`` -- disk (1e-3 Msun, Sigma ~ r^-0.5) + Ulrich envelope with a
+  20 deg cavity, 256x128x1 cells, 0.1-5000 au, two species.
+* ``spiral_3d``    -- the disk of ``disk_envelope`` with a two-armed log spiral,
+  256x128x128 cells, Henyey-Greenstein g = 0.6.
+* ``uv_points``    -- log-uniform baselines for the visibility workload.
+* ``sampler_batch`` -- parameter draws for a batched sampler step.
+"""
+import numpy as np
+
+from .inputs import AU, Dust, Model
+
+
+def powerlaw_dust(lam_um, g=None):
+    """kappa_abs = 1e3 (lam/1um)^-1 for lam > 1 um, flat below; albedo 0.5."""
+    lam_um = np.asarray(lam_um, dtype=float)
+    kabs = np.where(lam_um > 1.0, 1e3 * lam_um ** -1.0, 1e3)
+    d = Dust()
+    d.set_properties(lam_um * 1e-4, kabs, kabs.copy(),
+                     g=None if g is None else np.full_like(kabs, g))
+    return d
+
+
+def disk_c1(nr=64, nt=64, nphi=1, rmax=100.0):
+    m = Model()
+    m.set_spherical_grid(0.1, rmax, nr + 1, nt + 1, nphi + 1)
+    m.add_star(mass=0.5, luminosity=0.906329, temperature=4000.)
+    m.grid.set_wavelength_grid(0.1, 1.0e4, 100, log=True)
+    d = powerlaw_dust(m.grid.lam)
+    m.add_disk(mass=1e-4, rmin=0.1, rmax=100., plrho=2.25, h0=0.1, plh=1.25, dust=d)
+    return m
+
+
+def disk_envelope(nr=256, nt=128, nphi=1, mdisk=1e-3, plrho=1.75, plh=1.25, menv=0.1,
+                  rcent=50., envelope=True):
+    m = Model()
+    m.set_spherical_grid(0.1, 5000., nr + 1, nt + 1, nphi + 1)
+    m.add_star(mass=0.5, luminosity=0.906329, temperature=4000.)
+    m.grid.set_wavelength_grid(0.1, 1.0e4, 100, log=True)
+    d = powerlaw_dust(m.grid.lam)
+    m.add_disk(mass=mdisk, rmin=0.1, rmax=100., plrho=plrho, h0=0.1, plh=plh, dust=d)
+    if envelope:
+        m.add_ulrichextended_envelope(mass=menv, rmin=0.1, rmax=5000., rcent=rcent, cavpl=1.0,
+                                      cavrfact=0.01, theta_open=20., zoffset=1., dust=d,
+                                      rcent_ne_rdisk=True)
+    return m
+
+
+def spiral_3d(nr=256, nt=128, nphi=128, pitch_deg=15.0, amp=0.3, g=0.6, r0=1.0):
+    m = Model()
+    m.set_spherical_grid(0.1, 5000., nr + 1, nt + 1, nphi + 1)
+    m.add_star(mass=0.5, luminosity=0.906329, temperature=4000.)
+    m.grid.set_wavelength_grid(0.1, 1.0e4, 100, log=True)
+    d = powerlaw_dust(m.grid.lam, g=g)
+    m.add_disk(mass=1e-3, rmin=0.1, rmax=100., plrho=1.75, h0=0.1, plh=1.25, dust=d)
+    rr, tt, pp = np.meshgrid(m.grid.r, m.grid.theta, m.grid.phi, indexing="ij")
+    R = np.maximum(rr * np.sin(tt), 1e-30)
+    pert = 1.0 + amp * np.cos(2.0 * (pp - np.log(R / r0) / np.tan(np.radians(pitch_deg))))
+    m.grid.density[0] = m.grid.density[0] * pert
+    return m
+
+
+def uv_points(K=50000, bmin_m=15.0, bmax_m=2000.0, lam_m=1.3e-3, seed=0):
+    rng = np.random.default_rng(seed)
+    b = 10 ** rng.uniform(np.log10(bmin_m), np.log10(bmax_m), K)
+    pa = rng.uniform(0.0, np.pi, K)
+    return b * np.cos(pa) / lam_m, b * np.sin(pa) / lam_m
+
+
+def sampler_batch(n=64, seed=0):
+    """Parameter draws of one batched sampler step (log-uniform masses)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        out.append(dict(
+            mdisk=10 ** rng.uniform(-5, -2),
+            gamma=rng.uniform(0.0, 1.5),
+            plh=rng.uniform(1.0, 1.3),
+            menv=10 ** rng.uniform(-3, np.log10(0.3)),
+            incl=rng.uniform(0.0, 90.0),
+            nr=int(rng.choice([64, 128, 256])),
+            nt=int(rng.choice([64, 128])),
+            nphot=int(10 ** rng.uniform(6, 7)),
+        ))
+    return out
+
+
+def batch_model(p):
+    return disk_envelope(nr=p["nr"], nt=p["nt"], mdisk=p["mdisk"],
+                         plrho=p["gamma"] + p["plh"], plh=p["plh"], menv=p["menv"])
+
+
+__all__ = ["powerlaw_dust", "disk_c1", "disk_envelope", "spiral_3d", "uv_points",
+           "sampler_batch", "batch_model", "AU"]

@@ -1,0 +1,348 @@
+// image.hip -- ray-traced images and their adjoint on gfx950: what RADMC-3D's
+// camera computes for pdspy's Model.run_image (pdspy/modeling/Model.py:515-577,
+// radmc3d/run.py:157-289) with scattering_mode_max=0, i.e. the formal solution
+//     I = sum_k S_k (1 - exp(-dtau_k)) exp(-tau_k),  S = B_nu(T),
+// along parallel rays, plus blackbody point stars attenuated by the model
+// (pdspy's nostar=False).  Pixel (j,k) is centred at ((k - nx/2) pix,
+// (j - ny/2) pix) in the sky plane, the centring pdspy's zoomau produces
+// (Model.py:522-527).
+//
+// Kernels: a per-cell prep pass (alpha = sum_s rho_s kappa_s, S = B_nu(T)) so
+// the ray loop reads 8 bytes per crossed cell; one ray per lane; the adjoint
+// re-integrates each ray twice (first for I, then for dI/d dtau_k = e^-tau_k
+// e^-dtau_k S_k - (I - I_<=k)) instead of storing tau, and accumulates
+// dL/dalpha, dL/dS per cell with FP64 atomics; a post pass applies the chain
+// rule to log(rho_s) and T.
+#include <cmath>
+
+#include "rt_internal.h"
+
+using namespace rt;
+
+struct CamParams {
+    DevGrid g;
+    float n[3], ex[3], ey[3];
+    float cpa, spa;
+    float pix;       // au
+    int nx, ny, nsub, mode;
+    float tau_stop;
+    const float *alpha;  // [cell] per au
+    const float *src;    // [cell]
+};
+
+__device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, Pos &p) {
+    float xp = X * C.cpa + Y * C.spa;
+    float yp = -X * C.spa + Y * C.cpa;
+    float R = C.g.rw[C.g.nr];
+    float q2 = xp * xp + yp * yp;
+    if (q2 >= C.g.rw2[C.g.nr]) return false;
+    float h = sqrtf(C.g.rw2[C.g.nr] - q2);
+    p.x = xp * C.ex[0] + yp * C.ey[0] + h * C.n[0];
+    p.y = xp * C.ex[1] + yp * C.ey[1] + h * C.n[1];
+    p.z = xp * C.ex[2] + yp * C.ey[2] + h * C.n[2];
+    p.dx = -C.n[0];
+    p.dy = -C.n[1];
+    p.dz = -C.n[2];
+    snap_radius(p, R);
+    p.ir = C.g.nr - 1;
+    locate_angles(C.g, p);
+    return true;
+}
+
+__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
+    float I = 0.f, tau = 0.f;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+        int kind;
+        float tw = next_wall(C.g, p, kind);
+        if (p.ir >= 0) {
+            int c = cell_index(C.g, p);
+            float dtau = C.alpha[c] * tw;
+            if (C.mode == 0) {
+                I = fmaf(__expf(-tau) * (-expm1f(-dtau)), C.src[c], I);
+                tau += dtau;
+                if (tau > C.tau_stop) break;
+            } else {
+                tau += dtau;
+            }
+        }
+        if (cross_wall(C.g, p, tw, kind)) break;
+    }
+    return C.mode == 0 ? I : tau;
+}
+
+__global__ void __launch_bounds__(64) k_image(CamParams C, float *img) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= C.nx * C.ny) return;
+    int j = pix / C.nx, k = pix - j * C.nx;
+    float acc = 0.f;
+    for (int a = 0; a < C.nsub; ++a)
+        for (int b = 0; b < C.nsub; ++b) {
+            float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
+            float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
+            Pos p;
+            if (ray_start(C, X, Y, p)) acc += integrate_ray(C, p);
+        }
+    img[pix] = acc / (float)(C.nsub * C.nsub);
+}
+
+__global__ void __launch_bounds__(64) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
+                                                  double *g_src) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= C.nx * C.ny) return;
+    float gI = dimg[pix] / (float)(C.nsub * C.nsub);
+    if (gI == 0.f) return;
+    int j = pix / C.nx, k = pix - j * C.nx;
+    for (int a = 0; a < C.nsub; ++a)
+        for (int b = 0; b < C.nsub; ++b) {
+            float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
+            float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
+            Pos p0;
+            if (!ray_start(C, X, Y, p0)) continue;
+            float I = integrate_ray(C, p0);
+            // reverse sweep: recompute tau front to back, R_k = I - I_<=k
+            Pos p = p0;
+            float tau = 0.f, cum = 0.f;
+            for (int guard = 0; guard < (1 << 24); ++guard) {
+                int kind;
+                float tw = next_wall(C.g, p, kind);
+                if (p.ir >= 0) {
+                    int c = cell_index(C.g, p);
+                    float dtau = C.alpha[c] * tw;
+                    float att = __expf(-tau);
+                    float om = -expm1f(-dtau);
+                    float S = C.src[c];
+                    cum = fmaf(att * om, S, cum);
+                    float Rk = I - cum;
+                    float dI_ddtau = att * (1.f - om) * S - Rk;
+                    unsafeAtomicAdd(&g_alpha[c], (double)(gI * dI_ddtau * tw));
+                    unsafeAtomicAdd(&g_src[c], (double)(gI * att * om));
+                    tau += dtau;
+                    if (tau > C.tau_stop) break;
+                }
+                if (cross_wall(C.g, p, tw, kind)) break;
+            }
+        }
+}
+
+// stars: one thread per star; adds F exp(-tau) to the pixel holding the star,
+// or (adjoint) dL/dalpha along the star's line of sight
+__global__ void k_stars(CamParams C, int nstar, const float *spos, const float *sflux, float *img,
+                        const float *dimg, double *g_alpha) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nstar) return;
+    float qx = spos[3 * s], qy = spos[3 * s + 1], qz = spos[3 * s + 2];
+    float xp = qx * C.ex[0] + qy * C.ex[1] + qz * C.ex[2];
+    float yp = qx * C.ey[0] + qy * C.ey[1] + qz * C.ey[2];
+    float X = xp * C.cpa - yp * C.spa;
+    float Y = xp * C.spa + yp * C.cpa;
+    int k = (int)floorf(X / C.pix + 0.5f) + C.nx / 2;
+    int j = (int)floorf(Y / C.pix + 0.5f) + C.ny / 2;
+    if (k < 0 || k >= C.nx || j < 0 || j >= C.ny) return;
+    Pos p;
+    p.x = qx;
+    p.y = qy;
+    p.z = qz;
+    p.dx = C.n[0];
+    p.dy = C.n[1];
+    p.dz = C.n[2];
+    locate(C.g, p);
+    float tau = 0.f;
+    Pos p0 = p;
+    for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
+        int kind;
+        float tw = next_wall(C.g, p, kind);
+        if (p.ir >= 0) tau += C.alpha[cell_index(C.g, p)] * tw;
+        if (cross_wall(C.g, p, tw, kind)) break;
+    }
+    float Is = sflux[s] * __expf(-tau);
+    if (img) {
+        atomicAdd(&img[j * C.nx + k], Is);
+        return;
+    }
+    float gI = dimg[j * C.nx + k];
+    if (gI == 0.f) return;
+    p = p0;
+    for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
+        int kind;
+        float tw = next_wall(C.g, p, kind);
+        if (p.ir >= 0) unsafeAtomicAdd(&g_alpha[cell_index(C.g, p)], (double)(-gI * Is * tw));
+        if (cross_wall(C.g, p, tw, kind)) break;
+    }
+}
+
+struct KapArg {
+    float k[RT_MAX_SPEC];
+};
+
+// alpha (per au) and S = B_nu(T) per cell
+__global__ void k_cam_prep(const float *rho, int S, int ncell, KapArg kap, float lenu, const float *T,
+                           float pre, float hnu_k, int mode, float *alpha, float *src) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    float a = 0.f;
+    for (int s = 0; s < S; ++s) a = fmaf(rho[(size_t)s * ncell + c], kap.k[s], a);
+    alpha[c] = a * lenu;
+    src[c] = mode == 0 ? planck_f(pre, hnu_k, T[c]) : 0.f;
+}
+
+__global__ void k_cam_post(const double *g_alpha, const double *g_src, const float *rho, int S,
+                           int ncell, KapArg kap, float lenu, const float *T, float pre,
+                           float hnu_k, float *g_logrho, float *g_T) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    double ga = g_alpha[c];
+    for (int s = 0; s < S; ++s)
+        g_logrho[(size_t)s * ncell + c] =
+            (float)(ga * (double)rho[(size_t)s * ncell + c] * (double)kap.k[s] * (double)lenu);
+    g_T[c] = (float)(g_src[c] * (double)dplanck_f(pre, hnu_k, T[c]));
+}
+
+__global__ void k_zero_f64(double *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// opacity at the camera wavelength: log-log interpolation of the model's table
+// (power-law extrapolation beyond the ends)
+static double kappa_at(const rt_model *m, const std::vector<double> &tab, int s, double lam) {
+    const int nl = m->nlam;
+    const double *k = &tab[(size_t)s * nl];
+    if (nl == 1) return k[0];
+    int i = 0;
+    if (lam <= m->lam[0])
+        i = 0;
+    else if (lam >= m->lam[nl - 1])
+        i = nl - 2;
+    else
+        while (i < nl - 2 && m->lam[i + 1] < lam) ++i;
+    double l0 = std::log(m->lam[i]), l1 = std::log(m->lam[i + 1]);
+    if (!(k[i] > 0) || !(k[i + 1] > 0)) {
+        double w = (lam - m->lam[i]) / (m->lam[i + 1] - m->lam[i]);
+        return k[i] + w * (k[i + 1] - k[i]);
+    }
+    double w = (std::log(lam) - l0) / (l1 - l0);
+    return std::exp(std::log(k[i]) + w * (std::log(k[i + 1]) - std::log(k[i])));
+}
+
+static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg &kap, float &pre,
+                        float &hnu_k) {
+    rt_ctx *ctx = m->ctx;
+    if (!cam || cam->nx < 1 || cam->ny < 1 || !(cam->pix_cm > 0) || !(cam->lam_um > 0))
+        return set_err(ctx, RT_E_ARG, "invalid camera");
+    if (cam->mode != 0 && cam->mode != 1) return set_err(ctx, RT_E_ARG, "camera mode must be 0 or 1");
+    const double d2r = M_PI / 180.0;
+    double i = cam->incl_deg * d2r, ph = cam->phi_deg * d2r, pa = cam->posang_deg * d2r;
+    double n[3] = {std::sin(i) * std::sin(ph), -std::sin(i) * std::cos(ph), std::cos(i)};
+    double ex[3] = {std::cos(ph), std::sin(ph), 0.0};
+    double ey[3] = {n[1] * ex[2] - n[2] * ex[1], n[2] * ex[0] - n[0] * ex[2], n[0] * ex[1] - n[1] * ex[0]};
+    C.g = dev_grid(m);
+    for (int a = 0; a < 3; ++a) {
+        C.n[a] = (float)n[a];
+        C.ex[a] = (float)ex[a];
+        C.ey[a] = (float)ey[a];
+    }
+    C.cpa = (float)std::cos(pa);
+    C.spa = (float)std::sin(pa);
+    C.pix = (float)(cam->pix_cm / RT_LEN_UNIT);
+    C.nx = cam->nx;
+    C.ny = cam->ny;
+    C.nsub = cam->nsub > 0 ? cam->nsub : 1;
+    C.mode = cam->mode;
+    C.tau_stop = (float)(cam->tau_stop > 0 ? cam->tau_stop : 40.0);
+    for (int s = 0; s < RT_MAX_SPEC; ++s)
+        kap.k[s] = s < m->nspec ? (float)kappa_at(m, m->kabs, s, cam->lam_um) : 0.f;
+    double nu = 2.99792458e10 / (cam->lam_um * 1e-4);
+    pre = (float)(2.0 * 6.6260755e-27 * nu * nu * nu / (2.99792458e10 * 2.99792458e10));
+    hnu_k = (float)(6.6260755e-27 * nu / 1.380658e-16);
+    size_t nc = (size_t)m->ncell;
+    if (!m->d_alpha) {
+        HIPCHK(ctx, hipMalloc(&m->d_alpha, nc * sizeof(float)));
+        HIPCHK(ctx, hipMalloc(&m->d_src, nc * sizeof(float)));
+        HIPCHK(ctx, hipMalloc(&m->d_galpha, nc * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&m->d_gsrc, nc * sizeof(double)));
+    }
+    C.alpha = m->d_alpha;
+    C.src = m->d_src;
+    return 0;
+}
+
+static void star_fluxes(const rt_model *m, const rt_camera *cam, std::vector<float> &fl) {
+    double nu = 2.99792458e10 / (cam->lam_um * 1e-4);
+    fl.resize(m->nstar);
+    for (int s = 0; s < m->nstar; ++s) {
+        double R = m->stars[5 * s], T = m->stars[5 * s + 1];
+        double x = 6.6260755e-27 * nu / (1.380658e-16 * T);
+        double B = 2.0 * 6.6260755e-27 * nu * nu * nu / (2.99792458e10 * 2.99792458e10) / std::expm1(x);
+        fl[s] = (float)(M_PI * R * R * B / (cam->pix_cm * cam->pix_cm));
+    }
+}
+
+int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho, float *d_img) {
+    rt_ctx *ctx = m->ctx;
+    CamParams C{};
+    KapArg kap{};
+    float pre, hnu_k;
+    int rc = setup_camera(m, cam, C, kap, pre, hnu_k);
+    if (rc) return rc;
+    if (!d_T && cam->mode == 0) return set_err(ctx, RT_E_ARG, "temperature pointer is NULL");
+    const float *rho = d_rho ? d_rho : m->d_rho;
+    hipStream_t sm = ctx->stream;
+    hipLaunchKernelGGL(k_cam_prep, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, rho, m->nspec,
+                       m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->mode, m->d_alpha,
+                       m->d_src);
+    int npx = cam->nx * cam->ny;
+    hipLaunchKernelGGL(k_image, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_img);
+    if (cam->incl_star && cam->mode == 0 && m->nstar > 0) {
+        std::vector<float> fl;
+        star_fluxes(m, cam, fl);
+        float *d_fl = nullptr;
+        HIPCHK(ctx, hipMalloc(&d_fl, sizeof(float) * m->nstar));
+        HIPCHK(ctx, hipMemcpyAsync(d_fl, fl.data(), sizeof(float) * m->nstar, hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, d_fl, d_img,
+                           (const float *)nullptr, (double *)nullptr);
+        HIPCHK(ctx, hipStreamSynchronize(sm));
+        HIPCHK(ctx, hipFree(d_fl));
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,
+                      const float *d_dimg, float *d_g_logrho, float *d_g_T) {
+    rt_ctx *ctx = m->ctx;
+    if (cam && cam->mode != 0) return set_err(ctx, RT_E_ARG, "adjoint needs mode 0 (intensity)");
+    CamParams C{};
+    KapArg kap{};
+    float pre, hnu_k;
+    int rc = setup_camera(m, cam, C, kap, pre, hnu_k);
+    if (rc) return rc;
+    if (!d_T || !d_dimg || !d_g_logrho || !d_g_T) return set_err(ctx, RT_E_ARG, "NULL pointer");
+    const float *rho = d_rho ? d_rho : m->d_rho;
+    hipStream_t sm = ctx->stream;
+    size_t nc = (size_t)m->ncell;
+    hipLaunchKernelGGL(k_cam_prep, dim3(nblk(nc, 256)), dim3(256), 0, sm, rho, m->nspec, m->ncell,
+                       kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, 0, m->d_alpha, m->d_src);
+    hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, nc);
+    hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
+    int npx = cam->nx * cam->ny;
+    hipLaunchKernelGGL(k_image_adj, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_dimg, m->d_galpha,
+                       m->d_gsrc);
+    if (cam->incl_star && m->nstar > 0) {
+        std::vector<float> fl;
+        star_fluxes(m, cam, fl);
+        float *d_fl = nullptr;
+        HIPCHK(ctx, hipMalloc(&d_fl, sizeof(float) * m->nstar));
+        HIPCHK(ctx, hipMemcpyAsync(d_fl, fl.data(), sizeof(float) * m->nstar, hipMemcpyHostToDevice, sm));
+        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, d_fl,
+                           (float *)nullptr, d_dimg, m->d_galpha);
+        HIPCHK(ctx, hipStreamSynchronize(sm));
+        HIPCHK(ctx, hipFree(d_fl));
+    }
+    hipLaunchKernelGGL(k_cam_post, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, m->d_gsrc,
+                       rho, m->nspec, m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k,
+                       d_g_logrho, d_g_T);
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
+}

@@ -1,0 +1,436 @@
+// rt_device.h -- device helpers shared by the gfx950 kernels: Philox4x32-10,
+// the spherical-grid traversal (FP32 with FP64 re-evaluation of near-tangent
+// sphere/cone crossings), Planck functions and wave-level helpers.
+//
+// Geometry follows the grids pdspy writes for RADMC-3D (radmc3d/write.py:198-244):
+// r walls (lengths in au on the device), theta walls in [0, pi/2] with an
+// equatorial mirror (the lower half-space folds onto the upper one) or [0, pi],
+// phi walls over [0, 2 pi].  Cells are indexed c = ir + nr*(it + nt*ip)
+// (r fastest, dust_density.inp order, write.py:259-264).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define RT_INF 3.0e38f
+
+namespace rt {
+
+// ---------------------------------------------------------------- Philox
+// Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as easy as 1, 2, 3"
+// (SC'11): 10 rounds of the 4x32 bijection, Weyl key schedule.
+struct Philox {
+    uint32_t k0, k1;
+    uint32_t c0, c1, c2;  // block counter, packet id lo / hi
+    uint32_t stream;
+    uint32_t buf[4];
+    int idx;
+};
+
+__device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2,
+                                             uint32_t c3, uint32_t k0, uint32_t k1,
+                                             uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+__device__ __forceinline__ void rng_init(Philox &r, uint64_t seed, uint64_t pid,
+                                         uint32_t stream) {
+    r.k0 = (uint32_t)seed;
+    r.k1 = (uint32_t)(seed >> 32);
+    r.c0 = 0;
+    r.c1 = (uint32_t)pid;
+    r.c2 = (uint32_t)(pid >> 32);
+    r.stream = stream;
+    r.idx = 4;
+}
+
+// uniform in (0,1) from the top 24 bits, exact in FP32
+__device__ __forceinline__ float uni(Philox &r) {
+    if (r.idx == 4) {
+        philox_block(r.c0, r.c1, r.c2, r.stream, r.k0, r.k1, r.buf);
+        r.c0++;
+        r.idx = 0;
+    }
+    uint32_t x;
+    // register-indexed select (avoids scratch for a dynamic array index)
+    x = (r.idx == 0) ? r.buf[0] : (r.idx == 1) ? r.buf[1] : (r.idx == 2) ? r.buf[2] : r.buf[3];
+    r.idx++;
+    return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+// ---------------------------------------------------------------- grid
+struct DevGrid {
+    int nr, nt, np, mirror, ncell;
+    const float *rw;    // nr+1 radial walls (au)
+    const float *rw2;   // their squares
+    const double *rwd;  // FP64 walls
+    const float *tw;    // nt+1 theta walls
+    const float *ct;    // cos of theta walls (ct[nt] = 0 exactly in mirror mode)
+    const float *st;    // sin
+    const double *ctd;  // FP64 cos/sin for the near-tangent re-evaluation
+    const double *std_;
+    const float *pw;    // np+1 phi walls
+    const float *cp;
+    const float *sp;
+};
+
+enum { W_NONE = 0, W_ROUT, W_RIN, W_TLO, W_THI, W_PLO, W_PHI, W_HOLE };
+
+struct Pos {
+    float x, y, z, dx, dy, dz;
+    int ir, it, ip, hemi;  // ir = -1: inner hole; ir = nr: outside
+};
+
+__device__ __forceinline__ int cell_index(const DevGrid &g, const Pos &p) {
+    return p.ir + g.nr * (p.it + g.nt * p.ip);
+}
+
+__device__ __forceinline__ int search_walls(const float *w, int n, float v) {
+    // i in [0, n-1] with w[i] <= v < w[i+1], clamped
+    if (v <= w[0]) return 0;
+    if (v >= w[n]) return n - 1;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (w[mid] <= v)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void locate_angles(const DevGrid &g, Pos &p) {
+    float rho = sqrtf(p.x * p.x + p.y * p.y);
+    if (g.mirror) {
+        p.hemi = (p.z >= 0.f) ? 1 : -1;
+        p.it = search_walls(g.tw, g.nt, atan2f(rho, fabsf(p.z)));
+    } else {
+        p.hemi = 1;
+        p.it = search_walls(g.tw, g.nt, atan2f(rho, p.z));
+    }
+    if (g.np > 1) {
+        float ph = atan2f(p.y, p.x);
+        if (ph < 0.f) ph += 6.28318530718f;
+        p.ip = search_walls(g.pw, g.np, ph);
+    } else {
+        p.ip = 0;
+    }
+}
+
+__device__ __forceinline__ void locate(const DevGrid &g, Pos &p) {
+    float rr = p.x * p.x + p.y * p.y + p.z * p.z;
+    if (rr < g.rw2[0]) {
+        p.ir = -1;
+        p.hemi = (g.mirror && p.z < 0.f) ? -1 : 1;
+        p.it = 0;
+        p.ip = 0;
+        return;
+    }
+    if (rr >= g.rw2[g.nr]) {
+        p.ir = g.nr;
+        return;
+    }
+    p.ir = search_walls(g.rw, g.nr, sqrtf(rr));
+    locate_angles(g, p);
+}
+
+__device__ __forceinline__ float sph_out(float b, float rr, float R2) {
+    float c = rr - R2;
+    float D = fmaxf(fmaf(b, b, -c), 0.f);
+    float sq = sqrtf(D);
+    float t = (b <= 0.f) ? (sq - b) : (-c / (b + sq));
+    return fmaxf(t, 0.f);
+}
+
+// inner sphere; near-tangent cases (D small against b^2) are re-evaluated in
+// FP64 from the FP32 position so the hit/miss decision is consistent
+__device__ __forceinline__ float sph_in(const Pos &p, float b, float rr, float R2, double Rd) {
+    if (b >= 0.f) return RT_INF;
+    float c = rr - R2;
+    float D = fmaf(b, b, -c);
+    if (D < 1e-4f * b * b) {
+        double bd = (double)p.x * p.dx + (double)p.y * p.dy + (double)p.z * p.dz;
+        double rrd = (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z;
+        double cd = rrd - Rd * Rd;
+        double Dd = bd * bd - cd;
+        if (Dd < 0.0) return RT_INF;
+        double t = cd / (sqrt(Dd) - bd);
+        return t > 0.0 ? (float)t : 0.f;
+    }
+    float t = c / (sqrtf(D) - b);
+    return fmaxf(t, 0.f);
+}
+
+// Exit through the cone at polar angle th (folded frame): f = rho^2 cos^2 -
+// z^2 sin^2; root with f' > 0 is (-B+sqrt(D))/A, with f' < 0 is (-B-sqrt(D))/A;
+// on the nappe sign(z) = sign(cos th) the sign of dtheta/dt is
+// sign(f') * sign(cos th).  req = -1 leaves towards smaller theta, +1 larger.
+__device__ __forceinline__ float cone_root(double A, double B, double C, double D,
+                                           int want_plus) {
+    double sq = sqrt(D);
+    double t;
+    if (want_plus)
+        t = (B < 0) ? (-B + sq) / A : -C / (B + sq);
+    else
+        t = (B > 0) ? -(B + sq) / A : C / (-B + sq);
+    return (float)t;
+}
+
+__device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, float dy,
+                                          float dz, float cth, float sth, double cthd,
+                                          double sthd, int req) {
+    float c2 = cth * cth, s2 = sth * sth;
+    float A = (dx * dx + dy * dy) * c2 - dz * dz * s2;
+    float B = (x * dx + y * dy) * c2 - z * dz * s2;
+    float C = (x * x + y * y) * c2 - z * z * s2;
+    float D = B * B - A * C;
+    int want_plus = (req * (cth > 0.f ? 1 : -1)) > 0;
+    float t;
+    if (D < 1e-4f * B * B) {
+        double c2d = cthd * cthd, s2d = sthd * sthd;
+        double Ad = ((double)dx * dx + (double)dy * dy) * c2d - (double)dz * dz * s2d;
+        double Bd = ((double)x * dx + (double)y * dy) * c2d - (double)z * dz * s2d;
+        double Cd = ((double)x * x + (double)y * y) * c2d - (double)z * z * s2d;
+        double Dd = Bd * Bd - Ad * Cd;
+        if (Dd < 0.0) return RT_INF;
+        t = cone_root(Ad, Bd, Cd, Dd, want_plus);
+    } else {
+        float sq = sqrtf(D);
+        if (want_plus)
+            t = (B < 0.f) ? (-B + sq) / A : -C / (B + sq);
+        else
+            t = (B > 0.f) ? -(B + sq) / A : C / (-B + sq);
+    }
+    if (!(t >= 0.f) || !(t < RT_INF)) return RT_INF;
+    float zh = fmaf(t, dz, z);
+    if (zh * cth < 0.f) return RT_INF;
+    return t;
+}
+
+__device__ __forceinline__ float phi_hit(float x, float y, float dx, float dy, float cph,
+                                         float sph, int req) {
+    float g0 = -x * sph + y * cph;
+    float g1 = -dx * sph + dy * cph;
+    if (req < 0 ? !(g1 < 0.f) : !(g1 > 0.f)) return RT_INF;
+    float t = -g0 / g1;
+    if (!(t >= 0.f)) return RT_INF;
+    float xh = fmaf(t, dx, x), yh = fmaf(t, dy, y);
+    if (xh * cph + yh * sph < 0.f) return RT_INF;
+    return t;
+}
+
+// distance to the next wall of the current cell and which wall it is
+__device__ __forceinline__ float next_wall(const DevGrid &g, const Pos &p, int &kind) {
+    float rr = p.x * p.x + p.y * p.y + p.z * p.z;
+    float b = p.x * p.dx + p.y * p.dy + p.z * p.dz;
+    if (p.ir < 0) {
+        kind = W_HOLE;
+        return sph_out(b, rr, g.rw2[0]);
+    }
+    float tb = sph_out(b, rr, g.rw2[p.ir + 1]);
+    kind = W_ROUT;
+    float t = sph_in(p, b, rr, g.rw2[p.ir], g.rwd[p.ir]);
+    if (t < tb) {
+        tb = t;
+        kind = W_RIN;
+    }
+    float zf = p.hemi > 0 ? p.z : -p.z, dzf = p.hemi > 0 ? p.dz : -p.dz;
+    int j = p.it;
+    if (j > 0) {
+        if (!g.mirror && g.ct[j] == 0.f) {
+            if (p.dz > 0.f) {
+                t = -p.z / p.dz;
+                if (t < 0.f) t = RT_INF;
+                if (t < tb) {
+                    tb = t;
+                    kind = W_TLO;
+                }
+            }
+        } else {
+            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, g.ct[j], g.st[j], g.ctd[j], g.std_[j], -1);
+            if (t < tb) {
+                tb = t;
+                kind = W_TLO;
+            }
+        }
+    }
+    if (g.mirror && j + 1 == g.nt) {
+        if (dzf < 0.f) {
+            t = fmaxf(-zf / dzf, 0.f);
+            if (t < tb) {
+                tb = t;
+                kind = W_THI;
+            }
+        }
+    } else if (j + 1 < g.nt) {
+        if (!g.mirror && g.ct[j + 1] == 0.f) {
+            if (p.dz < 0.f) {
+                t = -p.z / p.dz;
+                if (t < 0.f) t = RT_INF;
+                if (t < tb) {
+                    tb = t;
+                    kind = W_THI;
+                }
+            }
+        } else {
+            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, g.ct[j + 1], g.st[j + 1], g.ctd[j + 1],
+                         g.std_[j + 1], +1);
+            if (t < tb) {
+                tb = t;
+                kind = W_THI;
+            }
+        }
+    }
+    if (g.np > 1) {
+        int k = p.ip;
+        t = phi_hit(p.x, p.y, p.dx, p.dy, g.cp[k], g.sp[k], -1);
+        if (t < tb) {
+            tb = t;
+            kind = W_PLO;
+        }
+        t = phi_hit(p.x, p.y, p.dx, p.dy, g.cp[k + 1], g.sp[k + 1], +1);
+        if (t < tb) {
+            tb = t;
+            kind = W_PHI;
+        }
+    }
+    return tb;
+}
+
+__device__ __forceinline__ void snap_radius(Pos &p, float R) {
+    float r = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+    if (r > 0.f) {
+        float f = R / r;
+        p.x *= f;
+        p.y *= f;
+        p.z *= f;
+    }
+}
+
+// move by t, cross the wall; returns true when the packet leaves the grid
+__device__ __forceinline__ bool cross_wall(const DevGrid &g, Pos &p, float t, int kind) {
+    p.x = fmaf(t, p.dx, p.x);
+    p.y = fmaf(t, p.dy, p.y);
+    p.z = fmaf(t, p.dz, p.z);
+    switch (kind) {
+    case W_HOLE:
+        snap_radius(p, g.rw[0]);
+        p.ir = 0;
+        locate_angles(g, p);
+        return false;
+    case W_ROUT:
+        p.ir++;
+        if (p.ir >= g.nr) return true;
+        snap_radius(p, g.rw[p.ir]);
+        return false;
+    case W_RIN:
+        snap_radius(p, g.rw[p.ir]);
+        p.ir--;
+        return false;
+    case W_TLO:
+        p.it--;
+        return false;
+    case W_THI:
+        if (g.mirror && p.it + 1 == g.nt) {
+            p.z = 0.f;
+            p.hemi = -p.hemi;
+        } else {
+            p.it++;
+        }
+        return false;
+    case W_PLO:
+        p.ip = (p.ip == 0) ? g.np - 1 : p.ip - 1;
+        return false;
+    case W_PHI:
+        p.ip = (p.ip + 1 == g.np) ? 0 : p.ip + 1;
+        return false;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- physics
+// pdspy CGS constants (pdspy/constants/physics.py)
+#define RT_H 6.6260755e-27
+#define RT_C 2.99792458e10
+#define RT_K 1.380658e-16
+
+__device__ __forceinline__ void iso_dir(Philox &r, float &dx, float &dy, float &dz) {
+    float mu = 2.f * uni(r) - 1.f;
+    float u2 = uni(r);
+    float s, c;
+    sincospif(2.f * u2, &s, &c);
+    float st = sqrtf(fmaxf(0.f, 1.f - mu * mu));
+    dx = st * c;
+    dy = st * s;
+    dz = mu;
+}
+
+__device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy, float &dz) {
+    float u = uni(r);
+    float ct;
+    if (fabsf(g) < 1e-6f) {
+        ct = 2.f * u - 1.f;
+    } else {
+        float f = (1.f - g * g) / (1.f - g + 2.f * g * u);
+        ct = (1.f + g * g - f * f) / (2.f * g);
+    }
+    ct = fminf(1.f, fmaxf(-1.f, ct));
+    float u2 = uni(r);
+    float sps, cps;
+    sincospif(2.f * u2, &sps, &cps);
+    float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+    float ux = dx, uy = dy, uz = dz;
+    if (fabsf(uz) > 0.99999f) {
+        dx = st * cps;
+        dy = st * sps;
+        dz = (uz > 0.f ? 1.f : -1.f) * ct;
+    } else {
+        float s = sqrtf(1.f - uz * uz);
+        dx = st * (ux * uz * cps - uy * sps) / s + ux * ct;
+        dy = st * (uy * uz * cps + ux * sps) / s + uy * ct;
+        dz = -st * cps * s + uz * ct;
+    }
+    float n = rsqrtf(dx * dx + dy * dy + dz * dz);
+    dx *= n;
+    dy *= n;
+    dz *= n;
+}
+
+// B_nu(T) in FP32 from h nu/k (pdspy/misc/B_nu.py:4-6); pre = 2 h nu^3 / c^2
+__device__ __forceinline__ float planck_f(float pre, float hnu_k, float T) {
+    if (!(T > 0.f)) return 0.f;
+    float x = hnu_k / T;
+    if (x > 80.f) return pre * __expf(-x);
+    return pre / expm1f(x);
+}
+
+// dB_nu/dT (pdspy/misc/dB_nu.py:4-7 up to sign) = pre * x e^x / (T (e^x-1)^2)
+__device__ __forceinline__ float dplanck_f(float pre, float hnu_k, float T) {
+    if (!(T > 0.f)) return 0.f;
+    float x = hnu_k / T;
+    if (x > 80.f) return pre * x * __expf(-x) / T;
+    float e = expm1f(x);
+    return pre * x * (e + 1.f) / (T * e * e);
+}
+
+// ---------------------------------------------------------------- waves
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+}  // namespace rt

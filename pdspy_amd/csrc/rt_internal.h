@@ -1,0 +1,129 @@
+// rt_internal.h -- host-side state behind the C ABI (include/pdspy_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pdspy_amd.h"
+#include "rt_device.h"
+
+#define RT_MAX_SPEC 8    // species per model
+#define RT_MAX_GROUP 4   // distinct absorption opacities per model
+#define RT_MAX_LAM 255   // wavelength bins (guide tables are uint8)
+#define RT_NGUIDE 128    // guide-table entries per CDF row
+#define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
+#define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int n_cu = 0;
+    int wallclock_khz = 0;
+};
+
+// counters[] slots used by the thermal kernels
+enum {
+    CNT_PACKET = 0,
+    CNT_CROSS,
+    CNT_ABS,
+    CNT_SCAT,
+    CNT_ESCAPE,
+    CNT_KILLED,
+    CNT_EVENTS,
+    CNT_MRW,
+    CNT_ABORT,
+    CNT_OVERFLOW,
+    CNT_N = 16
+};
+
+struct HostTables {
+    int ngroup = 0;
+    std::vector<int> grp;    // species -> group
+    std::vector<int> grep;   // group -> representative species
+    std::vector<double> nu, dnu;
+    std::vector<double> kmax;        // per group
+    std::vector<double> lnq;         // [G][ntemp]
+    std::vector<float> cdf;          // [G][ntemp][nlam]
+    std::vector<unsigned char> guide;  // [G][ntemp][NGUIDE]
+    std::vector<float> scdf;         // [nstar][nlam]
+    std::vector<unsigned char> sguide; // [nstar][NGUIDE]
+    std::vector<float> spick;        // [nstar]
+    std::vector<float> kross;        // [G][ntemp] Rosseland-mean extinction (MRW)
+    std::vector<float> kplanck;      // [G][ntemp] Planck-mean absorption (MRW)
+    double lum = 0;
+    double lnt0 = 0, dlnt = 0;
+};
+
+struct rt_model {
+    rt_ctx *ctx = nullptr;
+    int nr = 0, nt = 0, np = 0, ncell = 0, nspec = 0, nlam = 0, nstar = 0, mirror = 0;
+    int ntemp = 1000;
+    double temp0 = 0.01, temp1 = 1e5;
+    bool has_g = false;
+    std::vector<double> rw, tw, pw;              // walls (cm, rad, rad)
+    std::vector<double> lam, kabs, ksca, gsca;   // host opacity tables
+    std::vector<double> rho;                     // [S][cells] host copy
+    std::vector<double> stars;                   // [nstar][5]
+    std::vector<double> vol;                     // cell volumes (cm^3)
+    HostTables tab;
+    // device buffers
+    float *d_rw = nullptr, *d_rw2 = nullptr, *d_tw = nullptr, *d_ct = nullptr, *d_st = nullptr;
+    double *d_rwd = nullptr, *d_ctd = nullptr, *d_std = nullptr;
+    float *d_pw = nullptr, *d_cp = nullptr, *d_sp = nullptr;
+    float *d_rhoL = nullptr;      // [cells][S] rho * length unit
+    float *d_rho = nullptr;       // [S][cells] rho (g/cm^3)
+    float *d_ka = nullptr, *d_ks = nullptr, *d_g = nullptr;  // [S][nlam]
+    int *d_grp = nullptr;
+    float *d_gka = nullptr;       // [G][nlam]
+    float *d_ginvk = nullptr;     // [G] 2^shift / kmax
+    float *d_invdr = nullptr;     // [nr] 1/dr (au^-1)
+    float *d_cdf = nullptr;
+    unsigned char *d_guide = nullptr;
+    double *d_lnq = nullptr;
+    float *d_scdf = nullptr;
+    unsigned char *d_sguide = nullptr;
+    float *d_spick = nullptr;
+    float *d_spos = nullptr;      // [nstar][3] au
+    float *d_kross = nullptr, *d_kplanck = nullptr;
+    double *d_econv = nullptr;    // [cells][G] erg/s/g per tally unit (per packet energy 1)
+    unsigned long long *d_tally = nullptr;  // [cells][G]
+    float *d_xrow = nullptr;      // [cells][G] T-row coordinate of the lagged temperature
+    float *d_Tg = nullptr;        // [cells][G] temperature (K)
+    unsigned int *d_visits = nullptr;
+    unsigned long long *d_cnt = nullptr;
+    float *d_Tout = nullptr;      // [S][cells]
+    // image scratch
+    float *d_alpha = nullptr, *d_src = nullptr;
+    double *d_galpha = nullptr, *d_gsrc = nullptr;
+    float *d_rowI = nullptr;      // per-ray intensity for the adjoint
+    size_t rowI_cap = 0;
+    double eps_last = 0;          // packet energy of the last thermal run
+    bool have_T = false;
+};
+
+int set_err(rt_ctx *ctx, int code, const char *fmt, ...);
+#define HIPCHK(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_err((ctx), RT_E_HIP, "%s failed: %s (%s:%d)", #call,                \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+int build_tables(rt_model *m, std::string &err);
+rt::DevGrid dev_grid(const rt_model *m);
+int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
+                rt_thermal_stats *st);
+int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,
+              float *d_img);
+int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,
+                      const float *d_dimg, float *d_g_logrho, float *d_g_T);
+int vis_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
+            const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im);
+int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx, int ny,
+                    double dxy, const double *d_u, const double *d_v, int K, double dra,
+                    double ddec, float *d_g_img);
+int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, const float *mre,
+              const float *mim, int K, double *out, float *gre, float *gim);

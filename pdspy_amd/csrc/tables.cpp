@@ -1,0 +1,178 @@
+// tables.cpp -- FP64 host construction of the thermal-MC tables.
+//
+// Frequency grid nu_i = c / lam_i with trapezoid weights dnu_i; Planck function
+// from pdspy/misc/B_nu.py:4-6 and its temperature derivative (dB_nu.py:4-7);
+// emission per gram q(T) = 4 pi sum_i kabs_i B_i(T) dnu_i on the log-spaced
+// temperature grid of RADMC-3D's (ntemp, temp0, temp1) options
+// (pdspy/radmc3d/write.py:47-52); Bjorkman & Wood re-emission CDFs
+// P_i(T) ~ kabs_i dB_i/dT dnu_i; blackbody star spectra (stars.inp with -T,
+// write.py:148-172) with L = 4 pi R^2 * pi sum B_i dnu_i.  Everything is built
+// in log space so no row underflows at 0.01 K or overflows at 1e5 K.
+#include <cmath>
+#include <cstring>
+
+#include "rt_internal.h"
+
+namespace {
+
+const double kH = 6.6260755e-27, kC = 2.99792458e10, kK = 1.380658e-16;
+const double kPi = 3.14159265358979323846;
+
+double ln_expm1(double x) { return x > 30.0 ? x + std::log1p(-std::exp(-x)) : std::log(std::expm1(x)); }
+
+// ln B_nu(T)
+double lnB(double nu, double T) {
+    double x = kH * nu / (kK * T);
+    return std::log(2.0 * kH / (kC * kC)) + 3.0 * std::log(nu) - ln_expm1(x);
+}
+
+// ln dB_nu/dT = ln(2 h^2 nu^4 / (c^2 k T^2)) - x - 2 ln(1 - e^-x)
+double lndB(double nu, double T) {
+    double x = kH * nu / (kK * T);
+    return std::log(2.0 * kH * kH / (kC * kC * kK)) + 4.0 * std::log(nu) - 2.0 * std::log(T) - x -
+           2.0 * std::log(-std::expm1(-x));
+}
+
+// normalised cumulative distribution from log weights; empty rows emit in the
+// longest-wavelength bin
+void make_cdf(const std::vector<double> &lw, float *cdf, unsigned char *guide, int ng) {
+    int n = (int)lw.size();
+    double mx = -INFINITY;
+    for (double v : lw) mx = std::max(mx, v);
+    std::vector<double> acc(n);
+    if (mx == -INFINITY) {
+        for (int i = 0; i < n; ++i) acc[i] = (i == n - 1) ? 1.0 : 0.0;
+    } else {
+        double s = 0;
+        for (int i = 0; i < n; ++i) {
+            s += std::exp(lw[i] - mx);
+            acc[i] = s;
+        }
+        for (int i = 0; i < n; ++i) acc[i] /= s;
+        acc[n - 1] = 1.0;
+    }
+    for (int i = 0; i < n; ++i) cdf[i] = (float)acc[i];
+    cdf[n - 1] = 1.0f;
+    // guide[k] = smallest i with cdf[i] >= k/ng (in the FP32 values the kernel sees)
+    int i = 0;
+    for (int k = 0; k < ng; ++k) {
+        float thr = (float)k / (float)ng;
+        while (i < n - 1 && cdf[i] < thr) ++i;
+        guide[k] = (unsigned char)i;
+    }
+}
+
+}  // namespace
+
+int build_tables(rt_model *m, std::string &err) {
+    HostTables &t = m->tab;
+    const int nl = m->nlam, ns = m->nspec, nT = m->ntemp;
+    t.nu.resize(nl);
+    t.dnu.resize(nl);
+    for (int i = 0; i < nl; ++i) t.nu[i] = kC / (m->lam[i] * 1e-4);
+    if (nl == 1) {
+        t.dnu[0] = 1.0;
+    } else {
+        for (int i = 0; i < nl; ++i) {
+            int a = i > 0 ? i - 1 : 0, b = i < nl - 1 ? i + 1 : nl - 1;
+            t.dnu[i] = 0.5 * std::fabs(t.nu[b] - t.nu[a]);
+        }
+    }
+    // species sharing an absorption opacity share a temperature (group)
+    t.grp.assign(ns, 0);
+    t.grep.clear();
+    for (int s = 0; s < ns; ++s) {
+        int found = -1;
+        for (size_t q = 0; q < t.grep.size() && found < 0; ++q)
+            if (!std::memcmp(&m->kabs[(size_t)t.grep[q] * nl], &m->kabs[(size_t)s * nl],
+                             sizeof(double) * nl))
+                found = (int)q;
+        if (found < 0) {
+            found = (int)t.grep.size();
+            t.grep.push_back(s);
+        }
+        t.grp[s] = found;
+    }
+    t.ngroup = (int)t.grep.size();
+    if (t.ngroup > RT_MAX_GROUP) {
+        err = "more distinct absorption opacities than RT_MAX_GROUP";
+        return RT_E_LIMIT;
+    }
+    const int G = t.ngroup;
+    t.lnt0 = std::log(m->temp0);
+    t.dlnt = (std::log(m->temp1) - t.lnt0) / (nT - 1);
+    t.kmax.assign(G, 1.0);
+    t.lnq.assign((size_t)G * nT, 0.0);
+    t.cdf.assign((size_t)G * nT * nl, 0.f);
+    t.guide.assign((size_t)G * nT * RT_NGUIDE, 0);
+    t.kross.assign((size_t)G * nT, 0.f);
+    t.kplanck.assign((size_t)G * nT, 0.f);
+    std::vector<double> lw(nl), lwb(nl);
+    for (int q = 0; q < G; ++q) {
+        const int s = t.grep[q];
+        const double *ka = &m->kabs[(size_t)s * nl];
+        const double *ks = &m->ksca[(size_t)s * nl];
+        double km = 0;
+        for (int i = 0; i < nl; ++i) km = std::max(km, ka[i]);
+        t.kmax[q] = km > 0 ? km : 1.0;
+        for (int it = 0; it < nT; ++it) {
+            double T = std::exp(t.lnt0 + it * t.dlnt);
+            double mx = -INFINITY;
+            for (int i = 0; i < nl; ++i) {
+                lwb[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) + lnB(t.nu[i], T) : -INFINITY;
+                mx = std::max(mx, lwb[i]);
+            }
+            double s1 = 0;
+            for (int i = 0; i < nl; ++i) s1 += std::exp(lwb[i] - mx);
+            t.lnq[(size_t)q * nT + it] = std::log(4.0 * kPi) + mx + std::log(s1);
+            for (int i = 0; i < nl; ++i)
+                lw[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) + lndB(t.nu[i], T) : -INFINITY;
+            make_cdf(lw, &t.cdf[((size_t)q * nT + it) * nl],
+                     &t.guide[((size_t)q * nT + it) * RT_NGUIDE], RT_NGUIDE);
+            // Planck-mean absorption and Rosseland-mean extinction at T
+            double wb = 0, wkb = 0, wd = 0, wdk = 0, mb = -INFINITY, md = -INFINITY;
+            for (int i = 0; i < nl; ++i) {
+                mb = std::max(mb, std::log(t.dnu[i]) + lnB(t.nu[i], T));
+                md = std::max(md, std::log(t.dnu[i]) + lndB(t.nu[i], T));
+            }
+            for (int i = 0; i < nl; ++i) {
+                double b = std::exp(std::log(t.dnu[i]) + lnB(t.nu[i], T) - mb);
+                double d = std::exp(std::log(t.dnu[i]) + lndB(t.nu[i], T) - md);
+                wb += b;
+                wkb += b * ka[i];
+                wd += d;
+                double ke = ka[i] + ks[i];
+                wdk += ke > 0 ? d / ke : 0.0;
+            }
+            t.kplanck[(size_t)q * nT + it] = (float)(wkb / wb);
+            t.kross[(size_t)q * nT + it] = (float)(wdk > 0 ? wd / wdk : 0.0);
+        }
+    }
+    // stars
+    t.scdf.assign((size_t)std::max(1, m->nstar) * nl, 0.f);
+    t.sguide.assign((size_t)std::max(1, m->nstar) * RT_NGUIDE, 0);
+    t.spick.assign(std::max(1, m->nstar), 1.f);
+    double ltot = 0;
+    std::vector<double> lum(m->nstar);
+    for (int k = 0; k < m->nstar; ++k) {
+        double R = m->stars[5 * k], Ts = m->stars[5 * k + 1];
+        double mx = -INFINITY;
+        for (int i = 0; i < nl; ++i) {
+            lw[i] = std::log(t.dnu[i]) + lnB(t.nu[i], Ts);
+            mx = std::max(mx, lw[i]);
+        }
+        double s1 = 0;
+        for (int i = 0; i < nl; ++i) s1 += std::exp(lw[i] - mx);
+        lum[k] = 4.0 * kPi * R * R * kPi * std::exp(mx) * s1;
+        ltot += lum[k];
+        make_cdf(lw, &t.scdf[(size_t)k * nl], &t.sguide[(size_t)k * RT_NGUIDE], RT_NGUIDE);
+    }
+    double acc = 0;
+    for (int k = 0; k < m->nstar; ++k) {
+        acc += lum[k];
+        t.spick[k] = (float)(acc / ltot);
+    }
+    if (m->nstar > 0) t.spick[m->nstar - 1] = 1.f;
+    t.lum = ltot;
+    return 0;
+}

@@ -1,0 +1,201 @@
+// vis.hip -- model visibilities and chi^2 on gfx950.
+//
+// V(u,v) = sum_jk I[j][k] exp(-2 pi i (u x_k + v y_j)) with x_k = (k - nx/2) dxy,
+// y_j = (j - ny/2) dxy, times the phase-centre shift exp(-2 pi i (u dra + v ddec)).
+// On the pixel-conjugate grid this is pdspy's imtovis (imaging/imtovis.py:13-24,
+// fftshift(fft2(ifftshift(I)))); at arbitrary (u,v) it is the exact sampler
+// behind interpolate_model (interferometry/interpolate_model.py:20-27).
+// chi^2 = sum w ((Re_d - Re_m)^2 + (Im_d - Im_m)^2) (libinterferometry.pyx:619-633).
+//
+// Forward: one (u,v) point per lane; the image is staged through LDS in
+// column strips of 32 and read as wave-wide broadcasts; the 32 column phasors
+// sit in registers and each row phasor is one sincospi, so the inner loop is
+// two FMAs per pixel.  Adjoint: one pixel per lane in 16x16 tiles; per chunk of
+// 64 points the tile's column phasors (times the point's complex cotangent) and
+// row phasors are built once in LDS and each lane does two FMAs per point.
+#include <cmath>
+
+#include "rt_internal.h"
+
+#define VIS_BS 256
+#define VIS_STRIP 32
+#define VIS_MAXROWS 1024
+
+__global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny,
+                                                const double *u, const double *v, int K,
+                                                double dxy, double dra, double ddec, float *re,
+                                                float *im) {
+    __shared__ float strip[VIS_MAXROWS * VIS_STRIP / 4];  // rows x 8 columns per pass (32 KB)
+    const int q = blockIdx.x * VIS_BS + threadIdx.x;
+    const bool act = q < K;
+    float ud = 0.f, vd = 0.f;
+    if (act) {
+        ud = (float)(u[q] * dxy);
+        vd = (float)(v[q] * dxy);
+    }
+    float sr = 0.f, si = 0.f;
+    // process columns in strips of 8 so that a strip of up to 1024 rows fits LDS
+    const int W = 8;
+    for (int k0 = 0; k0 < nx; k0 += W) {
+        int wk = min(W, nx - k0);
+        __syncthreads();
+        for (int i = threadIdx.x; i < ny * W; i += VIS_BS) {
+            int j = i / W, kk = i - j * W;
+            strip[i] = (kk < wk) ? img[(size_t)j * nx + k0 + kk] : 0.f;
+        }
+        __syncthreads();
+        float ar[W], ai[W];
+#pragma unroll
+        for (int kk = 0; kk < W; ++kk) {
+            float s, c;
+            sincospif(-2.f * ud * (float)(k0 + kk - nx / 2), &s, &c);
+            ar[kk] = c;
+            ai[kk] = s;
+        }
+        for (int j = 0; j < ny; ++j) {
+            const float *row = strip + j * W;
+            float rr = 0.f, ri = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < W; ++kk) {
+                rr = fmaf(row[kk], ar[kk], rr);
+                ri = fmaf(row[kk], ai[kk], ri);
+            }
+            float bs, bc;
+            sincospif(-2.f * vd * (float)(j - ny / 2), &bs, &bc);
+            sr += rr * bc - ri * bs;
+            si += rr * bs + ri * bc;
+        }
+    }
+    if (act) {
+        double ph = -2.0 * (u[q] * dra + v[q] * ddec);
+        double s, c;
+        sincospi(ph, &s, &c);
+        re[q] = (float)(sr * c - si * s);
+        im[q] = (float)(sr * s + si * c);
+    }
+}
+
+#define ADJ_T 16
+#define ADJ_CH 64
+
+// g[j][k] = Re sum_q H_q a_qk b_qj,  H_q = (gre_q - i gim_q) shift_q,
+// a_qk = exp(-2 pi i u_q x_k), b_qj = exp(-2 pi i v_q y_j)
+__global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, const float *gim, int nx,
+                                                          int ny, const double *u, const double *v,
+                                                          int K, double dxy, double dra, double ddec,
+                                                          float *gimg) {
+    __shared__ float Ha_r[ADJ_CH][ADJ_T], Ha_i[ADJ_CH][ADJ_T];
+    __shared__ float b_r[ADJ_CH][ADJ_T], b_i[ADJ_CH][ADJ_T];
+    const int tx = threadIdx.x % ADJ_T, ty = threadIdx.x / ADJ_T;
+    const int k0 = blockIdx.x * ADJ_T, j0 = blockIdx.y * ADJ_T;
+    const int k = k0 + tx, j = j0 + ty;
+    float acc = 0.f;
+    for (int q0 = 0; q0 < K; q0 += ADJ_CH) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < ADJ_CH * ADJ_T; i += ADJ_T * ADJ_T) {
+            int qq = i / ADJ_T, t = i - qq * ADJ_T;
+            int q = q0 + qq;
+            float hr = 0.f, hi = 0.f, ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+            if (q < K) {
+                double s, c;
+                sincospi(-2.0 * (u[q] * dra + v[q] * ddec), &s, &c);
+                // H = (gre - i gim) * (c + i s)
+                float gr = gre[q], gi = gim[q];
+                hr = (float)(gr * c + gi * s);
+                hi = (float)(gr * s - gi * c);
+                float sa, ca, sb, cb;
+                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + t - nx / 2), &sa, &ca);
+                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + t - ny / 2), &sb, &cb);
+                ar = ca;
+                ai = sa;
+                br = cb;
+                bi = sb;
+            }
+            Ha_r[qq][t] = hr * ar - hi * ai;
+            Ha_i[qq][t] = hr * ai + hi * ar;
+            b_r[qq][t] = br;
+            b_i[qq][t] = bi;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int qq = 0; qq < ADJ_CH; ++qq)
+            acc = fmaf(Ha_r[qq][tx], b_r[qq][ty], fmaf(-Ha_i[qq][tx], b_i[qq][ty], acc));
+    }
+    if (k < nx && j < ny) gimg[(size_t)j * nx + k] = acc;
+}
+
+#define CHI_BS 256
+
+__global__ void __launch_bounds__(CHI_BS) k_chisq(const float *dre, const float *dim, const float *w,
+                                                  const float *mre, const float *mim, int K,
+                                                  double *partial, float *gre, float *gim) {
+    __shared__ double sh[CHI_BS];
+    int q = blockIdx.x * CHI_BS + threadIdx.x;
+    double x = 0.0;
+    if (q < K) {
+        double a = (double)dre[q] - mre[q], b = (double)dim[q] - mim[q];
+        x = (a * a + b * b) * (double)w[q];
+        if (gre) {
+            gre[q] = (float)(-2.0 * w[q] * a);
+            gim[q] = (float)(-2.0 * w[q] * b);
+        }
+    }
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = CHI_BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// fixed-order final sum (deterministic)
+__global__ void __launch_bounds__(CHI_BS) k_sum(const double *partial, int n, double *out) {
+    __shared__ double sh[CHI_BS];
+    double x = 0.0;
+    for (int i = threadIdx.x; i < n; i += CHI_BS) x += partial[i];
+    sh[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = CHI_BS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+int vis_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
+            const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im) {
+    if (nx < 1 || ny < 1 || ny > VIS_MAXROWS || K < 0 || !(dxy > 0))
+        return set_err(ctx, RT_E_ARG, "vis: invalid sizes (ny <= %d)", VIS_MAXROWS);
+    if (K == 0) return RT_OK;
+    hipLaunchKernelGGL(k_vis, dim3((K + VIS_BS - 1) / VIS_BS), dim3(VIS_BS), 0, ctx->stream, d_img, nx,
+                       ny, d_u, d_v, K, dxy, dra, ddec, d_re, d_im);
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx, int ny, double dxy,
+                    const double *d_u, const double *d_v, int K, double dra, double ddec,
+                    float *d_g_img) {
+    if (nx < 1 || ny < 1 || K < 0 || !(dxy > 0)) return set_err(ctx, RT_E_ARG, "vis adjoint: invalid sizes");
+    dim3 grid((nx + ADJ_T - 1) / ADJ_T, (ny + ADJ_T - 1) / ADJ_T);
+    hipLaunchKernelGGL(k_vis_adj, grid, dim3(ADJ_T * ADJ_T), 0, ctx->stream, d_gre, d_gim, nx, ny,
+                       d_u, d_v, K, dxy, dra, ddec, d_g_img);
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, const float *mre,
+              const float *mim, int K, double *out, float *gre, float *gim) {
+    if (K < 1 || !out) return set_err(ctx, RT_E_ARG, "chisq: invalid arguments");
+    if ((gre == nullptr) != (gim == nullptr)) return set_err(ctx, RT_E_ARG, "chisq: gre/gim both or neither");
+    int nb = (K + CHI_BS - 1) / CHI_BS;
+    double *partial = nullptr;
+    HIPCHK(ctx, hipMallocAsync((void **)&partial, sizeof(double) * nb, ctx->stream));
+    hipLaunchKernelGGL(k_chisq, dim3(nb), dim3(CHI_BS), 0, ctx->stream, dre, dim, w, mre, mim, K,
+                       partial, gre, gim);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(CHI_BS), 0, ctx->stream, partial, nb, out);
+    HIPCHK(ctx, hipFreeAsync(partial, ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
+}

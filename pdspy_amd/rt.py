@@ -1,0 +1,412 @@
+"""Host side of the MI355X continuum RT path, mirroring pdspy's Model API.
+
+The functions take a pdspy ``Model`` (or anything with the same ``grid``
+attributes: ``w1/w2/w3``, ``density``, ``dust``, ``stars``, ``lam``,
+``temperature``) and fill the same outputs pdspy's RADMC-3D path fills:
+
+* ``run_thermal``      -- Model.run_thermal / run_thermal_radmc3d
+  (pdspy/modeling/Model.py:54-72, 221-256): ``grid.temperature`` becomes a list
+  (one per species) of float64 ``[nr, ntheta, nphi]`` arrays, exactly the
+  reshape/transpose of Model.py:249-254.
+* ``run_image``        -- Model.run_image_radmc3d (Model.py:515-577): an
+  ``Image`` in Jy/pixel with x, y in arcsec (Model.py:566-572).
+* ``run_visibilities`` -- Model.run_visibilities_radmc3d (Model.py:682-721) and
+  imtovis (pdspy/imaging/imtovis.py:6-27).
+* ``interpolate_model`` -- pdspy/interferometry/interpolate_model.py:11-57
+  (exact DFT at arbitrary (u, v)).
+* ``GPUModelMixin``    -- adds ``code="mi355x"`` to a Model subclass and defers
+  every other ``code`` to pdspy's own implementation.
+
+All heavy work runs in the HIP library (``pdspy_amd/lib/libpdspy_amd.so``);
+there is no CPU fallback.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .inputs import AU, R_sun, Jy, arcsec, c as c_light, pc, Image, Visibilities
+
+# radmc3d.inp keys pdspy's write.control accepts (pdspy/radmc3d/write.py:1-21);
+# those that change the thermal MC are used, the rest are accepted and ignored
+_CONTROL_KEYS = {
+    "incl_dust", "incl_lines", "incl_freefree", "nphot_therm", "nphot_scat", "nphot_spec",
+    "iseed", "ifast", "enthres", "itempdecoup", "istar_sphere", "ntemp", "temp0", "temp1",
+    "scattering_mode_max", "rto_style", "camera_tracemode", "camera_nrrefine",
+    "camera_refine_criterion", "camera_incl_stars", "camera_starsphere_nrpix",
+    "camera_spher_cavity_relres", "camera_localobs_projection", "camera_min_dangle",
+    "camera_max_dangle", "camera_min_dr", "camera_diagnostics_subpix", "camera_secondorder",
+    "camera_interpol_jnu", "mc_weighted_photons", "optimized_motion", "lines_mode",
+    "lines_maxdoppler", "lines_partition_ntempint", "lines_partition_temp0",
+    "lines_partition_temp1", "lines_show_pictograms", "tgas_eq_tdust", "subbox_nx",
+    "subbox_ny", "subbox_nz", "subbox_x0", "subbox_x1", "subbox_y0", "subbox_y1", "subbox_z0",
+    "subbox_z1", "modified_random_walk", "mrw_gamma", "mrw_tauthres", "mrw_count_trigger",
+    "setthreads", "camera_scatsrc_allfreq", "mc_scat_maxtauabs", "writeimage_unformatted",
+    "camera_circ_nrphiinf", "camera_circ_dbdr",
+}
+# keys of the run_* wrappers themselves (Model.py:221-222, 515-519)
+_RUN_KEYS = {"verbose", "timelimit", "nice"}
+
+
+def _check_keywords(keywords):
+    bad = set(keywords) - _CONTROL_KEYS - _RUN_KEYS
+    if bad:
+        raise TypeError("unexpected keyword(s) for the RADMC-3D option set: %s" % sorted(bad))
+
+
+def _loglog_interp(x_new, x, y):
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    if x.shape == np.shape(x_new) and np.allclose(x, x_new, rtol=1e-10, atol=0):
+        return y.copy()
+    if x[0] > x[-1]:
+        x, y = x[::-1], y[::-1]
+    pos = y > 0
+    if np.all(pos):
+        lx, ly = np.log(x), np.log(y)
+        out = np.interp(np.log(x_new), lx, ly)
+        # power-law extrapolation beyond the table ends
+        lo = np.log(x_new) < lx[0]
+        hi = np.log(x_new) > lx[-1]
+        if lo.any():
+            s = (ly[1] - ly[0]) / (lx[1] - lx[0])
+            out[lo] = ly[0] + s * (np.log(x_new)[lo] - lx[0])
+        if hi.any():
+            s = (ly[-1] - ly[-2]) / (lx[-1] - lx[-2])
+            out[hi] = ly[-1] + s * (np.log(x_new)[hi] - lx[-1])
+        return np.exp(out)
+    return np.interp(x_new, x, y)
+
+
+def model_arrays(grid):
+    """pdspy Grid -> the flat FP64 arrays of rt_model_desc (units of
+    write_radmc3d, Model.py:733-778: walls in cm/rad/rad, rho r-fastest,
+    opacities on the grid's wavelengths in micron)."""
+    if getattr(grid, "coordsystem", "spherical") != "spherical":
+        raise ValueError("only spherical grids are supported")
+    lam = np.asarray(grid.lam, dtype=float)
+    rho = np.ascontiguousarray(np.array([np.asarray(d, dtype=float).transpose(2, 1, 0)
+                                         for d in grid.density]))
+    kabs, ksca, gg = [], [], []
+    has_g = any(getattr(d, "g", None) is not None for d in grid.dust)
+    for d in grid.dust:
+        lam_d = np.asarray(d.lam, dtype=float) * 1e4
+        kabs.append(_loglog_interp(lam, lam_d, d.kabs))
+        ksca.append(_loglog_interp(lam, lam_d, d.ksca))
+        g = getattr(d, "g", None)
+        gg.append(np.interp(np.log(lam), np.log(lam_d), g) if g is not None else np.zeros_like(lam))
+    stars = np.array([[s.radius * R_sun, s.temperature, s.x * AU, s.y * AU, s.z * AU]
+                      for s in grid.stars], dtype=float).reshape(-1, 5)
+    return dict(rw=np.asarray(grid.w1, dtype=float) * AU, tw=np.asarray(grid.w2, dtype=float),
+                pw=np.asarray(grid.w3, dtype=float), rho=rho, lam_um=lam,
+                kabs=np.ascontiguousarray(kabs), ksca=np.ascontiguousarray(ksca),
+                gsca=np.ascontiguousarray(gg) if has_g else None, stars=stars)
+
+
+class DeviceModel:
+    """A model resident on the GPU (``rt_model``)."""
+
+    def __init__(self, arrays, device=0, ntemp=1000, temp0=0.01, temp1=1e5):
+        self.ctx = _lib.context(device)
+        a = {k: (None if v is None else np.ascontiguousarray(v, dtype=float))
+             for k, v in arrays.items()}
+        self._keep = a
+        rho = a["rho"]
+        self.nspec = rho.shape[0]
+        self.nr, self.nt, self.np = len(a["rw"]) - 1, len(a["tw"]) - 1, len(a["pw"]) - 1
+        self.ncell = self.nr * self.nt * self.np
+        P = lambda x: None if x is None else x.ctypes.data
+        d = _lib.ModelDesc(self.nr, self.nt, self.np, P(a["rw"]), P(a["tw"]), P(a["pw"]),
+                           self.nspec, P(rho), len(a["lam_um"]), P(a["lam_um"]), P(a["kabs"]),
+                           P(a["ksca"]), P(a.get("gsca")), a["stars"].shape[0], P(a["stars"]),
+                           int(ntemp), float(temp0), float(temp1))
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.rt_model_create(self.ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        self.rho_host = rho.reshape(self.nspec, self.ncell)
+        self._rho_dev = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.ctx.L.rt_model_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def set_density(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=float).reshape(self.nspec, self.ncell)
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_model_set_density(self.h, rho.ctypes.data))
+        self.rho_host = rho
+        self._rho_dev = None
+
+    def thermal(self, nphot, seed=0, **opts):
+        o = _lib.ThermalOpts()
+        for k, v in opts.items():
+            if v is None:
+                continue
+            if not hasattr(o, k):
+                raise TypeError("unknown thermal option %r" % k)
+            setattr(o, k, v)
+        st = _lib.ThermalStats()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_thermal(self.h, int(nphot), int(seed), C.byref(o), C.byref(st)))
+        return st.as_dict()
+
+    def temperature(self, with_visits=False):
+        T = np.zeros((self.nspec, self.ncell))
+        vis = np.zeros(self.ncell, dtype=np.uint64) if with_visits else None
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_get_temperature(
+            self.h, T.ctypes.data, None if vis is None else vis.ctypes.data))
+        return (T, vis) if with_visits else T
+
+    def temperature_device(self):
+        import torch
+        out = torch.empty((self.nspec, self.ncell), dtype=torch.float32, device="cuda")
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_get_temperature_device(self.h, C.c_void_p(out.data_ptr())))
+        return out
+
+    def rho_device(self):
+        import torch
+        if self._rho_dev is None:
+            self._rho_dev = torch.tensor(self.rho_host, dtype=torch.float32, device="cuda")
+        return self._rho_dev
+
+    @staticmethod
+    def camera(npix, pix_cm, lam_um, incl=0.0, pa=0.0, phi=0.0, mode=0, incl_star=True,
+               nsub=1, tau_stop=40.0, ny=None):
+        return _lib.Camera(int(npix), int(ny if ny is not None else npix), int(nsub),
+                           float(pix_cm), float(incl), float(pa), float(phi), float(lam_um),
+                           int(mode), 1 if incl_star else 0, float(tau_stop))
+
+    def image(self, cam, T_dev, rho_dev=None):
+        """Intensity image (erg/s/cm^2/Hz/sr, or tau) as a float32 torch tensor [ny, nx]."""
+        import torch
+        img = torch.empty((cam.ny, cam.nx), dtype=torch.float32, device="cuda")
+        T_dev = T_dev.contiguous() if T_dev is not None else None
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_image(
+            self.h, C.byref(cam), None if T_dev is None else C.c_void_p(T_dev.data_ptr()),
+            None if rho_dev is None else C.c_void_p(rho_dev.contiguous().data_ptr()),
+            C.c_void_p(img.data_ptr())))
+        return img
+
+    def image_adjoint(self, cam, T_dev, rho_dev, dimg):
+        import torch
+        g_lr = torch.empty((self.nspec, self.ncell), dtype=torch.float32, device="cuda")
+        g_T = torch.empty((self.ncell,), dtype=torch.float32, device="cuda")
+        dimg = dimg.contiguous().float()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_image_adjoint(
+            self.h, C.byref(cam), C.c_void_p(T_dev.contiguous().data_ptr()),
+            None if rho_dev is None else C.c_void_p(rho_dev.contiguous().data_ptr()),
+            C.c_void_p(dimg.data_ptr()), C.c_void_p(g_lr.data_ptr()), C.c_void_p(g_T.data_ptr())))
+        return g_lr, g_T
+
+
+def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0, device=0):
+    """V at (u, v) of a float32 torch image [ny, nx]; returns (re, im) float32 tensors."""
+    import torch
+    ctx = _lib.context(device)
+    img = img.contiguous().float()
+    u = torch.as_tensor(u, dtype=torch.float64, device="cuda").contiguous()
+    v = torch.as_tensor(v, dtype=torch.float64, device="cuda").contiguous()
+    K = u.numel()
+    re = torch.empty(K, dtype=torch.float32, device="cuda")
+    im = torch.empty(K, dtype=torch.float32, device="cuda")
+    ny, nx = img.shape
+    ctx.check(ctx.L.rt_vis_dft(ctx.h, C.c_void_p(img.data_ptr()), nx, ny, float(dxy),
+                               C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), K, float(dra),
+                               float(ddec), C.c_void_p(re.data_ptr()), C.c_void_p(im.data_ptr())))
+    return re, im
+
+
+def vis_dft_adjoint(gre, gim, nx, ny, dxy, u, v, dra=0.0, ddec=0.0, device=0):
+    import torch
+    ctx = _lib.context(device)
+    gre = gre.contiguous().float()
+    gim = gim.contiguous().float()
+    u = torch.as_tensor(u, dtype=torch.float64, device="cuda").contiguous()
+    v = torch.as_tensor(v, dtype=torch.float64, device="cuda").contiguous()
+    g = torch.empty((ny, nx), dtype=torch.float32, device="cuda")
+    ctx.check(ctx.L.rt_vis_dft_adjoint(ctx.h, C.c_void_p(gre.data_ptr()), C.c_void_p(gim.data_ptr()),
+                                       nx, ny, float(dxy), C.c_void_p(u.data_ptr()),
+                                       C.c_void_p(v.data_ptr()), u.numel(), float(dra), float(ddec),
+                                       C.c_void_p(g.data_ptr())))
+    return g
+
+
+def chisq(dre, dim, w, mre, mim, with_grad=False, device=0):
+    """chi^2 (float64 tensor, shape [1]) and optionally d chi^2 / d(mre, mim)."""
+    import torch
+    ctx = _lib.context(device)
+    ts = [t.contiguous().float() for t in (dre, dim, w, mre, mim)]
+    K = ts[0].numel()
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+    gre = torch.empty(K, dtype=torch.float32, device="cuda") if with_grad else None
+    gim = torch.empty(K, dtype=torch.float32, device="cuda") if with_grad else None
+    ctx.check(ctx.L.rt_chisq(ctx.h, *[C.c_void_p(t.data_ptr()) for t in ts], K,
+                             C.c_void_p(out.data_ptr()),
+                             None if gre is None else C.c_void_p(gre.data_ptr()),
+                             None if gim is None else C.c_void_p(gim.data_ptr())))
+    return (out, gre, gim) if with_grad else out
+
+
+# ---------------------------------------------------------------- pdspy mirror
+def _device_model(model, device=0, ntemp=1000, temp0=0.01, temp1=1e5):
+    dm = getattr(model, "_pdspy_amd_dev", None)
+    key = (id(model.grid), tuple(id(d) for d in model.grid.density), ntemp, temp0, temp1, device)
+    if dm is None or dm[0] != key:
+        dev = DeviceModel(model_arrays(model.grid), device=device, ntemp=ntemp, temp0=temp0,
+                          temp1=temp1)
+        model._pdspy_amd_dev = (key, dev)
+        return dev
+    return dm[1]
+
+
+def run_thermal(model, nphot=1e6, seed=None, device=0, max_events=None, epoch_first=None,
+                epoch_growth=None, count_visits=False, return_stats=False, **keywords):
+    """Model.run_thermal(code="mi355x"): fills ``model.grid.temperature``.
+
+    Accepts the RADMC-3D keyword set pdspy forwards (``iseed`` is the seed,
+    ``modified_random_walk``/``mrw_*`` select MRW, ``timelimit`` bounds the
+    kernel time, ``ntemp``/``temp0``/``temp1`` set the emission tables);
+    thread, niceness and verbosity keywords are accepted and ignored.
+    """
+    _check_keywords(keywords)
+    if seed is None:
+        seed = keywords.get("iseed", 0) or 0
+    dm = _device_model(model, device, keywords.get("ntemp") or 1000,
+                       keywords.get("temp0") or 0.01, keywords.get("temp1") or 1e5)
+    opts = dict(max_events=max_events, epoch_first=epoch_first, epoch_growth=epoch_growth,
+                count_visits=1 if count_visits else 0,
+                mrw=1 if keywords.get("modified_random_walk") else 0,
+                mrw_tauthres=keywords.get("mrw_tauthres"),
+                mrw_count_trigger=keywords.get("mrw_count_trigger"),
+                mrw_gamma=keywords.get("mrw_gamma"),
+                max_seconds=keywords.get("timelimit"))
+    stats = dm.thermal(int(nphot), int(seed), **opts)
+    T, vis = dm.temperature(with_visits=True)
+    nr, nt, np_ = dm.nr, dm.nt, dm.np
+    model.grid.temperature = [np.transpose(T[s].reshape((np_, nt, nr)), axes=(2, 1, 0))
+                              for s in range(dm.nspec)]
+    model._pdspy_amd_visits = np.transpose(vis.reshape((np_, nt, nr)), axes=(2, 1, 0))
+    return stats if return_stats else None
+
+
+def imaging_temperature(grid):
+    """Density-weighted species-mean temperature, as write_radmc3d hands
+    RADMC-3D for imaging (Model.py:759-769); returns [cells] r-fastest."""
+    density = np.array(grid.density, dtype=float)
+    temperature = np.array(grid.temperature, dtype=float)
+    density[density == 0] = 1.0e-30
+    T = (density * temperature).sum(axis=0) / density.sum(axis=0)
+    return np.ascontiguousarray(T.transpose(2, 1, 0)).reshape(-1)
+
+
+def _camera_lams(model, lam, loadlambda):
+    if loadlambda:
+        return np.atleast_1d(np.asarray(model.camera_wavelength, dtype=float))
+    return np.atleast_1d(np.asarray([float(x) for x in np.atleast_1d(lam)], dtype=float))
+
+
+def _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device):
+    import torch
+    dm = _device_model(model, device)
+    if len(model.grid.temperature) == 0:
+        raise RuntimeError("run_thermal (or grid.add_temperature) must come before imaging")
+    T = torch.tensor(imaging_temperature(model.grid), dtype=torch.float32, device="cuda")
+    pix_cm = pixelsize * dpc * AU
+    chans = []
+    for lam_um in lams:
+        cam = dm.camera(npix, pix_cm, lam_um, incl=incl, pa=pa, phi=phi, incl_star=not nostar,
+                        nsub=nsub)
+        chans.append(dm.image(cam, T).double().cpu().numpy())
+    img = np.stack(chans, axis=-1)  # [ny, nx, nlam], erg/s/cm^2/Hz/sr
+    # Jy/pixel (Model.py:566-567)
+    img = img / Jy * (pix_cm / (dpc * pc)) ** 2
+    return img[:, :, :, None]
+
+
+def run_image(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="1300",
+              loadlambda=False, incl=0, pa=0, phi=0, dpc=1, nostar=False, nsub=1, device=0,
+              **keywords):
+    """Model.run_image(code="mi355x") (Model.py:515-577): ``model.images[name]``."""
+    _check_keywords({k: v for k, v in keywords.items()
+                     if k not in ("imolspec", "iline", "widthkms", "vkms", "linenlam",
+                                  "doppcatch", "unstructured")})
+    lams = _camera_lams(model, lam, loadlambda)
+    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device)
+    x = (np.arange(npix) - npix // 2) * float(pixelsize)
+    model.images[name] = Image(image, x=x.astype(float), y=x.astype(float).copy(),
+                               wave=lams * 1.0e-4)
+    return model.images[name]
+
+
+def fft_grid_uv(npix, pixelsize):
+    """u, v of imtovis (imtovis.py:18-23): fftshift(fftfreq(n, dx[rad])) meshgrid, raveled."""
+    uu = np.fft.fftshift(np.fft.fftfreq(npix, pixelsize * arcsec))
+    u, v = np.meshgrid(uu, uu)
+    return u.reshape(-1), v.reshape(-1)
+
+
+def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="1300",
+                     loadlambda=False, incl=0, pa=0, phi=0, dpc=1, nostar=False, nsub=1,
+                     device=0, **keywords):
+    """Model.run_visibilities(code="mi355x") (Model.py:682-721): the image's
+    visibilities on its pixel-conjugate grid (imtovis.py:6-27)."""
+    import torch
+    _check_keywords(keywords)
+    lams = _camera_lams(model, lam, loadlambda)
+    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device)
+    u, v = fft_grid_uv(npix, pixelsize)
+    real = np.empty((npix * npix, len(lams)))
+    imag = np.empty((npix * npix, len(lams)))
+    for i in range(len(lams)):
+        img = torch.tensor(image[:, :, i, 0], dtype=torch.float32, device="cuda")
+        re, im = vis_dft(img, pixelsize * arcsec, u, v, device=device)
+        real[:, i] = re.double().cpu().numpy()
+        imag[:, i] = im.double().cpu().numpy()
+    freq = c_light / (lams * 1.0e-4)
+    model.visibilities[name] = Visibilities(u, v, freq, real, imag, np.ones(real.shape))
+    return model.visibilities[name]
+
+
+def interpolate_model(u, v, freq, model_image, dRA=0.0, dDec=0.0, device=0, **unused):
+    """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57),
+    by exact DFT; dRA/dDec in arcsec shift the phase centre."""
+    import torch
+    dxy = (model_image.x[1] - model_image.x[0]) * arcsec
+    real, imag = [], []
+    for i in range(model_image.image.shape[2]):
+        img = torch.tensor(model_image.image[:, :, i, 0], dtype=torch.float32, device="cuda")
+        re, im = vis_dft(img, dxy, u, v, dRA * arcsec, dDec * arcsec, device=device)
+        real.append(re.double().cpu().numpy().reshape(-1, 1))
+        imag.append(im.double().cpu().numpy().reshape(-1, 1))
+    real = np.concatenate(real, axis=1)
+    imag = np.concatenate(imag, axis=1)
+    return Visibilities(np.asarray(u, float), np.asarray(v, float), np.asarray(freq, float),
+                        real, imag, np.ones(real.shape))
+
+
+class GPUModelMixin:
+    """Mix into a pdspy Model subclass: ``code="mi355x"`` runs here, every other
+    ``code`` goes to pdspy's own method (RADMC-3D or Hyperion)."""
+
+    def run_thermal(self, nphot=1e6, code="radmc3d", **keywords):
+        if code == "mi355x":
+            return run_thermal(self, nphot=nphot, **keywords)
+        return super().run_thermal(nphot=nphot, code=code, **keywords)
+
+    def run_image(self, name=None, nphot=1e6, code="radmc3d", **keywords):
+        if code == "mi355x":
+            return run_image(self, name=name, nphot=nphot, **keywords)
+        return super().run_image(name=name, nphot=nphot, code=code, **keywords)
+
+    def run_visibilities(self, name=None, nphot=1e6, code="radmc3d", **keywords):
+        if code == "mi355x":
+            return run_visibilities(self, name=name, nphot=nphot, **keywords)
+        return super().run_visibilities(name=name, nphot=nphot, code=code, **keywords)

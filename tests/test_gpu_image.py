@@ -1,0 +1,261 @@
+"""GPU images, adjoint, visibilities and chi^2 against closed forms and the
+FP64 oracle.  Tolerances are written per test (FP32 kernels vs FP64 oracle)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pdspy_amd import autograd as rtag
+from pdspy_amd import configs, inputs, rt
+from tests.helpers import oracle_arrays
+from tests.test_oracle import C_LIGHT, _uniform_sphere, np_planck
+
+pytestmark = pytest.mark.gpu
+AU = inputs.AU
+
+
+def _dev(arr):
+    return rt.DeviceModel(arr)
+
+
+@pytest.mark.parametrize("incl", [0.0, 37.0, 90.0])
+def test_image_uniform_shell_closed_form_gpu(incl):
+    m, T = _uniform_sphere()
+    dm = _dev(m)
+    npix, pix = 33, 2.6 * AU
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    cam = dm.camera(npix, pix, 1300.0, incl=incl, incl_star=False, tau_stop=1e30)
+    img = dm.image(cam, Td).double().cpu().numpy()
+    camt = dm.camera(npix, pix, 1300.0, incl=incl, mode=1, incl_star=False)
+    tau_img = dm.image(camt, Td).double().cpu().numpy()
+    rin, rout = m["rw"][0], m["rw"][-1]
+    k = np.arange(npix) - npix // 2
+    X, Y = np.meshgrid(k * pix, k * pix)
+    b2 = X ** 2 + Y ** 2
+    path = 2 * np.sqrt(np.clip(rout ** 2 - b2, 0, None)) - 2 * np.sqrt(np.clip(rin ** 2 - b2, 0, None))
+    tau = 1e-18 * path
+    B = np_planck(C_LIGHT / 1300e-4, 50.0)
+    # FP32 traversal: 1e-4 of the peak
+    assert np.allclose(tau_img, tau, rtol=1e-4, atol=1e-4 * tau.max())
+    assert np.allclose(img, B * (-np.expm1(-tau)), rtol=1e-4, atol=1e-4 * B * tau.max())
+
+
+def _disk_setup(nr=48, nt=32, nphi=1, seed=0):
+    arr = oracle_arrays(configs.disk_envelope(nr=nr, nt=nt, nphi=nphi))
+    rng = np.random.default_rng(seed)
+    ncell = nr * nt * nphi
+    T = 20.0 + 200.0 * rng.uniform(size=ncell)
+    return arr, T
+
+
+@pytest.mark.parametrize("incl,pa", [(60.0, 0.0), (25.0, 33.0)])
+def test_image_matches_oracle(incl, pa):
+    arr, T = _disk_setup()
+    dm = _dev(arr)
+    kap = rt._loglog_interp(np.array([1300.0]), arr["lam_um"], arr["kabs"][0])[0]
+    npix, pix = 64, 4.0 * AU
+    cam = dm.camera(npix, pix, 1300.0, incl=incl, pa=pa, incl_star=True)
+    img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+    ref = oracle.image(arr, T, [kap, kap], npix, pix, incl=incl, posang=pa, lam_um=1300.0,
+                       incl_star=True)
+    err = np.abs(img - ref).max() / np.abs(ref).max()
+    print("image max rel err %.2e" % err)
+    assert err < 2e-3
+    # FP32 vs FP64 traversal: total flux to 1e-4
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+
+
+def test_image_3d_and_tau_mode_match_oracle():
+    arr, T = _disk_setup(nr=24, nt=16, nphi=12)
+    dm = _dev(arr)
+    kap = rt._loglog_interp(np.array([870.0]), arr["lam_um"], arr["kabs"][0])[0]
+    npix, pix = 40, 6.0 * AU
+    for mode in (0, 1):
+        cam = dm.camera(npix, pix, 870.0, incl=45.0, pa=10.0, phi=30.0, mode=mode, incl_star=False)
+        img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+        ref = oracle.image(arr, T, [kap, kap], npix, pix, incl=45.0, posang=10.0, phi=30.0,
+                           lam_um=870.0, mode=mode)
+        err = np.abs(img - ref).max() / np.abs(ref).max()
+        print("mode %d max rel err %.2e" % (mode, err))
+        assert err < 2e-3
+
+
+def test_image_adjoint_matches_oracle_fp64_gradient():
+    arr, T = _disk_setup(nr=32, nt=16)
+    dm = _dev(arr)
+    kap = rt._loglog_interp(np.array([1300.0]), arr["lam_um"], arr["kabs"][0])[0]
+    npix, pix = 48, 5.0 * AU
+    rng = np.random.default_rng(1)
+    dimg = rng.normal(size=(npix, npix))
+    cam = dm.camera(npix, pix, 1300.0, incl=60.0, incl_star=True)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    g_lr, g_T = dm.image_adjoint(cam, Td, None, torch.tensor(dimg, dtype=torch.float32, device="cuda"))
+    o_lr, o_T = oracle.image_adjoint(arr, T, [kap, kap], dimg, npix, pix, incl=60.0, lam_um=1300.0,
+                                     incl_star=True)
+    e_lr = np.linalg.norm(g_lr.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
+    e_T = np.linalg.norm(g_T.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
+    print("relative gradient error: logrho %.2e  T %.2e" % (e_lr, e_T))
+    assert e_lr < 2e-3 and e_T < 2e-3
+
+
+def test_image_adjoint_central_differences_gpu():
+    """Relative error of the hand-written adjoint against central finite
+    differences of the GPU forward pass (50 random cells, FP32 forward)."""
+    arr, T = _disk_setup(nr=32, nt=16, seed=3)
+    dm = _dev(arr)
+    npix, pix = 40, 5.0 * AU
+    cam = dm.camera(npix, pix, 1300.0, incl=60.0, incl_star=False)
+    logrho = torch.log(torch.tensor(arr["rho"].reshape(2, -1), dtype=torch.float64,
+                                    device="cuda").clamp_min(1e-40)).float()
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    rng = np.random.default_rng(4)
+    w = torch.tensor(rng.normal(size=(npix, npix)), dtype=torch.float32, device="cuda")
+    lr = logrho.clone().requires_grad_(True)
+    Tt = Td.clone().requires_grad_(True)
+    loss = (rtag.image(lr, Tt, dm, cam) * w).sum()
+    loss.backward()
+    g = lr.grad[0].double().cpu().numpy()
+    gT = Tt.grad.double().cpu().numpy()
+    # cells that matter: largest gradients
+    cells = np.argsort(-np.abs(g))[:200]
+    cells = rng.choice(cells, 50, replace=False)
+    errs, errsT = [], []
+    for c in cells:
+        h = 2e-2
+        lp = logrho.clone()
+        lp[0, c] += h
+        lm = logrho.clone()
+        lm[0, c] -= h
+        fp = (dm.image(cam, Td, torch.exp(lp)).double() * w.double()).sum().item()
+        fm = (dm.image(cam, Td, torch.exp(lm)).double() * w.double()).sum().item()
+        fd = (fp - fm) / (2 * h)
+        errs.append(abs(fd - g[c]) / (abs(g[c]) + 1e-30))
+        dT = 1e-2 * T[c]
+        Tp = Td.clone()
+        Tp[c] += dT
+        Tm = Td.clone()
+        Tm[c] -= dT
+        rho = torch.exp(logrho)
+        fpT = (dm.image(cam, Tp, rho).double() * w.double()).sum().item()
+        fmT = (dm.image(cam, Tm, rho).double() * w.double()).sum().item()
+        if abs(gT[c]) > 0:
+            errsT.append(abs((fpT - fmT) / (2 * dT) - gT[c]) / abs(gT[c]))
+    print("FD relative error (50 cells): logrho median %.2e max %.2e; T median %.2e"
+          % (np.median(errs), np.max(errs), np.median(errsT)))
+    assert np.median(errs) < 2e-2
+    assert np.median(errsT) < 2e-2
+
+
+def test_vis_dft_matches_numpy_fft_and_oracle():
+    rng = np.random.default_rng(5)
+    n, dx = 64, 1e-7
+    img = rng.uniform(size=(n, n))
+    u, v = rt.fft_grid_uv(n, dx / inputs.arcsec)
+    re, im = rt.vis_dft(torch.tensor(img, dtype=torch.float32, device="cuda"), dx, u, v)
+    V = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img))).ravel()
+    scale = np.abs(V).max()
+    assert np.abs(re.double().cpu().numpy() - V.real).max() < 1e-5 * scale
+    assert np.abs(im.double().cpu().numpy() - V.imag).max() < 1e-5 * scale
+    uu, vv = configs.uv_points(K=3000, lam_m=1.3e-3, seed=1)
+    dxy = 0.0078125 * inputs.arcsec
+    img2 = rng.uniform(size=(256, 256))
+    re2, im2 = rt.vis_dft(torch.tensor(img2, dtype=torch.float32, device="cuda"), dxy, uu, vv,
+                          0.1 * inputs.arcsec, -0.05 * inputs.arcsec)
+    ore, oim = oracle.vis_dft(img2, dxy, uu, vv, 0.1 * inputs.arcsec, -0.05 * inputs.arcsec)
+    scale = np.abs(ore + 1j * oim).max()
+    err = max(np.abs(re2.double().cpu().numpy() - ore).max(), np.abs(im2.double().cpu().numpy() - oim).max())
+    print("vis max abs err / max|V| = %.2e" % (err / scale))
+    assert err < 2e-5 * scale
+
+
+def test_vis_adjoint_matches_oracle():
+    rng = np.random.default_rng(6)
+    ny = nx = 48
+    K = 1500
+    dxy = 0.03 * inputs.arcsec
+    u, v = configs.uv_points(K=K, lam_m=1.3e-3, seed=2)
+    gr, gi = rng.normal(size=K), rng.normal(size=K)
+    g = rt.vis_dft_adjoint(torch.tensor(gr, dtype=torch.float32, device="cuda"),
+                           torch.tensor(gi, dtype=torch.float32, device="cuda"), nx, ny, dxy, u, v,
+                           0.02 * inputs.arcsec, 0.01 * inputs.arcsec).double().cpu().numpy()
+    o = oracle.vis_dft_adjoint(gr, gi, nx, ny, dxy, u, v, 0.02 * inputs.arcsec, 0.01 * inputs.arcsec)
+    assert np.linalg.norm(g - o) / np.linalg.norm(o) < 1e-5
+
+
+def test_chisq_and_gradient():
+    rng = np.random.default_rng(7)
+    K = 10007
+    a = [rng.normal(size=K).astype(np.float32) for _ in range(4)]
+    w = rng.uniform(0.5, 2, K).astype(np.float32)
+    t = [torch.tensor(x, device="cuda") for x in (a[0], a[1], w, a[2], a[3])]
+    out, gre, gim = rt.chisq(*t, with_grad=True)
+    want = oracle.chisq(a[0], a[1], w, a[2], a[3])
+    assert out.item() == pytest.approx(want, rel=1e-6)
+    assert np.allclose(gre.cpu().numpy(), -2 * w * (a[0] - a[2]), rtol=1e-5)
+    # deterministic reduction
+    out2 = rt.chisq(*t)
+    assert out2.item() == out.item()
+
+
+def test_chi2_of_visibilities_gradient_matches_oracle_chain():
+    """d chi^2 / d log rho through image -> DFT -> chi^2 (config-3 chain, small)."""
+    arr, T = _disk_setup(nr=32, nt=16, seed=8)
+    dm = _dev(arr)
+    npix = 48
+    pixsize = 0.05  # arcsec
+    dpc = 140.0
+    pix_cm = pixsize * dpc * AU
+    dxy = pixsize * inputs.arcsec
+    cam = dm.camera(npix, pix_cm, 1300.0, incl=60.0, incl_star=True)
+    u, v = configs.uv_points(K=2000, bmax_m=300.0, lam_m=1.3e-3, seed=3)
+    logrho = torch.log(torch.tensor(arr["rho"].reshape(2, -1), dtype=torch.float32,
+                                    device="cuda").clamp_min(1e-38))
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    with torch.no_grad():
+        img = rtag.image(logrho, Td, dm, cam)
+        re, im = rtag.visibilities(img, torch.tensor(u, device="cuda"), torch.tensor(v, device="cuda"), dxy)
+    rng = np.random.default_rng(9)
+    sig = 0.01 * torch.sqrt(re ** 2 + im ** 2).max().item()
+    dre = re + torch.tensor(rng.normal(size=len(u)) * sig * 3, dtype=torch.float32, device="cuda")
+    dim = im + torch.tensor(rng.normal(size=len(u)) * sig * 3, dtype=torch.float32, device="cuda")
+    w = torch.full_like(dre, 1.0 / sig ** 2)
+    lr = logrho.clone().requires_grad_(True)
+    Tt = Td.clone().requires_grad_(True)
+    mre, mim = rtag.visibilities(rtag.image(lr, Tt, dm, cam), torch.tensor(u, device="cuda"),
+                                 torch.tensor(v, device="cuda"), dxy)
+    loss = rtag.chi2(mre, mim, dre, dim, w)
+    loss.backward()
+    # FP64 oracle chain
+    kap = rt._loglog_interp(np.array([1300.0]), arr["lam_um"], arr["kabs"][0])[0]
+    oimg = oracle.image(arr, T, [kap, kap], npix, pix_cm, incl=60.0, lam_um=1300.0, incl_star=True)
+    ore, oim = oracle.vis_dft(oimg, dxy, u, v)
+    d_re, d_im, ww = dre.double().cpu().numpy(), dim.double().cpu().numpy(), w.double().cpu().numpy()
+    gre, gim = -2 * ww * (d_re - ore), -2 * ww * (d_im - oim)
+    gimg = oracle.vis_dft_adjoint(gre, gim, npix, npix, dxy, u, v)
+    o_lr, o_T = oracle.image_adjoint(arr, T, [kap, kap], gimg, npix, pix_cm, incl=60.0,
+                                     lam_um=1300.0, incl_star=True)
+    e_lr = np.linalg.norm(lr.grad.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
+    e_T = np.linalg.norm(Tt.grad.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
+    print("chi2 chain relative gradient error: logrho %.2e T %.2e" % (e_lr, e_T))
+    assert e_lr < 1e-2 and e_T < 1e-2
+
+
+def test_run_image_and_visibilities_units():
+    m = configs.disk_envelope(nr=32, nt=16)
+    rt.run_thermal(m, nphot=20000, iseed=1, max_events=20000)
+    im = rt.run_image(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140)
+    assert im.image.shape == (32, 32, 1, 1)
+    assert im.x[16] == 0.0 and im.x[17] == pytest.approx(0.1)
+    assert im.wave[0] == pytest.approx(0.13)
+    # Jy/pixel conversion (Model.py:566-567) against an explicit rendering in cgs
+    dm = rt._device_model(m)
+    T = torch.tensor(rt.imaging_temperature(m.grid), dtype=torch.float32, device="cuda")
+    pix_cm = 0.1 * 140 * AU
+    cgs = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60), T).double().cpu().numpy()
+    assert np.allclose(im.image[:, :, 0, 0], cgs / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2,
+                       rtol=1e-6)
+    vis = rt.run_visibilities(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140)
+    V = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(im.image[:, :, 0, 0])))
+    scale = np.abs(V).max()
+    assert np.abs(vis.real[:, 0] - V.real.ravel()).max() < 1e-5 * scale
+    assert np.abs(vis.imag[:, 0] - V.imag.ravel()).max() < 1e-5 * scale

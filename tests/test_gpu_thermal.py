@@ -1,0 +1,134 @@
+"""GPU thermal Monte Carlo: parity with the FP64 oracle (same Philox streams,
+same epoch schedule), bitwise reproducibility, closed forms, error paths."""
+import numpy as np
+import pytest
+
+import oracle
+from pdspy_amd import configs, rt
+from tests.helpers import dT_stats, oracle_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_T(arr, nphot, seed, **opts):
+    dm = rt.DeviceModel(arr)
+    st = dm.thermal(nphot, seed, count_visits=1, **opts)
+    T, vis = dm.temperature(with_visits=True)
+    return T, vis, st
+
+
+def test_thermal_matches_oracle_disk():
+    print("case: disk 32x32, 1e5 packets, seed 7", flush=True)
+    arr = oracle_arrays(configs.disk_c1(nr=32, nt=32))
+    n = 100000
+    Tg, vg, sg = _gpu_T(arr, n, 7, max_events=20000)
+    To, vo, so = oracle.thermal(arr, n, seed=7, max_events=20000, lagged=True)
+    med, p99, ncell = dT_stats(To, Tg, vo)
+    print("median %.2e p99 %.2e cells %d" % (med, p99, ncell))
+    assert ncell > 500
+    assert med <= 0.01 and p99 <= 0.05
+    assert sg["n_epochs"] == so["n_epochs"]
+    assert abs(int(vg.sum()) - int(vo.sum())) <= 0.01 * vo.sum()
+    for k in ("n_abs", "n_scat", "n_cross"):
+        assert sg[k] == pytest.approx(so[k], rel=0.02), k
+    assert sg["n_escape"] + sg["n_killed"] == n
+
+
+def test_thermal_matches_oracle_disk_envelope_two_species():
+    print("case: disk+envelope 48x24, 5e4 packets, seed 1", flush=True)
+    arr = oracle_arrays(configs.disk_envelope(nr=48, nt=24))
+    n = 50000
+    Tg, vg, sg = _gpu_T(arr, n, 1, max_events=20000)
+    To, vo, so = oracle.thermal(arr, n, seed=1, max_events=20000, lagged=True)
+    med, p99, ncell = dT_stats(To, Tg, vo)
+    print("median %.2e p99 %.2e cells %d" % (med, p99, ncell))
+    assert med <= 0.01 and p99 <= 0.05
+    # species share one opacity: equal temperatures wherever both are present
+    both = (arr["rho"][0].ravel() > 0) & (arr["rho"][1].ravel() > 0)
+    assert np.array_equal(Tg[0][both], Tg[1][both])
+
+
+def test_thermal_hg_3d_matches_oracle():
+    print("case: 3D spiral 16x8x8 HG g=0.6, 2e4 packets", flush=True)
+    arr = oracle_arrays(configs.spiral_3d(nr=16, nt=8, nphi=8))
+    n = 20000
+    Tg, vg, sg = _gpu_T(arr, n, 2, max_events=20000)
+    To, vo, so = oracle.thermal(arr, n, seed=2, max_events=20000, lagged=True)
+    med, p99, ncell = dT_stats(To, Tg, vo)
+    print("median %.2e p99 %.2e cells %d" % (med, p99, ncell))
+    assert med <= 0.01 and p99 <= 0.05
+
+
+def test_thermal_bitwise_reproducible_across_launch_shapes():
+    print("case: determinism disk 24x24, 3e4 packets", flush=True)
+    arr = oracle_arrays(configs.disk_c1(nr=24, nt=24))
+    dm = rt.DeviceModel(arr)
+    out = []
+    for bs, bpc in [(256, 0), (256, 0), (64, 1), (128, 3)]:
+        dm.thermal(30000, 9, max_events=20000, block_size=bs, blocks_per_cu=bpc)
+        out.append(dm.temperature())
+    for T in out[1:]:
+        assert np.array_equal(T, out[0])
+    dm.thermal(30000, 10, max_events=20000)
+    assert not np.array_equal(dm.temperature(), out[0])
+
+
+def test_thermal_thin_shell_closed_form_gpu():
+    from tests.test_oracle import _thin_shell_model, _star_lum
+    m = _thin_shell_model()
+    T, vis, st = _gpu_T(m, 400000, 4)
+    L = _star_lum(m["lam_um"], 1.392e11, 4000.0)
+    assert st["lum"] == pytest.approx(L, rel=1e-9)
+    rw = m["rw"]
+    for i in range(4):
+        e = L * 10.0 * (rw[i + 1] - rw[i]) / (4.0 / 3.0 * np.pi * (rw[i + 1] ** 3 - rw[i] ** 3))
+        lo, hi = 1.0, 3000.0
+        for _ in range(80):
+            mid = np.sqrt(lo * hi)
+            if oracle.emis_per_gram(np.full(100, 10.0), m["lam_um"], mid) < e:
+                lo = mid
+            else:
+                hi = mid
+        Tc = T[0].reshape(4, 4)[:, i]
+        assert np.all(np.abs(Tc / lo - 1) < 5e-3), (i, Tc, lo)
+
+
+def test_thermal_event_cap_kills_and_counts():
+    arr = oracle_arrays(configs.disk_c1(nr=16, nt=16))
+    T, vis, st = _gpu_T(arr, 5000, 3, max_events=10)
+    assert st["n_killed"] > 0
+    assert st["n_killed"] + st["n_escape"] == 5000
+
+
+def test_thermal_watchdog_returns_error_not_hang():
+    arr = oracle_arrays(configs.disk_c1(nr=16, nt=16))
+    dm = rt.DeviceModel(arr)
+    from pdspy_amd._lib import RTError
+    with pytest.raises(RTError) as e:
+        dm.thermal(2000000, 0, max_events=1000000, max_seconds=1e-4)
+    assert e.value.code == -4
+
+
+def test_model_api_fills_grid_temperature():
+    m = configs.disk_envelope(nr=32, nt=16)
+    st = rt.run_thermal(m, nphot=20000, iseed=5, max_events=20000, return_stats=True,
+                        modified_random_walk=False, setthreads=8, verbose=False)
+    assert len(m.grid.temperature) == 2
+    for T, d in zip(m.grid.temperature, m.grid.density):
+        assert T.shape == d.shape and T.dtype == np.float64
+        assert np.all(T[d > 0] > 0) or np.mean(T[d > 0] > 0) > 0.9
+    assert st["n_escape"] + st["n_killed"] == 20000
+
+
+def test_invalid_inputs_rejected():
+    from pdspy_amd._lib import RTError
+    arr = oracle_arrays(configs.disk_c1(nr=8, nt=8))
+    bad = dict(arr)
+    bad["rho"] = arr["rho"].copy()
+    bad["rho"][0, 0, 0, 0] = np.nan
+    with pytest.raises(RTError):
+        rt.DeviceModel(bad)
+    bad = dict(arr)
+    bad["rw"] = arr["rw"][::-1].copy()
+    with pytest.raises(RTError):
+        rt.DeviceModel(bad)
