@@ -58,7 +58,7 @@ static bool finite_all(const double *p, size_t n) {
 
 static void free_model(rt_model *m) {
     void *ptrs[] = {m->d_rw, m->d_rw2, m->d_tw, m->d_ct, m->d_st, m->d_rwd, m->d_ctd, m->d_std,
-                    m->d_pw, m->d_cp, m->d_sp, m->d_rhoL, m->d_rho, m->d_ka, m->d_ks, m->d_g,
+                    m->d_pw, m->d_cp, m->d_sp, m->d_rhoL, m->d_rho, m->d_rhopos, m->d_ka, m->d_ks, m->d_g,
                     m->d_grp, m->d_gka, m->d_ginvk, m->d_invdr, m->d_cdf, m->d_guide, m->d_lnq,
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
@@ -71,12 +71,17 @@ static int upload_density(rt_model *m) {
     rt_ctx *ctx = m->ctx;
     const int S = m->nspec, nc = m->ncell;
     std::vector<float> rhoL((size_t)nc * S), rho((size_t)S * nc);
+    std::vector<unsigned char> pos((size_t)S * nc);
     for (int s = 0; s < S; ++s)
         for (int c = 0; c < nc; ++c) {
             double r = m->rho[(size_t)s * nc + c];
             rho[(size_t)s * nc + c] = (float)r;
             rhoL[(size_t)c * S + s] = (float)(r * RT_LEN_UNIT);
+            // the species' temperature is reported wherever its FP64 density is
+            // positive, also where the FP32 copy underflows to 0
+            pos[(size_t)s * nc + c] = r > 0 ? 1 : 0;
         }
+    HIPCHK(ctx, hipMemcpy(m->d_rhopos, pos.data(), pos.size(), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(m->d_rhoL, rhoL.data(), rhoL.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(m->d_rho, rho.data(), rho.size() * sizeof(float), hipMemcpyHostToDevice));
     return 0;
@@ -223,6 +228,7 @@ static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
         for (int a = 0; a < 3; ++a) spos[3 * s + a] = (float)(m->stars[5 * s + 2 + a] / L);
     const size_t ncg = (size_t)nc * G;
     if (upload<float>(ctx, &m->d_rhoL, nullptr, nrho) || upload<float>(ctx, &m->d_rho, nullptr, nrho) ||
+        upload<unsigned char>(ctx, &m->d_rhopos, nullptr, nrho) ||
         upload(ctx, &m->d_ka, ka.data(), nk) || upload(ctx, &m->d_ks, ks.data(), nk) ||
         upload(ctx, &m->d_g, gg.data(), nk) || upload(ctx, &m->d_grp, m->tab.grp.data(), m->nspec) ||
         upload(ctx, &m->d_gka, gka.data(), gka.size()) || upload(ctx, &m->d_ginvk, ginvk.data(), ginvk.size()) ||

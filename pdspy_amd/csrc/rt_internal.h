@@ -74,6 +74,7 @@ struct rt_model {
     float *d_pw = nullptr, *d_cp = nullptr, *d_sp = nullptr;
     float *d_rhoL = nullptr;      // [cells][S] rho * length unit
     float *d_rho = nullptr;       // [S][cells] rho (g/cm^3)
+    unsigned char *d_rhopos = nullptr;  // [S][cells] 1 where the FP64 density is > 0
     float *d_ka = nullptr, *d_ks = nullptr, *d_g = nullptr;  // [S][nlam]
     int *d_grp = nullptr;
     float *d_gka = nullptr;       // [G][nlam]

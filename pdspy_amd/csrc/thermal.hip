@@ -328,12 +328,12 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 }
 
 // per-species output temperature: group temperature where rho_s > 0, else 0
-__global__ void k_species_T(const float *Tg, const float *rho, const int *grp, int S, int G,
-                            int ncell, float *Tout) {
+__global__ void k_species_T(const float *Tg, const unsigned char *rhopos, const int *grp, int S,
+                            int G, int ncell, float *Tout) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S * ncell) return;
     int s = i / ncell, c = i - s * ncell;
-    Tout[i] = rho[i] > 0.f ? Tg[(size_t)c * G + grp[s]] : 0.f;
+    Tout[i] = rhopos[i] ? Tg[(size_t)c * G + grp[s]] : 0.f;
 }
 
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -436,7 +436,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         nep++;
     }
     hipLaunchKernelGGL(k_species_T, dim3(nblk((size_t)m->nspec * m->ncell, 256)), dim3(256), 0, sm,
-                       m->d_Tg, m->d_rho, m->d_grp, m->nspec, G, m->ncell, m->d_Tout);
+                       m->d_Tg, m->d_rhopos, m->d_grp, m->nspec, G, m->ncell, m->d_Tout);
     hipEvent_t e_end = mk();
     HIPCHK(ctx, hipEventRecord(e_end, sm));
     HIPCHK(ctx, hipEventSynchronize(e_end));
