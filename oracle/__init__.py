@@ -52,13 +52,15 @@ class _ThermOpts(C.Structure):
                 ("temp0", C.c_double), ("temp1", C.c_double),
                 ("max_events", C.c_uint64), ("lagged", C.c_int),
                 ("epoch_first", C.c_uint64), ("epoch_growth", C.c_double),
-                ("nthreads", C.c_int)]
+                ("nthreads", C.c_int), ("mrw", C.c_int), ("mrw_tauthres", C.c_double),
+                ("mrw_count", C.c_int), ("mrw_gamma", C.c_double)]
 
 
 class _ThermStats(C.Structure):
     _fields_ = [("n_cross", C.c_uint64), ("n_abs", C.c_uint64), ("n_scat", C.c_uint64),
                 ("n_escape", C.c_uint64), ("n_killed", C.c_uint64),
-                ("n_events", C.c_uint64), ("eps", C.c_double), ("lum", C.c_double),
+                ("n_events", C.c_uint64), ("n_mrw", C.c_uint64), ("eps", C.c_double),
+                ("lum", C.c_double),
                 ("tally_tau_sum", C.c_double), ("n_epochs", C.c_int)]
 
 
@@ -90,6 +92,10 @@ def _setup(L):
     L.or_emis_per_gram.argtypes = [P, P, C.c_int, C.c_double]
     L.or_emis_per_gram.restype = C.c_double
     L.or_freq_grid.argtypes = [P, C.c_int, P, P]
+    L.or_mrw_survival.argtypes = [C.c_double]
+    L.or_mrw_survival.restype = C.c_double
+    L.or_mrw_invert.argtypes = [C.c_double]
+    L.or_mrw_invert.restype = C.c_double
 
 
 def _f64(a):
@@ -128,13 +134,15 @@ class _Pinned:
 
 
 def thermal(m, nphot, seed=0, ntemp=1000, temp0=0.01, temp1=1e5, max_events=0,
-            lagged=True, epoch_first=None, epoch_growth=1.25, nthreads=0):
+            lagged=True, epoch_first=None, epoch_growth=1.25, nthreads=0, mrw=False,
+            mrw_tauthres=10.0, mrw_count=100, mrw_gamma=2.0):
     """Returns (T[nspec][cells] float64, visits[cells] uint64, stats dict)."""
     pm = _Pinned(m)
     if epoch_first is None:
         epoch_first = max(1024, int(nphot) // 4096)
     o = _ThermOpts(int(nphot), int(seed), ntemp, temp0, temp1, int(max_events),
-                   1 if lagged else 0, int(epoch_first), float(epoch_growth), int(nthreads))
+                   1 if lagged else 0, int(epoch_first), float(epoch_growth), int(nthreads),
+                   1 if mrw else 0, float(mrw_tauthres), int(mrw_count), float(mrw_gamma))
     T = np.zeros((pm.nspec, pm.ncell))
     visits = np.zeros(pm.ncell, dtype=np.uint64)
     st = _ThermStats()
@@ -256,3 +264,11 @@ def freq_grid(lam_um):
     dnu = np.zeros(len(lam))
     lib().or_freq_grid(lam.ctypes.data, len(lam), nu.ctypes.data, dnu.ctypes.data)
     return nu, dnu
+
+
+def mrw_survival(y):
+    return lib().or_mrw_survival(float(y))
+
+
+def mrw_invert(xi):
+    return lib().or_mrw_invert(float(xi))

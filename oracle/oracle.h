@@ -53,10 +53,14 @@ typedef struct {
     uint64_t epoch_first; /* first epoch size in packets (lagged mode)      */
     double epoch_growth;  /* later epochs: max(first, N_done*(growth-1))    */
     int nthreads;         /* OpenMP threads (lagged mode only)              */
+    int mrw;              /* modified random walk on/off                    */
+    double mrw_tauthres;  /* Rosseland tau to the nearest wall to trigger   */
+    int mrw_count;        /* interactions in the cell before it may trigger */
+    double mrw_gamma;     /* sphere radius = d_min - gamma / alpha_R        */
 } or_thermal_opts;
 
 typedef struct {
-    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events;
+    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw;
     double eps;           /* packet energy L_tot/nphot, erg/s               */
     double lum;           /* total stellar luminosity on the grid, erg/s    */
     double tally_tau_sum; /* sum over cells/groups of eps*tally/eps        */
@@ -128,6 +132,11 @@ double or_dplanck_dT(double nu, double T);
 /* q(T) = 4 pi sum_i kabs_i B_i(T) dnu_i  (erg/s/g) and its inverse.        */
 double or_emis_per_gram(const double *kabs, const double *lam_um, int nlam,
                         double T);
+/* Modified-random-walk escape law: survival probability of a diffusing packet
+ * started at the centre of a sphere, P(y) = 2 sum_n (-1)^(n+1) exp(-n^2 pi^2 y)
+ * with y = D t / R0^2, and its inverse. */
+double or_mrw_survival(double y);
+double or_mrw_invert(double xi);
 
 #ifdef __cplusplus
 }

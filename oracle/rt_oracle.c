@@ -549,6 +549,8 @@ typedef struct {
     double *cdf;     /* [ngroup][ntemp][nlam] */
     double *scdf;    /* [nstar][nlam] */
     double *spick;   /* [nstar] cumulative luminosity fractions */
+    double *kross;   /* [nspec][ntemp] Rosseland-mean extinction */
+    double *kplanck; /* [ngroup][ntemp] Planck-mean absorption */
     double lum;
 } tab_t;
 
@@ -562,6 +564,8 @@ static void tab_free(tab_t *t) {
     free(t->cdf);
     free(t->scdf);
     free(t->spick);
+    free(t->kross);
+    free(t->kplanck);
 }
 
 static double logsumexp(const double *v, int n) {
@@ -645,6 +649,43 @@ static int tab_init(tab_t *t, const or_model *m, const or_thermal_opts *o) {
                                     : -INFINITY;
             cdf_from_log(lw, nl, t->cdf + ((size_t)q * t->ntemp + it) * nl);
         }
+    }
+    /* Planck-mean absorption per group and Rosseland-mean extinction per
+     * species on the same temperature grid (used by the modified random walk) */
+    t->kross = malloc(sizeof(double) * ns * t->ntemp);
+    t->kplanck = malloc(sizeof(double) * ng * t->ntemp);
+    for (int it = 0; it < t->ntemp; ++it) {
+        double T = exp(t->lnt0 + it * t->dlnt);
+        double *lb = malloc(sizeof(double) * nl), *ld = malloc(sizeof(double) * nl);
+        double mb = -INFINITY, md = -INFINITY;
+        for (int i = 0; i < nl; ++i) {
+            lb[i] = log(t->dnu[i]) + ln_planck(t->nu[i], T);
+            ld[i] = log(t->dnu[i]) + ln_dplanck(t->nu[i], T);
+            if (lb[i] > mb) mb = lb[i];
+            if (ld[i] > md) md = ld[i];
+        }
+        for (int q = 0; q < ng; ++q) {
+            const double *ka = m->kabs + (size_t)t->grep[q] * nl;
+            double wb = 0, wkb = 0;
+            for (int i = 0; i < nl; ++i) {
+                double b = exp(lb[i] - mb);
+                wb += b;
+                wkb += b * ka[i];
+            }
+            t->kplanck[q * t->ntemp + it] = wkb / wb;
+        }
+        for (int s = 0; s < ns; ++s) {
+            double wd = 0, wdk = 0;
+            for (int i = 0; i < nl; ++i) {
+                double d = exp(ld[i] - md);
+                double ke = m->kabs[(size_t)s * nl + i] + m->ksca[(size_t)s * nl + i];
+                wd += d;
+                if (ke > 0) wdk += d / ke;
+            }
+            t->kross[s * t->ntemp + it] = wdk > 0 ? wd / wdk : 0.0;
+        }
+        free(lb);
+        free(ld);
     }
     /* stars: blackbody point sources, L = 4 pi R^2 * pi * sum B dnu */
     t->scdf = malloc(sizeof(double) * (m->nstar > 0 ? m->nstar : 1) * nl);
@@ -747,6 +788,68 @@ static void hg_dir(rng_t *r, double g, double *d) {
     d[2] /= n;
 }
 
+/* ------------------------------------------------- modified random walk */
+/* Survival probability of a packet diffusing from the centre of a sphere of
+ * radius R0, y = D t / R0^2 (Fleck & Canfield 1984; Min et al. 2009):
+ * P(y) = 2 sum_{n>=1} (-1)^(n+1) exp(-n^2 pi^2 y); for small y the Jacobi
+ * dual 1 - (pi y)^-1/2 sum_m exp(-(m+1/2)^2 / y) converges faster. */
+double or_mrw_survival(double y) {
+    if (y <= 0) return 1.0;
+    if (y < 0.1) {
+        double s = 0;
+        for (int m = -40; m <= 40; ++m) s += exp(-(m + 0.5) * (m + 0.5) / y);
+        return 1.0 - s / sqrt(PI_D * y);
+    }
+    double s = 0;
+    for (int n = 1; n <= 60; ++n) s += ((n & 1) ? 2.0 : -2.0) * exp(-(double)n * n * PI_D * PI_D * y);
+    return s;
+}
+
+double or_mrw_invert(double xi) {
+    double lo = log(1e-9), hi = log(60.0);
+    if (xi >= or_mrw_survival(1e-9)) return 1e-9;
+    if (xi <= or_mrw_survival(60.0)) return 60.0;
+    for (int k = 0; k < 80; ++k) {
+        double mid = 0.5 * (lo + hi);
+        if (or_mrw_survival(exp(mid)) > xi)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return exp(0.5 * (lo + hi));
+}
+
+/* perpendicular distance to the nearest wall of the current cell that bounds
+ * the medium (the equatorial mirror and the polar axis are not walls) */
+static double wall_dist(const geom_t *g, const pk_t *p) {
+    double r = sqrt(p->x * p->x + p->y * p->y + p->z * p->z);
+    double d = fmin(r - g->rw[p->ir], g->rw[p->ir + 1] - r);
+    double rho = sqrt(p->x * p->x + p->y * p->y);
+    double th = g->mirror ? atan2(rho, fabs(p->z)) : atan2(rho, p->z);
+    int j = p->it;
+    if (j > 0) d = fmin(d, r * sin(fmin(fabs(th - g->tw[j]), 0.5 * PI_D)));
+    if (j + 1 < g->nt) d = fmin(d, r * sin(fmin(fabs(g->tw[j + 1] - th), 0.5 * PI_D)));
+    if (g->np > 1) {
+        double ph = atan2(p->y, p->x);
+        if (ph < 0) ph += TWO_PI_D;
+        int k = p->ip;
+        d = fmin(d, rho * sin(fmin(fabs(ph - g->pw[k]), 0.5 * PI_D)));
+        d = fmin(d, rho * sin(fmin(fabs(g->pw[k + 1] - ph), 0.5 * PI_D)));
+    }
+    return d;
+}
+
+/* table value at temperature row coordinate x (linear between rows) */
+static double trow(const double *tab, int ntemp, double T, double lnt0, double dlnt) {
+    double x = (T > 0) ? (log(T) - lnt0) / dlnt : 0.0;
+    if (x < 0) x = 0;
+    if (x > ntemp - 1) x = ntemp - 1;
+    int lo = (int)x;
+    if (lo > ntemp - 2) lo = ntemp - 2;
+    double w = x - lo;
+    return (1 - w) * tab[lo] + w * tab[lo + 1];
+}
+
 /* ------------------------------------------------------------ thermal MC */
 typedef struct {
     const or_model *m;
@@ -762,7 +865,7 @@ typedef struct {
 } mc_t;
 
 typedef struct {
-    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events;
+    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw;
 } cnt_t;
 
 static void deposit(mc_t *mc, int c, int ifr, double s) {
@@ -782,6 +885,116 @@ static void deposit(mc_t *mc, int c, int ifr, double s) {
             if (old > ((uint64_t)1 << 62)) mc->overflow = 1;
         }
     }
+}
+
+/* add an explicit absorbed amount (kappa * path, cm^3/g) to group q of cell c */
+static void deposit_q(mc_t *mc, int c, int q, double kpath) {
+    int ng = mc->t->ngroup;
+    uint64_t add = (uint64_t)llround(kpath * mc->W[(size_t)c * ng + q]);
+    if (add) {
+        uint64_t old;
+#pragma omp atomic capture
+        {
+            old = mc->tally[(size_t)c * ng + q];
+            mc->tally[(size_t)c * ng + q] += add;
+        }
+        if (old > ((uint64_t)1 << 62)) mc->overflow = 1;
+    }
+}
+
+/* the temperature re-emission sees: lagged snapshot or the running tally */
+static double cur_T(mc_t *mc, int c, int q) {
+    int ng = mc->t->ngroup;
+    if (mc->o->lagged) return mc->Tcur[(size_t)c * ng + q];
+    uint64_t tv;
+#pragma omp atomic read
+    tv = mc->tally[(size_t)c * ng + q];
+    return temp_from_e(mc->t, q, (double)tv * mc->econv[(size_t)c * ng + q]);
+}
+
+/* rotate unit vector d by polar angle (cos ct) and azimuth ps about itself */
+static void rotate_dir(double *d, double ct, double ps) {
+    double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+    double cps = cos(ps), sps = sin(ps);
+    double ux = d[0], uy = d[1], uz = d[2];
+    if (fabs(uz) > 0.99999) {
+        d[0] = st * cps;
+        d[1] = st * sps;
+        d[2] = (uz > 0 ? 1.0 : -1.0) * ct;
+    } else {
+        double s = sqrt(1.0 - uz * uz);
+        d[0] = st * (ux * uz * cps - uy * sps) / s + ux * ct;
+        d[1] = st * (uy * uz * cps + ux * sps) / s + uy * ct;
+        d[2] = -st * cps * s + uz * ct;
+    }
+    double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    d[0] /= n;
+    d[1] /= n;
+    d[2] /= n;
+}
+
+/* one modified-random-walk step (Min et al. 2009; Robitaille 2010) for a
+ * packet at an interaction point in cell c; returns 1 if it was taken */
+static int mrw_step(mc_t *mc, pk_t *p, int c, rng_t *r, int *ifr) {
+    const or_model *m = mc->m;
+    const geom_t *g = mc->g;
+    const tab_t *t = mc->t;
+    const int ns = m->nspec, ng = t->ngroup, nl = t->nlam;
+    double aR = 0;
+    for (int s = 0; s < ns; ++s) {
+        double rho = m->rho[(size_t)s * g->ncell + c];
+        if (rho > 0)
+            aR += rho * trow(t->kross + (size_t)s * t->ntemp, t->ntemp, cur_T(mc, c, t->grp[s]),
+                             t->lnt0, t->dlnt);
+    }
+    double dmin = wall_dist(g, p);
+    if (!(aR * dmin >= mc->o->mrw_tauthres)) return 0;
+    double R0 = dmin - mc->o->mrw_gamma / aR;
+    if (!(R0 > 0)) return 0;
+    /* diffusion path length to the sphere: y = D t / R0^2, D = 1/(3 alpha_R) */
+    double y = or_mrw_invert(uni(r));
+    double ell = 3.0 * aR * R0 * R0 * y;
+    double kP[8];
+    for (int q = 0; q < ng; ++q) {
+        kP[q] = trow(t->kplanck + (size_t)q * t->ntemp, t->ntemp, cur_T(mc, c, q), t->lnt0, t->dlnt);
+        deposit_q(mc, c, q, kP[q] * ell);
+    }
+    /* exit point uniform on the sphere, direction cosine-weighted outwards */
+    double nrm[3];
+    iso_dir(r, nrm);
+    p->x += R0 * nrm[0];
+    p->y += R0 * nrm[1];
+    p->z += R0 * nrm[2];
+    if (g->mirror) p->hemi = (p->z >= 0) ? 1 : -1;
+    double ct = sqrt(uni(r));
+    double ps = TWO_PI_D * uni(r);
+    rotate_dir(nrm, ct, ps);
+    p->dx = nrm[0];
+    p->dy = nrm[1];
+    p->dz = nrm[2];
+    /* new frequency: re-emission spectrum of a species picked by rho * kappa_P */
+    double wsum = 0;
+    for (int s = 0; s < ns; ++s) wsum += m->rho[(size_t)s * g->ncell + c] * kP[t->grp[s]];
+    double us = uni(r) * wsum, acc = 0;
+    int sp = ns - 1;
+    for (int s = 0; s < ns; ++s) {
+        acc += m->rho[(size_t)s * g->ncell + c] * kP[t->grp[s]];
+        if (us <= acc) {
+            sp = s;
+            break;
+        }
+    }
+    int q = t->grp[sp];
+    double T = cur_T(mc, c, q);
+    double x = (T > 0) ? (log(T) - t->lnt0) / t->dlnt : 0.0;
+    if (x < 0) x = 0;
+    if (x > t->ntemp - 1) x = t->ntemp - 1;
+    int lo = (int)x;
+    if (lo > t->ntemp - 2) lo = t->ntemp - 2;
+    double w = x - lo;
+    int row = lo + (uni(r) < w ? 1 : 0);
+    *ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, uni(r));
+    return 1;
 }
 
 static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
@@ -815,7 +1028,7 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
         mc->visits[cell_of(g, &p)]++;
     }
     double tau = -log(uni(&r));
-    uint64_t nev = 0;
+    uint64_t nev = 0, nev_cell = 0;
     for (;;) {
         int kind;
         double tw = next_wall(g, &p, &kind);
@@ -838,6 +1051,7 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
             p.z += s * p.dz;
             cnt->n_events++;
             nev++;
+            nev_cell++;
             double ue = uni(&r);
             if (ue * aext < asca) {
                 cnt->n_scat++;
@@ -906,11 +1120,14 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
                 cnt->n_killed++;
                 return;
             }
+            if (mc->o->mrw && nev_cell >= (uint64_t)mc->o->mrw_count && mrw_step(mc, &p, c, &r, &ifr))
+                cnt->n_mrw++;
             tau = -log(uni(&r));
         } else {
             if (c >= 0) deposit(mc, c, ifr, tw);
             tau -= aext * tw;
             cnt->n_cross++;
+            nev_cell = 0;
             if (cross_wall(g, &p, tw, kind)) {
                 cnt->n_escape++;
                 return;
@@ -988,9 +1205,9 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             if (sz < first) sz = first;
             uint64_t end = done + sz;
             if (end > o->nphot) end = o->nphot;
-            uint64_t a = 0, b = 0, c2 = 0, d = 0, e = 0, f = 0;
+            uint64_t a = 0, b = 0, c2 = 0, d = 0, e = 0, f = 0, h = 0;
             int64_t lo = (int64_t)done, hi = (int64_t)end;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e, f)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e, f, h)
             for (int64_t p = lo; p < hi; ++p) {
                 cnt_t cc;
                 memset(&cc, 0, sizeof(cc));
@@ -1001,6 +1218,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
                 d += cc.n_escape;
                 e += cc.n_killed;
                 f += cc.n_events;
+                h += cc.n_mrw;
             }
             tot.n_cross += a;
             tot.n_abs += b;
@@ -1008,6 +1226,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             tot.n_escape += d;
             tot.n_killed += e;
             tot.n_events += f;
+            tot.n_mrw += h;
             done = end;
             nep++;
             snapshot(&mc);
@@ -1037,6 +1256,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
         st->n_escape = tot.n_escape;
         st->n_killed = tot.n_killed;
         st->n_events = tot.n_events;
+        st->n_mrw = tot.n_mrw;
         st->eps = eps;
         st->lum = t.lum;
         st->tally_tau_sum = tsum;
