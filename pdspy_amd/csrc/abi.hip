@@ -60,7 +60,7 @@ static void free_model(rt_model *m) {
     void *ptrs[] = {m->d_rw, m->d_rw2, m->d_tw, m->d_ct, m->d_st, m->d_rwd, m->d_ctd, m->d_std,
                     m->d_pw, m->d_cp, m->d_sp, m->d_rhoL, m->d_rho, m->d_rhopos, m->d_ka, m->d_ks, m->d_g,
                     m->d_grp, m->d_gka, m->d_ginvk, m->d_invdr, m->d_cdf, m->d_guide, m->d_lnq,
-                    m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck,
+                    m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI};
     for (void *p : ptrs)
@@ -242,6 +242,7 @@ static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
         upload(ctx, &m->d_spos, spos.data(), spos.size()) ||
         upload(ctx, &m->d_kross, m->tab.kross.data(), m->tab.kross.size()) ||
         upload(ctx, &m->d_kplanck, m->tab.kplanck.data(), m->tab.kplanck.size()) ||
+        upload(ctx, &m->d_mrwy, m->tab.mrw_y.data(), m->tab.mrw_y.size()) ||
         upload(ctx, &m->d_econv, econv.data(), econv.size()) ||
         upload<unsigned long long>(ctx, &m->d_tally, nullptr, ncg) ||
         upload<float>(ctx, &m->d_xrow, nullptr, ncg) || upload<float>(ctx, &m->d_Tg, nullptr, ncg) ||

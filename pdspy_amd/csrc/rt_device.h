@@ -382,17 +382,9 @@ __device__ __forceinline__ void iso_dir(Philox &r, float &dx, float &dy, float &
     dz = mu;
 }
 
-__device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy, float &dz) {
-    float u = uni(r);
-    float ct;
-    if (fabsf(g) < 1e-6f) {
-        ct = 2.f * u - 1.f;
-    } else {
-        float f = (1.f - g * g) / (1.f - g + 2.f * g * u);
-        ct = (1.f + g * g - f * f) / (2.f * g);
-    }
-    ct = fminf(1.f, fmaxf(-1.f, ct));
-    float u2 = uni(r);
+// rotate the unit vector (dx,dy,dz) by polar angle (cosine ct) and azimuth
+// 2 pi u2 about itself
+__device__ __forceinline__ void rotate_dir(float ct, float u2, float &dx, float &dy, float &dz) {
     float sps, cps;
     sincospif(2.f * u2, &sps, &cps);
     float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
@@ -411,6 +403,65 @@ __device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy,
     dx *= n;
     dy *= n;
     dz *= n;
+}
+
+// Henyey-Greenstein scattering about the current direction
+__device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy, float &dz) {
+    float u = uni(r);
+    float ct;
+    if (fabsf(g) < 1e-6f) {
+        ct = 2.f * u - 1.f;
+    } else {
+        float f = (1.f - g * g) / (1.f - g + 2.f * g * u);
+        ct = (1.f + g * g - f * f) / (2.f * g);
+    }
+    ct = fminf(1.f, fmaxf(-1.f, ct));
+    rotate_dir(ct, uni(r), dx, dy, dz);
+}
+
+// perpendicular distance to the nearest wall of the current cell that bounds
+// the medium; the equatorial mirror and the polar axis are not walls
+__device__ __forceinline__ float wall_dist(const DevGrid &g, const Pos &p) {
+    float r = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+    float d = fminf(r - g.rw[p.ir], g.rw[p.ir + 1] - r);
+    float rho = sqrtf(p.x * p.x + p.y * p.y);
+    float th = g.mirror ? atan2f(rho, fabsf(p.z)) : atan2f(rho, p.z);
+    const float hp = 1.57079632679f;
+    int j = p.it;
+    if (j > 0) d = fminf(d, r * sinf(fminf(fabsf(th - g.tw[j]), hp)));
+    if (j + 1 < g.nt) d = fminf(d, r * sinf(fminf(fabsf(g.tw[j + 1] - th), hp)));
+    if (g.np > 1) {
+        float ph = atan2f(p.y, p.x);
+        if (ph < 0.f) ph += 6.28318530718f;
+        int k = p.ip;
+        d = fminf(d, rho * sinf(fminf(fabsf(ph - g.pw[k]), hp)));
+        d = fminf(d, rho * sinf(fminf(fabsf(g.pw[k + 1] - ph), hp)));
+    }
+    return d;
+}
+
+// table value at the fractional temperature-row coordinate x
+__device__ __forceinline__ float trow(const float *tab, int ntemp, float x) {
+    x = fminf(fmaxf(x, 0.f), (float)(ntemp - 1));
+    int lo = min((int)x, ntemp - 2);
+    float w = x - (float)lo;
+    return fmaf(w, tab[lo + 1] - tab[lo], tab[lo]);
+}
+
+// diffusion escape law: y(xi) with P(y) = xi; table on xi_k = k/(NY-1), and
+// P ~ 2 exp(-pi^2 y) below the first node
+__device__ __forceinline__ float mrw_escape_y(const float *Y, int ny, float xi) {
+    const float inv = 1.f / (float)(ny - 1);
+    if (xi < inv) return __logf(2.f / xi) * 0.10132118364f;  // 1/pi^2
+    float f = xi * (float)(ny - 1);
+    int k = min((int)f, ny - 2);
+    float w = f - (float)k;
+    return fmaf(w, Y[k + 1] - Y[k], Y[k]);
+}
+
+// a[i] for i < 4 without dynamic register indexing
+__device__ __forceinline__ float pick4(const float (&a)[4], int i) {
+    return i == 0 ? a[0] : i == 1 ? a[1] : i == 2 ? a[2] : a[3];
 }
 
 // B_nu(T) in FP32 from h nu/k (pdspy/misc/B_nu.py:4-6); pre = 2 h nu^3 / c^2

@@ -14,6 +14,7 @@
 #define RT_NGUIDE 128    // guide-table entries per CDF row
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
+#define RT_MRW_NY 1024   // entries of the diffusion escape-law inverse table
 
 struct rt_ctx {
     int device = 0;
@@ -50,7 +51,8 @@ struct HostTables {
     std::vector<float> scdf;         // [nstar][nlam]
     std::vector<unsigned char> sguide; // [nstar][NGUIDE]
     std::vector<float> spick;        // [nstar]
-    std::vector<float> kross;        // [G][ntemp] Rosseland-mean extinction (MRW)
+    std::vector<float> kross;        // [S][ntemp] Rosseland-mean extinction (MRW)
+    std::vector<float> mrw_y;        // [RT_MRW_NY] inverse of the escape law
     std::vector<float> kplanck;      // [G][ntemp] Planck-mean absorption (MRW)
     double lum = 0;
     double lnt0 = 0, dlnt = 0;
@@ -87,7 +89,7 @@ struct rt_model {
     unsigned char *d_sguide = nullptr;
     float *d_spick = nullptr;
     float *d_spos = nullptr;      // [nstar][3] au
-    float *d_kross = nullptr, *d_kplanck = nullptr;
+    float *d_kross = nullptr, *d_kplanck = nullptr, *d_mrwy = nullptr;
     double *d_econv = nullptr;    // [cells][G] erg/s/g per tally unit (per packet energy 1)
     unsigned long long *d_tally = nullptr;  // [cells][G]
     float *d_xrow = nullptr;      // [cells][G] T-row coordinate of the lagged temperature

@@ -105,7 +105,7 @@ int build_tables(rt_model *m, std::string &err) {
     t.lnq.assign((size_t)G * nT, 0.0);
     t.cdf.assign((size_t)G * nT * nl, 0.f);
     t.guide.assign((size_t)G * nT * RT_NGUIDE, 0);
-    t.kross.assign((size_t)G * nT, 0.f);
+    t.kross.assign((size_t)ns * nT, 0.f);
     t.kplanck.assign((size_t)G * nT, 0.f);
     std::vector<double> lw(nl), lwb(nl);
     for (int q = 0; q < G; ++q) {
@@ -129,25 +129,60 @@ int build_tables(rt_model *m, std::string &err) {
                 lw[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) + lndB(t.nu[i], T) : -INFINITY;
             make_cdf(lw, &t.cdf[((size_t)q * nT + it) * nl],
                      &t.guide[((size_t)q * nT + it) * RT_NGUIDE], RT_NGUIDE);
-            // Planck-mean absorption and Rosseland-mean extinction at T
-            double wb = 0, wkb = 0, wd = 0, wdk = 0, mb = -INFINITY, md = -INFINITY;
-            for (int i = 0; i < nl; ++i) {
-                mb = std::max(mb, std::log(t.dnu[i]) + lnB(t.nu[i], T));
-                md = std::max(md, std::log(t.dnu[i]) + lndB(t.nu[i], T));
-            }
+            // Planck-mean absorption at T
+            double wb = 0, wkb = 0, mb = -INFINITY;
+            for (int i = 0; i < nl; ++i) mb = std::max(mb, std::log(t.dnu[i]) + lnB(t.nu[i], T));
             for (int i = 0; i < nl; ++i) {
                 double b = std::exp(std::log(t.dnu[i]) + lnB(t.nu[i], T) - mb);
-                double d = std::exp(std::log(t.dnu[i]) + lndB(t.nu[i], T) - md);
                 wb += b;
                 wkb += b * ka[i];
-                wd += d;
-                double ke = ka[i] + ks[i];
-                wdk += ke > 0 ? d / ke : 0.0;
             }
             t.kplanck[(size_t)q * nT + it] = (float)(wkb / wb);
-            t.kross[(size_t)q * nT + it] = (float)(wdk > 0 ? wd / wdk : 0.0);
         }
     }
+    // Rosseland-mean extinction per species (kabs + ksca), weights dB/dT dnu
+    for (int s = 0; s < ns; ++s) {
+        const double *ka = &m->kabs[(size_t)s * nl];
+        const double *ks = &m->ksca[(size_t)s * nl];
+        for (int it = 0; it < nT; ++it) {
+            double T = std::exp(t.lnt0 + it * t.dlnt);
+            double md = -INFINITY;
+            for (int i = 0; i < nl; ++i) md = std::max(md, std::log(t.dnu[i]) + lndB(t.nu[i], T));
+            double wd = 0, wdk = 0;
+            for (int i = 0; i < nl; ++i) {
+                double d = std::exp(std::log(t.dnu[i]) + lndB(t.nu[i], T) - md);
+                double ke = ka[i] + ks[i];
+                wd += d;
+                if (ke > 0) wdk += d / ke;
+            }
+            t.kross[(size_t)s * nT + it] = (float)(wdk > 0 ? wd / wdk : 0.0);
+        }
+    }
+    // modified random walk: y(xi) inverting the survival probability of a
+    // packet diffusing from the centre of a sphere, P(y) = 2 sum_n (-1)^(n+1)
+    // exp(-n^2 pi^2 y) (y = D t / R0^2), on xi_k = k / (NY - 1)
+    auto surv = [](double y) {
+        if (y < 0.1) {  // Jacobi dual form, fast for small y
+            double s = 0;
+            for (int mm = -40; mm <= 40; ++mm) s += std::exp(-(mm + 0.5) * (mm + 0.5) / y);
+            return 1.0 - s / std::sqrt(kPi * y);
+        }
+        double s = 0;
+        for (int n = 1; n <= 60; ++n) s += ((n & 1) ? 2.0 : -2.0) * std::exp(-(double)n * n * kPi * kPi * y);
+        return s;
+    };
+    t.mrw_y.assign(RT_MRW_NY, 0.f);
+    for (int k = 1; k < RT_MRW_NY; ++k) {
+        double xi = (double)k / (RT_MRW_NY - 1);
+        double lo = std::log(1e-9), hi = std::log(60.0);
+        for (int it = 0; it < 90; ++it) {
+            double mid = 0.5 * (lo + hi);
+            if (surv(std::exp(mid)) > xi) lo = mid; else hi = mid;
+        }
+        t.mrw_y[k] = (float)std::exp(0.5 * (lo + hi));
+    }
+    t.mrw_y[RT_MRW_NY - 1] = 0.f;  // xi = 1: no time has passed
+    t.mrw_y[0] = (float)(std::log(2.0 * (RT_MRW_NY - 1)) / (kPi * kPi));
     // stars
     t.scdf.assign((size_t)std::max(1, m->nstar) * nl, 0.f);
     t.sguide.assign((size_t)std::max(1, m->nstar) * RT_NGUIDE, 0);

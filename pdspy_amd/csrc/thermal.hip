@@ -47,7 +47,9 @@ struct ThermParams {
     const float *spick;
     const float *spos;
     const float *xrow;            // [cell][G]
-    const float *kross, *kplanck; // [G][ntemp]
+    const float *kross;           // [S][ntemp] Rosseland-mean extinction
+    const float *kplanck;         // [G][ntemp] Planck-mean absorption
+    const float *mrwy;            // [RT_MRW_NY] escape-law inverse
     unsigned long long *tally;    // [cell][G]
     unsigned int *visits;         // nullable
     unsigned long long *cnt;
@@ -85,15 +87,49 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     float *s_ks = s_ka + S * NL;
     float *s_gka = s_ks + S * NL;
     float *s_g = s_gka + G * NL;  // only when P.hg
+    // grid walls in LDS: every step reads several of them
+    const int NR = P.g.nr, NT = P.g.nt, NP = P.g.np;
+    float *s_rw = s_g + (P.hg ? S * NL : 0);
+    float *s_rw2 = s_rw + NR + 1;
+    float *s_tw = s_rw2 + NR + 1;
+    float *s_ct = s_tw + NT + 1;
+    float *s_st = s_ct + NT + 1;
+    float *s_pw = s_st + NT + 1;
+    float *s_cp = s_pw + NP + 1;
+    float *s_sp = s_cp + NP + 1;
+    float *s_invdr = s_sp + NP + 1;
     for (int i = threadIdx.x; i < S * NL; i += blockDim.x) {
         s_ka[i] = P.ka[i];
         s_ks[i] = P.ks[i];
         if (P.hg) s_g[i] = P.gg[i];
     }
     for (int i = threadIdx.x; i < G * NL; i += blockDim.x) s_gka[i] = P.gka[i];
+    for (int i = threadIdx.x; i <= NR; i += blockDim.x) {
+        s_rw[i] = P.g.rw[i];
+        s_rw2[i] = P.g.rw2[i];
+        if (i < NR) s_invdr[i] = P.invdr[i];
+    }
+    for (int i = threadIdx.x; i <= NT; i += blockDim.x) {
+        s_tw[i] = P.g.tw[i];
+        s_ct[i] = P.g.ct[i];
+        s_st[i] = P.g.st[i];
+    }
+    for (int i = threadIdx.x; i <= NP; i += blockDim.x) {
+        s_pw[i] = P.g.pw[i];
+        s_cp[i] = P.g.cp[i];
+        s_sp[i] = P.g.sp[i];
+    }
     __syncthreads();
 
-    const DevGrid &g = P.g;
+    DevGrid g = P.g;
+    g.rw = s_rw;
+    g.rw2 = s_rw2;
+    g.tw = s_tw;
+    g.ct = s_ct;
+    g.st = s_st;
+    g.pw = s_pw;
+    g.cp = s_cp;
+    g.sp = s_sp;
     const uint64_t range = P.pend - P.pbegin;
     const int lane = __lane_id();
     const uint64_t t_start = wall_clock64();
@@ -102,7 +138,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     Philox r;
     int ifr = 0;
     float tau = 0.f;
-    uint32_t nev = 0, nstep = 0;
+    uint32_t nev = 0, nstep = 0, nev_cell = 0;
     bool alive = false, exhausted = false;
     unsigned long long acc[RT_MAX_GROUP];
 #pragma unroll
@@ -138,6 +174,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                         locate(g, p);
                         nev = 0;
                         nstep = 0;
+                        nev_cell = 0;
                         if (p.ir >= g.nr) {
                             n_esc++;
                         } else {
@@ -186,7 +223,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         if (aext * tw > tau) {
             // interaction inside the cell
             float sseg = tau / aext;
-            float wf = P.invdr[p.ir] * sseg;
+            float wf = s_invdr[p.ir] * sseg;
 #pragma unroll
             for (int q = 0; q < RT_MAX_GROUP; ++q)
                 if (q < G) acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
@@ -195,6 +232,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             p.z = fmaf(sseg, p.dz, p.z);
             n_ev++;
             nev++;
+            nev_cell++;
             float ue = uni(r);
             const float *rc = P.rhoL + (size_t)c * S;
             if (ue * aext < asca) {
@@ -240,10 +278,66 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 alive = false;
                 continue;
             }
+            if (P.mrw && nev_cell >= (uint32_t)P.mrw_count) {
+                // modified random walk: jump to a sphere inside the cell when the
+                // Rosseland optical depth to the nearest wall is large
+                float aR = 0.f;
+                for (int s = 0; s < S; ++s) {
+                    float rho = rc[s];
+                    if (rho > 0.f)
+                        aR = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
+                                            P.xrow[(size_t)c * G + P.grp[s]]), aR);
+                }
+                float dmin = wall_dist(g, p);
+                float R0 = dmin - P.mrw_gamma / aR;
+                if (aR * dmin >= P.mrw_tau && R0 > 0.f) {
+                    float y = mrw_escape_y(P.mrwy, RT_MRW_NY, uni(r));
+                    float ell = 3.f * aR * R0 * R0 * y;
+                    float wl = s_invdr[p.ir] * ell;
+                    float kP[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int q = 0; q < RT_MAX_GROUP; ++q)
+                        if (q < G) {
+                            kP[q] = trow(P.kplanck + (size_t)q * P.ntemp, P.ntemp,
+                                         P.xrow[(size_t)c * G + q]);
+                            acc[q] += (unsigned long long)rintf(kP[q] * P.ginvk[q] * wl);
+                        }
+                    float nx, ny, nz;
+                    iso_dir(r, nx, ny, nz);
+                    p.x = fmaf(R0, nx, p.x);
+                    p.y = fmaf(R0, ny, p.y);
+                    p.z = fmaf(R0, nz, p.z);
+                    if (g.mirror) p.hemi = (p.z >= 0.f) ? 1 : -1;
+                    float ct = sqrtf(uni(r));
+                    rotate_dir(ct, uni(r), nx, ny, nz);
+                    p.dx = nx;
+                    p.dy = ny;
+                    p.dz = nz;
+                    float wsum = 0.f;
+                    for (int s = 0; s < S; ++s) wsum = fmaf(rc[s], pick4(kP, P.grp[s]), wsum);
+                    float us = uni(r) * wsum, a = 0.f;
+                    int sp = S - 1;
+                    for (int s = 0; s < S; ++s) {
+                        a = fmaf(rc[s], pick4(kP, P.grp[s]), a);
+                        if (us <= a) {
+                            sp = s;
+                            break;
+                        }
+                    }
+                    int q = P.grp[sp];
+                    float x = fminf(fmaxf(P.xrow[(size_t)c * G + q], 0.f), (float)(P.ntemp - 1));
+                    int lo = min((int)x, P.ntemp - 2);
+                    float w = x - (float)lo;
+                    int row = lo + (uni(r) < w ? 1 : 0);
+                    size_t ro = (size_t)q * P.ntemp + row;
+                    ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, uni(r));
+                    n_mrw++;
+                }
+            }
             tau = -__logf(uni(r));
         } else {
             if (c >= 0) {
-                float wf = P.invdr[p.ir] * tw;
+                float wf = s_invdr[p.ir] * tw;
 #pragma unroll
                 for (int q = 0; q < RT_MAX_GROUP; ++q)
                     if (q < G)
@@ -252,6 +346,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             }
             tau -= aext * tw;
             n_cross++;
+            nev_cell = 0;
             if (++nstep > P.max_steps) {  // pathological zero-length wall ping-pong
                 n_kill++;
                 alive = false;
@@ -389,6 +484,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.spos = m->d_spos;
     P.xrow = m->d_xrow;
     P.kross = m->d_kross;
+    P.mrwy = m->d_mrwy;
     P.kplanck = m->d_kplanck;
     P.tally = m->d_tally;
     P.visits = opt.count_visits ? m->d_visits : nullptr;
@@ -401,7 +497,10 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.mrw_count = opt.mrw_count_trigger > 0 ? opt.mrw_count_trigger : 100;
     P.mrw_tau = (float)(opt.mrw_tauthres > 0 ? opt.mrw_tauthres : 10.0);
     P.mrw_gamma = (float)(opt.mrw_gamma > 0 ? opt.mrw_gamma : 2.0);
-    size_t lds = sizeof(float) * ((size_t)m->nspec * m->nlam * (P.hg ? 3 : 2) + (size_t)G * m->nlam);
+    size_t lds = sizeof(float) * ((size_t)m->nspec * m->nlam * (P.hg ? 3 : 2) + (size_t)G * m->nlam +
+                                  2 * (size_t)(m->nr + 1) + (size_t)m->nr + 3 * (size_t)(m->nt + 1) +
+                                  3 * (size_t)(m->np + 1));
+    if (lds > 64 * 1024) return set_err(ctx, RT_E_LIMIT, "thermal kernel LDS %zu B > 64 KiB", lds);
     unsigned grid = (unsigned)(ctx->n_cu * bpc);
 
     std::vector<hipEvent_t> ev;

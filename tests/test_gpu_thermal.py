@@ -59,6 +59,20 @@ def test_thermal_hg_3d_matches_oracle():
     assert med <= 0.01 and p99 <= 0.05
 
 
+def test_thermal_mrw_matches_oracle():
+    print("case: disk+envelope 64x32 MRW, 4e4 packets, seed 4", flush=True)
+    arr = oracle_arrays(configs.disk_envelope(nr=64, nt=32))
+    n = 40000
+    Tg, vg, sg = _gpu_T(arr, n, 4, max_events=1000000, mrw=1)
+    To, vo, so = oracle.thermal(arr, n, seed=4, max_events=1000000, lagged=True, mrw=True)
+    med, p99, ncell = dT_stats(To, Tg, vo)
+    print("median %.2e p99 %.2e cells %d  mrw steps gpu %d oracle %d"
+          % (med, p99, ncell, sg["n_mrw"], so["n_mrw"]))
+    assert med <= 0.01 and p99 <= 0.05
+    assert sg["n_mrw"] == pytest.approx(so["n_mrw"], rel=0.05)
+    assert sg["n_killed"] == 0
+
+
 def test_thermal_bitwise_reproducible_across_launch_shapes():
     print("case: determinism disk 24x24, 3e4 packets", flush=True)
     arr = oracle_arrays(configs.disk_c1(nr=24, nt=24))

@@ -339,3 +339,37 @@ def test_chisq_closed_form():
     w = rng.uniform(0.5, 2, 100)
     want = np.sum(((a[0] - a[2]) ** 2 + (a[1] - a[3]) ** 2) * w)
     assert oracle.chisq(a[0], a[1], w, a[2], a[3]) == pytest.approx(want, rel=1e-12)
+
+
+# ------------------------------------------------------------------ modified random walk
+def test_mrw_escape_law_mean_first_passage():
+    """Diffusion from the centre of a sphere: mean first-passage time R0^2/(6D),
+    i.e. int_0^inf P(y) dy = 1/6; the inverse samples the same law."""
+    from scipy.integrate import quad
+    assert quad(oracle.mrw_survival, 0, 60, limit=200)[0] == pytest.approx(1 / 6, rel=1e-10)
+    assert oracle.mrw_survival(0.0) == 1.0
+    # the two series representations join continuously at y = 0.1
+    assert oracle.mrw_survival(0.1 - 1e-12) == pytest.approx(oracle.mrw_survival(0.1), abs=1e-10)
+    xi = (np.arange(4000) + 0.5) / 4000
+    ys = np.array([oracle.mrw_invert(x) for x in xi])
+    assert ys.mean() == pytest.approx(1 / 6, rel=2e-3)
+    for x, y in zip(xi[::500], ys[::500]):
+        assert oracle.mrw_survival(y) == pytest.approx(x, abs=1e-9)
+
+
+def test_mrw_temperatures_agree_with_plain_walk():
+    """The MRW approximation changes the disk temperatures by less than the
+    seed-to-seed Monte Carlo noise of the plain random walk, with far fewer
+    interactions per packet."""
+    m = oracle_arrays(configs.disk_c1(nr=32, nt=32))
+    n = 40000
+    T0, v0, s0 = oracle.thermal(m, n, seed=1, max_events=1000000, lagged=True, nthreads=8)
+    T1, v1, _ = oracle.thermal(m, n, seed=2, max_events=1000000, lagged=True, nthreads=8)
+    Tm, vm, sm = oracle.thermal(m, n, seed=3, max_events=1000000, lagged=True, nthreads=8,
+                                mrw=True)
+    noise = dT_stats(T0, T1, np.minimum(v0, v1))
+    mrw = dT_stats(T0, Tm, np.minimum(v0, vm))
+    assert sm["n_mrw"] > 0 and sm["n_killed"] == 0
+    assert sm["n_events"] < 0.5 * s0["n_events"]
+    assert mrw[0] <= 1.5 * noise[0] + 1e-3, (mrw, noise)
+    assert mrw[1] <= 1.5 * noise[1] + 5e-3, (mrw, noise)
