@@ -91,6 +91,8 @@ typedef struct {
     int watchdog_fired;
     double kernel_ms; /* device time of the MC launches (HIP events) */
     double total_ms;  /* device time of the whole run incl. temperature updates */
+    double max_epoch_ms;         /* longest single MC launch */
+    uint64_t max_packet_steps;   /* most steps (crossings + interactions) of one packet */
 } rt_thermal_stats;
 
 /* Camera (RADMC-3D `image` options pdspy passes, Model.py:515-534):

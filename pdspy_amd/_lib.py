@@ -50,7 +50,8 @@ class ThermalStats(C.Structure):
                 ("n_escape", C.c_uint64), ("n_killed", C.c_uint64), ("n_events", C.c_uint64),
                 ("n_mrw", C.c_uint64), ("eps", C.c_double), ("lum", C.c_double),
                 ("n_epochs", C.c_int), ("watchdog_fired", C.c_int), ("kernel_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("max_epoch_ms", C.c_double),
+                ("max_packet_steps", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -448,12 +448,12 @@ __device__ __forceinline__ float trow(const float *tab, int ntemp, float x) {
     return fmaf(w, tab[lo + 1] - tab[lo], tab[lo]);
 }
 
-// diffusion escape law: y(xi) with P(y) = xi; table on xi_k = k/(NY-1), and
-// P ~ 2 exp(-pi^2 y) below the first node
-__device__ __forceinline__ float mrw_escape_y(const float *Y, int ny, float xi) {
-    const float inv = 1.f / (float)(ny - 1);
-    if (xi < inv) return __logf(2.f / xi) * 0.10132118364f;  // 1/pi^2
-    float f = xi * (float)(ny - 1);
+// diffusion escape law: y(xi) with P(y) = xi, tabulated on a uniform grid of
+// logit(xi) over [-lmax, lmax] (24-bit uniforms stay inside it)
+__device__ __forceinline__ float mrw_escape_y(const float *Y, int ny, float lmax, float xi) {
+    float L = __logf(xi) - __logf(1.f - xi);
+    float f = (L + lmax) * ((float)(ny - 1) / (2.f * lmax));
+    f = fminf(fmaxf(f, 0.f), (float)(ny - 1));
     int k = min((int)f, ny - 2);
     float w = f - (float)k;
     return fmaf(w, Y[k + 1] - Y[k], Y[k]);

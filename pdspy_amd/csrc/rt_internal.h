@@ -14,7 +14,8 @@
 #define RT_NGUIDE 128    // guide-table entries per CDF row
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
-#define RT_MRW_NY 1024   // entries of the diffusion escape-law inverse table
+#define RT_MRW_NY 2048   // entries of the diffusion escape-law inverse table
+#define RT_MRW_LMAX 18.0f  // its logit range: ln(xi/(1-xi)) in [-LMAX, LMAX]
 
 struct rt_ctx {
     int device = 0;
@@ -36,6 +37,7 @@ enum {
     CNT_MRW,
     CNT_ABORT,
     CNT_OVERFLOW,
+    CNT_MAXPKT,
     CNT_N = 16
 };
 

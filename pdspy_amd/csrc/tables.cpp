@@ -171,9 +171,13 @@ int build_tables(rt_model *m, std::string &err) {
         for (int n = 1; n <= 60; ++n) s += ((n & 1) ? 2.0 : -2.0) * std::exp(-(double)n * n * kPi * kPi * y);
         return s;
     };
+    // tabulated on a uniform grid of logit(xi) = ln(xi / (1 - xi)), in which y is
+    // close to linear at both ends (y ~ (ln 2 - ln xi)/pi^2 as xi -> 0 and
+    // y ~ 1 / (4 ln(1/(1-xi))) as xi -> 1)
     t.mrw_y.assign(RT_MRW_NY, 0.f);
-    for (int k = 1; k < RT_MRW_NY; ++k) {
-        double xi = (double)k / (RT_MRW_NY - 1);
+    for (int k = 0; k < RT_MRW_NY; ++k) {
+        double L = -RT_MRW_LMAX + 2.0 * RT_MRW_LMAX * k / (RT_MRW_NY - 1);
+        double xi = 1.0 / (1.0 + std::exp(-L));
         double lo = std::log(1e-9), hi = std::log(60.0);
         for (int it = 0; it < 90; ++it) {
             double mid = 0.5 * (lo + hi);
@@ -181,8 +185,6 @@ int build_tables(rt_model *m, std::string &err) {
         }
         t.mrw_y[k] = (float)std::exp(0.5 * (lo + hi));
     }
-    t.mrw_y[RT_MRW_NY - 1] = 0.f;  // xi = 1: no time has passed
-    t.mrw_y[0] = (float)(std::log(2.0 * (RT_MRW_NY - 1)) / (kPi * kPi));
     // stars
     t.scdf.assign((size_t)std::max(1, m->nstar) * nl, 0.f);
     t.sguide.assign((size_t)std::max(1, m->nstar) * RT_NGUIDE, 0);

@@ -144,6 +144,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 #pragma unroll
     for (int q = 0; q < RT_MAX_GROUP; ++q) acc[q] = 0;
     uint32_t n_cross = 0, n_abs = 0, n_scat = 0, n_esc = 0, n_kill = 0, n_ev = 0, n_mrw = 0;
+    uint32_t max_pkt = 0;  // most steps of one packet on this lane
 
     for (uint32_t iter = 0;; ++iter) {
         // ---- refill dead lanes with new packets (one atomic per wave)
@@ -161,6 +162,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     uint64_t idx = base + rank;
                     if (idx < range) {
                         uint64_t pid = P.pbegin + idx;
+                        max_pkt = max(max_pkt, nev + nstep);
                         rng_init(r, P.seed, pid, 0u);
                         float u = uni(r);
                         int ks = 0;
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
                 if (aR * dmin >= P.mrw_tau && R0 > 0.f) {
-                    float y = mrw_escape_y(P.mrwy, RT_MRW_NY, uni(r));
+                    float y = mrw_escape_y(P.mrwy, RT_MRW_NY, RT_MRW_LMAX, uni(r));
                     float ell = 3.f * aR * R0 * R0 * y;
                     float wl = s_invdr[p.ir] * ell;
                     float kP[4] = {0.f, 0.f, 0.f, 0.f};
@@ -353,6 +355,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 continue;
             }
             if (cross_wall(g, p, tw, kind)) {
+                max_pkt = max(max_pkt, nev + nstep);
                 n_esc++;
                 alive = false;
                 continue;
@@ -368,6 +371,9 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
         if (lane == 0 && x) atomicAdd(&P.cnt[CNT_CROSS + k], x);
     }
+    unsigned long long mp = max_pkt;
+    for (int off = 32; off > 0; off >>= 1) mp = max(mp, (unsigned long long)__shfl_down(mp, off));
+    if (lane == 0 && mp) atomicMax(&P.cnt[CNT_MAXPKT], mp);
 }
 
 // lagged temperatures from the tallies: e = tally * econv * eps, log-log
@@ -543,10 +549,12 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     HIPCHK(ctx, hipMemcpy(cnt, m->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost));
     float tot = 0, kms = 0;
     (void)hipEventElapsedTime(&tot, e_begin, e_end);
+    float kmax = 0;
     for (auto &pr : kev) {
         float x = 0;
         (void)hipEventElapsedTime(&x, pr.first, pr.second);
         kms += x;
+        kmax = std::max(kmax, x);
     }
     for (auto e : ev) (void)hipEventDestroy(e);
     m->eps_last = eps;
@@ -565,6 +573,8 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->watchdog_fired = cnt[CNT_ABORT] ? 1 : 0;
         st->kernel_ms = kms;
         st->total_ms = tot;
+        st->max_epoch_ms = kmax;
+        st->max_packet_steps = cnt[CNT_MAXPKT];
     }
     if (cnt[CNT_ABORT])
         return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded %.1f s; packets were killed",

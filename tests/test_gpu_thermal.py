@@ -60,9 +60,9 @@ def test_thermal_hg_3d_matches_oracle():
 
 
 def test_thermal_mrw_matches_oracle():
-    print("case: disk+envelope 64x32 MRW, 4e4 packets, seed 4", flush=True)
+    print("case: disk+envelope 64x32 MRW, 1.5e5 packets, seed 4", flush=True)
     arr = oracle_arrays(configs.disk_envelope(nr=64, nt=32))
-    n = 40000
+    n = 150000
     Tg, vg, sg = _gpu_T(arr, n, 4, max_events=1000000, mrw=1)
     To, vo, so = oracle.thermal(arr, n, seed=4, max_events=1000000, lagged=True, mrw=True)
     med, p99, ncell = dT_stats(To, Tg, vo)
