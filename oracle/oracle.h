@@ -57,6 +57,8 @@ typedef struct {
     double mrw_tauthres;  /* Rosseland tau to the nearest wall to trigger   */
     int mrw_count;        /* interactions in the cell before it may trigger */
     double mrw_gamma;     /* sphere radius = d_min - gamma / alpha_R        */
+    int lag;              /* lagged mode: epoch e re-emits with the temperatures
+                             of epochs <= e - lag (default 2)               */
 } or_thermal_opts;
 
 typedef struct {

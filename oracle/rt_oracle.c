@@ -1200,14 +1200,21 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
 #endif
         uint64_t done = 0;
         uint64_t first = o->epoch_first ? o->epoch_first : 1;
+        /* ring of `lag` snapshots: before epoch e, slot e % lag holds the
+         * temperatures of epochs <= e - lag (zeros for e < lag) */
+        int lag = o->lag > 0 ? o->lag : 2;
+        double *ring = calloc((size_t)lag * nc * ng, sizeof(double));
+        double *Tfinal = mc.Tcur;
+        int e = 0;
         while (done < o->nphot) {
+            mc.Tcur = ring + (size_t)(e % lag) * nc * ng;
             uint64_t sz = (uint64_t)floor((double)done * (o->epoch_growth - 1.0));
             if (sz < first) sz = first;
             uint64_t end = done + sz;
             if (end > o->nphot) end = o->nphot;
-            uint64_t a = 0, b = 0, c2 = 0, d = 0, e = 0, f = 0, h = 0;
+            uint64_t a = 0, b = 0, c2 = 0, d = 0, e2 = 0, f = 0, h = 0;
             int64_t lo = (int64_t)done, hi = (int64_t)end;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e, f, h)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e2, f, h)
             for (int64_t p = lo; p < hi; ++p) {
                 cnt_t cc;
                 memset(&cc, 0, sizeof(cc));
@@ -1216,7 +1223,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
                 b += cc.n_abs;
                 c2 += cc.n_scat;
                 d += cc.n_escape;
-                e += cc.n_killed;
+                e2 += cc.n_killed;
                 f += cc.n_events;
                 h += cc.n_mrw;
             }
@@ -1224,13 +1231,17 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             tot.n_abs += b;
             tot.n_scat += c2;
             tot.n_escape += d;
-            tot.n_killed += e;
+            tot.n_killed += e2;
             tot.n_events += f;
             tot.n_mrw += h;
             done = end;
             nep++;
+            /* S_e from the tallies of epochs <= e, into the slot epoch e + lag reads */
             snapshot(&mc);
+            e++;
         }
+        mc.Tcur = Tfinal;
+        free(ring);
     }
     snapshot(&mc);
     /* expected number of absorptions implied by the path-length tallies:

@@ -98,6 +98,9 @@ struct rt_model {
     float *d_Tg = nullptr;        // [cells][G] temperature (K)
     unsigned int *d_visits = nullptr;
     unsigned long long *d_cnt = nullptr;
+    void *d_pipe = nullptr;                 // epoch pipeline counters and flags
+    unsigned long long *d_tally2 = nullptr; // [2][cells][G] working tallies by epoch parity
+    float *d_xring = nullptr;               // [3][cells][G] temperature-row snapshots
     float *d_Tout = nullptr;      // [S][cells]
     // image scratch
     float *d_alpha = nullptr, *d_src = nullptr;

@@ -7,22 +7,30 @@
 //  * flights of optical depth -ln(xi) through the spherical grid; every segment
 //    of length s in cell c adds kabs_g(nu) * s to the cell's absorbed-energy
 //    tally of absorption group g (u64 fixed point, unit 2^-28 kmax_g dr_c), so
-//    the sum is exact and independent of execution order;
+//    the sums are exact and independent of execution order;
 //  * at an interaction the packet scatters with probability alpha_sca/alpha_ext
 //    (isotropic or Henyey-Greenstein) or is absorbed and re-emitted at once,
-//    isotropically, with a frequency drawn from kabs dB/dT at the cell's current
-//    temperature (T interpolated between the rows of the (ntemp, temp0, temp1)
-//    table by a mixture draw);
-//  * "current temperature" is the one implied by the tallies of all packets of
-//    the earlier epochs (epoch k covers packets [N_k, N_k+1), N_k+1 = N_k +
-//    max(first, N_k (growth-1))), which makes the result bit-reproducible for a
-//    seed however the packets are scheduled.
+//    isotropically, with a frequency drawn from kabs dB/dT at the cell's
+//    temperature (interpolated between rows of the (ntemp, temp0, temp1) table
+//    by a mixture draw); optionally the modified random walk replaces long
+//    random walks deep inside optically thick cells;
+//  * the packets are split into epochs [N_k, N_k+1), N_k+1 = N_k +
+//    max(first, N_k (growth-1)); a packet of epoch e re-emits with the
+//    temperatures implied by the tallies of epochs <= e-2.  The result is
+//    bit-reproducible for a seed however packets are scheduled.
 //
-// Execution: a persistent kernel, one packet per lane.  A lane whose packet has
-// left the grid takes the next packet id at once (one atomic per wavefront per
-// refill, ballot-compacted), so the 64 lanes of a wave stay busy until the
-// epoch's packets run out.  Opacities live in LDS; densities ([cell][species],
-// fp32) and tallies stay L2-resident for the grid sizes of interest.
+// Execution: ONE persistent launch per run.  Each lane carries one packet; a
+// lane whose packet has finished takes the next one at once (one atomic per
+// wavefront per refill, from a per-epoch counter).  Epoch e+2 may start as soon
+// as epoch e has finished and its temperature snapshot is published, so the
+// long random walks at the end of one epoch overlap the bulk of the next.
+// Snapshots are computed cooperatively: any wave that finds the next epoch
+// complete takes a 64-entry chunk, folds its working tallies into the
+// cumulative ones and writes the temperature rows; the wave finishing the last
+// chunk publishes it.  Working tallies alternate by epoch parity and snapshots
+// rotate through three slots, which is all that can be live at once.
+// Opacities and grid walls live in LDS; the current cell's densities and
+// temperature rows stay in registers.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -30,6 +38,8 @@
 #include "rt_internal.h"
 
 using namespace rt;
+
+#define RT_MAX_EPOCH 256
 
 struct ThermParams {
     DevGrid g;
@@ -46,19 +56,54 @@ struct ThermParams {
     const unsigned char *sguide;  // [nstar][NG]
     const float *spick;
     const float *spos;
-    const float *xrow;            // [cell][G]
     const float *kross;           // [S][ntemp] Rosseland-mean extinction
     const float *kplanck;         // [G][ntemp] Planck-mean absorption
     const float *mrwy;            // [RT_MRW_NY] escape-law inverse
-    unsigned long long *tally;    // [cell][G]
     unsigned int *visits;         // nullable
     unsigned long long *cnt;
-    uint64_t pbegin, pend, seed, max_events;
+    uint64_t seed, max_events;
     uint32_t max_steps;  // wall crossings per packet
     uint64_t max_ticks;
     int mrw, mrw_count;
     float mrw_tau, mrw_gamma;
+    // epoch pipeline
+    int nepoch, ncg, nchunk;
+    const unsigned long long *ep_start;  // [E+1]
+    unsigned long long *ep_claim;        // [E]
+    unsigned long long *ep_done;         // [E]
+    unsigned int *snap_claim;            // [E]
+    unsigned int *snap_done;             // [E]
+    unsigned int *snap_ready;            // [E]
+    unsigned int *next_snap;             // [1]
+    unsigned long long *tally2;          // [2][cell][G] working tallies by epoch parity
+    unsigned long long *cum;             // [cell][G] tallies of all folded epochs
+    float *xring;                        // [3][cell][G] temperature-row snapshots
+    const double *econv;                 // [cell][G]
+    const double *lnq;                   // [G][ntemp]
+    double eps, lnt0, dlnt;
 };
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ unsigned long long ld_acq64(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_acq32(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_rlx32(const unsigned *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_rlx64(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel32(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // smallest i with u <= cdf[i], starting from the guide entry
 __device__ __forceinline__ int sample_guided(const float *cdf, const unsigned char *guide,
@@ -68,16 +113,82 @@ __device__ __forceinline__ int sample_guided(const float *cdf, const unsigned ch
     return i;
 }
 
-__device__ __forceinline__ void flush_tally(const ThermParams &P, int c,
+__device__ __forceinline__ float pick8(const float (&a)[RT_MAX_SPEC], int i) {
+    float v = a[0];
+#pragma unroll
+    for (int k = 1; k < RT_MAX_SPEC; ++k) v = (i == k) ? a[k] : v;
+    return v;
+}
+
+// fractional temperature row for an absorbed power per gram e (ln-q table)
+__device__ __forceinline__ float xrow_of(const double *lq, int ntemp, double e) {
+    if (!(e > 0.0)) return -1.f;
+    double le = log(e);
+    int m;
+    if (le <= lq[0])
+        m = 0;
+    else if (le >= lq[ntemp - 1])
+        m = ntemp - 2;
+    else {
+        int lo = 0, hi = ntemp - 1;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (lq[mid] <= le)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        m = lo;
+    }
+    return (float)(m + (le - lq[m]) / (lq[m + 1] - lq[m]));
+}
+
+__device__ __forceinline__ void flush_tally(const ThermParams &P, unsigned long long *tb, int c,
                                             unsigned long long (&acc)[RT_MAX_GROUP]) {
 #pragma unroll
     for (int q = 0; q < RT_MAX_GROUP; ++q) {
         if (q < P.ngroup && acc[q]) {
-            unsigned long long old = atomicAdd(&P.tally[(size_t)c * P.ngroup + q], acc[q]);
+            unsigned long long old = atomicAdd(&tb[(size_t)c * P.ngroup + q], acc[q]);
             if (old > (1ull << 62)) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
             acc[q] = 0;
         }
     }
+}
+
+// One chunk of the next temperature snapshot, if its epoch is complete.
+// Called with the whole wave active.  Returns true if the wave did work.
+__device__ bool snapshot_help(const ThermParams &P, int lane) {
+    unsigned k = uniform32(ld_rlx32(P.next_snap));
+    if (k >= (unsigned)P.nepoch) return false;
+    unsigned long long size = P.ep_start[k + 1] - P.ep_start[k];
+    unsigned long long dn = ld_acq64(&P.ep_done[k]);
+    dn = __shfl(dn, 0);
+    if (dn < size) return false;
+    unsigned ch = 0;
+    if (lane == 0) ch = atomicAdd(&P.snap_claim[k], 1u);
+    ch = uniform32(__shfl(ch, 0));
+    if (ch >= (unsigned)P.nchunk) return false;
+    int i = (int)ch * 64 + lane;
+    if (i < P.ncg) {
+        unsigned long long *tp = P.tally2 + (size_t)(k & 1u) * P.ncg + i;
+        unsigned long long cv = ld_rlx64(P.cum + i) + ld_rlx64(tp);
+        st_rlx64(P.cum + i, cv);
+        st_rlx64(tp, 0ull);
+        int q = i % P.ngroup;
+        float x = xrow_of(P.lnq + (size_t)q * P.ntemp, P.ntemp, (double)cv * P.econv[i] * P.eps);
+        __hip_atomic_store(P.xring + (size_t)(k % 3u) * P.ncg + i, x, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+    if (lane == 0) {
+        unsigned old = __hip_atomic_fetch_add(&P.snap_done[k], 1u, __ATOMIC_ACQ_REL,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1u == (unsigned)P.nchunk) {
+            st_rel32(&P.snap_ready[k], 1u);
+            st_rel32(P.next_snap, k + 1u);
+        }
+    }
+    return true;
 }
 
 __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
@@ -130,78 +241,114 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     g.pw = s_pw;
     g.cp = s_cp;
     g.sp = s_sp;
-    const uint64_t range = P.pend - P.pbegin;
+
     const int lane = __lane_id();
     const uint64_t t_start = wall_clock64();
+    const int E = P.nepoch;
 
     Pos p;
     Philox r;
     int ifr = 0;
     float tau = 0.f;
     uint32_t nev = 0, nstep = 0, nev_cell = 0;
-    bool alive = false, exhausted = false;
+    bool alive = false;
+    int pe = 0;                                // epoch of this lane's packet
+    const float *xs = nullptr;                 // its temperature snapshot (null: T = 0)
+    unsigned long long *tb = P.tally2;         // its working tallies
+    int ccell = -1;                            // cell whose data sit in registers
+    float rcache[RT_MAX_SPEC];
+    float xcache[RT_MAX_GROUP];
+#pragma unroll
+    for (int s = 0; s < RT_MAX_SPEC; ++s) rcache[s] = 0.f;
+#pragma unroll
+    for (int q = 0; q < RT_MAX_GROUP; ++q) xcache[q] = -1.f;
     unsigned long long acc[RT_MAX_GROUP];
 #pragma unroll
     for (int q = 0; q < RT_MAX_GROUP; ++q) acc[q] = 0;
     uint32_t n_cross = 0, n_abs = 0, n_scat = 0, n_esc = 0, n_kill = 0, n_ev = 0, n_mrw = 0;
-    uint32_t max_pkt = 0;  // most steps of one packet on this lane
+    uint32_t max_pkt = 0;
+    int cur_e = 0;          // wave-uniform: epoch packets are claimed from
+    bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
 
     for (uint32_t iter = 0;; ++iter) {
-        // ---- refill dead lanes with new packets (one atomic per wave)
-        if (!exhausted) {
-            uint64_t need = __ballot(!alive);
-            if (need) {
-                int cnt = __popcll(need);
-                int leader = __ffsll((unsigned long long)need) - 1;
-                unsigned long long base = 0;
-                if (lane == leader) base = atomicAdd(&P.cnt[CNT_PACKET], (unsigned long long)cnt);
-                base = __shfl(base, leader);
-                if (base + cnt >= range) exhausted = true;
-                if (!alive) {
-                    uint64_t rank = __popcll(need & ((1ull << lane) - 1ull));
-                    uint64_t idx = base + rank;
-                    if (idx < range) {
-                        uint64_t pid = P.pbegin + idx;
-                        max_pkt = max(max_pkt, nev + nstep);
-                        rng_init(r, P.seed, pid, 0u);
-                        float u = uni(r);
-                        int ks = 0;
-                        while (ks < P.nstar - 1 && u > P.spick[ks]) ++ks;
-                        ifr = sample_guided(P.scdf + (size_t)ks * NL, P.sguide + (size_t)ks * RT_NGUIDE,
-                                            uni(r));
-                        iso_dir(r, p.dx, p.dy, p.dz);
-                        p.x = P.spos[3 * ks];
-                        p.y = P.spos[3 * ks + 1];
-                        p.z = P.spos[3 * ks + 2];
-                        locate(g, p);
-                        nev = 0;
-                        nstep = 0;
-                        nev_cell = 0;
-                        if (p.ir >= g.nr) {
-                            n_esc++;
-                        } else {
-                            alive = true;
-                            if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
-                            tau = -__logf(uni(r));
+        const bool any0 = __any(alive);
+        // ---- help with the next temperature snapshot
+        if (!any0 || (iter & 63u) == 0u) snapshot_help(P, lane);
+
+        // ---- refill dead lanes with packets of the current epoch
+        if (cur_e < E) {
+            if (!epoch_ok)
+                epoch_ok = cur_e < 2 || uniform32(ld_acq32(&P.snap_ready[cur_e - 2])) != 0u;
+            if (epoch_ok) {
+                uint64_t need = __ballot(!alive);
+                if (need) {
+                    int cnt = __popcll(need);
+                    int leader = __ffsll((unsigned long long)need) - 1;
+                    unsigned long long base = 0;
+                    if (lane == leader)
+                        base = atomicAdd(&P.ep_claim[cur_e], (unsigned long long)cnt);
+                    base = __shfl(base, leader);
+                    const unsigned long long e0 = P.ep_start[cur_e];
+                    const unsigned long long size = P.ep_start[cur_e + 1] - e0;
+                    if (!alive) {
+                        uint64_t idx = base + __popcll(need & ((1ull << lane) - 1ull));
+                        if (idx < size) {
+                            uint64_t pid = e0 + idx;
+                            pe = cur_e;
+                            xs = pe >= 2 ? P.xring + (size_t)((pe - 2) % 3) * P.ncg : nullptr;
+                            tb = P.tally2 + (size_t)(pe & 1) * P.ncg;
+                            rng_init(r, P.seed, pid, 0u);
+                            float u = uni(r);
+                            int ks = 0;
+                            while (ks < P.nstar - 1 && u > P.spick[ks]) ++ks;
+                            ifr = sample_guided(P.scdf + (size_t)ks * NL,
+                                                P.sguide + (size_t)ks * RT_NGUIDE, uni(r));
+                            iso_dir(r, p.dx, p.dy, p.dz);
+                            p.x = P.spos[3 * ks];
+                            p.y = P.spos[3 * ks + 1];
+                            p.z = P.spos[3 * ks + 2];
+                            locate(g, p);
+                            nev = 0;
+                            nstep = 0;
+                            nev_cell = 0;
+                            ccell = -1;
+                            if (p.ir >= g.nr) {
+                                n_esc++;
+                                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+                            } else {
+                                alive = true;
+                                if (p.ir >= 0 && P.visits)
+                                    atomicAdd(&P.visits[cell_index(g, p)], 1u);
+                                tau = -__logf(uni(r));
+                            }
                         }
+                    }
+                    if (base + cnt >= size) {
+                        cur_e++;
+                        epoch_ok = false;
                     }
                 }
             }
         }
-        if (!__any(alive)) break;
+        const bool any1 = __any(alive);
+        if (!any1) {
+            // nothing to step: all claimed and all snapshots published -> done
+            if (cur_e >= E && uniform32(ld_rlx32(P.next_snap)) >= (unsigned)E) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
 
         // ---- watchdog (wave-uniform check)
         if ((iter & 1023u) == 0u) {
             bool stop = (wall_clock64() - t_start) > P.max_ticks ||
-                        __atomic_load_n(&P.cnt[CNT_ABORT], __ATOMIC_RELAXED) != 0ull;
+                        uniform32((unsigned)ld_rlx64(&P.cnt[CNT_ABORT])) != 0u;
             if (stop) {
                 if (lane == 0) atomicOr(&P.cnt[CNT_ABORT], 1ull);
                 if (alive) {
-                    if (p.ir >= 0) flush_tally(P, cell_index(g, p), acc);
+                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc);
                     n_kill++;
                     alive = false;
                 }
-                exhausted = true;
                 break;
             }
         }
@@ -214,12 +361,21 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         int c = -1;
         if (p.ir >= 0) {
             c = cell_index(g, p);
-            const float *rc = P.rhoL + (size_t)c * S;
-            for (int s = 0; s < S; ++s) {
-                float rho = rc[s];
-                aabs = fmaf(rho, s_ka[s * NL + ifr], aabs);
-                asca = fmaf(rho, s_ks[s * NL + ifr], asca);
+            if (c != ccell) {
+                const float *rc = P.rhoL + (size_t)c * S;
+#pragma unroll
+                for (int s = 0; s < RT_MAX_SPEC; ++s) rcache[s] = s < S ? rc[s] : 0.f;
+#pragma unroll
+                for (int q = 0; q < RT_MAX_GROUP; ++q)
+                    xcache[q] = (q < G && xs) ? xs[(size_t)c * G + q] : -1.f;
+                ccell = c;
             }
+#pragma unroll
+            for (int s = 0; s < RT_MAX_SPEC; ++s)
+                if (s < S) {
+                    aabs = fmaf(rcache[s], s_ka[s * NL + ifr], aabs);
+                    asca = fmaf(rcache[s], s_ks[s * NL + ifr], asca);
+                }
         }
         float aext = aabs + asca;
         if (aext * tw > tau) {
@@ -236,14 +392,13 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             nev++;
             nev_cell++;
             float ue = uni(r);
-            const float *rc = P.rhoL + (size_t)c * S;
             if (ue * aext < asca) {
                 n_scat++;
                 if (P.hg) {
                     float us = uni(r) * asca, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
-                        a = fmaf(rc[s], s_ks[s * NL + ifr], a);
+                        a = fmaf(pick8(rcache, s), s_ks[s * NL + ifr], a);
                         if (us <= a) {
                             sp = s;
                             break;
@@ -258,15 +413,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 float us = uni(r) * aabs, a = 0.f;
                 int sp = S - 1;
                 for (int s = 0; s < S; ++s) {
-                    a = fmaf(rc[s], s_ka[s * NL + ifr], a);
+                    a = fmaf(pick8(rcache, s), s_ka[s * NL + ifr], a);
                     if (us <= a) {
                         sp = s;
                         break;
                     }
                 }
                 int q = P.grp[sp];
-                float x = P.xrow[(size_t)c * G + q];
-                x = fminf(fmaxf(x, 0.f), (float)(P.ntemp - 1));
+                float x = fminf(fmaxf(pick4(xcache, q), 0.f), (float)(P.ntemp - 1));
                 int lo = min((int)x, P.ntemp - 2);
                 float w = x - (float)lo;
                 int row = lo + (uni(r) < w ? 1 : 0);
@@ -275,9 +429,11 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 iso_dir(r, p.dx, p.dy, p.dz);
             }
             if (P.max_events && nev >= P.max_events) {
-                flush_tally(P, c, acc);
+                flush_tally(P, tb, c, acc);
+                max_pkt = max(max_pkt, nev + nstep);
                 n_kill++;
                 alive = false;
+                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 continue;
             }
             if (P.mrw && nev_cell >= (uint32_t)P.mrw_count) {
@@ -285,10 +441,10 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 // Rosseland optical depth to the nearest wall is large
                 float aR = 0.f;
                 for (int s = 0; s < S; ++s) {
-                    float rho = rc[s];
+                    float rho = pick8(rcache, s);
                     if (rho > 0.f)
                         aR = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
-                                            P.xrow[(size_t)c * G + P.grp[s]]), aR);
+                                            pick4(xcache, P.grp[s])), aR);
                 }
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
@@ -300,8 +456,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 #pragma unroll
                     for (int q = 0; q < RT_MAX_GROUP; ++q)
                         if (q < G) {
-                            kP[q] = trow(P.kplanck + (size_t)q * P.ntemp, P.ntemp,
-                                         P.xrow[(size_t)c * G + q]);
+                            kP[q] = trow(P.kplanck + (size_t)q * P.ntemp, P.ntemp, xcache[q]);
                             acc[q] += (unsigned long long)rintf(kP[q] * P.ginvk[q] * wl);
                         }
                     float nx, ny, nz;
@@ -316,18 +471,18 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     p.dy = ny;
                     p.dz = nz;
                     float wsum = 0.f;
-                    for (int s = 0; s < S; ++s) wsum = fmaf(rc[s], pick4(kP, P.grp[s]), wsum);
+                    for (int s = 0; s < S; ++s) wsum = fmaf(pick8(rcache, s), pick4(kP, P.grp[s]), wsum);
                     float us = uni(r) * wsum, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
-                        a = fmaf(rc[s], pick4(kP, P.grp[s]), a);
+                        a = fmaf(pick8(rcache, s), pick4(kP, P.grp[s]), a);
                         if (us <= a) {
                             sp = s;
                             break;
                         }
                     }
                     int q = P.grp[sp];
-                    float x = fminf(fmaxf(P.xrow[(size_t)c * G + q], 0.f), (float)(P.ntemp - 1));
+                    float x = fminf(fmaxf(pick4(xcache, q), 0.f), (float)(P.ntemp - 1));
                     int lo = min((int)x, P.ntemp - 2);
                     float w = x - (float)lo;
                     int row = lo + (uni(r) < w ? 1 : 0);
@@ -344,20 +499,23 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 for (int q = 0; q < RT_MAX_GROUP; ++q)
                     if (q < G)
                         acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
-                flush_tally(P, c, acc);
+                flush_tally(P, tb, c, acc);
             }
             tau -= aext * tw;
             n_cross++;
             nev_cell = 0;
+            bool end = false;
             if (++nstep > P.max_steps) {  // pathological zero-length wall ping-pong
                 n_kill++;
-                alive = false;
-                continue;
-            }
-            if (cross_wall(g, p, tw, kind)) {
-                max_pkt = max(max_pkt, nev + nstep);
+                end = true;
+            } else if (cross_wall(g, p, tw, kind)) {
                 n_esc++;
+                end = true;
+            }
+            if (end) {
+                max_pkt = max(max_pkt, nev + nstep);
                 alive = false;
+                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 continue;
             }
             if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
@@ -376,41 +534,16 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     if (lane == 0 && mp) atomicMax(&P.cnt[CNT_MAXPKT], mp);
 }
 
-// lagged temperatures from the tallies: e = tally * econv * eps, log-log
-// interpolation in ln q(T); xrow = fractional row of the T table (-1 if T=0)
+// final temperatures from the cumulative tallies
 __global__ void k_snapshot(const unsigned long long *tally, const double *econv, double eps,
                            const double *lnq, int ntemp, double lnt0, double dlnt, int ncg,
                            int G, float *xrow, float *Tg) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncg) return;
     int q = i % G;
-    double e = (double)tally[i] * econv[i] * eps;
-    if (!(e > 0.0)) {
-        xrow[i] = -1.f;
-        Tg[i] = 0.f;
-        return;
-    }
-    const double *lq = lnq + (size_t)q * ntemp;
-    double le = log(e);
-    int m;
-    if (le <= lq[0])
-        m = 0;
-    else if (le >= lq[ntemp - 1])
-        m = ntemp - 2;
-    else {
-        int lo = 0, hi = ntemp - 1;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (lq[mid] <= le)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        m = lo;
-    }
-    double x = m + (le - lq[m]) / (lq[m + 1] - lq[m]);
-    xrow[i] = (float)x;
-    Tg[i] = (float)exp(lnt0 + x * dlnt);
+    float x = xrow_of(lnq + (size_t)q * ntemp, ntemp, (double)tally[i] * econv[i] * eps);
+    xrow[i] = x;
+    Tg[i] = x < 0.f ? 0.f : (float)exp(lnt0 + (double)x * dlnt);
 }
 
 __global__ void k_fill_u64(unsigned long long *p, size_t n, unsigned long long v) {
@@ -439,6 +572,30 @@ __global__ void k_species_T(const float *Tg, const unsigned char *rhopos, const 
 
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// epoch boundaries: N_0 = 0, N_k+1 = min(nphot, N_k + max(first, floor(N_k (growth-1))))
+static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t first, double growth) {
+    std::vector<unsigned long long> st{0};
+    uint64_t done = 0;
+    while (done < nphot) {
+        uint64_t sz = (uint64_t)std::floor((double)done * (growth - 1.0));
+        if (sz < first) sz = first;
+        done = std::min(nphot, done + sz);
+        st.push_back(done);
+    }
+    return st;
+}
+
+static int ensure_pipe_buffers(rt_model *m) {
+    rt_ctx *ctx = m->ctx;
+    if (m->d_pipe) return 0;
+    const size_t ncg = (size_t)m->ncell * m->tab.ngroup;
+    HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * (4 * RT_MAX_EPOCH + 2) +
+                                          sizeof(unsigned int) * (4 * RT_MAX_EPOCH + 4)));
+    HIPCHK(ctx, hipMalloc(&m->d_tally2, sizeof(unsigned long long) * 2 * ncg));
+    HIPCHK(ctx, hipMalloc(&m->d_xring, sizeof(float) * 3 * ncg));
+    return 0;
+}
+
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                 rt_thermal_stats *st) {
     rt_ctx *ctx = m->ctx;
@@ -453,15 +610,36 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     double max_s = opt.max_seconds > 0 ? opt.max_seconds : 60.0;
     int bs = opt.block_size > 0 ? opt.block_size : 256;
     if (bs % 64 || bs > 256) return set_err(ctx, RT_E_ARG, "block_size must be 64..256, multiple of 64");
-    int bpc = opt.blocks_per_cu > 0 ? opt.blocks_per_cu : (2048 / bs);
     const int G = m->tab.ngroup;
     const size_t ncg = (size_t)m->ncell * G;
     const double eps = m->tab.lum / (double)nphot;
+    std::vector<unsigned long long> est = epoch_starts(nphot, first, growth);
+    const int E = (int)est.size() - 1;
+    if (E > RT_MAX_EPOCH)
+        return set_err(ctx, RT_E_LIMIT, "%d epochs > %d: raise epoch_first or epoch_growth", E,
+                       RT_MAX_EPOCH);
+    if (ensure_pipe_buffers(m)) return RT_E_HIP;
 
-    // reset tallies, lagged temperatures, counters, visits
+    // pipeline state layout in d_pipe
+    unsigned long long *ep_start = (unsigned long long *)m->d_pipe;
+    unsigned long long *ep_claim = ep_start + RT_MAX_EPOCH + 1;
+    unsigned long long *ep_done = ep_claim + RT_MAX_EPOCH;
+    unsigned int *u32base = (unsigned int *)(ep_done + RT_MAX_EPOCH + 1);
+    unsigned int *snap_claim = u32base;
+    unsigned int *snap_done = snap_claim + RT_MAX_EPOCH;
+    unsigned int *snap_ready = snap_done + RT_MAX_EPOCH;
+    unsigned int *next_snap = snap_ready + RT_MAX_EPOCH;
+
+    // reset tallies, counters, pipeline state (fill kernels: no memset nodes if captured)
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, ncg, 0ull);
-    hipLaunchKernelGGL(k_fill_f32, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_xrow, ncg, -1.f);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(2 * ncg, 256)), dim3(256), 0, sm, m->d_tally2, 2 * ncg, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)CNT_N, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(2 * RT_MAX_EPOCH + 1, 256)), dim3(256), 0, sm, ep_claim,
+                       (size_t)(2 * RT_MAX_EPOCH + 1), 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(3 * RT_MAX_EPOCH + 4, 256)), dim3(256), 0, sm, u32base,
+                       (size_t)(3 * RT_MAX_EPOCH + 4), 0u);
+    HIPCHK(ctx, hipMemcpyAsync(ep_start, est.data(), sizeof(unsigned long long) * est.size(),
+                               hipMemcpyHostToDevice, sm));
     if (opt.count_visits)
         hipLaunchKernelGGL(k_fill_u32, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, m->d_visits,
                            (size_t)m->ncell, 0u);
@@ -488,11 +666,9 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.sguide = m->d_sguide;
     P.spick = m->d_spick;
     P.spos = m->d_spos;
-    P.xrow = m->d_xrow;
     P.kross = m->d_kross;
-    P.mrwy = m->d_mrwy;
     P.kplanck = m->d_kplanck;
-    P.tally = m->d_tally;
+    P.mrwy = m->d_mrwy;
     P.visits = opt.count_visits ? m->d_visits : nullptr;
     P.cnt = m->d_cnt;
     P.seed = seed;
@@ -503,60 +679,64 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.mrw_count = opt.mrw_count_trigger > 0 ? opt.mrw_count_trigger : 100;
     P.mrw_tau = (float)(opt.mrw_tauthres > 0 ? opt.mrw_tauthres : 10.0);
     P.mrw_gamma = (float)(opt.mrw_gamma > 0 ? opt.mrw_gamma : 2.0);
+    P.nepoch = E;
+    P.ncg = (int)ncg;
+    P.nchunk = (int)((ncg + 63) / 64);
+    P.ep_start = ep_start;
+    P.ep_claim = ep_claim;
+    P.ep_done = ep_done;
+    P.snap_claim = snap_claim;
+    P.snap_done = snap_done;
+    P.snap_ready = snap_ready;
+    P.next_snap = next_snap;
+    P.tally2 = m->d_tally2;
+    P.cum = m->d_tally;
+    P.xring = m->d_xring;
+    P.econv = m->d_econv;
+    P.lnq = m->d_lnq;
+    P.eps = eps;
+    P.lnt0 = m->tab.lnt0;
+    P.dlnt = m->tab.dlnt;
     size_t lds = sizeof(float) * ((size_t)m->nspec * m->nlam * (P.hg ? 3 : 2) + (size_t)G * m->nlam +
                                   2 * (size_t)(m->nr + 1) + (size_t)m->nr + 3 * (size_t)(m->nt + 1) +
                                   3 * (size_t)(m->np + 1));
     if (lds > 64 * 1024) return set_err(ctx, RT_E_LIMIT, "thermal kernel LDS %zu B > 64 KiB", lds);
+    int bpc = opt.blocks_per_cu;
+    if (bpc <= 0) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_thermal, bs, lds) != hipSuccess || occ < 1)
+            occ = 1;
+        bpc = occ;
+    }
     unsigned grid = (unsigned)(ctx->n_cu * bpc);
 
-    std::vector<hipEvent_t> ev;
-    auto mk = [&]() {
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        ev.push_back(e);
-        return e;
-    };
-    hipEvent_t e_begin = mk();
+    hipEvent_t e_begin, e_k0, e_k1, e_end;
+    (void)hipEventCreate(&e_begin);
+    (void)hipEventCreate(&e_k0);
+    (void)hipEventCreate(&e_k1);
+    (void)hipEventCreate(&e_end);
     HIPCHK(ctx, hipEventRecord(e_begin, sm));
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> kev;
-    uint64_t done = 0;
-    int nep = 0;
-    while (done < nphot) {
-        uint64_t sz = (uint64_t)std::floor((double)done * (growth - 1.0));
-        if (sz < first) sz = first;
-        uint64_t end = std::min(nphot, done + sz);
-        P.pbegin = done;
-        P.pend = end;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)1, 0ull);
-        hipEvent_t a = mk(), b = mk();
-        (void)hipEventRecord(a, sm);
-        hipLaunchKernelGGL(k_thermal, dim3(grid), dim3(bs), lds, sm, P);
-        (void)hipEventRecord(b, sm);
-        kev.push_back({a, b});
-        hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally,
-                           m->d_econv, eps, m->d_lnq, m->ntemp, m->tab.lnt0, m->tab.dlnt,
-                           (int)ncg, G, m->d_xrow, m->d_Tg);
-        HIPCHK(ctx, hipGetLastError());
-        done = end;
-        nep++;
-    }
+    HIPCHK(ctx, hipEventRecord(e_k0, sm));
+    hipLaunchKernelGGL(k_thermal, dim3(grid), dim3(bs), lds, sm, P);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(e_k1, sm));
+    hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, m->d_econv, eps,
+                       m->d_lnq, m->ntemp, m->tab.lnt0, m->tab.dlnt, (int)ncg, G, m->d_xrow, m->d_Tg);
     hipLaunchKernelGGL(k_species_T, dim3(nblk((size_t)m->nspec * m->ncell, 256)), dim3(256), 0, sm,
                        m->d_Tg, m->d_rhopos, m->d_grp, m->nspec, G, m->ncell, m->d_Tout);
-    hipEvent_t e_end = mk();
     HIPCHK(ctx, hipEventRecord(e_end, sm));
     HIPCHK(ctx, hipEventSynchronize(e_end));
     unsigned long long cnt[CNT_N];
     HIPCHK(ctx, hipMemcpy(cnt, m->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+    unsigned int nsnap = 0;
+    HIPCHK(ctx, hipMemcpy(&nsnap, next_snap, sizeof(nsnap), hipMemcpyDeviceToHost));
     float tot = 0, kms = 0;
     (void)hipEventElapsedTime(&tot, e_begin, e_end);
-    float kmax = 0;
-    for (auto &pr : kev) {
-        float x = 0;
-        (void)hipEventElapsedTime(&x, pr.first, pr.second);
-        kms += x;
-        kmax = std::max(kmax, x);
-    }
-    for (auto e : ev) (void)hipEventDestroy(e);
+    (void)hipEventElapsedTime(&kms, e_k0, e_k1);
+    (void)hipEventDestroy(e_begin);
+    (void)hipEventDestroy(e_k0);
+    (void)hipEventDestroy(e_k1);
+    (void)hipEventDestroy(e_end);
     m->eps_last = eps;
     m->have_T = true;
     if (st) {
@@ -569,16 +749,18 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->n_mrw = cnt[CNT_MRW];
         st->eps = eps;
         st->lum = m->tab.lum;
-        st->n_epochs = nep;
+        st->n_epochs = E;
         st->watchdog_fired = cnt[CNT_ABORT] ? 1 : 0;
         st->kernel_ms = kms;
         st->total_ms = tot;
-        st->max_epoch_ms = kmax;
+        st->max_epoch_ms = kms;
         st->max_packet_steps = cnt[CNT_MAXPKT];
     }
     if (cnt[CNT_ABORT])
         return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded %.1f s; packets were killed",
                        max_s);
     if (cnt[CNT_OVERFLOW]) return set_err(ctx, RT_E_OVERFLOW, "fixed-point tally overflow");
+    if (nsnap != (unsigned)E)
+        return set_err(ctx, RT_E_HIP, "thermal pipeline ended after %u of %d epochs", nsnap, E);
     return RT_OK;
 }
