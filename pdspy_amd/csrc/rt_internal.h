@@ -100,7 +100,8 @@ struct rt_model {
     unsigned long long *d_cnt = nullptr;
     void *d_pipe = nullptr;                 // epoch pipeline counters and flags
     unsigned long long *d_tally2 = nullptr; // [2][cells][G] working tallies by epoch parity
-    float *d_xring = nullptr;               // [3][cells][G] temperature-row snapshots
+    float *d_xring = nullptr;               // [epochs][slot] temperature-row snapshots
+    size_t xring_cap = 0;                   // floats allocated in d_xring
     float *d_Tout = nullptr;      // [S][cells]
     // image scratch
     float *d_alpha = nullptr, *d_src = nullptr;

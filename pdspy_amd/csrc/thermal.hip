@@ -27,8 +27,15 @@
 // Snapshots are computed cooperatively: any wave that finds the next epoch
 // complete takes a 64-entry chunk, folds its working tallies into the
 // cumulative ones and writes the temperature rows; the wave finishing the last
-// chunk publishes it.  Working tallies alternate by epoch parity and snapshots
-// rotate through three slots, which is all that can be live at once.
+// chunk publishes it.  Working tallies alternate by epoch parity.
+//
+// Memory model: the L2 of each XCD is not coherent with the others, and an
+// agent-scope acquire/release would invalidate/write back a whole L2.  So no
+// such fence appears here: counters, flags and tallies are only touched by
+// device-coherent atomics, each epoch's snapshot gets its own slot that is
+// written once before its flag is raised and never rewritten during the run
+// (so no cache can hold a stale copy), and workgroup-scope fences keep the
+// program order of the atomics.
 // Opacities and grid walls live in LDS; the current cell's densities and
 // temperature rows stay in registers.
 #include <algorithm>
@@ -77,19 +84,14 @@ struct ThermParams {
     unsigned int *next_snap;             // [1]
     unsigned long long *tally2;          // [2][cell][G] working tallies by epoch parity
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
-    float *xring;                        // [3][cell][G] temperature-row snapshots
+    float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
+    size_t slot;                         // floats per snapshot slot (>= cells * G, aligned)
     const double *econv;                 // [cell][G]
     const double *lnq;                   // [G][ntemp]
     double eps, lnt0, dlnt;
 };
 
 // ---------------------------------------------------------------- helpers
-__device__ __forceinline__ unsigned long long ld_acq64(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned ld_acq32(const unsigned *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ unsigned ld_rlx32(const unsigned *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -99,8 +101,16 @@ __device__ __forceinline__ unsigned long long ld_rlx64(const unsigned long long 
 __device__ __forceinline__ void st_rlx64(unsigned long long *p, unsigned long long v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_rel32(unsigned *p, unsigned v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_rlx32(unsigned *p, unsigned v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// program-order fences that wait for outstanding memory operations of this
+// wave without touching the L2 (the data they order is device-coherent)
+__device__ __forceinline__ void order_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+__device__ __forceinline__ void order_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+__device__ __forceinline__ void epoch_done(unsigned long long *p) {
+    order_release();
+    __hip_atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -113,10 +123,12 @@ __device__ __forceinline__ int sample_guided(const float *cdf, const unsigned ch
     return i;
 }
 
-__device__ __forceinline__ float pick8(const float (&a)[RT_MAX_SPEC], int i) {
+// a[i] for a register array without dynamic indexing (which would spill)
+template <int N>
+__device__ __forceinline__ float pickn(const float (&a)[N], int i) {
     float v = a[0];
 #pragma unroll
-    for (int k = 1; k < RT_MAX_SPEC; ++k) v = (i == k) ? a[k] : v;
+    for (int k = 1; k < N; ++k) v = (i == k) ? a[k] : v;
     return v;
 }
 
@@ -143,10 +155,11 @@ __device__ __forceinline__ float xrow_of(const double *lq, int ntemp, double e) 
     return (float)(m + (le - lq[m]) / (lq[m + 1] - lq[m]));
 }
 
+template <int MG>
 __device__ __forceinline__ void flush_tally(const ThermParams &P, unsigned long long *tb, int c,
-                                            unsigned long long (&acc)[RT_MAX_GROUP]) {
+                                            unsigned long long (&acc)[MG]) {
 #pragma unroll
-    for (int q = 0; q < RT_MAX_GROUP; ++q) {
+    for (int q = 0; q < MG; ++q) {
         if (q < P.ngroup && acc[q]) {
             unsigned long long old = atomicAdd(&tb[(size_t)c * P.ngroup + q], acc[q]);
             if (old > (1ull << 62)) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
@@ -161,13 +174,14 @@ __device__ bool snapshot_help(const ThermParams &P, int lane) {
     unsigned k = uniform32(ld_rlx32(P.next_snap));
     if (k >= (unsigned)P.nepoch) return false;
     unsigned long long size = P.ep_start[k + 1] - P.ep_start[k];
-    unsigned long long dn = ld_acq64(&P.ep_done[k]);
+    unsigned long long dn = ld_rlx64(&P.ep_done[k]);
     dn = __shfl(dn, 0);
     if (dn < size) return false;
     unsigned ch = 0;
     if (lane == 0) ch = atomicAdd(&P.snap_claim[k], 1u);
     ch = uniform32(__shfl(ch, 0));
     if (ch >= (unsigned)P.nchunk) return false;
+    order_acquire();
     int i = (int)ch * 64 + lane;
     if (i < P.ncg) {
         unsigned long long *tp = P.tally2 + (size_t)(k & 1u) * P.ncg + i;
@@ -176,21 +190,23 @@ __device__ bool snapshot_help(const ThermParams &P, int lane) {
         st_rlx64(tp, 0ull);
         int q = i % P.ngroup;
         float x = xrow_of(P.lnq + (size_t)q * P.ntemp, P.ntemp, (double)cv * P.econv[i] * P.eps);
-        __hip_atomic_store(P.xring + (size_t)(k % 3u) * P.ncg + i, x, __ATOMIC_RELAXED,
+        __hip_atomic_store(P.xsnap + (size_t)k * P.slot + i, x, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    __atomic_thread_fence(__ATOMIC_RELEASE);
+    order_release();
     if (lane == 0) {
-        unsigned old = __hip_atomic_fetch_add(&P.snap_done[k], 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+        unsigned old = atomicAdd(&P.snap_done[k], 1u);
         if (old + 1u == (unsigned)P.nchunk) {
-            st_rel32(&P.snap_ready[k], 1u);
-            st_rel32(P.next_snap, k + 1u);
+            order_release();
+            st_rlx32(&P.snap_ready[k], 1u);
+            st_rlx32(P.next_snap, k + 1u);
         }
     }
     return true;
 }
 
+// MS, MG: compile-time bounds on species and absorption groups (register caches)
+template <int MS, int MG>
 __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     extern __shared__ float lds[];
     const int S = P.nspec, NL = P.nlam, G = P.ngroup;
@@ -256,15 +272,15 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     const float *xs = nullptr;                 // its temperature snapshot (null: T = 0)
     unsigned long long *tb = P.tally2;         // its working tallies
     int ccell = -1;                            // cell whose data sit in registers
-    float rcache[RT_MAX_SPEC];
-    float xcache[RT_MAX_GROUP];
+    float rcache[MS];
+    float xcache[MG];
 #pragma unroll
-    for (int s = 0; s < RT_MAX_SPEC; ++s) rcache[s] = 0.f;
+    for (int s = 0; s < MS; ++s) rcache[s] = 0.f;
 #pragma unroll
-    for (int q = 0; q < RT_MAX_GROUP; ++q) xcache[q] = -1.f;
-    unsigned long long acc[RT_MAX_GROUP];
+    for (int q = 0; q < MG; ++q) xcache[q] = -1.f;
+    unsigned long long acc[MG];
 #pragma unroll
-    for (int q = 0; q < RT_MAX_GROUP; ++q) acc[q] = 0;
+    for (int q = 0; q < MG; ++q) acc[q] = 0;
     uint32_t n_cross = 0, n_abs = 0, n_scat = 0, n_esc = 0, n_kill = 0, n_ev = 0, n_mrw = 0;
     uint32_t max_pkt = 0;
     int cur_e = 0;          // wave-uniform: epoch packets are claimed from
@@ -277,8 +293,10 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 
         // ---- refill dead lanes with packets of the current epoch
         if (cur_e < E) {
-            if (!epoch_ok)
-                epoch_ok = cur_e < 2 || uniform32(ld_acq32(&P.snap_ready[cur_e - 2])) != 0u;
+            if (!epoch_ok) {
+                epoch_ok = cur_e < 2 || uniform32(ld_rlx32(&P.snap_ready[cur_e - 2])) != 0u;
+                if (epoch_ok) order_acquire();
+            }
             if (epoch_ok) {
                 uint64_t need = __ballot(!alive);
                 if (need) {
@@ -295,7 +313,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                         if (idx < size) {
                             uint64_t pid = e0 + idx;
                             pe = cur_e;
-                            xs = pe >= 2 ? P.xring + (size_t)((pe - 2) % 3) * P.ncg : nullptr;
+                            xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
                             tb = P.tally2 + (size_t)(pe & 1) * P.ncg;
                             rng_init(r, P.seed, pid, 0u);
                             float u = uni(r);
@@ -314,8 +332,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             ccell = -1;
                             if (p.ir >= g.nr) {
                                 n_esc++;
-                                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE,
-                                                       __HIP_MEMORY_SCOPE_AGENT);
+                                epoch_done(&P.ep_done[pe]);
                             } else {
                                 alive = true;
                                 if (p.ir >= 0 && P.visits)
@@ -335,7 +352,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         if (!any1) {
             // nothing to step: all claimed and all snapshots published -> done
             if (cur_e >= E && uniform32(ld_rlx32(P.next_snap)) >= (unsigned)E) break;
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(32);
         }
 
         // ---- watchdog (wave-uniform check)
@@ -364,14 +381,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (c != ccell) {
                 const float *rc = P.rhoL + (size_t)c * S;
 #pragma unroll
-                for (int s = 0; s < RT_MAX_SPEC; ++s) rcache[s] = s < S ? rc[s] : 0.f;
+                for (int s = 0; s < MS; ++s) rcache[s] = s < S ? rc[s] : 0.f;
 #pragma unroll
-                for (int q = 0; q < RT_MAX_GROUP; ++q)
+                for (int q = 0; q < MG; ++q)
                     xcache[q] = (q < G && xs) ? xs[(size_t)c * G + q] : -1.f;
                 ccell = c;
             }
 #pragma unroll
-            for (int s = 0; s < RT_MAX_SPEC; ++s)
+            for (int s = 0; s < MS; ++s)
                 if (s < S) {
                     aabs = fmaf(rcache[s], s_ka[s * NL + ifr], aabs);
                     asca = fmaf(rcache[s], s_ks[s * NL + ifr], asca);
@@ -383,7 +400,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             float sseg = tau / aext;
             float wf = s_invdr[p.ir] * sseg;
 #pragma unroll
-            for (int q = 0; q < RT_MAX_GROUP; ++q)
+            for (int q = 0; q < MG; ++q)
                 if (q < G) acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
             p.x = fmaf(sseg, p.dx, p.x);
             p.y = fmaf(sseg, p.dy, p.y);
@@ -398,7 +415,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     float us = uni(r) * asca, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
-                        a = fmaf(pick8(rcache, s), s_ks[s * NL + ifr], a);
+                        a = fmaf(pickn<MS>(rcache, s), s_ks[s * NL + ifr], a);
                         if (us <= a) {
                             sp = s;
                             break;
@@ -413,14 +430,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 float us = uni(r) * aabs, a = 0.f;
                 int sp = S - 1;
                 for (int s = 0; s < S; ++s) {
-                    a = fmaf(pick8(rcache, s), s_ka[s * NL + ifr], a);
+                    a = fmaf(pickn<MS>(rcache, s), s_ka[s * NL + ifr], a);
                     if (us <= a) {
                         sp = s;
                         break;
                     }
                 }
                 int q = P.grp[sp];
-                float x = fminf(fmaxf(pick4(xcache, q), 0.f), (float)(P.ntemp - 1));
+                float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                 int lo = min((int)x, P.ntemp - 2);
                 float w = x - (float)lo;
                 int row = lo + (uni(r) < w ? 1 : 0);
@@ -433,7 +450,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 max_pkt = max(max_pkt, nev + nstep);
                 n_kill++;
                 alive = false;
-                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                epoch_done(&P.ep_done[pe]);
                 continue;
             }
             if (P.mrw && nev_cell >= (uint32_t)P.mrw_count) {
@@ -441,10 +458,10 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 // Rosseland optical depth to the nearest wall is large
                 float aR = 0.f;
                 for (int s = 0; s < S; ++s) {
-                    float rho = pick8(rcache, s);
+                    float rho = pickn<MS>(rcache, s);
                     if (rho > 0.f)
                         aR = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
-                                            pick4(xcache, P.grp[s])), aR);
+                                            pickn<MG>(xcache, P.grp[s])), aR);
                 }
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
@@ -452,9 +469,11 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     float y = mrw_escape_y(P.mrwy, RT_MRW_NY, RT_MRW_LMAX, uni(r));
                     float ell = 3.f * aR * R0 * R0 * y;
                     float wl = s_invdr[p.ir] * ell;
-                    float kP[4] = {0.f, 0.f, 0.f, 0.f};
+                    float kP[MG];
 #pragma unroll
-                    for (int q = 0; q < RT_MAX_GROUP; ++q)
+                    for (int q = 0; q < MG; ++q) kP[q] = 0.f;
+#pragma unroll
+                    for (int q = 0; q < MG; ++q)
                         if (q < G) {
                             kP[q] = trow(P.kplanck + (size_t)q * P.ntemp, P.ntemp, xcache[q]);
                             acc[q] += (unsigned long long)rintf(kP[q] * P.ginvk[q] * wl);
@@ -471,18 +490,18 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     p.dy = ny;
                     p.dz = nz;
                     float wsum = 0.f;
-                    for (int s = 0; s < S; ++s) wsum = fmaf(pick8(rcache, s), pick4(kP, P.grp[s]), wsum);
+                    for (int s = 0; s < S; ++s) wsum = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, P.grp[s]), wsum);
                     float us = uni(r) * wsum, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
-                        a = fmaf(pick8(rcache, s), pick4(kP, P.grp[s]), a);
+                        a = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, P.grp[s]), a);
                         if (us <= a) {
                             sp = s;
                             break;
                         }
                     }
                     int q = P.grp[sp];
-                    float x = fminf(fmaxf(pick4(xcache, q), 0.f), (float)(P.ntemp - 1));
+                    float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                     int lo = min((int)x, P.ntemp - 2);
                     float w = x - (float)lo;
                     int row = lo + (uni(r) < w ? 1 : 0);
@@ -496,7 +515,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (c >= 0) {
                 float wf = s_invdr[p.ir] * tw;
 #pragma unroll
-                for (int q = 0; q < RT_MAX_GROUP; ++q)
+                for (int q = 0; q < MG; ++q)
                     if (q < G)
                         acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
                 flush_tally(P, tb, c, acc);
@@ -515,7 +534,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (end) {
                 max_pkt = max(max_pkt, nev + nstep);
                 alive = false;
-                __hip_atomic_fetch_add(&P.ep_done[pe], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                epoch_done(&P.ep_done[pe]);
                 continue;
             }
             if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
@@ -592,7 +611,20 @@ static int ensure_pipe_buffers(rt_model *m) {
     HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * (4 * RT_MAX_EPOCH + 2) +
                                           sizeof(unsigned int) * (4 * RT_MAX_EPOCH + 4)));
     HIPCHK(ctx, hipMalloc(&m->d_tally2, sizeof(unsigned long long) * 2 * ncg));
-    HIPCHK(ctx, hipMalloc(&m->d_xring, sizeof(float) * 3 * ncg));
+    return 0;
+}
+
+// one snapshot slot per epoch (slots aligned to 4 KiB)
+static size_t snap_slot(size_t ncg) { return (ncg + 1023) / 1024 * 1024; }
+
+static int ensure_snap_buffers(rt_model *m, int E) {
+    rt_ctx *ctx = m->ctx;
+    const size_t need = snap_slot((size_t)m->ncell * m->tab.ngroup) * (size_t)E;
+    if (m->d_xring && m->xring_cap >= need) return 0;
+    if (m->d_xring) HIPCHK(ctx, hipFree(m->d_xring));
+    m->d_xring = nullptr;
+    HIPCHK(ctx, hipMalloc(&m->d_xring, sizeof(float) * need));
+    m->xring_cap = need;
     return 0;
 }
 
@@ -618,7 +650,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     if (E > RT_MAX_EPOCH)
         return set_err(ctx, RT_E_LIMIT, "%d epochs > %d: raise epoch_first or epoch_growth", E,
                        RT_MAX_EPOCH);
-    if (ensure_pipe_buffers(m)) return RT_E_HIP;
+    if (ensure_pipe_buffers(m) || ensure_snap_buffers(m, E)) return RT_E_HIP;
 
     // pipeline state layout in d_pipe
     unsigned long long *ep_start = (unsigned long long *)m->d_pipe;
@@ -691,7 +723,8 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.next_snap = next_snap;
     P.tally2 = m->d_tally2;
     P.cum = m->d_tally;
-    P.xring = m->d_xring;
+    P.xsnap = m->d_xring;
+    P.slot = snap_slot(ncg);
     P.econv = m->d_econv;
     P.lnq = m->d_lnq;
     P.eps = eps;
@@ -701,10 +734,18 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
                                   2 * (size_t)(m->nr + 1) + (size_t)m->nr + 3 * (size_t)(m->nt + 1) +
                                   3 * (size_t)(m->np + 1));
     if (lds > 64 * 1024) return set_err(ctx, RT_E_LIMIT, "thermal kernel LDS %zu B > 64 KiB", lds);
+    // instantiation by species / group counts (register caches sized to fit)
+    void (*kern)(ThermParams);
+    if (m->nspec <= 2 && G <= 1)
+        kern = k_thermal<2, 1>;
+    else if (m->nspec <= 4 && G <= 2)
+        kern = k_thermal<4, 2>;
+    else
+        kern = k_thermal<RT_MAX_SPEC, RT_MAX_GROUP>;
     int bpc = opt.blocks_per_cu;
     if (bpc <= 0) {
         int occ = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_thermal, bs, lds) != hipSuccess || occ < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, bs, lds) != hipSuccess || occ < 1)
             occ = 1;
         bpc = occ;
     }
@@ -717,7 +758,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     (void)hipEventCreate(&e_end);
     HIPCHK(ctx, hipEventRecord(e_begin, sm));
     HIPCHK(ctx, hipEventRecord(e_k0, sm));
-    hipLaunchKernelGGL(k_thermal, dim3(grid), dim3(bs), lds, sm, P);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), lds, sm, P);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(e_k1, sm));
     hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, m->d_econv, eps,
