@@ -72,7 +72,8 @@ typedef struct {
 typedef struct {
     uint64_t max_events;   /* per-packet interaction cap (default 1e6)         */
     uint64_t epoch_first;  /* packets in the first epoch (default max(1024,N/4096)) */
-    double epoch_growth;   /* epoch k+1 adds max(first, N_done*(growth-1)); 1.25 */
+    double epoch_growth;   /* epoch k+1 adds max(first, N_done*(growth-1)); 1.25;
+                              epoch e re-emits with the temperatures of slots <= e-2 */
     int count_visits;      /* 1: count packet entries per cell                  */
     int mrw;               /* 1: modified random walk (RADMC-3D modified_random_walk) */
     double mrw_tauthres;   /* MRW trigger: Rosseland tau to the nearest wall (10) */
@@ -81,6 +82,8 @@ typedef struct {
     double max_seconds;    /* watchdog per launch (default 60 s)                 */
     int block_size;        /* threads per block (0 = 256)                         */
     int blocks_per_cu;     /* persistent blocks per CU (0 = auto)                 */
+    uint64_t slice_events; /* a packet of epoch e deposits into temperature slot
+                              e + floor(interactions / slice_events) (default 1e4) */
 } rt_thermal_opts;
 
 typedef struct {

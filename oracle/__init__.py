@@ -53,7 +53,8 @@ class _ThermOpts(C.Structure):
                 ("max_events", C.c_uint64), ("lagged", C.c_int),
                 ("epoch_first", C.c_uint64), ("epoch_growth", C.c_double),
                 ("nthreads", C.c_int), ("mrw", C.c_int), ("mrw_tauthres", C.c_double),
-                ("mrw_count", C.c_int), ("mrw_gamma", C.c_double), ("lag", C.c_int)]
+                ("mrw_count", C.c_int), ("mrw_gamma", C.c_double), ("lag", C.c_int),
+                ("slice_events", C.c_uint64)]
 
 
 class _ThermStats(C.Structure):
@@ -135,7 +136,7 @@ class _Pinned:
 
 def thermal(m, nphot, seed=0, ntemp=1000, temp0=0.01, temp1=1e5, max_events=0,
             lagged=True, epoch_first=None, epoch_growth=1.25, nthreads=0, mrw=False,
-            mrw_tauthres=10.0, mrw_count=100, mrw_gamma=2.0, lag=2):
+            mrw_tauthres=10.0, mrw_count=100, mrw_gamma=2.0, lag=2, slice_events=10000):
     """Returns (T[nspec][cells] float64, visits[cells] uint64, stats dict)."""
     pm = _Pinned(m)
     if epoch_first is None:
@@ -143,7 +144,7 @@ def thermal(m, nphot, seed=0, ntemp=1000, temp0=0.01, temp1=1e5, max_events=0,
     o = _ThermOpts(int(nphot), int(seed), ntemp, temp0, temp1, int(max_events),
                    1 if lagged else 0, int(epoch_first), float(epoch_growth), int(nthreads),
                    1 if mrw else 0, float(mrw_tauthres), int(mrw_count), float(mrw_gamma),
-                   int(lag))
+                   int(lag), int(slice_events))
     T = np.zeros((pm.nspec, pm.ncell))
     visits = np.zeros(pm.ncell, dtype=np.uint64)
     st = _ThermStats()

@@ -57,8 +57,11 @@ typedef struct {
     double mrw_tauthres;  /* Rosseland tau to the nearest wall to trigger   */
     int mrw_count;        /* interactions in the cell before it may trigger */
     double mrw_gamma;     /* sphere radius = d_min - gamma / alpha_R        */
-    int lag;              /* lagged mode: epoch e re-emits with the temperatures
-                             of epochs <= e - lag (default 2)               */
+    int lag;              /* lagged mode: epoch e re-emits with the snapshot of
+                             slots <= e - lag (default 2)                   */
+    uint64_t slice_events;/* lagged mode: a packet of epoch e deposits into
+                             slot e + floor(events / slice_events) (capped at
+                             the epoch count); 0 = whole packets in slot e  */
 } or_thermal_opts;
 
 typedef struct {

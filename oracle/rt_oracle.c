@@ -856,7 +856,9 @@ typedef struct {
     const geom_t *g;
     const tab_t *t;
     const or_thermal_opts *o;
-    uint64_t *tally;   /* [ncell][ngroup] fixed point */
+    uint64_t *tally;   /* [ncell][ngroup] fixed point: folded (cumulative) tallies */
+    uint64_t *slots;   /* lagged mode: [nslot][ncell][ngroup] tallies by deposit slot */
+    int nslot;         /* number of epochs + 1 (the last slot takes all later deposits) */
     double *W;         /* [ncell][ngroup] units per (kabs*s) */
     double *Tcur;      /* [ncell][ngroup] lagged temperatures */
     double *econv;     /* [ncell][ngroup] erg/s/g per tally unit */
@@ -868,7 +870,7 @@ typedef struct {
     uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw;
 } cnt_t;
 
-static void deposit(mc_t *mc, int c, int ifr, double s) {
+static void deposit(mc_t *mc, uint64_t *tl, int c, int ifr, double s) {
     const tab_t *t = mc->t;
     int ng = t->ngroup;
     for (int q = 0; q < ng; ++q) {
@@ -879,8 +881,8 @@ static void deposit(mc_t *mc, int c, int ifr, double s) {
             uint64_t old;
 #pragma omp atomic capture
             {
-                old = mc->tally[(size_t)c * ng + q];
-                mc->tally[(size_t)c * ng + q] += add;
+                old = tl[(size_t)c * ng + q];
+                tl[(size_t)c * ng + q] += add;
             }
             if (old > ((uint64_t)1 << 62)) mc->overflow = 1;
         }
@@ -888,15 +890,15 @@ static void deposit(mc_t *mc, int c, int ifr, double s) {
 }
 
 /* add an explicit absorbed amount (kappa * path, cm^3/g) to group q of cell c */
-static void deposit_q(mc_t *mc, int c, int q, double kpath) {
+static void deposit_q(mc_t *mc, uint64_t *tl, int c, int q, double kpath) {
     int ng = mc->t->ngroup;
     uint64_t add = (uint64_t)llround(kpath * mc->W[(size_t)c * ng + q]);
     if (add) {
         uint64_t old;
 #pragma omp atomic capture
         {
-            old = mc->tally[(size_t)c * ng + q];
-            mc->tally[(size_t)c * ng + q] += add;
+            old = tl[(size_t)c * ng + q];
+            tl[(size_t)c * ng + q] += add;
         }
         if (old > ((uint64_t)1 << 62)) mc->overflow = 1;
     }
@@ -935,7 +937,7 @@ static void rotate_dir(double *d, double ct, double ps) {
 
 /* one modified-random-walk step (Min et al. 2009; Robitaille 2010) for a
  * packet at an interaction point in cell c; returns 1 if it was taken */
-static int mrw_step(mc_t *mc, pk_t *p, int c, rng_t *r, int *ifr) {
+static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) {
     const or_model *m = mc->m;
     const geom_t *g = mc->g;
     const tab_t *t = mc->t;
@@ -957,7 +959,7 @@ static int mrw_step(mc_t *mc, pk_t *p, int c, rng_t *r, int *ifr) {
     double kP[8];
     for (int q = 0; q < ng; ++q) {
         kP[q] = trow(t->kplanck + (size_t)q * t->ntemp, t->ntemp, cur_T(mc, c, q), t->lnt0, t->dlnt);
-        deposit_q(mc, c, q, kP[q] * ell);
+        deposit_q(mc, tl, c, q, kP[q] * ell);
     }
     /* exit point uniform on the sphere, direction cosine-weighted outwards */
     double nrm[3];
@@ -997,7 +999,7 @@ static int mrw_step(mc_t *mc, pk_t *p, int c, rng_t *r, int *ifr) {
     return 1;
 }
 
-static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
+static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
     const or_model *m = mc->m;
     const geom_t *g = mc->g;
     const tab_t *t = mc->t;
@@ -1029,6 +1031,12 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
     }
     double tau = -log(uni(&r));
     uint64_t nev = 0, nev_cell = 0;
+    /* deposit slot: epoch + floor(events / slice) in lagged mode */
+    const uint64_t J = mc->o->lagged ? mc->o->slice_events : 0;
+    int slot = epoch;
+    const size_t ncg = (size_t)g->ncell * ng;
+    uint64_t *tl = mc->o->lagged ? mc->slots + (size_t)(slot < mc->nslot - 1 ? slot : mc->nslot - 1) * ncg
+                                 : mc->tally;
     for (;;) {
         int kind;
         double tw = next_wall(g, &p, &kind);
@@ -1045,13 +1053,17 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
         double aext = aabs + asca;
         if (aext * tw > tau) {
             double s = tau / aext;
-            deposit(mc, c, ifr, s);
+            deposit(mc, tl, c, ifr, s);
             p.x += s * p.dx;
             p.y += s * p.dy;
             p.z += s * p.dz;
             cnt->n_events++;
             nev++;
             nev_cell++;
+            if (J && nev % J == 0 && slot < mc->nslot - 1) {
+                slot++;
+                tl = mc->slots + (size_t)slot * ncg;
+            }
             double ue = uni(&r);
             if (ue * aext < asca) {
                 cnt->n_scat++;
@@ -1120,11 +1132,11 @@ static void run_packet(mc_t *mc, uint64_t pid, cnt_t *cnt) {
                 cnt->n_killed++;
                 return;
             }
-            if (mc->o->mrw && nev_cell >= (uint64_t)mc->o->mrw_count && mrw_step(mc, &p, c, &r, &ifr))
+            if (mc->o->mrw && nev_cell >= (uint64_t)mc->o->mrw_count && mrw_step(mc, tl, &p, c, &r, &ifr))
                 cnt->n_mrw++;
             tau = -log(uni(&r));
         } else {
-            if (c >= 0) deposit(mc, c, ifr, tw);
+            if (c >= 0) deposit(mc, tl, c, ifr, tw);
             tau -= aext * tw;
             cnt->n_cross++;
             nev_cell = 0;
@@ -1192,7 +1204,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
     memset(&tot, 0, sizeof(tot));
     int nep = 0;
     if (!o->lagged) {
-        for (uint64_t p = 0; p < o->nphot; ++p) run_packet(&mc, p, &tot);
+        for (uint64_t p = 0; p < o->nphot; ++p) run_packet(&mc, p, 0, &tot);
         nep = 1;
     } else {
 #ifdef _OPENMP
@@ -1200,6 +1212,20 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
 #endif
         uint64_t done = 0;
         uint64_t first = o->epoch_first ? o->epoch_first : 1;
+        /* count epochs, allocate one deposit slot per epoch plus a final one */
+        {
+            uint64_t dd = 0;
+            int ne = 0;
+            while (dd < o->nphot) {
+                uint64_t sz = (uint64_t)floor((double)dd * (o->epoch_growth - 1.0));
+                if (sz < first) sz = first;
+                dd += sz;
+                if (dd > o->nphot) dd = o->nphot;
+                ne++;
+            }
+            mc.nslot = ne + 1;
+            mc.slots = calloc((size_t)mc.nslot * nc * ng, sizeof(uint64_t));
+        }
         /* ring of `lag` snapshots: before epoch e, slot e % lag holds the
          * temperatures of epochs <= e - lag (zeros for e < lag) */
         int lag = o->lag > 0 ? o->lag : 2;
@@ -1218,7 +1244,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             for (int64_t p = lo; p < hi; ++p) {
                 cnt_t cc;
                 memset(&cc, 0, sizeof(cc));
-                run_packet(&mc, (uint64_t)p, &cc);
+                run_packet(&mc, (uint64_t)p, e, &cc);
                 a += cc.n_cross;
                 b += cc.n_abs;
                 c2 += cc.n_scat;
@@ -1236,10 +1262,20 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             tot.n_mrw += h;
             done = end;
             nep++;
-            /* S_e from the tallies of epochs <= e, into the slot epoch e + lag reads */
+            /* fold deposit slot e (final now: it only receives deposits of
+             * epochs <= e) and take S_e, into the ring entry epoch e + lag reads */
+            {
+                uint64_t *sl = mc.slots + (size_t)e * nc * ng;
+                for (size_t i = 0; i < (size_t)nc * ng; ++i) mc.tally[i] += sl[i];
+            }
             snapshot(&mc);
             e++;
         }
+        for (int a = e; a < mc.nslot; ++a) {
+            uint64_t *sl = mc.slots + (size_t)a * nc * ng;
+            for (size_t i = 0; i < (size_t)nc * ng; ++i) mc.tally[i] += sl[i];
+        }
+        free(mc.slots);
         mc.Tcur = Tfinal;
         free(ring);
     }

@@ -42,7 +42,8 @@ class ThermalOpts(C.Structure):
                 ("epoch_growth", C.c_double), ("count_visits", C.c_int), ("mrw", C.c_int),
                 ("mrw_tauthres", C.c_double), ("mrw_count_trigger", C.c_int),
                 ("mrw_gamma", C.c_double), ("max_seconds", C.c_double),
-                ("block_size", C.c_int), ("blocks_per_cu", C.c_int)]
+                ("block_size", C.c_int), ("blocks_per_cu", C.c_int),
+                ("slice_events", C.c_uint64)]
 
 
 class ThermalStats(C.Structure):

@@ -99,7 +99,8 @@ struct rt_model {
     unsigned int *d_visits = nullptr;
     unsigned long long *d_cnt = nullptr;
     void *d_pipe = nullptr;                 // epoch pipeline counters and flags
-    unsigned long long *d_tally2 = nullptr; // [2][cells][G] working tallies by epoch parity
+    unsigned long long *d_tally2 = nullptr; // [epochs+1][slot] tallies by deposit slot
+    size_t tslot_cap = 0;                   // entries allocated in d_tally2
     float *d_xring = nullptr;               // [epochs][slot] temperature-row snapshots
     size_t xring_cap = 0;                   // floats allocated in d_xring
     float *d_Tout = nullptr;      // [S][cells]

@@ -15,19 +15,21 @@
 //    by a mixture draw); optionally the modified random walk replaces long
 //    random walks deep inside optically thick cells;
 //  * the packets are split into epochs [N_k, N_k+1), N_k+1 = N_k +
-//    max(first, N_k (growth-1)); a packet of epoch e re-emits with the
-//    temperatures implied by the tallies of epochs <= e-2.  The result is
-//    bit-reproducible for a seed however packets are scheduled.
+//    max(first, N_k (growth-1)); a packet of epoch e deposits into temperature
+//    slot e + floor(interactions / slice) and re-emits with the temperatures
+//    implied by the deposits of slots <= e-2.  The result is bit-reproducible
+//    for a seed however packets are scheduled.
 //
 // Execution: ONE persistent launch per run.  Each lane carries one packet; a
 // lane whose packet has finished takes the next one at once (one atomic per
 // wavefront per refill, from a per-epoch counter).  Epoch e+2 may start as soon
-// as epoch e has finished and its temperature snapshot is published, so the
-// long random walks at the end of one epoch overlap the bulk of the next.
+// as slot e is complete -- every packet of epochs <= e has finished or moved on
+// to a later slot -- and its snapshot is published, so a long random walk
+// holds up the pipeline for at most one slice of interactions.
 // Snapshots are computed cooperatively: any wave that finds the next epoch
 // complete takes a 64-entry chunk, folds its working tallies into the
 // cumulative ones and writes the temperature rows; the wave finishing the last
-// chunk publishes it.  Working tallies alternate by epoch parity.
+// chunk publishes it.
 //
 // Memory model: the L2 of each XCD is not coherent with the others, and an
 // agent-scope acquire/release would invalidate/write back a whole L2.  So no
@@ -77,12 +79,15 @@ struct ThermParams {
     int nepoch, ncg, nchunk;
     const unsigned long long *ep_start;  // [E+1]
     unsigned long long *ep_claim;        // [E]
-    unsigned long long *ep_done;         // [E]
+    unsigned long long *slot_start;      // [E] packets of epoch e that have started
+    unsigned long long *slot_in;         // [E] packets that moved into slot e
+    unsigned long long *slot_out;        // [E] packets that left slot e (moved on or ended)
+    uint64_t slice;                      // interactions per slot (0: whole packets)
     unsigned int *snap_claim;            // [E]
     unsigned int *snap_done;             // [E]
     unsigned int *snap_ready;            // [E]
     unsigned int *next_snap;             // [1]
-    unsigned long long *tally2;          // [2][cell][G] working tallies by epoch parity
+    unsigned long long *tslot;           // [E+1][slot] tallies by deposit slot (last: all later)
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
     float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
     size_t slot;                         // floats per snapshot slot (>= cells * G, aligned)
@@ -108,7 +113,7 @@ __device__ __forceinline__ void st_rlx32(unsigned *p, unsigned v) {
 // wave without touching the L2 (the data they order is device-coherent)
 __device__ __forceinline__ void order_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
 __device__ __forceinline__ void order_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
-__device__ __forceinline__ void epoch_done(unsigned long long *p) {
+__device__ __forceinline__ void count_up(unsigned long long *p) {
     order_release();
     __hip_atomic_fetch_add(p, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -173,10 +178,14 @@ __device__ __forceinline__ void flush_tally(const ThermParams &P, unsigned long 
 __device__ bool snapshot_help(const ThermParams &P, int lane) {
     unsigned k = uniform32(ld_rlx32(P.next_snap));
     if (k >= (unsigned)P.nepoch) return false;
+    // slot k is complete when all of epoch k has started and every packet that
+    // entered it has left (slots < k are complete: their snapshots are out)
     unsigned long long size = P.ep_start[k + 1] - P.ep_start[k];
-    unsigned long long dn = ld_rlx64(&P.ep_done[k]);
-    dn = __shfl(dn, 0);
-    if (dn < size) return false;
+    unsigned long long st = __shfl(ld_rlx64(&P.slot_start[k]), 0);
+    if (st < size) return false;
+    unsigned long long in = __shfl(ld_rlx64(&P.slot_in[k]), 0);
+    unsigned long long out = __shfl(ld_rlx64(&P.slot_out[k]), 0);
+    if (out < st + in) return false;
     unsigned ch = 0;
     if (lane == 0) ch = atomicAdd(&P.snap_claim[k], 1u);
     ch = uniform32(__shfl(ch, 0));
@@ -184,10 +193,8 @@ __device__ bool snapshot_help(const ThermParams &P, int lane) {
     order_acquire();
     int i = (int)ch * 64 + lane;
     if (i < P.ncg) {
-        unsigned long long *tp = P.tally2 + (size_t)(k & 1u) * P.ncg + i;
-        unsigned long long cv = ld_rlx64(P.cum + i) + ld_rlx64(tp);
+        unsigned long long cv = ld_rlx64(P.cum + i) + ld_rlx64(P.tslot + (size_t)k * P.slot + i);
         st_rlx64(P.cum + i, cv);
-        st_rlx64(tp, 0ull);
         int q = i % P.ngroup;
         float x = xrow_of(P.lnq + (size_t)q * P.ntemp, P.ntemp, (double)cv * P.econv[i] * P.eps);
         __hip_atomic_store(P.xsnap + (size_t)k * P.slot + i, x, __ATOMIC_RELAXED,
@@ -269,8 +276,9 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     uint32_t nev = 0, nstep = 0, nev_cell = 0;
     bool alive = false;
     int pe = 0;                                // epoch of this lane's packet
+    int slot = 0;                              // its current deposit slot
     const float *xs = nullptr;                 // its temperature snapshot (null: T = 0)
-    unsigned long long *tb = P.tally2;         // its working tallies
+    unsigned long long *tb = P.tslot;          // its deposit slot's tallies
     int ccell = -1;                            // cell whose data sit in registers
     float rcache[MS];
     float xcache[MG];
@@ -313,8 +321,9 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                         if (idx < size) {
                             uint64_t pid = e0 + idx;
                             pe = cur_e;
+                            slot = pe;
                             xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
-                            tb = P.tally2 + (size_t)(pe & 1) * P.ncg;
+                            tb = P.tslot + (size_t)pe * P.slot;
                             rng_init(r, P.seed, pid, 0u);
                             float u = uni(r);
                             int ks = 0;
@@ -332,13 +341,20 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             ccell = -1;
                             if (p.ir >= g.nr) {
                                 n_esc++;
-                                epoch_done(&P.ep_done[pe]);
+                                count_up(&P.slot_out[pe]);
                             } else {
                                 alive = true;
                                 if (p.ir >= 0 && P.visits)
                                     atomicAdd(&P.visits[cell_index(g, p)], 1u);
                                 tau = -__logf(uni(r));
                             }
+                        }
+                    }
+                    {
+                        unsigned long long nst = base >= size ? 0ull : std::min((unsigned long long)cnt, size - base);
+                        if (lane == leader && nst) {
+                            order_release();
+                            atomicAdd(&P.slot_start[cur_e], nst);
                         }
                     }
                     if (base + cnt >= size) {
@@ -408,6 +424,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             n_ev++;
             nev++;
             nev_cell++;
+            if (P.slice && nev % P.slice == 0u && slot < E) {
+                // next slot: earlier deposits are final for snapshots <= slot
+                flush_tally(P, tb, c, acc);
+                if (slot + 1 < E) count_up(&P.slot_in[slot + 1]);
+                count_up(&P.slot_out[slot]);
+                slot++;
+                tb = P.tslot + (size_t)slot * P.slot;
+            }
             float ue = uni(r);
             if (ue * aext < asca) {
                 n_scat++;
@@ -450,7 +474,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 max_pkt = max(max_pkt, nev + nstep);
                 n_kill++;
                 alive = false;
-                epoch_done(&P.ep_done[pe]);
+                if (slot < E) count_up(&P.slot_out[slot]);
                 continue;
             }
             if (P.mrw && nev_cell >= (uint32_t)P.mrw_count) {
@@ -534,7 +558,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (end) {
                 max_pkt = max(max_pkt, nev + nstep);
                 alive = false;
-                epoch_done(&P.ep_done[pe]);
+                if (slot < E) count_up(&P.slot_out[slot]);
                 continue;
             }
             if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
@@ -604,28 +628,43 @@ static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t fir
     return st;
 }
 
+// pipeline counters and flags: u64 [ep_start E+1 | ep_claim | slot_start |
+// slot_in | slot_out] then u32 [snap_claim | snap_done | snap_ready | next_snap]
+#define PIPE_U64 (5 * RT_MAX_EPOCH + 1)
+#define PIPE_U32 (3 * RT_MAX_EPOCH + 4)
+
 static int ensure_pipe_buffers(rt_model *m) {
     rt_ctx *ctx = m->ctx;
     if (m->d_pipe) return 0;
-    const size_t ncg = (size_t)m->ncell * m->tab.ngroup;
-    HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * (4 * RT_MAX_EPOCH + 2) +
-                                          sizeof(unsigned int) * (4 * RT_MAX_EPOCH + 4)));
-    HIPCHK(ctx, hipMalloc(&m->d_tally2, sizeof(unsigned long long) * 2 * ncg));
+    HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * PIPE_U64 + sizeof(unsigned int) * PIPE_U32));
     return 0;
 }
 
-// one snapshot slot per epoch (slots aligned to 4 KiB)
+// one temperature snapshot and one tally array per slot (aligned to 4 KiB)
 static size_t snap_slot(size_t ncg) { return (ncg + 1023) / 1024 * 1024; }
 
-static int ensure_snap_buffers(rt_model *m, int E) {
+static int ensure_slot_buffers(rt_model *m, int E) {
     rt_ctx *ctx = m->ctx;
-    const size_t need = snap_slot((size_t)m->ncell * m->tab.ngroup) * (size_t)E;
-    if (m->d_xring && m->xring_cap >= need) return 0;
-    if (m->d_xring) HIPCHK(ctx, hipFree(m->d_xring));
-    m->d_xring = nullptr;
-    HIPCHK(ctx, hipMalloc(&m->d_xring, sizeof(float) * need));
-    m->xring_cap = need;
+    const size_t sl = snap_slot((size_t)m->ncell * m->tab.ngroup);
+    if (!m->d_xring || m->xring_cap < sl * (size_t)E) {
+        if (m->d_xring) HIPCHK(ctx, hipFree(m->d_xring));
+        m->d_xring = nullptr;
+        HIPCHK(ctx, hipMalloc(&m->d_xring, sizeof(float) * sl * (size_t)E));
+        m->xring_cap = sl * (size_t)E;
+    }
+    if (!m->d_tally2 || m->tslot_cap < sl * (size_t)(E + 1)) {
+        if (m->d_tally2) HIPCHK(ctx, hipFree(m->d_tally2));
+        m->d_tally2 = nullptr;
+        HIPCHK(ctx, hipMalloc(&m->d_tally2, sizeof(unsigned long long) * sl * (size_t)(E + 1)));
+        m->tslot_cap = sl * (size_t)(E + 1);
+    }
     return 0;
+}
+
+// cumulative tallies plus the last slot (deposits after the last epoch's snapshot)
+__global__ void k_add_u64(unsigned long long *dst, const unsigned long long *src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
 }
 
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
@@ -650,13 +689,17 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     if (E > RT_MAX_EPOCH)
         return set_err(ctx, RT_E_LIMIT, "%d epochs > %d: raise epoch_first or epoch_growth", E,
                        RT_MAX_EPOCH);
-    if (ensure_pipe_buffers(m) || ensure_snap_buffers(m, E)) return RT_E_HIP;
+    if (ensure_pipe_buffers(m) || ensure_slot_buffers(m, E)) return RT_E_HIP;
+    const size_t sl = snap_slot(ncg);
+    const uint64_t slice = opt.slice_events ? opt.slice_events : 10000ull;
 
     // pipeline state layout in d_pipe
     unsigned long long *ep_start = (unsigned long long *)m->d_pipe;
     unsigned long long *ep_claim = ep_start + RT_MAX_EPOCH + 1;
-    unsigned long long *ep_done = ep_claim + RT_MAX_EPOCH;
-    unsigned int *u32base = (unsigned int *)(ep_done + RT_MAX_EPOCH + 1);
+    unsigned long long *slot_start = ep_claim + RT_MAX_EPOCH;
+    unsigned long long *slot_in = slot_start + RT_MAX_EPOCH;
+    unsigned long long *slot_out = slot_in + RT_MAX_EPOCH;
+    unsigned int *u32base = (unsigned int *)(ep_start + PIPE_U64);
     unsigned int *snap_claim = u32base;
     unsigned int *snap_done = snap_claim + RT_MAX_EPOCH;
     unsigned int *snap_ready = snap_done + RT_MAX_EPOCH;
@@ -664,12 +707,13 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
 
     // reset tallies, counters, pipeline state (fill kernels: no memset nodes if captured)
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, ncg, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(2 * ncg, 256)), dim3(256), 0, sm, m->d_tally2, 2 * ncg, 0ull);
+    const size_t nts = sl * (size_t)(E + 1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nts, 256)), dim3(256), 0, sm, m->d_tally2, nts, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)CNT_N, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(2 * RT_MAX_EPOCH + 1, 256)), dim3(256), 0, sm, ep_claim,
-                       (size_t)(2 * RT_MAX_EPOCH + 1), 0ull);
-    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(3 * RT_MAX_EPOCH + 4, 256)), dim3(256), 0, sm, u32base,
-                       (size_t)(3 * RT_MAX_EPOCH + 4), 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(4 * RT_MAX_EPOCH, 256)), dim3(256), 0, sm, ep_claim,
+                       (size_t)(4 * RT_MAX_EPOCH), 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(PIPE_U32, 256)), dim3(256), 0, sm, u32base,
+                       (size_t)PIPE_U32, 0u);
     HIPCHK(ctx, hipMemcpyAsync(ep_start, est.data(), sizeof(unsigned long long) * est.size(),
                                hipMemcpyHostToDevice, sm));
     if (opt.count_visits)
@@ -716,12 +760,15 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.nchunk = (int)((ncg + 63) / 64);
     P.ep_start = ep_start;
     P.ep_claim = ep_claim;
-    P.ep_done = ep_done;
+    P.slot_start = slot_start;
+    P.slot_in = slot_in;
+    P.slot_out = slot_out;
+    P.slice = slice;
     P.snap_claim = snap_claim;
     P.snap_done = snap_done;
     P.snap_ready = snap_ready;
     P.next_snap = next_snap;
-    P.tally2 = m->d_tally2;
+    P.tslot = m->d_tally2;
     P.cum = m->d_tally;
     P.xsnap = m->d_xring;
     P.slot = snap_slot(ncg);
@@ -761,6 +808,8 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), lds, sm, P);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipEventRecord(e_k1, sm));
+    hipLaunchKernelGGL(k_add_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally,
+                       m->d_tally2 + sl * (size_t)E, ncg);
     hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, m->d_econv, eps,
                        m->d_lnq, m->ntemp, m->tab.lnt0, m->tab.dlnt, (int)ncg, G, m->d_xrow, m->d_Tg);
     hipLaunchKernelGGL(k_species_T, dim3(nblk((size_t)m->nspec * m->ncell, 256)), dim3(256), 0, sm,
