@@ -96,6 +96,14 @@ typedef struct {
     double total_ms;  /* device time of the whole run incl. temperature updates */
     double max_epoch_ms;         /* longest single MC launch */
     uint64_t max_packet_steps;   /* most steps (crossings + interactions) of one packet */
+    uint64_t wave_iters;  /* loop iterations summed over wavefronts; lane utilisation is
+                             (n_cross + n_events) / (64 wave_iters)                      */
+    uint64_t idle_iters;  /* of which a wavefront had no packet to step                 */
+    uint64_t n_nudged;    /* packets moved 2^-16 r along their ray after bouncing
+                             between two cells at a wall without moving             */
+    uint64_t phase_clocks[4]; /* shader clocks summed over wavefronts spent in the
+                                 refill, step, snapshot and idle phases; zero unless the
+                                 library is built with RT_THERMAL_PROFILE              */
 } rt_thermal_stats;
 
 /* Camera (RADMC-3D `image` options pdspy passes, Model.py:515-534):

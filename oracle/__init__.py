@@ -62,7 +62,8 @@ class _ThermStats(C.Structure):
                 ("n_escape", C.c_uint64), ("n_killed", C.c_uint64),
                 ("n_events", C.c_uint64), ("n_mrw", C.c_uint64), ("eps", C.c_double),
                 ("lum", C.c_double),
-                ("tally_tau_sum", C.c_double), ("n_epochs", C.c_int)]
+                ("tally_tau_sum", C.c_double), ("n_epochs", C.c_int),
+                ("n_nudged", C.c_uint64)]
 
 
 class _Camera(C.Structure):

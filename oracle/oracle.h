@@ -70,6 +70,7 @@ typedef struct {
     double lum;           /* total stellar luminosity on the grid, erg/s    */
     double tally_tau_sum; /* sum over cells/groups of eps*tally/eps        */
     int n_epochs;
+    uint64_t n_nudged;    /* packets moved on by the ping-pong guard        */
 } or_thermal_stats;
 
 /* Thermal Monte Carlo.  T_out[nspec][cells] K (r fastest), visits[cells]

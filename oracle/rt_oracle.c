@@ -488,6 +488,25 @@ static inline int cell_of(const geom_t *g, const pk_t *p) {
     return p->ir + g->nr * (p->it + g->nt * p->ip);
 }
 
+/* Ping-pong guard (the FP64 counterpart of the device's): after 8 consecutive
+ * crossings shorter than 2^-44 r (r: outer radius of the cell the step began
+ * in) the packet moves 2^-36 r along its ray and is located again.  Returns 1
+ * when it moved the packet. */
+static int unstick(const geom_t *g, pk_t *p, double tw, double rcell, int *nzero) {
+    if (tw > rcell * 0x1p-44) {
+        *nzero = 0;
+        return 0;
+    }
+    if (++*nzero < 8) return 0;
+    *nzero = 0;
+    double d = rcell * 0x1p-36;
+    p->x += d * p->dx;
+    p->y += d * p->dy;
+    p->z += d * p->dz;
+    locate(g, p);
+    return 1;
+}
+
 int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg,
                  int *cell, double *len, int *hemi) {
     geom_t g;
@@ -521,9 +540,11 @@ int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg
     } else {
         locate(&g, &p);
     }
+    int nzero = 0;
     while (p.ir >= -1 && p.ir < g.nr) {
         int kind;
         double t = next_wall(&g, &p, &kind);
+        double rcell = g.rw[p.ir + 1];
         if (n < maxseg) {
             cell[n] = (p.ir < 0) ? -1 : cell_of(&g, &p);
             len[n] = t;
@@ -531,6 +552,7 @@ int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg
         }
         n++;
         if (cross_wall(&g, &p, t, kind)) break;
+        unstick(&g, &p, t, rcell, &nzero);
         if (n > 10000000) break;
     }
     geom_free(&g);
@@ -867,7 +889,7 @@ typedef struct {
 } mc_t;
 
 typedef struct {
-    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw;
+    uint64_t n_cross, n_abs, n_scat, n_escape, n_killed, n_events, n_mrw, n_nudged;
 } cnt_t;
 
 static void deposit(mc_t *mc, uint64_t *tl, int c, int ifr, double s) {
@@ -1031,6 +1053,7 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
     }
     double tau = -log(uni(&r));
     uint64_t nev = 0, nev_cell = 0;
+    int nzero = 0;
     /* deposit slot: epoch + floor(events / slice) in lagged mode */
     const uint64_t J = mc->o->lagged ? mc->o->slice_events : 0;
     int slot = epoch;
@@ -1060,6 +1083,7 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
             cnt->n_events++;
             nev++;
             nev_cell++;
+            nzero = 0;
             if (J && nev % J == 0 && slot < mc->nslot - 1) {
                 slot++;
                 tl = mc->slots + (size_t)slot * ncg;
@@ -1140,9 +1164,17 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
             tau -= aext * tw;
             cnt->n_cross++;
             nev_cell = 0;
+            double rcell = g->rw[p.ir + 1];
             if (cross_wall(g, &p, tw, kind)) {
                 cnt->n_escape++;
                 return;
+            }
+            if (unstick(g, &p, tw, rcell, &nzero)) {
+                cnt->n_nudged++;
+                if (p.ir >= g->nr) {
+                    cnt->n_escape++;
+                    return;
+                }
             }
             if (p.ir >= 0 && mc->visits) {
 #pragma omp atomic
@@ -1238,9 +1270,9 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             if (sz < first) sz = first;
             uint64_t end = done + sz;
             if (end > o->nphot) end = o->nphot;
-            uint64_t a = 0, b = 0, c2 = 0, d = 0, e2 = 0, f = 0, h = 0;
+            uint64_t a = 0, b = 0, c2 = 0, d = 0, e2 = 0, f = 0, h = 0, nu = 0;
             int64_t lo = (int64_t)done, hi = (int64_t)end;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e2, f, h)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : a, b, c2, d, e2, f, h, nu)
             for (int64_t p = lo; p < hi; ++p) {
                 cnt_t cc;
                 memset(&cc, 0, sizeof(cc));
@@ -1252,6 +1284,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
                 e2 += cc.n_killed;
                 f += cc.n_events;
                 h += cc.n_mrw;
+                nu += cc.n_nudged;
             }
             tot.n_cross += a;
             tot.n_abs += b;
@@ -1260,6 +1293,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
             tot.n_killed += e2;
             tot.n_events += f;
             tot.n_mrw += h;
+            tot.n_nudged += nu;
             done = end;
             nep++;
             /* fold deposit slot e (final now: it only receives deposits of
@@ -1308,6 +1342,7 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
         st->lum = t.lum;
         st->tally_tau_sum = tsum;
         st->n_epochs = nep;
+        st->n_nudged = tot.n_nudged;
     }
     int ovf = mc.overflow;
     free(mc.tally);
@@ -1383,9 +1418,11 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
     double I = 0, tau = 0;
     int ns = m->nspec, n = 0;
     double nu = C_C / (cam->lam_um * 1e-4);
+    int nzero = 0;
     for (int guard = 0; guard < 100000000; ++guard) {
         int kind;
         double tw = next_wall(g, &p, &kind);
+        double rcell = g->rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_of(g, &p);
             double a = 0;
@@ -1405,6 +1442,7 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
             if (cam->mode == 0 && tau > cam->tau_stop) break;
         }
         if (cross_wall(g, &p, tw, kind)) break;
+        if (unstick(g, &p, tw, rcell, &nzero) && p.ir >= g->nr) break;
     }
     if (nseg) *nseg = n;
     return cam->mode == 0 ? I : tau;
@@ -1424,10 +1462,11 @@ static double star_tau(const or_model *m, const geom_t *g, const double *kap,
     p.dz = n[2];
     locate(g, &p);
     double tau = 0;
-    int k = 0;
+    int k = 0, nzero = 0;
     while (p.ir >= -1 && p.ir < g->nr) {
         int kind;
         double tw = next_wall(g, &p, &kind);
+        double rcell = g->rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_of(g, &p);
             double a = 0;
@@ -1441,6 +1480,7 @@ static double star_tau(const or_model *m, const geom_t *g, const double *kap,
             tau += a * tw;
         }
         if (cross_wall(g, &p, tw, kind)) break;
+        unstick(g, &p, tw, rcell, &nzero);
     }
     if (nseg) *nseg = k;
     return tau;

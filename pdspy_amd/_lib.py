@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libpdspy_amd.so")
+# PDSPY_AMD_LIB selects another build of the same ABI (e.g. a profiling build)
+LIB_PATH = os.environ.get("PDSPY_AMD_LIB") or os.path.join(HERE, "lib", "libpdspy_amd.so")
 
 # symbol list (must match include/pdspy_amd.h)
 EXPORTS = [
@@ -52,10 +53,14 @@ class ThermalStats(C.Structure):
                 ("n_mrw", C.c_uint64), ("eps", C.c_double), ("lum", C.c_double),
                 ("n_epochs", C.c_int), ("watchdog_fired", C.c_int), ("kernel_ms", C.c_double),
                 ("total_ms", C.c_double), ("max_epoch_ms", C.c_double),
-                ("max_packet_steps", C.c_uint64)]
+                ("max_packet_steps", C.c_uint64), ("wave_iters", C.c_uint64),
+                ("idle_iters", C.c_uint64), ("n_nudged", C.c_uint64),
+                ("phase_clocks", C.c_uint64 * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["phase_clocks"] = list(self.phase_clocks)
+        return d
 
 
 class Camera(C.Structure):

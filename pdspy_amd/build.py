@@ -27,28 +27,38 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
-def build(verbose=False, extra=()):
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(verbose=False, extra=(), out=None):
+    """Compile every source and link the library at ``out`` (default LIB).
+
+    ``extra`` adds hipcc flags, e.g. ("-DRT_THERMAL_PROFILE=1",) for the
+    phase-clock build."""
+    lib = out or LIB
+    libdir = os.path.dirname(os.path.abspath(lib))
+    os.makedirs(libdir, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
     for src in sources():
-        obj = os.path.join(LIBDIR, os.path.basename(src) + ".o")
+        obj = os.path.join(libdir, os.path.basename(src) + ".o")
         cmd = [hipcc, *HIPCC_FLAGS, *extra, "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     for o in objs:
         os.remove(o)
-    return LIB
+    return lib
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
-    print(LIB)
+    args = sys.argv[1:]
+    out = None
+    if "-o" in args:
+        out = args[args.index("-o") + 1]
+    extra = [a for a in args if a.startswith("-D")]
+    print(build(verbose="-v" in args, extra=extra, out=out))

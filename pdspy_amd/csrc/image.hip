@@ -51,9 +51,11 @@ __device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, 
 
 __device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
     float I = 0.f, tau = 0.f;
+    uint32_t nzero = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {
         int kind;
         float tw = next_wall(C.g, p, kind);
+        const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_index(C.g, p);
             float dtau = C.alpha[c] * tw;
@@ -66,6 +68,7 @@ __device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
             }
         }
         if (cross_wall(C.g, p, tw, kind)) break;
+        if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
     }
     return C.mode == 0 ? I : tau;
 }
@@ -102,9 +105,11 @@ __global__ void __launch_bounds__(64) k_image_adj(CamParams C, const float *dimg
             // reverse sweep: recompute tau front to back, R_k = I - I_<=k
             Pos p = p0;
             float tau = 0.f, cum = 0.f;
+            uint32_t nzero = 0;
             for (int guard = 0; guard < (1 << 24); ++guard) {
                 int kind;
                 float tw = next_wall(C.g, p, kind);
+                const float rcell = C.g.rw[p.ir + 1];
                 if (p.ir >= 0) {
                     int c = cell_index(C.g, p);
                     float dtau = C.alpha[c] * tw;
@@ -120,6 +125,7 @@ __global__ void __launch_bounds__(64) k_image_adj(CamParams C, const float *dimg
                     if (tau > C.tau_stop) break;
                 }
                 if (cross_wall(C.g, p, tw, kind)) break;
+                if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
             }
         }
 }
@@ -148,11 +154,14 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, const float *
     locate(C.g, p);
     float tau = 0.f;
     Pos p0 = p;
+    uint32_t nzero = 0;
     for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
         int kind;
         float tw = next_wall(C.g, p, kind);
+        const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) tau += C.alpha[cell_index(C.g, p)] * tw;
         if (cross_wall(C.g, p, tw, kind)) break;
+        unstick(C.g, p, tw, rcell, nzero);
     }
     float Is = sflux[s] * __expf(-tau);
     if (img) {
@@ -162,11 +171,14 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, const float *
     float gI = dimg[j * C.nx + k];
     if (gI == 0.f) return;
     p = p0;
+    nzero = 0;
     for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
         int kind;
         float tw = next_wall(C.g, p, kind);
+        const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) unsafeAtomicAdd(&g_alpha[cell_index(C.g, p)], (double)(-gI * Is * tw));
         if (cross_wall(C.g, p, tw, kind)) break;
+        unstick(C.g, p, tw, rcell, nzero);
     }
 }
 

@@ -365,6 +365,28 @@ __device__ __forceinline__ bool cross_wall(const DevGrid &g, Pos &p, float t, in
     return false;
 }
 
+// Ping-pong guard.  A ray almost tangent to a sphere or cone can find the wall
+// it has just crossed again at a distance below FP32 resolution and bounce
+// between two cells without moving.  After 8 consecutive crossings shorter
+// than 2^-20 r (r: outer radius of the cell the step started in) the position
+// is advanced by 2^-16 r along the ray and its cell is found again from the
+// position.  Returns true when it moved the packet.
+__device__ __forceinline__ bool unstick(const DevGrid &g, Pos &p, float tw, float rcell,
+                                        uint32_t &nzero) {
+    if (tw > rcell * 9.5367431640625e-07f) {
+        nzero = 0;
+        return false;
+    }
+    if (++nzero < 8u) return false;
+    nzero = 0;
+    float d = rcell * 1.52587890625e-05f;
+    p.x = fmaf(d, p.dx, p.x);
+    p.y = fmaf(d, p.dy, p.y);
+    p.z = fmaf(d, p.dz, p.z);
+    locate(g, p);
+    return true;
+}
+
 // ---------------------------------------------------------------- physics
 // pdspy CGS constants (pdspy/constants/physics.py)
 #define RT_H 6.6260755e-27

@@ -35,10 +35,14 @@ enum {
     CNT_KILLED,
     CNT_EVENTS,
     CNT_MRW,
+    CNT_NUDGE,
     CNT_ABORT,
     CNT_OVERFLOW,
     CNT_MAXPKT,
-    CNT_N = 16
+    CNT_WITER,     // loop iterations summed over wavefronts
+    CNT_IDLE,      // of which with no packet to step
+    CNT_TK0,       // 4 phase clocks (RT_THERMAL_PROFILE builds only)
+    CNT_N = CNT_TK0 + 4
 };
 
 struct HostTables {

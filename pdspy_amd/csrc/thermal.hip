@@ -50,6 +50,11 @@ using namespace rt;
 
 #define RT_MAX_EPOCH 256
 
+// RT_THERMAL_PROFILE=1 adds shader-clock accounting of the loop phases
+#ifndef RT_THERMAL_PROFILE
+#define RT_THERMAL_PROFILE 0
+#endif
+
 struct ThermParams {
     DevGrid g;
     int nspec, ngroup, nlam, ntemp, nstar, hg;
@@ -129,9 +134,9 @@ __device__ __forceinline__ int sample_guided(const float *cdf, const unsigned ch
 }
 
 // a[i] for a register array without dynamic indexing (which would spill)
-template <int N>
-__device__ __forceinline__ float pickn(const float (&a)[N], int i) {
-    float v = a[0];
+template <int N, typename T>
+__device__ __forceinline__ T pickn(const T (&a)[N], int i) {
+    T v = a[0];
 #pragma unroll
     for (int k = 1; k < N; ++k) v = (i == k) ? a[k] : v;
     return v;
@@ -160,14 +165,20 @@ __device__ __forceinline__ float xrow_of(const double *lq, int ntemp, double e) 
     return (float)(m + (le - lq[m]) / (lq[m + 1] - lq[m]));
 }
 
+// Adds the packet's pending deposits to the tallies of cell c.  Overflow is
+// checked without waiting for the atomics: every add and the value each add
+// found (kept in old[], OR-ed into ovf at the next flush or at the end) must
+// stay below 2^62, so no tally can wrap unnoticed.
 template <int MG>
 __device__ __forceinline__ void flush_tally(const ThermParams &P, unsigned long long *tb, int c,
-                                            unsigned long long (&acc)[MG]) {
+                                            unsigned long long (&acc)[MG],
+                                            unsigned long long (&old)[MG],
+                                            unsigned long long &ovf) {
 #pragma unroll
     for (int q = 0; q < MG; ++q) {
         if (q < P.ngroup && acc[q]) {
-            unsigned long long old = atomicAdd(&tb[(size_t)c * P.ngroup + q], acc[q]);
-            if (old > (1ull << 62)) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
+            ovf |= old[q] | acc[q];
+            old[q] = atomicAdd(&tb[(size_t)c * P.ngroup + q], acc[q]);
             acc[q] = 0;
         }
     }
@@ -237,7 +248,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         s_ks[i] = P.ks[i];
         if (P.hg) s_g[i] = P.gg[i];
     }
-    for (int i = threadIdx.x; i < G * NL; i += blockDim.x) s_gka[i] = P.gka[i];
+    for (int i = threadIdx.x; i < G * NL; i += blockDim.x) s_gka[i] = P.gka[i] * P.ginvk[i / NL];
     for (int i = threadIdx.x; i <= NR; i += blockDim.x) {
         s_rw[i] = P.g.rw[i];
         s_rw2[i] = P.g.rw2[i];
@@ -273,7 +284,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     Philox r;
     int ifr = 0;
     float tau = 0.f;
-    uint32_t nev = 0, nstep = 0, nev_cell = 0;
+    uint32_t nev = 0, nstep = 0, nev_cell = 0, nzero = 0;
     bool alive = false;
     int pe = 0;                                // epoch of this lane's packet
     int slot = 0;                              // its current deposit slot
@@ -286,18 +297,46 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     for (int s = 0; s < MS; ++s) rcache[s] = 0.f;
 #pragma unroll
     for (int q = 0; q < MG; ++q) xcache[q] = -1.f;
-    unsigned long long acc[MG];
+    unsigned long long acc[MG], aold[MG];
+    unsigned long long ovf = 0;  // OR of tally values seen and of scaled adds (overflow check)
+    float ginv[MG];              // 2^shift / kmax per group
 #pragma unroll
-    for (int q = 0; q < MG; ++q) acc[q] = 0;
+    for (int q = 0; q < MG; ++q) {
+        acc[q] = 0;
+        aold[q] = 0;
+        ginv[q] = q < G ? P.ginvk[q] : 0.f;
+    }
+    int grpr[MS];                // absorption group of each species
+#pragma unroll
+    for (int s = 0; s < MS; ++s) grpr[s] = s < S ? P.grp[s] : 0;
     uint32_t n_cross = 0, n_abs = 0, n_scat = 0, n_esc = 0, n_kill = 0, n_ev = 0, n_mrw = 0;
+    uint32_t n_nudge = 0;
     uint32_t max_pkt = 0;
     int cur_e = 0;          // wave-uniform: epoch packets are claimed from
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
+    uint64_t n_idle = 0;
+    uint32_t iter = 0;
+#if RT_THERMAL_PROFILE
+    uint64_t tk[4] = {0, 0, 0, 0};  // refill, step, snapshot, idle
+    uint64_t tc = clock64();
+#define RT_PHASE(k)                  \
+    do {                             \
+        uint64_t tn_ = clock64();    \
+        tk[k] += tn_ - tc;           \
+        tc = tn_;                    \
+    } while (0)
+#else
+#define RT_PHASE(k) \
+    do {            \
+    } while (0)
+#endif
 
-    for (uint32_t iter = 0;; ++iter) {
+    for (;; ++iter) {
+        RT_PHASE(1);
         const bool any0 = __any(alive);
         // ---- help with the next temperature snapshot
         if (!any0 || (iter & 63u) == 0u) snapshot_help(P, lane);
+        RT_PHASE(2);
 
         // ---- refill dead lanes with packets of the current epoch
         if (cur_e < E) {
@@ -338,6 +377,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             nev = 0;
                             nstep = 0;
                             nev_cell = 0;
+                            nzero = 0;
                             ccell = -1;
                             if (p.ir >= g.nr) {
                                 n_esc++;
@@ -364,11 +404,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 }
             }
         }
+        RT_PHASE(0);
         const bool any1 = __any(alive);
         if (!any1) {
             // nothing to step: all claimed and all snapshots published -> done
             if (cur_e >= E && uniform32(ld_rlx32(P.next_snap)) >= (unsigned)E) break;
+            n_idle++;
             __builtin_amdgcn_s_sleep(32);
+            RT_PHASE(3);
         }
 
         // ---- watchdog (wave-uniform check)
@@ -378,7 +421,8 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (stop) {
                 if (lane == 0) atomicOr(&P.cnt[CNT_ABORT], 1ull);
                 if (alive) {
-                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc);
+                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, aold, ovf);
+                    max_pkt = max(max_pkt, nev + nstep);
                     n_kill++;
                     alive = false;
                 }
@@ -417,16 +461,17 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             float wf = s_invdr[p.ir] * sseg;
 #pragma unroll
             for (int q = 0; q < MG; ++q)
-                if (q < G) acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
+                if (q < G) acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * wf);
             p.x = fmaf(sseg, p.dx, p.x);
             p.y = fmaf(sseg, p.dy, p.y);
             p.z = fmaf(sseg, p.dz, p.z);
             n_ev++;
             nev++;
             nev_cell++;
+            nzero = 0;
             if (P.slice && nev % P.slice == 0u && slot < E) {
                 // next slot: earlier deposits are final for snapshots <= slot
-                flush_tally(P, tb, c, acc);
+                flush_tally(P, tb, c, acc, aold, ovf);
                 if (slot + 1 < E) count_up(&P.slot_in[slot + 1]);
                 count_up(&P.slot_out[slot]);
                 slot++;
@@ -460,7 +505,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                         break;
                     }
                 }
-                int q = P.grp[sp];
+                int q = pickn<MS>(grpr, sp);
                 float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                 int lo = min((int)x, P.ntemp - 2);
                 float w = x - (float)lo;
@@ -470,7 +515,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 iso_dir(r, p.dx, p.dy, p.dz);
             }
             if (P.max_events && nev >= P.max_events) {
-                flush_tally(P, tb, c, acc);
+                flush_tally(P, tb, c, acc, aold, ovf);
                 max_pkt = max(max_pkt, nev + nstep);
                 n_kill++;
                 alive = false;
@@ -485,7 +530,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     float rho = pickn<MS>(rcache, s);
                     if (rho > 0.f)
                         aR = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
-                                            pickn<MG>(xcache, P.grp[s])), aR);
+                                            pickn<MG>(xcache, pickn<MS>(grpr, s))), aR);
                 }
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
@@ -500,7 +545,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     for (int q = 0; q < MG; ++q)
                         if (q < G) {
                             kP[q] = trow(P.kplanck + (size_t)q * P.ntemp, P.ntemp, xcache[q]);
-                            acc[q] += (unsigned long long)rintf(kP[q] * P.ginvk[q] * wl);
+                            acc[q] += (unsigned long long)rintf(kP[q] * ginv[q] * wl);
                         }
                     float nx, ny, nz;
                     iso_dir(r, nx, ny, nz);
@@ -514,17 +559,17 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     p.dy = ny;
                     p.dz = nz;
                     float wsum = 0.f;
-                    for (int s = 0; s < S; ++s) wsum = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, P.grp[s]), wsum);
+                    for (int s = 0; s < S; ++s) wsum = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, pickn<MS>(grpr, s)), wsum);
                     float us = uni(r) * wsum, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
-                        a = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, P.grp[s]), a);
+                        a = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, pickn<MS>(grpr, s)), a);
                         if (us <= a) {
                             sp = s;
                             break;
                         }
                     }
-                    int q = P.grp[sp];
+                    int q = pickn<MS>(grpr, sp);
                     float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                     int lo = min((int)x, P.ntemp - 2);
                     float w = x - (float)lo;
@@ -541,19 +586,26 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 #pragma unroll
                 for (int q = 0; q < MG; ++q)
                     if (q < G)
-                        acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * P.ginvk[q] * wf);
-                flush_tally(P, tb, c, acc);
+                        acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * wf);
+                flush_tally(P, tb, c, acc, aold, ovf);
             }
             tau -= aext * tw;
             n_cross++;
             nev_cell = 0;
             bool end = false;
-            if (++nstep > P.max_steps) {  // pathological zero-length wall ping-pong
+            const float rcell = g.rw[p.ir + 1];
+            if (++nstep > P.max_steps) {  // last-resort cap on crossings
                 n_kill++;
                 end = true;
             } else if (cross_wall(g, p, tw, kind)) {
                 n_esc++;
                 end = true;
+            } else if (unstick(g, p, tw, rcell, nzero)) {
+                n_nudge++;
+                if (p.ir >= g.nr) {
+                    n_esc++;
+                    end = true;
+                }
             }
             if (end) {
                 max_pkt = max(max_pkt, nev + nstep);
@@ -565,9 +617,9 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         }
     }
     // ---- statistics: wave-reduce and one atomic per counter per wave
-    unsigned long long v[7] = {n_cross, n_abs, n_scat, n_esc, n_kill, n_ev, n_mrw};
+    unsigned long long v[8] = {n_cross, n_abs, n_scat, n_esc, n_kill, n_ev, n_mrw, n_nudge};
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 8; ++k) {
         unsigned long long x = v[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
         if (lane == 0 && x) atomicAdd(&P.cnt[CNT_CROSS + k], x);
@@ -575,6 +627,17 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     unsigned long long mp = max_pkt;
     for (int off = 32; off > 0; off >>= 1) mp = max(mp, (unsigned long long)__shfl_down(mp, off));
     if (lane == 0 && mp) atomicMax(&P.cnt[CNT_MAXPKT], mp);
+#pragma unroll
+    for (int q = 0; q < MG; ++q) ovf |= aold[q];
+    if (__any((ovf >> 62) != 0ull) && lane == 0) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
+    if (lane == 0) {
+        atomicAdd(&P.cnt[CNT_WITER], (unsigned long long)iter + 1ull);
+        if (n_idle) atomicAdd(&P.cnt[CNT_IDLE], (unsigned long long)n_idle);
+#if RT_THERMAL_PROFILE
+        for (int k = 0; k < 4; ++k) atomicAdd(&P.cnt[CNT_TK0 + k], (unsigned long long)tk[k]);
+#endif
+    }
+#undef RT_PHASE
 }
 
 // final temperatures from the cumulative tallies
@@ -845,6 +908,10 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->total_ms = tot;
         st->max_epoch_ms = kms;
         st->max_packet_steps = cnt[CNT_MAXPKT];
+        st->wave_iters = cnt[CNT_WITER];
+        st->n_nudged = cnt[CNT_NUDGE];
+        st->idle_iters = cnt[CNT_IDLE];
+        for (int k = 0; k < 4; ++k) st->phase_clocks[k] = cnt[CNT_TK0 + k];
     }
     if (cnt[CNT_ABORT])
         return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded %.1f s; packets were killed",

@@ -18,9 +18,11 @@ def _gpu_T(arr, nphot, seed, **opts):
 
 
 def test_thermal_matches_oracle_disk():
-    print("case: disk 32x32, 1e5 packets, seed 7", flush=True)
+    # 1e6 packets: independent seeds of the oracle already differ by median
+    # 0.14 %, p99 5.2 % here; the same seed shares most packet histories
+    print("case: disk 32x32, 1e6 packets, seed 7", flush=True)
     arr = oracle_arrays(configs.disk_c1(nr=32, nt=32))
-    n = 100000
+    n = 1000000
     Tg, vg, sg = _gpu_T(arr, n, 7, max_events=20000)
     To, vo, so = oracle.thermal(arr, n, seed=7, max_events=20000, lagged=True)
     med, p99, ncell = dT_stats(To, Tg, vo)
