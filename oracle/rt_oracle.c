@@ -73,32 +73,17 @@ void or_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
     out[3] = c3;
 }
 
-typedef struct {
-    uint32_t key[2];
-    uint32_t ctr[4];
-    uint32_t buf[4];
-    int idx;
-} rng_t;
-
-static void rng_init(rng_t *r, uint64_t seed, uint64_t pid, uint32_t stream) {
-    r->key[0] = (uint32_t)seed;
-    r->key[1] = (uint32_t)(seed >> 32);
-    r->ctr[0] = 0;
-    r->ctr[1] = (uint32_t)pid;
-    r->ctr[2] = (uint32_t)(pid >> 32);
-    r->ctr[3] = stream;
-    r->idx = 4;
-}
-
-/* uniform in (0,1) from the top 24 bits: ((x>>8)+0.5)*2^-24, exact in FP32 */
-static double uni(rng_t *r) {
-    if (r->idx == 4) {
-        or_philox4x32_10(r->ctr, r->key, r->buf);
-        r->ctr[0]++;
-        r->idx = 0;
+/* Random event k of packet pid (k = 0 at emission, then one per interaction
+ * and one per modified-random-walk step) takes the 8 uniforms of the Philox
+ * blocks with counter (k, pid_lo, pid_hi, 0) and (k, pid_lo, pid_hi, 1) under
+ * key = seed; each uniform is ((x>>8)+0.5)*2^-24, exact in FP32. */
+static void rng8(uint64_t seed, uint64_t pid, uint32_t k, double u[8]) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint32_t b = 0; b < 2; ++b) {
+        uint32_t ctr[4] = {k, (uint32_t)pid, (uint32_t)(pid >> 32), b}, out[4];
+        or_philox4x32_10(ctr, key, out);
+        for (int i = 0; i < 4; ++i) u[4 * b + i] = ((double)(out[i] >> 8) + 0.5) * (1.0 / 16777216.0);
     }
-    uint32_t x = r->buf[r->idx++];
-    return ((double)(x >> 8) + 0.5) * (1.0 / 16777216.0);
 }
 
 /* --------------------------------------------------------------- Planck */
@@ -770,17 +755,19 @@ static int sample_cdf(const double *cdf, int n, double u) {
     return hi;
 }
 
-static void iso_dir(rng_t *r, double *d) {
-    double mu = 2.0 * uni(r) - 1.0;
-    double ph = TWO_PI_D * uni(r);
+/* isotropic direction: cos(theta) = 2 u1 - 1, azimuth 2 pi u2 */
+static void iso_dir(double u1, double u2, double *d) {
+    double mu = 2.0 * u1 - 1.0;
+    double ph = TWO_PI_D * u2;
     double s = sqrt(fmax(0.0, 1.0 - mu * mu));
     d[0] = s * cos(ph);
     d[1] = s * sin(ph);
     d[2] = mu;
 }
 
-static void hg_dir(rng_t *r, double g, double *d) {
-    double u = uni(r);
+/* Henyey-Greenstein scattering of d: cosine from u1, azimuth 2 pi u2 */
+static void hg_dir(double u1, double u2, double g, double *d) {
+    double u = u1;
     double ct;
     if (fabs(g) < 1e-6)
         ct = 2.0 * u - 1.0;
@@ -790,7 +777,7 @@ static void hg_dir(rng_t *r, double g, double *d) {
     }
     if (ct > 1) ct = 1;
     if (ct < -1) ct = -1;
-    double ps = TWO_PI_D * uni(r);
+    double ps = TWO_PI_D * u2;
     double st = sqrt(fmax(0.0, 1.0 - ct * ct));
     double cps = cos(ps), sps = sin(ps);
     double ux = d[0], uy = d[1], uz = d[2];
@@ -959,7 +946,8 @@ static void rotate_dir(double *d, double ct, double ps) {
 
 /* one modified-random-walk step (Min et al. 2009; Robitaille 2010) for a
  * packet at an interaction point in cell c; returns 1 if it was taken */
-static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) {
+static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, uint64_t pid, uint32_t *krng,
+                    int *ifr) {
     const or_model *m = mc->m;
     const geom_t *g = mc->g;
     const tab_t *t = mc->t;
@@ -975,8 +963,12 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) 
     if (!(aR * dmin >= mc->o->mrw_tauthres)) return 0;
     double R0 = dmin - mc->o->mrw_gamma / aR;
     if (!(R0 > 0)) return 0;
+    /* one random event: u0 path length, u1 u2 exit point, u3 u4 exit direction,
+     * u5 species, u6 temperature row, u7 frequency */
+    double v[8];
+    rng8(mc->o->seed, pid, (*krng)++, v);
     /* diffusion path length to the sphere: y = D t / R0^2, D = 1/(3 alpha_R) */
-    double y = or_mrw_invert(uni(r));
+    double y = or_mrw_invert(v[0]);
     double ell = 3.0 * aR * R0 * R0 * y;
     double kP[8];
     for (int q = 0; q < ng; ++q) {
@@ -985,13 +977,13 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) 
     }
     /* exit point uniform on the sphere, direction cosine-weighted outwards */
     double nrm[3];
-    iso_dir(r, nrm);
+    iso_dir(v[1], v[2], nrm);
     p->x += R0 * nrm[0];
     p->y += R0 * nrm[1];
     p->z += R0 * nrm[2];
     if (g->mirror) p->hemi = (p->z >= 0) ? 1 : -1;
-    double ct = sqrt(uni(r));
-    double ps = TWO_PI_D * uni(r);
+    double ct = sqrt(v[3]);
+    double ps = TWO_PI_D * v[4];
     rotate_dir(nrm, ct, ps);
     p->dx = nrm[0];
     p->dy = nrm[1];
@@ -999,7 +991,7 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) 
     /* new frequency: re-emission spectrum of a species picked by rho * kappa_P */
     double wsum = 0;
     for (int s = 0; s < ns; ++s) wsum += m->rho[(size_t)s * g->ncell + c] * kP[t->grp[s]];
-    double us = uni(r) * wsum, acc = 0;
+    double us = v[5] * wsum, acc = 0;
     int sp = ns - 1;
     for (int s = 0; s < ns; ++s) {
         acc += m->rho[(size_t)s * g->ncell + c] * kP[t->grp[s]];
@@ -1016,8 +1008,8 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, rng_t *r, int *ifr) 
     int lo = (int)x;
     if (lo > t->ntemp - 2) lo = t->ntemp - 2;
     double w = x - lo;
-    int row = lo + (uni(r) < w ? 1 : 0);
-    *ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, uni(r));
+    int row = lo + (v[6] < w ? 1 : 0);
+    *ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, v[7]);
     return 1;
 }
 
@@ -1026,14 +1018,15 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
     const geom_t *g = mc->g;
     const tab_t *t = mc->t;
     const int nl = t->nlam, ns = m->nspec, ng = t->ngroup;
-    rng_t r;
-    rng_init(&r, mc->o->seed, pid, 0);
-    double u = uni(&r);
+    /* emission (random event 0): u0 star, u1 frequency, u2 u3 direction, u4 tau */
+    double u[8];
+    rng8(mc->o->seed, pid, 0, u);
+    uint32_t krng = 1;
     int ks = 0;
-    while (ks < m->nstar - 1 && u > t->spick[ks]) ks++;
-    int ifr = sample_cdf(t->scdf + (size_t)ks * nl, nl, uni(&r));
+    while (ks < m->nstar - 1 && u[0] > t->spick[ks]) ks++;
+    int ifr = sample_cdf(t->scdf + (size_t)ks * nl, nl, u[1]);
     double d[3];
-    iso_dir(&r, d);
+    iso_dir(u[2], u[3], d);
     pk_t p;
     memset(&p, 0, sizeof(p));
     p.x = m->star[5 * ks + 2];
@@ -1051,7 +1044,7 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
 #pragma omp atomic
         mc->visits[cell_of(g, &p)]++;
     }
-    double tau = -log(uni(&r));
+    double tau = -log(u[4]);
     uint64_t nev = 0, nev_cell = 0;
     int nzero = 0;
     /* deposit slot: epoch + floor(events / slice) in lagged mode */
@@ -1088,13 +1081,15 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
                 slot++;
                 tl = mc->slots + (size_t)slot * ncg;
             }
-            double ue = uni(&r);
-            if (ue * aext < asca) {
+            /* interaction (one random event): u0 scatter or absorb, u1 species,
+             * u2 temperature row, u3 frequency, u4 u5 direction, u6 next tau */
+            rng8(mc->o->seed, pid, krng++, u);
+            if (u[0] * aext < asca) {
                 cnt->n_scat++;
                 int hg = 0;
                 double gg = 0;
                 if (m->gsca) {
-                    double us = uni(&r) * asca, acc = 0;
+                    double us = u[1] * asca, acc = 0;
                     int sp = ns - 1;
                     for (int s = 0; s < ns; ++s) {
                         acc += m->rho[(size_t)s * g->ncell + c] * m->ksca[(size_t)s * nl + ifr];
@@ -1108,19 +1103,19 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
                 }
                 if (hg) {
                     double dd[3] = {p.dx, p.dy, p.dz};
-                    hg_dir(&r, gg, dd);
+                    hg_dir(u[4], u[5], gg, dd);
                     p.dx = dd[0];
                     p.dy = dd[1];
                     p.dz = dd[2];
                 } else {
-                    iso_dir(&r, d);
+                    iso_dir(u[4], u[5], d);
                     p.dx = d[0];
                     p.dy = d[1];
                     p.dz = d[2];
                 }
             } else {
                 cnt->n_abs++;
-                double us = uni(&r) * aabs, acc = 0;
+                double us = u[1] * aabs, acc = 0;
                 int sp = ns - 1;
                 for (int s = 0; s < ns; ++s) {
                     acc += m->rho[(size_t)s * g->ncell + c] * m->kabs[(size_t)s * nl + ifr];
@@ -1145,9 +1140,9 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
                 int lo = (int)x;
                 if (lo > t->ntemp - 2) lo = t->ntemp - 2;
                 double w = x - lo;
-                int row = lo + (uni(&r) < w ? 1 : 0);
-                ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, uni(&r));
-                iso_dir(&r, d);
+                int row = lo + (u[2] < w ? 1 : 0);
+                ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, u[3]);
+                iso_dir(u[4], u[5], d);
                 p.dx = d[0];
                 p.dy = d[1];
                 p.dz = d[2];
@@ -1156,9 +1151,10 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
                 cnt->n_killed++;
                 return;
             }
-            if (mc->o->mrw && nev_cell >= (uint64_t)mc->o->mrw_count && mrw_step(mc, tl, &p, c, &r, &ifr))
+            tau = -log(u[6]);
+            if (mc->o->mrw && nev_cell >= (uint64_t)mc->o->mrw_count &&
+                mrw_step(mc, tl, &p, c, pid, &krng, &ifr))
                 cnt->n_mrw++;
-            tau = -log(uni(&r));
         } else {
             if (c >= 0) deposit(mc, tl, c, ifr, tw);
             tau -= aext * tw;

@@ -18,13 +18,6 @@ namespace rt {
 // ---------------------------------------------------------------- Philox
 // Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as easy as 1, 2, 3"
 // (SC'11): 10 rounds of the 4x32 bijection, Weyl key schedule.
-struct Philox {
-    uint32_t k0, k1;
-    uint32_t c0, c1, c2;  // block counter, packet id lo / hi
-    uint32_t stream;
-    uint32_t buf[4];
-    int idx;
-};
 
 __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t c2,
                                              uint32_t c3, uint32_t k0, uint32_t k1,
@@ -49,29 +42,27 @@ __device__ __forceinline__ void philox_block(uint32_t c0, uint32_t c1, uint32_t 
     out[3] = c3;
 }
 
-__device__ __forceinline__ void rng_init(Philox &r, uint64_t seed, uint64_t pid,
-                                         uint32_t stream) {
-    r.k0 = (uint32_t)seed;
-    r.k1 = (uint32_t)(seed >> 32);
-    r.c0 = 0;
-    r.c1 = (uint32_t)pid;
-    r.c2 = (uint32_t)(pid >> 32);
-    r.stream = stream;
-    r.idx = 4;
+// uniform in (0,1) from the top 24 bits of a word, exact in FP32
+__device__ __forceinline__ float u24(uint32_t x) {
+    return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
-// uniform in (0,1) from the top 24 bits, exact in FP32
-__device__ __forceinline__ float uni(Philox &r) {
-    if (r.idx == 4) {
-        philox_block(r.c0, r.c1, r.c2, r.stream, r.k0, r.k1, r.buf);
-        r.c0++;
-        r.idx = 0;
+// The random numbers of a packet come in events: event k (k = 0 at emission,
+// then one per interaction and one per modified-random-walk step) uses the 8
+// uniforms of the blocks with counter (k, pid_lo, pid_hi, 0) and
+// (k, pid_lo, pid_hi, 1) under key = seed.  A fixed number of Philox blocks per
+// event keeps the wavefront from re-running the generator at every draw.
+__device__ __forceinline__ void rng8(uint64_t seed, uint64_t pid, uint32_t k, float u[8]) {
+    uint32_t a[4], b[4];
+    philox_block(k, (uint32_t)pid, (uint32_t)(pid >> 32), 0u, (uint32_t)seed,
+                 (uint32_t)(seed >> 32), a);
+    philox_block(k, (uint32_t)pid, (uint32_t)(pid >> 32), 1u, (uint32_t)seed,
+                 (uint32_t)(seed >> 32), b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        u[i] = u24(a[i]);
+        u[4 + i] = u24(b[i]);
     }
-    uint32_t x;
-    // register-indexed select (avoids scratch for a dynamic array index)
-    x = (r.idx == 0) ? r.buf[0] : (r.idx == 1) ? r.buf[1] : (r.idx == 2) ? r.buf[2] : r.buf[3];
-    r.idx++;
-    return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
 }
 
 // ---------------------------------------------------------------- grid
@@ -393,9 +384,9 @@ __device__ __forceinline__ bool unstick(const DevGrid &g, Pos &p, float tw, floa
 #define RT_C 2.99792458e10
 #define RT_K 1.380658e-16
 
-__device__ __forceinline__ void iso_dir(Philox &r, float &dx, float &dy, float &dz) {
-    float mu = 2.f * uni(r) - 1.f;
-    float u2 = uni(r);
+// isotropic direction: cos(theta) = 2 u1 - 1, azimuth 2 pi u2
+__device__ __forceinline__ void iso_dir(float u1, float u2, float &dx, float &dy, float &dz) {
+    float mu = 2.f * u1 - 1.f;
     float s, c;
     sincospif(2.f * u2, &s, &c);
     float st = sqrtf(fmaxf(0.f, 1.f - mu * mu));
@@ -427,9 +418,11 @@ __device__ __forceinline__ void rotate_dir(float ct, float u2, float &dx, float 
     dz *= n;
 }
 
-// Henyey-Greenstein scattering about the current direction
-__device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy, float &dz) {
-    float u = uni(r);
+// Henyey-Greenstein scattering about the current direction: cosine from u1,
+// azimuth 2 pi u2
+__device__ __forceinline__ void hg_dir(float u1, float u2, float g, float &dx, float &dy,
+                                       float &dz) {
+    float u = u1;
     float ct;
     if (fabsf(g) < 1e-6f) {
         ct = 2.f * u - 1.f;
@@ -438,7 +431,7 @@ __device__ __forceinline__ void hg_dir(Philox &r, float g, float &dx, float &dy,
         ct = (1.f + g * g - f * f) / (2.f * g);
     }
     ct = fminf(1.f, fmaxf(-1.f, ct));
-    rotate_dir(ct, uni(r), dx, dy, dz);
+    rotate_dir(ct, u2, dx, dy, dz);
 }
 
 // perpendicular distance to the nearest wall of the current cell that bounds

@@ -2,8 +2,9 @@
 // pdspy's Model.run_thermal (pdspy/modeling/Model.py:221-256, radmc3d/run.py:9-38).
 //
 // Algorithm (Bjorkman & Wood 2001 with a path-length absorbed-energy estimator):
-//  * nphot packets of energy eps = L/nphot leave blackbody point stars; packet p
-//    draws from Philox4x32-10 with key = seed, counter = (block, p, 0).
+//  * nphot packets of energy eps = L/nphot leave blackbody point stars; random
+//    event k of packet p (emission, interaction, modified-random-walk step)
+//    uses the Philox4x32-10 blocks with counter (k, p, 0|1) and key = seed.
 //  * flights of optical depth -ln(xi) through the spherical grid; every segment
 //    of length s in cell c adds kabs_g(nu) * s to the cell's absorbed-energy
 //    tally of absorption group g (u64 fixed point, unit 2^-28 kmax_g dr_c), so
@@ -281,7 +282,8 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     const int E = P.nepoch;
 
     Pos p;
-    Philox r;
+    uint64_t pid = 0;   // packet id (Philox counter words 1, 2)
+    uint32_t krng = 0;  // its next random event (Philox counter word 0)
     int ifr = 0;
     float tau = 0.f;
     uint32_t nev = 0, nstep = 0, nev_cell = 0, nzero = 0;
@@ -358,18 +360,20 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     if (!alive) {
                         uint64_t idx = base + __popcll(need & ((1ull << lane) - 1ull));
                         if (idx < size) {
-                            uint64_t pid = e0 + idx;
+                            pid = e0 + idx;
                             pe = cur_e;
                             slot = pe;
                             xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
                             tb = P.tslot + (size_t)pe * P.slot;
-                            rng_init(r, P.seed, pid, 0u);
-                            float u = uni(r);
+                            // emission (random event 0): star, frequency, direction, tau
+                            float u[8];
+                            rng8(P.seed, pid, 0u, u);
+                            krng = 1;
                             int ks = 0;
-                            while (ks < P.nstar - 1 && u > P.spick[ks]) ++ks;
+                            while (ks < P.nstar - 1 && u[0] > P.spick[ks]) ++ks;
                             ifr = sample_guided(P.scdf + (size_t)ks * NL,
-                                                P.sguide + (size_t)ks * RT_NGUIDE, uni(r));
-                            iso_dir(r, p.dx, p.dy, p.dz);
+                                                P.sguide + (size_t)ks * RT_NGUIDE, u[1]);
+                            iso_dir(u[2], u[3], p.dx, p.dy, p.dz);
                             p.x = P.spos[3 * ks];
                             p.y = P.spos[3 * ks + 1];
                             p.z = P.spos[3 * ks + 2];
@@ -386,7 +390,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                                 alive = true;
                                 if (p.ir >= 0 && P.visits)
                                     atomicAdd(&P.visits[cell_index(g, p)], 1u);
-                                tau = -__logf(uni(r));
+                                tau = -__logf(u[4]);
                             }
                         }
                     }
@@ -477,11 +481,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 slot++;
                 tb = P.tslot + (size_t)slot * P.slot;
             }
-            float ue = uni(r);
-            if (ue * aext < asca) {
+            // interaction (one random event): u0 scatter or absorb, u1 species,
+            // u2 temperature row, u3 frequency, u4 u5 direction, u6 next tau
+            float u[8];
+            rng8(P.seed, pid, krng++, u);
+            if (u[0] * aext < asca) {
                 n_scat++;
                 if (P.hg) {
-                    float us = uni(r) * asca, a = 0.f;
+                    float us = u[1] * asca, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
                         a = fmaf(pickn<MS>(rcache, s), s_ks[s * NL + ifr], a);
@@ -490,13 +497,13 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             break;
                         }
                     }
-                    hg_dir(r, s_g[sp * NL + ifr], p.dx, p.dy, p.dz);
+                    hg_dir(u[4], u[5], s_g[sp * NL + ifr], p.dx, p.dy, p.dz);
                 } else {
-                    iso_dir(r, p.dx, p.dy, p.dz);
+                    iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
                 }
             } else {
                 n_abs++;
-                float us = uni(r) * aabs, a = 0.f;
+                float us = u[1] * aabs, a = 0.f;
                 int sp = S - 1;
                 for (int s = 0; s < S; ++s) {
                     a = fmaf(pickn<MS>(rcache, s), s_ka[s * NL + ifr], a);
@@ -509,11 +516,12 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                 int lo = min((int)x, P.ntemp - 2);
                 float w = x - (float)lo;
-                int row = lo + (uni(r) < w ? 1 : 0);
+                int row = lo + (u[2] < w ? 1 : 0);
                 size_t ro = (size_t)q * P.ntemp + row;
-                ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, uni(r));
-                iso_dir(r, p.dx, p.dy, p.dz);
+                ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, u[3]);
+                iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
             }
+            tau = -__logf(u[6]);
             if (P.max_events && nev >= P.max_events) {
                 flush_tally(P, tb, c, acc, aold, ovf);
                 max_pkt = max(max_pkt, nev + nstep);
@@ -535,7 +543,11 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
                 if (aR * dmin >= P.mrw_tau && R0 > 0.f) {
-                    float y = mrw_escape_y(P.mrwy, RT_MRW_NY, RT_MRW_LMAX, uni(r));
+                    // one random event: u0 path length, u1 u2 exit point, u3 u4
+                    // exit direction, u5 species, u6 temperature row, u7 frequency
+                    float v[8];
+                    rng8(P.seed, pid, krng++, v);
+                    float y = mrw_escape_y(P.mrwy, RT_MRW_NY, RT_MRW_LMAX, v[0]);
                     float ell = 3.f * aR * R0 * R0 * y;
                     float wl = s_invdr[p.ir] * ell;
                     float kP[MG];
@@ -548,19 +560,19 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             acc[q] += (unsigned long long)rintf(kP[q] * ginv[q] * wl);
                         }
                     float nx, ny, nz;
-                    iso_dir(r, nx, ny, nz);
+                    iso_dir(v[1], v[2], nx, ny, nz);
                     p.x = fmaf(R0, nx, p.x);
                     p.y = fmaf(R0, ny, p.y);
                     p.z = fmaf(R0, nz, p.z);
                     if (g.mirror) p.hemi = (p.z >= 0.f) ? 1 : -1;
-                    float ct = sqrtf(uni(r));
-                    rotate_dir(ct, uni(r), nx, ny, nz);
+                    float ct = sqrtf(v[3]);
+                    rotate_dir(ct, v[4], nx, ny, nz);
                     p.dx = nx;
                     p.dy = ny;
                     p.dz = nz;
                     float wsum = 0.f;
                     for (int s = 0; s < S; ++s) wsum = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, pickn<MS>(grpr, s)), wsum);
-                    float us = uni(r) * wsum, a = 0.f;
+                    float us = v[5] * wsum, a = 0.f;
                     int sp = S - 1;
                     for (int s = 0; s < S; ++s) {
                         a = fmaf(pickn<MS>(rcache, s), pickn<MG>(kP, pickn<MS>(grpr, s)), a);
@@ -573,13 +585,12 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     float x = fminf(fmaxf(pickn<MG>(xcache, q), 0.f), (float)(P.ntemp - 1));
                     int lo = min((int)x, P.ntemp - 2);
                     float w = x - (float)lo;
-                    int row = lo + (uni(r) < w ? 1 : 0);
+                    int row = lo + (v[6] < w ? 1 : 0);
                     size_t ro = (size_t)q * P.ntemp + row;
-                    ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, uni(r));
+                    ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, v[7]);
                     n_mrw++;
                 }
             }
-            tau = -__logf(uni(r));
         } else {
             if (c >= 0) {
                 float wf = s_invdr[p.ir] * tw;
