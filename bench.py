@@ -4,9 +4,11 @@ Workload (the production-size disk + envelope): spherical grid 256 x 128 x 1
 cells (0.1-5000 au, equatorial mirror), flared disk (1e-3 Msun, Sigma ~ r^-0.5)
 plus an Ulrich envelope (0.1 Msun, Rc = 50 au, 20 deg cavity), two dust species
 with kappa_abs = 1e3 (lam/um)^-1 (albedo 0.5, isotropic), a 4000 K 2 Rsun star,
-100 wavelengths; nphot packets per step (default 1e8).  One step = one thermal
-Monte Carlo run; the image part renders the 256 x 256 1.3 mm image at 60 deg
-(2" field at 140 pc) of the resulting temperatures and its adjoint.
+100 wavelengths, modified random walk on (pdspy's fitting setting); nphot
+packets per step (default 1e8).  One step = one thermal Monte Carlo run plus
+the copy of the per-species temperatures to the host (what run_thermal hands
+back); the image part renders the 256 x 256 1.3 mm image at 60 deg (2" field at
+140 pc) of the resulting temperatures and its adjoint.
 
 Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
@@ -23,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+CALIB_BYTES = 256 << 20  # device copy whose FETCH_SIZE / WRITE_SIZE scale the counters
 
 
 def parse():
@@ -32,34 +35,45 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nphot", type=float, default=1e8)
     p.add_argument("--max-events", type=float, default=1e6)
-    p.add_argument("--mrw", type=int, default=0)
+    p.add_argument("--mrw", type=int, default=1)
     p.add_argument("--cpu-seconds", type=float, default=15.0,
-                   help="target CPU time of the bounded oracle sample")
+                   help="target wall time of the bounded CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-image", action="store_true")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "thermal_traffic.json"))
     return p.parse_args()
 
 
-def cpu_baseline(arr, nphot_gpu, max_events, target_s, mrw):
-    """The FP64 oracle (same algorithm, OpenMP over packets of one epoch) on the
-    host's cores, on a bounded sample of packets of the same workload."""
+def dT_stats(T_ref, T, visits, min_visits=100):
+    """median and 99th percentile of |dT|/T over cells with >= min_visits entries
+    and a positive reference temperature (all species)."""
+    sel = (np.asarray(visits) >= min_visits)[None, :] & (T_ref > 0)
+    d = np.abs(T[sel] - T_ref[sel]) / T_ref[sel]
+    return float(np.median(d)), float(np.percentile(d, 99)), int(sel.sum())
+
+
+def cpu_baseline(arr, max_events, target_s, mrw):
+    """The FP64 oracle (same physics) on the host's cores: immediate
+    re-emission, packets free-running over OpenMP threads that share the
+    tallies (no epoch barriers), on a bounded sample of the same workload."""
     import oracle
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ncores = min(ncores, 16)
-    n = 20000
+    n = 10000
     t0 = time.perf_counter()
-    oracle.thermal(arr, n, seed=100, max_events=int(max_events), lagged=True, nthreads=ncores,
-                   epoch_first=max(1024, int(nphot_gpu) // 4096))
+    oracle.thermal(arr, n, seed=100, max_events=int(max_events), lagged=False, nthreads=ncores,
+                   mrw=bool(mrw))
     dt = time.perf_counter() - t0
-    n2 = int(min(max(n, n * target_s / max(dt, 1e-3)), nphot_gpu))
+    n2 = int(max(n, n * target_s / max(dt, 1e-3)))
     t0 = time.perf_counter()
-    _, _, st = oracle.thermal(arr, n2, seed=101, max_events=int(max_events), lagged=True,
-                              nthreads=ncores, epoch_first=max(1024, int(nphot_gpu) // 4096))
+    _, _, st = oracle.thermal(arr, n2, seed=101, max_events=int(max_events), lagged=False,
+                              nthreads=ncores, mrw=bool(mrw))
     dt = time.perf_counter() - t0
+    steps = (st["n_cross"] + st["n_events"]) / n2
     return {"value": n2 / dt, "unit": "packets/s", "cores": ncores, "kind": "port",
-            "sample": "%d packets (seed 101) of the same workload, FP64 oracle, epoch-lagged "
-                      "OpenMP, %.1f s wall" % (n2, dt)}
+            "sample": "%d packets (seed 101) of the same workload: FP64 oracle, immediate "
+                      "re-emission, OpenMP over packets, %.1f s wall, %.0f steps/packet"
+                      % (n2, dt, steps)}
 
 
 def main():
@@ -70,49 +84,63 @@ def main():
         sys.exit(2)
     import torch
     from pdspy_amd import configs, inputs, rt
-    from tests.helpers import dT_stats
 
     nphot = int(a.nphot)
     model = configs.disk_envelope()
     arr = rt.model_arrays(model.grid)
     dm = rt.DeviceModel(arr)
     S = dm.nspec
-    opts = dict(max_events=int(a.max_events), mrw=a.mrw, count_visits=0)
+    opts = dict(max_events=int(a.max_events), mrw=a.mrw)
 
-    for w in range(a.warmup):
-        dm.thermal(nphot, 1000 + w, **opts)
+    # calibration copy of known size (scales FETCH_SIZE / WRITE_SIZE counters)
+    src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
     torch.cuda.synchronize()
-    stats = []
+    del src, dst
+
+    # warmup (the first one also counts cell visits for the accuracy mask)
+    vis = None
+    for w in range(a.warmup):
+        dm.thermal(nphot, 1000 + w, count_visits=1 if w == 0 else 0, **opts)
+        if w == 0:
+            _, vis = dm.temperature(with_visits=True)
+    torch.cuda.synchronize()
+    stats, temps = [], []
     t0 = time.perf_counter()
     for k in range(a.steps):
         stats.append(dm.thermal(nphot, k, **opts))
+        temps.append(dm.temperature())
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     value = nphot * a.steps / wall
 
-    # roofline of the MC kernel: B_alg = N_cross * 4 S + N_abs * 16 bytes
-    kern_ms = sum(s["kernel_ms"] for s in stats)
-    b_alg = sum(s["n_cross"] * 4 * S + s["n_abs"] * 16 for s in stats)
+    # roofline of the MC kernel (one launch per step): algorithmic bytes
+    # B_alg = N_cross * 4 S + N_abs * 16 over the launch's HIP-event duration
+    kern_ms = sum(s["kernel_ms"] for s in stats) / a.steps
+    b_alg = sum(s["n_cross"] * 4 * S + s["n_abs"] * 16 for s in stats) / a.steps
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(a.traffic_json):
         try:
-            traffic = json.load(open(a.traffic_json)).get("bytes_per_launch")
-        except Exception:
+            tj = json.load(open(a.traffic_json))
+            if int(tj.get("nphot", -1)) == nphot and int(tj.get("mrw", -1)) == a.mrw:
+                traffic = tj.get("bytes_per_launch")
+        except (OSError, ValueError):
             traffic = None
+    lane_steps = sum(s["n_cross"] + s["n_events"] for s in stats)
+    wave_iters = sum(s["wave_iters"] for s in stats)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_thermal", "b_alg_per_step": b_alg / a.steps,
-            "kernel_ms_per_step": kern_ms / a.steps,
-            "launches_per_step": stats[0]["n_epochs"]}
+            "kernel": "k_thermal", "b_alg_per_launch": b_alg, "kernel_ms_per_launch": kern_ms,
+            "launches_per_step": 1,
+            "steps_per_s": lane_steps / (kern_ms * a.steps * 1e-3),
+            "lane_utilisation": lane_steps / (64.0 * max(wave_iters, 1))}
 
-    # accuracy at this nphot: seed-to-seed scatter of the per-cell temperature
-    dm.thermal(nphot, 0, count_visits=1, **{k: v for k, v in opts.items() if k != "count_visits"})
-    T0, vis = dm.temperature(with_visits=True)
-    h0 = hashlib.sha256(T0.astype(np.float32).tobytes()).hexdigest()[:16]
-    dm.thermal(nphot, 1, **opts)
-    T1 = dm.temperature()
-    med, p99, ncell = dT_stats(T0, T1, vis)
+    med = p99 = ncell = None
+    if vis is not None and len(temps) >= 2:
+        med, p99, ncell = dT_stats(temps[0], temps[1], vis)
+    h0 = hashlib.sha256(temps[0].astype(np.float32).tobytes()).hexdigest()[:16]
 
     out = {
         "metric": "photon packets/s (thermal MC) + image rays/s w/ adjoint, at <=1% median dT/T",
@@ -120,20 +148,22 @@ def main():
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "disk+envelope 256x128x1 cells, 2 species, 100 lambda, "
-                               "nphot=%g per step" % nphot,
+                               "nphot=%g per step, MRW %s" % (nphot, "on" if a.mrw else "off"),
                    "nphot": nphot, "cells": dm.ncell, "species": S, "max_events": int(a.max_events),
                    "mrw": a.mrw},
         "roofline": roof,
         "accuracy": {"median_dT_over_T": med, "p99_dT_over_T": p99, "cells_ge_100_visits": ncell,
-                     "compared": "seed 0 vs seed 1 at equal nphot", "T_hash_seed0": h0},
+                     "compared": "timed steps seed 0 vs seed 1 at equal nphot",
+                     "T_hash_seed0": h0},
         "mc_stats": {k: stats[-1][k] for k in ("n_cross", "n_abs", "n_scat", "n_killed",
-                                               "n_events", "n_epochs")},
+                                               "n_events", "n_mrw", "n_epochs", "n_nudged",
+                                               "max_packet_steps")},
     }
 
     if not a.no_image:
         # 1.3 mm, 256x256, 2" field at 140 pc, 60 deg; forward and adjoint
-        T = torch.tensor(rt.imaging_temperature(_with_T(model, T0, dm)), dtype=torch.float32,
-                         device="cuda")
+        T = torch.tensor(rt.imaging_temperature(_with_T(model, temps[0], dm)),
+                         dtype=torch.float32, device="cuda")
         pix_cm = 2.0 / 256 * 140 * inputs.AU
         cam = dm.camera(256, pix_cm, 1300.0, incl=60.0)
         dimg = torch.ones((256, 256), dtype=torch.float32, device="cuda")
@@ -157,7 +187,7 @@ def main():
                         "ms_fwd": t_f * 1e3, "ms_fwd_adj": t_fa * 1e3,
                         "config": "256x256 px, 1.3 mm, incl 60, 2 arcsec field at 140 pc"}
     if not a.no_cpu:
-        cb = cpu_baseline(arr, nphot, a.max_events, a.cpu_seconds, a.mrw)
+        cb = cpu_baseline(arr, a.max_events, a.cpu_seconds, a.mrw)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = value / cb["value"]
     print(json.dumps(out), flush=True)

@@ -101,8 +101,9 @@ typedef struct {
     uint64_t idle_iters;  /* of which a wavefront had no packet to step                 */
     uint64_t n_nudged;    /* packets moved 2^-16 r along their ray after bouncing
                              between two cells at a wall without moving             */
-    uint64_t phase_clocks[4]; /* shader clocks summed over wavefronts spent in the
-                                 refill, step, snapshot and idle phases; zero unless the
+    uint64_t phase_clocks[8]; /* shader clocks summed over wavefronts spent in: refill,
+                                 loop overhead, snapshot, idle, geometry, crossing,
+                                 interaction, modified random walk; zero unless the
                                  library is built with RT_THERMAL_PROFILE              */
 } rt_thermal_stats;
 

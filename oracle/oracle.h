@@ -52,7 +52,9 @@ typedef struct {
     int lagged;           /* 0: immediate B&W; 1: epoch-lagged temperatures */
     uint64_t epoch_first; /* first epoch size in packets (lagged mode)      */
     double epoch_growth;  /* later epochs: max(first, N_done*(growth-1))    */
-    int nthreads;         /* OpenMP threads (lagged mode only)              */
+    int nthreads;         /* OpenMP threads: lagged mode 0 = all; immediate
+                             mode runs free-running in parallel when > 1
+                             (schedule-dependent result, for timing)        */
     int mrw;              /* modified random walk on/off                    */
     double mrw_tauthres;  /* Rosseland tau to the nearest wall to trigger   */
     int mrw_count;        /* interactions in the cell before it may trigger */

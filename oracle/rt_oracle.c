@@ -831,19 +831,26 @@ double or_mrw_invert(double xi) {
 /* perpendicular distance to the nearest wall of the current cell that bounds
  * the medium (the equatorial mirror and the polar axis are not walls) */
 static double wall_dist(const geom_t *g, const pk_t *p) {
+    /* cone at theta_j: r sin|theta - theta_j| = |rho cos theta_j - z sin theta_j|
+     * while |theta - theta_j| <= pi/2 (else the axis, r); phi half-plane at
+     * phi_k: |y cos phi_k - x sin phi_k| while |phi - phi_k| <= pi/2 (else rho) */
     double r = sqrt(p->x * p->x + p->y * p->y + p->z * p->z);
     double d = fmin(r - g->rw[p->ir], g->rw[p->ir + 1] - r);
     double rho = sqrt(p->x * p->x + p->y * p->y);
-    double th = g->mirror ? atan2(rho, fabs(p->z)) : atan2(rho, p->z);
+    double zf = g->mirror ? fabs(p->z) : p->z;
     int j = p->it;
-    if (j > 0) d = fmin(d, r * sin(fmin(fabs(th - g->tw[j]), 0.5 * PI_D)));
-    if (j + 1 < g->nt) d = fmin(d, r * sin(fmin(fabs(g->tw[j + 1] - th), 0.5 * PI_D)));
+    for (int w = j; w <= j + 1; ++w) {
+        if (w == 0 || w == g->nt) continue;
+        double sn = rho * g->ct[w] - zf * g->st[w];
+        double cs = zf * g->ct[w] + rho * g->st[w];
+        d = fmin(d, cs >= 0 ? fabs(sn) : r);
+    }
     if (g->np > 1) {
-        double ph = atan2(p->y, p->x);
-        if (ph < 0) ph += TWO_PI_D;
-        int k = p->ip;
-        d = fmin(d, rho * sin(fmin(fabs(ph - g->pw[k]), 0.5 * PI_D)));
-        d = fmin(d, rho * sin(fmin(fabs(g->pw[k + 1] - ph), 0.5 * PI_D)));
+        for (int k = p->ip; k <= p->ip + 1; ++k) {
+            double sn = p->y * g->cp[k] - p->x * g->sp[k];
+            double cs = p->x * g->cp[k] + p->y * g->sp[k];
+            d = fmin(d, cs >= 0 ? fabs(sn) : rho);
+        }
     }
     return d;
 }
@@ -1231,7 +1238,39 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
     cnt_t tot;
     memset(&tot, 0, sizeof(tot));
     int nep = 0;
-    if (!o->lagged) {
+    if (!o->lagged && o->nthreads > 1) {
+        /* free-running immediate re-emission on several threads (the way an
+         * OpenMP mctherm runs): packets read the shared tallies as they go, so
+         * the result depends on the schedule; used for timing */
+#ifdef _OPENMP
+        omp_set_num_threads(o->nthreads);
+#endif
+        uint64_t a = 0, b = 0, c2 = 0, d = 0, e2 = 0, f = 0, h = 0, nu = 0;
+        int64_t hi = (int64_t)o->nphot;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : a, b, c2, d, e2, f, h, nu)
+        for (int64_t p = 0; p < hi; ++p) {
+            cnt_t cc;
+            memset(&cc, 0, sizeof(cc));
+            run_packet(&mc, (uint64_t)p, 0, &cc);
+            a += cc.n_cross;
+            b += cc.n_abs;
+            c2 += cc.n_scat;
+            d += cc.n_escape;
+            e2 += cc.n_killed;
+            f += cc.n_events;
+            h += cc.n_mrw;
+            nu += cc.n_nudged;
+        }
+        tot.n_cross = a;
+        tot.n_abs = b;
+        tot.n_scat = c2;
+        tot.n_escape = d;
+        tot.n_killed = e2;
+        tot.n_events = f;
+        tot.n_mrw = h;
+        tot.n_nudged = nu;
+        nep = 1;
+    } else if (!o->lagged) {
         for (uint64_t p = 0; p < o->nphot; ++p) run_packet(&mc, p, 0, &tot);
         nep = 1;
     } else {

@@ -55,7 +55,7 @@ class ThermalStats(C.Structure):
                 ("total_ms", C.c_double), ("max_epoch_ms", C.c_double),
                 ("max_packet_steps", C.c_uint64), ("wave_iters", C.c_uint64),
                 ("idle_iters", C.c_uint64), ("n_nudged", C.c_uint64),
-                ("phase_clocks", C.c_uint64 * 4)]
+                ("phase_clocks", C.c_uint64 * 8)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

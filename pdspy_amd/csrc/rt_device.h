@@ -435,22 +435,34 @@ __device__ __forceinline__ void hg_dir(float u1, float u2, float g, float &dx, f
 }
 
 // perpendicular distance to the nearest wall of the current cell that bounds
-// the medium; the equatorial mirror and the polar axis are not walls
+// the medium; the equatorial mirror and the polar axis are not walls.  Cone at
+// theta_j: r sin|theta - theta_j| = |rho cos theta_j - z sin theta_j| while
+// |theta - theta_j| <= pi/2 (else the axis, r); phi half-plane at phi_k:
+// |y cos phi_k - x sin phi_k| while |phi - phi_k| <= pi/2 (else rho).
 __device__ __forceinline__ float wall_dist(const DevGrid &g, const Pos &p) {
     float r = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
     float d = fminf(r - g.rw[p.ir], g.rw[p.ir + 1] - r);
     float rho = sqrtf(p.x * p.x + p.y * p.y);
-    float th = g.mirror ? atan2f(rho, fabsf(p.z)) : atan2f(rho, p.z);
-    const float hp = 1.57079632679f;
+    float zf = g.mirror ? fabsf(p.z) : p.z;
     int j = p.it;
-    if (j > 0) d = fminf(d, r * sinf(fminf(fabsf(th - g.tw[j]), hp)));
-    if (j + 1 < g.nt) d = fminf(d, r * sinf(fminf(fabsf(g.tw[j + 1] - th), hp)));
+    if (j > 0) {
+        float sn = rho * g.ct[j] - zf * g.st[j];
+        float cs = zf * g.ct[j] + rho * g.st[j];
+        d = fminf(d, cs >= 0.f ? fabsf(sn) : r);
+    }
+    if (j + 1 < g.nt) {
+        float sn = rho * g.ct[j + 1] - zf * g.st[j + 1];
+        float cs = zf * g.ct[j + 1] + rho * g.st[j + 1];
+        d = fminf(d, cs >= 0.f ? fabsf(sn) : r);
+    }
     if (g.np > 1) {
-        float ph = atan2f(p.y, p.x);
-        if (ph < 0.f) ph += 6.28318530718f;
         int k = p.ip;
-        d = fminf(d, rho * sinf(fminf(fabsf(ph - g.pw[k]), hp)));
-        d = fminf(d, rho * sinf(fminf(fabsf(g.pw[k + 1] - ph), hp)));
+        float sn = p.y * g.cp[k] - p.x * g.sp[k];
+        float cs = p.x * g.cp[k] + p.y * g.sp[k];
+        d = fminf(d, cs >= 0.f ? fabsf(sn) : rho);
+        sn = p.y * g.cp[k + 1] - p.x * g.sp[k + 1];
+        cs = p.x * g.cp[k + 1] + p.y * g.sp[k + 1];
+        d = fminf(d, cs >= 0.f ? fabsf(sn) : rho);
     }
     return d;
 }

@@ -41,8 +41,8 @@ enum {
     CNT_MAXPKT,
     CNT_WITER,     // loop iterations summed over wavefronts
     CNT_IDLE,      // of which with no packet to step
-    CNT_TK0,       // 4 phase clocks (RT_THERMAL_PROFILE builds only)
-    CNT_N = CNT_TK0 + 4
+    CNT_TK0,       // 8 phase clocks (RT_THERMAL_PROFILE builds only)
+    CNT_N = CNT_TK0 + 8
 };
 
 struct HostTables {

@@ -55,6 +55,7 @@ using namespace rt;
 #ifndef RT_THERMAL_PROFILE
 #define RT_THERMAL_PROFILE 0
 #endif
+#define RT_NPHASE 8
 
 struct ThermParams {
     DevGrid g;
@@ -88,7 +89,7 @@ struct ThermParams {
     unsigned long long *slot_start;      // [E] packets of epoch e that have started
     unsigned long long *slot_in;         // [E] packets that moved into slot e
     unsigned long long *slot_out;        // [E] packets that left slot e (moved on or ended)
-    uint64_t slice;                      // interactions per slot (0: whole packets)
+    uint32_t slice;                      // interactions per slot
     unsigned int *snap_claim;            // [E]
     unsigned int *snap_done;             // [E]
     unsigned int *snap_ready;            // [E]
@@ -287,12 +288,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     int ifr = 0;
     float tau = 0.f;
     uint32_t nev = 0, nstep = 0, nev_cell = 0, nzero = 0;
+    uint32_t slice_left = 0;  // interactions until the packet moves to the next slot
     bool alive = false;
     int pe = 0;                                // epoch of this lane's packet
     int slot = 0;                              // its current deposit slot
     const float *xs = nullptr;                 // its temperature snapshot (null: T = 0)
     unsigned long long *tb = P.tslot;          // its deposit slot's tallies
     int ccell = -1;                            // cell whose data sit in registers
+    float aRc = -1.f;                          // its Rosseland extinction (-1: not yet)
     float rcache[MS];
     float xcache[MG];
 #pragma unroll
@@ -319,13 +322,22 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     uint64_t n_idle = 0;
     uint32_t iter = 0;
 #if RT_THERMAL_PROFILE
-    uint64_t tk[4] = {0, 0, 0, 0};  // refill, step, snapshot, idle
-    uint64_t tc = clock64();
-#define RT_PHASE(k)                  \
-    do {                             \
-        uint64_t tn_ = clock64();    \
-        tk[k] += tn_ - tc;           \
-        tc = tn_;                    \
+    // wave time by phase: refill, loop overhead, snapshot, idle, geometry,
+    // crossing, interaction, modified random walk.  The wave's last time stamp
+    // lives in LDS and the first active lane books each interval, so time
+    // spent in divergent branches is counted once per wave.
+    __shared__ unsigned long long s_tc[4];
+    uint64_t tk[RT_NPHASE];
+    for (int k = 0; k < RT_NPHASE; ++k) tk[k] = 0;
+    const int wslot = threadIdx.x >> 6;
+    if (lane == 0) s_tc[wslot] = clock64();
+#define RT_PHASE(k)                                                      \
+    do {                                                                 \
+        uint64_t tn_ = clock64();                                        \
+        if (lane == __ffsll((unsigned long long)__ballot(1)) - 1) {     \
+            tk[k] += tn_ - s_tc[wslot];                                  \
+            s_tc[wslot] = tn_;                                           \
+        }                                                                \
     } while (0)
 #else
 #define RT_PHASE(k) \
@@ -382,6 +394,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                             nstep = 0;
                             nev_cell = 0;
                             nzero = 0;
+                            slice_left = P.slice;
                             ccell = -1;
                             if (p.ir >= g.nr) {
                                 n_esc++;
@@ -436,21 +449,23 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
         if (!alive) continue;
 
         // ---- one step: to the next wall or to the next interaction
+        // (a new cell's data are requested first, the wall distances computed
+        // while they arrive)
+        const int c = p.ir >= 0 ? cell_index(g, p) : -1;
+        if (c >= 0 && c != ccell) {
+            const float *rc = P.rhoL + (size_t)c * S;
+#pragma unroll
+            for (int s = 0; s < MS; ++s) rcache[s] = s < S ? rc[s] : 0.f;
+#pragma unroll
+            for (int q = 0; q < MG; ++q)
+                xcache[q] = (q < G && xs) ? xs[(size_t)c * G + q] : -1.f;
+            ccell = c;
+            aRc = -1.f;
+        }
         int kind;
         float tw = next_wall(g, p, kind);
         float aabs = 0.f, asca = 0.f;
-        int c = -1;
-        if (p.ir >= 0) {
-            c = cell_index(g, p);
-            if (c != ccell) {
-                const float *rc = P.rhoL + (size_t)c * S;
-#pragma unroll
-                for (int s = 0; s < MS; ++s) rcache[s] = s < S ? rc[s] : 0.f;
-#pragma unroll
-                for (int q = 0; q < MG; ++q)
-                    xcache[q] = (q < G && xs) ? xs[(size_t)c * G + q] : -1.f;
-                ccell = c;
-            }
+        if (c >= 0) {
 #pragma unroll
             for (int s = 0; s < MS; ++s)
                 if (s < S) {
@@ -459,6 +474,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 }
         }
         float aext = aabs + asca;
+        RT_PHASE(4);
         if (aext * tw > tau) {
             // interaction inside the cell
             float sseg = tau / aext;
@@ -473,7 +489,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             nev++;
             nev_cell++;
             nzero = 0;
-            if (P.slice && nev % P.slice == 0u && slot < E) {
+            if (--slice_left == 0u && slot < E) {
                 // next slot: earlier deposits are final for snapshots <= slot
                 flush_tally(P, tb, c, acc, aold, ovf);
                 if (slot + 1 < E) count_up(&P.slot_in[slot + 1]);
@@ -481,6 +497,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 slot++;
                 tb = P.tslot + (size_t)slot * P.slot;
             }
+            if (slice_left == 0u) slice_left = P.slice;
             // interaction (one random event): u0 scatter or absorb, u1 species,
             // u2 temperature row, u3 frequency, u4 u5 direction, u6 next tau
             float u[8];
@@ -522,6 +539,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
             }
             tau = -__logf(u[6]);
+            RT_PHASE(6);
             if (P.max_events && nev >= P.max_events) {
                 flush_tally(P, tb, c, acc, aold, ovf);
                 max_pkt = max(max_pkt, nev + nstep);
@@ -533,13 +551,17 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (P.mrw && nev_cell >= (uint32_t)P.mrw_count) {
                 // modified random walk: jump to a sphere inside the cell when the
                 // Rosseland optical depth to the nearest wall is large
-                float aR = 0.f;
-                for (int s = 0; s < S; ++s) {
-                    float rho = pickn<MS>(rcache, s);
-                    if (rho > 0.f)
-                        aR = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
-                                            pickn<MG>(xcache, pickn<MS>(grpr, s))), aR);
+                if (aRc < 0.f) {
+                    float a = 0.f;
+                    for (int s = 0; s < S; ++s) {
+                        float rho = pickn<MS>(rcache, s);
+                        if (rho > 0.f)
+                            a = fmaf(rho, trow(P.kross + (size_t)s * P.ntemp, P.ntemp,
+                                               pickn<MG>(xcache, pickn<MS>(grpr, s))), a);
+                    }
+                    aRc = a;
                 }
+                const float aR = aRc;
                 float dmin = wall_dist(g, p);
                 float R0 = dmin - P.mrw_gamma / aR;
                 if (aR * dmin >= P.mrw_tau && R0 > 0.f) {
@@ -591,6 +613,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     n_mrw++;
                 }
             }
+            RT_PHASE(7);
         } else {
             if (c >= 0) {
                 float wf = s_invdr[p.ir] * tw;
@@ -618,6 +641,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                     end = true;
                 }
             }
+            RT_PHASE(5);
             if (end) {
                 max_pkt = max(max_pkt, nev + nstep);
                 alive = false;
@@ -644,10 +668,14 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     if (lane == 0) {
         atomicAdd(&P.cnt[CNT_WITER], (unsigned long long)iter + 1ull);
         if (n_idle) atomicAdd(&P.cnt[CNT_IDLE], (unsigned long long)n_idle);
-#if RT_THERMAL_PROFILE
-        for (int k = 0; k < 4; ++k) atomicAdd(&P.cnt[CNT_TK0 + k], (unsigned long long)tk[k]);
-#endif
     }
+#if RT_THERMAL_PROFILE
+    for (int k = 0; k < RT_NPHASE; ++k) {
+        unsigned long long x = tk[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+        if (lane == 0) atomicAdd(&P.cnt[CNT_TK0 + k], x);
+    }
+#endif
 #undef RT_PHASE
 }
 
@@ -837,7 +865,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.slot_start = slot_start;
     P.slot_in = slot_in;
     P.slot_out = slot_out;
-    P.slice = slice;
+    P.slice = (uint32_t)std::min<uint64_t>(slice, 0x7fffffffull);
     P.snap_claim = snap_claim;
     P.snap_done = snap_done;
     P.snap_ready = snap_ready;
@@ -922,7 +950,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->wave_iters = cnt[CNT_WITER];
         st->n_nudged = cnt[CNT_NUDGE];
         st->idle_iters = cnt[CNT_IDLE];
-        for (int k = 0; k < 4; ++k) st->phase_clocks[k] = cnt[CNT_TK0 + k];
+        for (int k = 0; k < 8; ++k) st->phase_clocks[k] = cnt[CNT_TK0 + k];
     }
     if (cnt[CNT_ABORT])
         return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded %.1f s; packets were killed",
