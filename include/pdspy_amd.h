@@ -144,6 +144,11 @@ int rt_thermal(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts
 /* Per-species temperatures [nspec][cells] (r fastest, dust_temperature.dat
  * order, read.pyx:433-478) and visits[cells] (may be NULL). */
 int rt_get_temperature(rt_model *m, double *h_T, uint64_t *h_visits);
+/* Pipeline timeline of the last rt_thermal (no counterpart in RADMC-3D; for
+ * tuning): ms from the start of the Monte Carlo launch to the publication of
+ * each epoch's temperature snapshot (n_epochs values, -1 if never published),
+ * then to the launch's end.  Writes min(count, cap) values, returns count. */
+int rt_thermal_timeline(const rt_model *m, double *h_ms, int cap);
 /* Device copy of the same temperatures as float [nspec][cells]. */
 int rt_get_temperature_device(rt_model *m, float *d_T);
 

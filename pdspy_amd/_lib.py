@@ -14,7 +14,8 @@ LIB_PATH = os.environ.get("PDSPY_AMD_LIB") or os.path.join(HERE, "lib", "libpdsp
 EXPORTS = [
     "rt_create", "rt_destroy", "rt_set_stream", "rt_last_error", "rt_device_info",
     "rt_model_create", "rt_model_destroy", "rt_model_ncells", "rt_model_set_density",
-    "rt_thermal", "rt_get_temperature", "rt_get_temperature_device", "rt_image",
+    "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
+    "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_chisq",
 ]
 
@@ -94,6 +95,7 @@ def load():
         "rt_model_set_density": ([P, P], I),
         "rt_thermal": ([P, U64, U64, C.POINTER(ThermalOpts), C.POINTER(ThermalStats)], I),
         "rt_get_temperature": ([P, P, P], I),
+        "rt_thermal_timeline": ([P, P, I], I),
         "rt_get_temperature_device": ([P, P], I),
         "rt_image": ([P, C.POINTER(Camera), P, P, P], I),
         "rt_image_adjoint": ([P, C.POINTER(Camera), P, P, P, P, P], I),

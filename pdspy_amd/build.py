@@ -19,6 +19,10 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # hardware FP64 atomics for the adjoint accumulators
     "-munsafe-fp-atomics",
+    # FP32 divide and square root by the hardware approximations (<= 2.5 ulp)
+    # instead of the correctly rounded expansions; near-tangent geometry is
+    # re-evaluated in FP64 anyway
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

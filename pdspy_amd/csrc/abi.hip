@@ -359,6 +359,14 @@ int rt_get_temperature(rt_model *m, double *h_T, uint64_t *h_visits) {
     return RT_OK;
 }
 
+int rt_thermal_timeline(const rt_model *m, double *h_ms, int cap) {
+    if (!m) return RT_E_ARG;
+    int n = (int)m->timeline.size();
+    if (h_ms)
+        for (int i = 0; i < n && i < cap; ++i) h_ms[i] = m->timeline[i];
+    return n;
+}
+
 int rt_get_temperature_device(rt_model *m, float *d_T) {
     if (!m || !d_T) return RT_E_ARG;
     rt_ctx *ctx = m->ctx;

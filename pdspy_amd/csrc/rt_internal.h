@@ -114,6 +114,7 @@ struct rt_model {
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
     double eps_last = 0;          // packet energy of the last thermal run
+    std::vector<double> timeline; // last run: ms from kernel start to each snapshot, then the end
     bool have_T = false;
 };
 

@@ -94,6 +94,7 @@ struct ThermParams {
     unsigned int *snap_done;             // [E]
     unsigned int *snap_ready;            // [E]
     unsigned int *next_snap;             // [1]
+    unsigned long long *snap_time;       // [E] clock at each snapshot, then t_begin, t_end
     unsigned long long *tslot;           // [E+1][slot] tallies by deposit slot (last: all later)
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
     float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
@@ -220,6 +221,7 @@ __device__ bool snapshot_help(const ThermParams &P, int lane) {
             order_release();
             st_rlx32(&P.snap_ready[k], 1u);
             st_rlx32(P.next_snap, k + 1u);
+            st_rlx64(&P.snap_time[k], wall_clock64());
         }
     }
     return true;
@@ -281,6 +283,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     const int lane = __lane_id();
     const uint64_t t_start = wall_clock64();
     const int E = P.nepoch;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_rlx64(&P.snap_time[RT_MAX_EPOCH], t_start);
 
     Pos p;
     uint64_t pid = 0;   // packet id (Philox counter words 1, 2)
@@ -666,6 +669,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     for (int q = 0; q < MG; ++q) ovf |= aold[q];
     if (__any((ovf >> 62) != 0ull) && lane == 0) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
     if (lane == 0) {
+        atomicMax(&P.snap_time[RT_MAX_EPOCH + 1], (unsigned long long)wall_clock64());
         atomicAdd(&P.cnt[CNT_WITER], (unsigned long long)iter + 1ull);
         if (n_idle) atomicAdd(&P.cnt[CNT_IDLE], (unsigned long long)n_idle);
     }
@@ -731,8 +735,9 @@ static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t fir
 }
 
 // pipeline counters and flags: u64 [ep_start E+1 | ep_claim | slot_start |
-// slot_in | slot_out] then u32 [snap_claim | snap_done | snap_ready | next_snap]
-#define PIPE_U64 (5 * RT_MAX_EPOCH + 1)
+// slot_in | slot_out | snap_time | t_begin, t_end] then u32 [snap_claim |
+// snap_done | snap_ready | next_snap]
+#define PIPE_U64 (6 * RT_MAX_EPOCH + 3)
 #define PIPE_U32 (3 * RT_MAX_EPOCH + 4)
 
 static int ensure_pipe_buffers(rt_model *m) {
@@ -801,6 +806,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     unsigned long long *slot_start = ep_claim + RT_MAX_EPOCH;
     unsigned long long *slot_in = slot_start + RT_MAX_EPOCH;
     unsigned long long *slot_out = slot_in + RT_MAX_EPOCH;
+    unsigned long long *snap_time = slot_out + RT_MAX_EPOCH;  // then t_begin, t_end
     unsigned int *u32base = (unsigned int *)(ep_start + PIPE_U64);
     unsigned int *snap_claim = u32base;
     unsigned int *snap_done = snap_claim + RT_MAX_EPOCH;
@@ -812,8 +818,8 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     const size_t nts = sl * (size_t)(E + 1);
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nts, 256)), dim3(256), 0, sm, m->d_tally2, nts, 0ull);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)CNT_N, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(4 * RT_MAX_EPOCH, 256)), dim3(256), 0, sm, ep_claim,
-                       (size_t)(4 * RT_MAX_EPOCH), 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(5 * RT_MAX_EPOCH + 2, 256)), dim3(256), 0, sm, ep_claim,
+                       (size_t)(5 * RT_MAX_EPOCH + 2), 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(PIPE_U32, 256)), dim3(256), 0, sm, u32base,
                        (size_t)PIPE_U32, 0u);
     HIPCHK(ctx, hipMemcpyAsync(ep_start, est.data(), sizeof(unsigned long long) * est.size(),
@@ -869,6 +875,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.snap_claim = snap_claim;
     P.snap_done = snap_done;
     P.snap_ready = snap_ready;
+    P.snap_time = snap_time;
     P.next_snap = next_snap;
     P.tslot = m->d_tally2;
     P.cum = m->d_tally;
@@ -929,6 +936,17 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     (void)hipEventDestroy(e_k0);
     (void)hipEventDestroy(e_k1);
     (void)hipEventDestroy(e_end);
+    {
+        std::vector<unsigned long long> tl(RT_MAX_EPOCH + 2);
+        HIPCHK(ctx, hipMemcpy(tl.data(), snap_time, sizeof(unsigned long long) * tl.size(),
+                              hipMemcpyDeviceToHost));
+        const double tick_ms = 1.0 / (double)ctx->wallclock_khz;
+        const unsigned long long t0 = tl[RT_MAX_EPOCH];
+        m->timeline.assign(E + 1, -1.0);
+        for (int k = 0; k < E; ++k)
+            if (tl[k]) m->timeline[k] = (double)(long long)(tl[k] - t0) * tick_ms;
+        if (tl[RT_MAX_EPOCH + 1]) m->timeline[E] = (double)(long long)(tl[RT_MAX_EPOCH + 1] - t0) * tick_ms;
+    }
     m->eps_last = eps;
     m->have_T = true;
     if (st) {

@@ -162,6 +162,15 @@ class DeviceModel:
             self.h, T.ctypes.data, None if vis is None else vis.ctypes.data))
         return (T, vis) if with_visits else T
 
+    def timeline(self):
+        """ms from the start of the last thermal launch to each epoch's snapshot,
+        and (last entry) to the launch's end."""
+        n = self.ctx.L.rt_thermal_timeline(self.h, None, 0)
+        out = np.zeros(max(n, 0))
+        if n > 0:
+            self.ctx.L.rt_thermal_timeline(self.h, out.ctypes.data, n)
+        return out
+
     def temperature_device(self):
         import torch
         out = torch.empty((self.nspec, self.ncell), dtype=torch.float32, device="cuda")
