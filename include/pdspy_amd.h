@@ -181,6 +181,48 @@ int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d
              const float *d_mre, const float *d_mim, int K, double *d_out, float *d_gre,
              float *d_gim);
 
+/* ---- batches: one sampler step for many models (no counterpart in pdspy,
+ * whose fitting loop runs one model per MPI rank: run_disk_model.py:31-511,
+ * utils/emcee.py:183-203).  The models are created as above and keep their
+ * own grids, densities (rt_model_set_density between steps) and outputs. */
+typedef struct rt_batch rt_batch;
+
+/* What one model contributes to a step: its image, the data it is fitted to
+ * and where the results go (device buffers, caller-owned). */
+typedef struct {
+    rt_camera cam;                  /* image of the model (mode 0)                 */
+    double dxy;                     /* pixel size in radians for the visibilities  */
+    double scale;                   /* image unit -> visibility unit, e.g. Jy per
+                                       pixel per erg/s/cm^2/Hz/sr                   */
+    const double *d_u, *d_v;        /* [K] (u,v) in wavelengths                     */
+    int K;
+    double dra, ddec;               /* phase-centre shift (rad)                     */
+    const float *d_re, *d_im, *d_w; /* [K] data visibilities and weights            */
+    float *d_img;                   /* out [ny][nx] image (intensity)               */
+    double *d_chi2;                 /* out [1] chi^2                                */
+    float *d_g_logrho;              /* out [nspec][cells] d chi^2 / d log rho, and  */
+    float *d_g_T;                   /* out [cells] d chi^2 / d T_img; both NULL:
+                                       no adjoint                                   */
+} rt_step_item;
+
+int rt_batch_create(rt_ctx *ctx, rt_model *const *models, int nmodel, rt_batch **out);
+int rt_batch_destroy(rt_batch *b);
+/* Thermal MC of every model in ONE persistent launch: nphot[i] packets with
+ * Philox key seeds[i], the same options for all; stats: nmodel entries or
+ * NULL.  Each model's temperatures equal those of rt_thermal with the same
+ * arguments, bit for bit. */
+int rt_batch_thermal(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+                     const rt_thermal_opts *opts, rt_thermal_stats *stats);
+/* One sampler step: the batched thermal launch, then per model the imaging
+ * temperature (density-weighted species mean, Model.py:759-769), image,
+ * visibilities, chi^2 (+ gradient) and the adjoint.  use_graph = 1 captures
+ * the step as a hipGraph on first use and replays it while nphot, opts and
+ * items stay the same (seeds and densities may change).  step_ms: device
+ * time of the step (may be NULL). */
+int rt_batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+                  const rt_thermal_opts *opts, const rt_step_item *items, int use_graph,
+                  rt_thermal_stats *stats, double *step_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,10 +6,10 @@ fill the same outputs; the work runs in hand-written HIP kernels reached
 through a C ABI (``include/pdspy_amd.h``).  ``GPUModelMixin`` adds
 ``code="mi355x"`` to a pdspy Model subclass.
 """
-from .rt import (DeviceModel, GPUModelMixin, chisq, fft_grid_uv, imaging_temperature,
+from .rt import (Batch, DeviceModel, GPUModelMixin, chisq, fft_grid_uv, imaging_temperature,
                  interpolate_model, model_arrays, run_image, run_thermal, run_visibilities,
                  vis_dft, vis_dft_adjoint)
 
-__all__ = ["DeviceModel", "GPUModelMixin", "chisq", "fft_grid_uv", "imaging_temperature",
+__all__ = ["Batch", "DeviceModel", "GPUModelMixin", "chisq", "fft_grid_uv", "imaging_temperature",
            "interpolate_model", "model_arrays", "run_image", "run_thermal", "run_visibilities",
            "vis_dft", "vis_dft_adjoint"]

@@ -17,6 +17,7 @@ EXPORTS = [
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_chisq",
+    "rt_batch_create", "rt_batch_destroy", "rt_batch_thermal", "rt_batch_step",
 ]
 
 RT_ERRORS = {-1: "RT_E_ARG", -2: "RT_E_HIP", -3: "RT_E_NOMEM", -4: "RT_E_WATCHDOG",
@@ -71,6 +72,15 @@ class Camera(C.Structure):
                 ("tau_stop", C.c_double)]
 
 
+class StepItem(C.Structure):
+    _fields_ = [("cam", Camera), ("dxy", C.c_double), ("scale", C.c_double),
+                ("d_u", C.c_void_p), ("d_v", C.c_void_p), ("K", C.c_int),
+                ("dra", C.c_double), ("ddec", C.c_double),
+                ("d_re", C.c_void_p), ("d_im", C.c_void_p), ("d_w", C.c_void_p),
+                ("d_img", C.c_void_p), ("d_chi2", C.c_void_p),
+                ("d_g_logrho", C.c_void_p), ("d_g_T", C.c_void_p)]
+
+
 _lib = None
 
 
@@ -102,6 +112,10 @@ def load():
         "rt_vis_dft": ([P, P, I, I, D, P, P, I, D, D, P, P], I),
         "rt_vis_dft_adjoint": ([P, P, P, I, I, D, P, P, I, D, D, P], I),
         "rt_chisq": ([P, P, P, P, P, P, I, P, P, P], I),
+        "rt_batch_create": ([P, P, I, C.POINTER(P)], I),
+        "rt_batch_destroy": ([P], I),
+        "rt_batch_thermal": ([P, P, P, C.POINTER(ThermalOpts), P], I),
+        "rt_batch_step": ([P, P, P, C.POINTER(ThermalOpts), P, I, P, C.POINTER(C.c_double)], I),
     }
     for name in EXPORTS:
         f = getattr(L, name)

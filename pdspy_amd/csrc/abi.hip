@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstring>
 
+#include <cstdlib>
+
 #include "rt_internal.h"
 
 int set_err(rt_ctx *ctx, int code, const char *fmt, ...) {
@@ -63,9 +65,11 @@ static void free_model(rt_model *m) {
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
-                    m->d_xring};
+                    m->d_xring, m->d_run};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    std::free(m->h_run);
+    m->h_run = nullptr;
 }
 
 static int upload_density(rt_model *m) {
@@ -93,6 +97,7 @@ static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
     if (d->nr < 1 || d->ntheta < 1 || d->nphi < 1 || d->nspec < 1 || d->nlam < 1)
         return set_err(ctx, RT_E_ARG, "grid, species and wavelength counts must be >= 1");
     if (d->nspec > RT_MAX_SPEC) return set_err(ctx, RT_E_LIMIT, "nspec > %d", RT_MAX_SPEC);
+    if (d->nstar > RT_MAX_STAR) return set_err(ctx, RT_E_LIMIT, "nstar > %d", RT_MAX_STAR);
     if (d->nlam > RT_MAX_LAM) return set_err(ctx, RT_E_LIMIT, "nlam > %d", RT_MAX_LAM);
     if (!d->h_r_walls || !d->h_theta_walls || !d->h_phi_walls || !d->h_rho || !d->h_lam_um ||
         !d->h_kabs || !d->h_ksca || (d->nstar > 0 && !d->h_stars))
@@ -404,7 +409,28 @@ int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d
              const float *d_mre, const float *d_mim, int K, double *d_out, float *d_gre,
              float *d_gim) {
     if (!ctx) return RT_E_ARG;
-    return chisq_run(ctx, d_dre, d_dim, d_w, d_mre, d_mim, K, d_out, d_gre, d_gim);
+    return chisq_run(ctx, d_dre, d_dim, d_w, d_mre, d_mim, K, d_out, d_gre, d_gim, nullptr);
+}
+
+// ---------------------------------------------------------------- batches
+int rt_batch_create(rt_ctx *ctx, rt_model *const *models, int nmodel, rt_batch **out) {
+    if (!ctx) return RT_E_ARG;
+    return batch_create(ctx, models, nmodel, out);
+}
+
+int rt_batch_destroy(rt_batch *b) { return batch_destroy(b); }
+
+int rt_batch_thermal(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+                     const rt_thermal_opts *opts, rt_thermal_stats *stats) {
+    if (!b) return RT_E_ARG;
+    return batch_thermal(b, nphot, seeds, opts, stats);
+}
+
+int rt_batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+                  const rt_thermal_opts *opts, const rt_step_item *items, int use_graph,
+                  rt_thermal_stats *stats, double *step_ms) {
+    if (!b) return RT_E_ARG;
+    return batch_step(b, nphot, seeds, opts, items, use_graph, stats, step_ms);
 }
 
 }  // extern "C"

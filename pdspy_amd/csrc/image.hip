@@ -132,7 +132,12 @@ __global__ void __launch_bounds__(64) k_image_adj(CamParams C, const float *dimg
 
 // stars: one thread per star; adds F exp(-tau) to the pixel holding the star,
 // or (adjoint) dL/dalpha along the star's line of sight
-__global__ void k_stars(CamParams C, int nstar, const float *spos, const float *sflux, float *img,
+// point-source fluxes (erg/s/cm^2/Hz/sr over one pixel) passed by value
+struct StarArg {
+    float f[RT_MAX_STAR];
+};
+
+__global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux, float *img,
                         const float *dimg, double *g_alpha) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nstar) return;
@@ -163,7 +168,7 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, const float *
         if (cross_wall(C.g, p, tw, kind)) break;
         unstick(C.g, p, tw, rcell, nzero);
     }
-    float Is = sflux[s] * __expf(-tau);
+    float Is = sflux.f[s] * __expf(-tau);
     if (img) {
         atomicAdd(&img[j * C.nx + k], Is);
         return;
@@ -238,6 +243,19 @@ static double kappa_at(const rt_model *m, const std::vector<double> &tab, int s,
     return std::exp(std::log(k[i]) + w * (std::log(k[i + 1]) - std::log(k[i])));
 }
 
+// per-cell scratch of the camera kernels (allocated once per model)
+int image_buffers(rt_model *m) {
+    rt_ctx *ctx = m->ctx;
+    size_t nc = (size_t)m->ncell;
+    if (!m->d_alpha) {
+        HIPCHK(ctx, hipMalloc(&m->d_alpha, nc * sizeof(float)));
+        HIPCHK(ctx, hipMalloc(&m->d_src, nc * sizeof(float)));
+        HIPCHK(ctx, hipMalloc(&m->d_galpha, nc * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&m->d_gsrc, nc * sizeof(double)));
+    }
+    return RT_OK;
+}
+
 static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg &kap, float &pre,
                         float &hnu_k) {
     rt_ctx *ctx = m->ctx;
@@ -268,26 +286,21 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
     double nu = 2.99792458e10 / (cam->lam_um * 1e-4);
     pre = (float)(2.0 * 6.6260755e-27 * nu * nu * nu / (2.99792458e10 * 2.99792458e10));
     hnu_k = (float)(6.6260755e-27 * nu / 1.380658e-16);
-    size_t nc = (size_t)m->ncell;
-    if (!m->d_alpha) {
-        HIPCHK(ctx, hipMalloc(&m->d_alpha, nc * sizeof(float)));
-        HIPCHK(ctx, hipMalloc(&m->d_src, nc * sizeof(float)));
-        HIPCHK(ctx, hipMalloc(&m->d_galpha, nc * sizeof(double)));
-        HIPCHK(ctx, hipMalloc(&m->d_gsrc, nc * sizeof(double)));
-    }
+    int rc = image_buffers(m);
+    if (rc) return rc;
     C.alpha = m->d_alpha;
     C.src = m->d_src;
     return 0;
 }
 
-static void star_fluxes(const rt_model *m, const rt_camera *cam, std::vector<float> &fl) {
+static void star_fluxes(const rt_model *m, const rt_camera *cam, StarArg &fl) {
     double nu = 2.99792458e10 / (cam->lam_um * 1e-4);
-    fl.resize(m->nstar);
-    for (int s = 0; s < m->nstar; ++s) {
+    for (int s = 0; s < RT_MAX_STAR; ++s) fl.f[s] = 0.f;
+    for (int s = 0; s < m->nstar && s < RT_MAX_STAR; ++s) {
         double R = m->stars[5 * s], T = m->stars[5 * s + 1];
         double x = 6.6260755e-27 * nu / (1.380658e-16 * T);
         double B = 2.0 * 6.6260755e-27 * nu * nu * nu / (2.99792458e10 * 2.99792458e10) / std::expm1(x);
-        fl[s] = (float)(M_PI * R * R * B / (cam->pix_cm * cam->pix_cm));
+        fl.f[s] = (float)(M_PI * R * R * B / (cam->pix_cm * cam->pix_cm));
     }
 }
 
@@ -307,15 +320,10 @@ int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *
     int npx = cam->nx * cam->ny;
     hipLaunchKernelGGL(k_image, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_img);
     if (cam->incl_star && cam->mode == 0 && m->nstar > 0) {
-        std::vector<float> fl;
+        StarArg fl;
         star_fluxes(m, cam, fl);
-        float *d_fl = nullptr;
-        HIPCHK(ctx, hipMalloc(&d_fl, sizeof(float) * m->nstar));
-        HIPCHK(ctx, hipMemcpyAsync(d_fl, fl.data(), sizeof(float) * m->nstar, hipMemcpyHostToDevice, sm));
-        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, d_fl, d_img,
+        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl, d_img,
                            (const float *)nullptr, (double *)nullptr);
-        HIPCHK(ctx, hipStreamSynchronize(sm));
-        HIPCHK(ctx, hipFree(d_fl));
     }
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
@@ -342,15 +350,10 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     hipLaunchKernelGGL(k_image_adj, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_dimg, m->d_galpha,
                        m->d_gsrc);
     if (cam->incl_star && m->nstar > 0) {
-        std::vector<float> fl;
+        StarArg fl;
         star_fluxes(m, cam, fl);
-        float *d_fl = nullptr;
-        HIPCHK(ctx, hipMalloc(&d_fl, sizeof(float) * m->nstar));
-        HIPCHK(ctx, hipMemcpyAsync(d_fl, fl.data(), sizeof(float) * m->nstar, hipMemcpyHostToDevice, sm));
-        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, d_fl,
+        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl,
                            (float *)nullptr, d_dimg, m->d_galpha);
-        HIPCHK(ctx, hipStreamSynchronize(sm));
-        HIPCHK(ctx, hipFree(d_fl));
     }
     hipLaunchKernelGGL(k_cam_post, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, m->d_gsrc,
                        rho, m->nspec, m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k,

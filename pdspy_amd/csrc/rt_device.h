@@ -125,21 +125,28 @@ __device__ __forceinline__ void locate_angles(const DevGrid &g, Pos &p) {
     }
 }
 
+// Cell of the position.  Every index is written once at the end (separate
+// early returns writing different fields would keep Pos out of registers).
 __device__ __forceinline__ void locate(const DevGrid &g, Pos &p) {
     float rr = p.x * p.x + p.y * p.y + p.z * p.z;
-    if (rr < g.rw2[0]) {
-        p.ir = -1;
-        p.hemi = (g.mirror && p.z < 0.f) ? -1 : 1;
-        p.it = 0;
-        p.ip = 0;
-        return;
+    const bool hole = rr < g.rw2[0], outside = !hole && rr >= g.rw2[g.nr];
+    int ir;
+    Pos q = p;
+    q.it = 0;
+    q.ip = 0;
+    q.hemi = (g.mirror && p.z < 0.f) ? -1 : 1;
+    if (hole) {
+        ir = -1;
+    } else if (outside) {
+        ir = g.nr;
+    } else {
+        ir = search_walls(g.rw, g.nr, sqrtf(rr));
+        locate_angles(g, q);
     }
-    if (rr >= g.rw2[g.nr]) {
-        p.ir = g.nr;
-        return;
-    }
-    p.ir = search_walls(g.rw, g.nr, sqrtf(rr));
-    locate_angles(g, p);
+    p.ir = ir;
+    p.it = q.it;
+    p.ip = q.ip;
+    p.hemi = q.hemi;
 }
 
 __device__ __forceinline__ float sph_out(float b, float rr, float R2) {
@@ -315,44 +322,34 @@ __device__ __forceinline__ void snap_radius(Pos &p, float R) {
     }
 }
 
-// move by t, cross the wall; returns true when the packet leaves the grid
+// move by t, cross the wall; returns true when the packet leaves the grid.
+// The cell indices change by +-1 through selects (no branch writes through a
+// pointer into the packet state, which would keep it out of registers).
 __device__ __forceinline__ bool cross_wall(const DevGrid &g, Pos &p, float t, int kind) {
     p.x = fmaf(t, p.dx, p.x);
     p.y = fmaf(t, p.dy, p.y);
     p.z = fmaf(t, p.dz, p.z);
-    switch (kind) {
-    case W_HOLE:
+    if (kind == W_HOLE) {
         snap_radius(p, g.rw[0]);
         p.ir = 0;
         locate_angles(g, p);
         return false;
-    case W_ROUT:
-        p.ir++;
-        if (p.ir >= g.nr) return true;
-        snap_radius(p, g.rw[p.ir]);
-        return false;
-    case W_RIN:
-        snap_radius(p, g.rw[p.ir]);
-        p.ir--;
-        return false;
-    case W_TLO:
-        p.it--;
-        return false;
-    case W_THI:
-        if (g.mirror && p.it + 1 == g.nt) {
-            p.z = 0.f;
-            p.hemi = -p.hemi;
-        } else {
-            p.it++;
-        }
-        return false;
-    case W_PLO:
-        p.ip = (p.ip == 0) ? g.np - 1 : p.ip - 1;
-        return false;
-    case W_PHI:
-        p.ip = (p.ip + 1 == g.np) ? 0 : p.ip + 1;
-        return false;
     }
+    const bool mirror_hit = kind == W_THI && g.mirror && p.it + 1 == g.nt;
+    const int dir_r = (kind == W_ROUT ? 1 : 0) - (kind == W_RIN ? 1 : 0);
+    const int dir_t = (kind == W_THI && !mirror_hit ? 1 : 0) - (kind == W_TLO ? 1 : 0);
+    const int dir_p = (kind == W_PHI ? 1 : 0) - (kind == W_PLO ? 1 : 0);
+    p.ir += dir_r;
+    if (p.ir >= g.nr) return true;
+    // the wall crossed: inner wall of the new cell going out, outer going in
+    if (dir_r != 0) snap_radius(p, g.rw[dir_r > 0 ? p.ir : p.ir + 1]);
+    p.it += dir_t;
+    if (mirror_hit) {
+        p.z = 0.f;
+        p.hemi = -p.hemi;
+    }
+    int ip = p.ip + dir_p;
+    p.ip = ip < 0 ? g.np - 1 : (ip >= g.np ? 0 : ip);
     return false;
 }
 

@@ -11,6 +11,7 @@
 #define RT_MAX_SPEC 8    // species per model
 #define RT_MAX_GROUP 4   // distinct absorption opacities per model
 #define RT_MAX_LAM 255   // wavelength bins (guide tables are uint8)
+#define RT_MAX_STAR 16   // point stars per model
 #define RT_NGUIDE 128    // guide-table entries per CDF row
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
@@ -113,6 +114,9 @@ struct rt_model {
     double *d_galpha = nullptr, *d_gsrc = nullptr;
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
+    void *h_run = nullptr;        // host parameter block of the thermal kernel (ThermParams)
+    void *d_run = nullptr;        // its device copy
+    std::vector<unsigned long long> h_est;  // epoch starts of the last planned run
     double eps_last = 0;          // packet energy of the last thermal run
     std::vector<double> timeline; // last run: ms from kernel start to each snapshot, then the end
     bool have_T = false;
@@ -127,7 +131,45 @@ int set_err(rt_ctx *ctx, int code, const char *fmt, ...);
                            hipGetErrorString(e_), __FILE__, __LINE__);                     \
     } while (0)
 
+// a planned thermal run (thermal.hip)
+struct ThermPlan {
+    int E = 0;            // epochs
+    double eps = 0;       // packet energy
+    bool count_visits = false;
+    int lds_wave = 0;     // LDS floats per wavefront
+};
+
 int build_tables(rt_model *m, std::string &err);
+int thermal_plan(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
+                 ThermPlan &rp);
+// upload_est: also copy the epoch starts (a captured step has them uploaded once)
+int thermal_enqueue_reset(rt_model *m, const ThermPlan &rp, hipStream_t sm, bool upload_est);
+void thermal_enqueue_finish(rt_model *m, const ThermPlan &rp, hipStream_t sm);
+// d_runs: device array of nrun parameter blocks, or (nrun = 1) h_run1: the
+// single run's host block passed as a kernel argument
+int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, int nrun,
+                           int lds_wave, int max_s, int max_g, const rt_thermal_opts *o,
+                           unsigned long long *d_abort, unsigned int *d_done, hipStream_t sm);
+int thermal_collect(rt_model *m, const ThermPlan &rp, bool aborted, float kms, float tot,
+                    rt_thermal_stats *st);
+size_t thermal_params_size();
+int thermal_params_copy(const rt_model *m, void *dst_host);
+void thermal_params_set_done(rt_model *m, unsigned int *d_done);
+struct PipeUpload {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+PipeUpload thermal_epoch_starts(rt_model *m);
+int image_buffers(rt_model *m);
+// batches (batch.hip)
+int batch_create(rt_ctx *ctx, rt_model *const *models, int n, rt_batch **out);
+int batch_destroy(rt_batch *b);
+int batch_thermal(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+                  const rt_thermal_opts *o, rt_thermal_stats *stats);
+int batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
+               const rt_thermal_opts *o, const rt_step_item *items, int use_graph,
+               rt_thermal_stats *stats, double *step_ms);
 rt::DevGrid dev_grid(const rt_model *m);
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                 rt_thermal_stats *st);
@@ -140,5 +182,7 @@ int vis_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const d
 int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx, int ny,
                     double dxy, const double *d_u, const double *d_v, int K, double dra,
                     double ddec, float *d_g_img);
+// ws: nullable workspace of chisq_workspace(K) doubles (else allocated stream-ordered)
 int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, const float *mre,
-              const float *mim, int K, double *out, float *gre, float *gim);
+              const float *mim, int K, double *out, float *gre, float *gim, double *ws);
+size_t chisq_workspace(int K);

@@ -44,6 +44,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 
 #include "rt_internal.h"
 
@@ -79,7 +81,6 @@ struct ThermParams {
     unsigned long long *cnt;
     uint64_t seed, max_events;
     uint32_t max_steps;  // wall crossings per packet
-    uint64_t max_ticks;
     int mrw, mrw_count;
     float mrw_tau, mrw_gamma;
     // epoch pipeline
@@ -95,6 +96,8 @@ struct ThermParams {
     unsigned int *snap_ready;            // [E]
     unsigned int *next_snap;             // [1]
     unsigned long long *snap_time;       // [E] clock at each snapshot, then t_begin, t_end
+    unsigned long long *ep_cur;          // [1] first epoch with unclaimed packets
+    unsigned int *runs_done;             // [1] of the batch: runs with all snapshots out
     unsigned long long *tslot;           // [E+1][slot] tallies by deposit slot (last: all later)
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
     float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
@@ -102,6 +105,15 @@ struct ThermParams {
     const double *econv;                 // [cell][G]
     const double *lnq;                   // [G][ntemp]
     double eps, lnt0, dlnt;
+};
+
+// one launch over a batch of runs
+struct BatchArgs {
+    int nrun;                      // runs (parameter blocks passed beside, device memory)
+    int lds_wave;                  // floats of LDS per wavefront
+    uint64_t max_ticks;            // watchdog (wall-clock ticks)
+    unsigned long long *abort;     // [1] watchdog flag
+    unsigned int *runs_done;       // [1] runs whose last snapshot is published
 };
 
 // ---------------------------------------------------------------- helpers
@@ -127,6 +139,12 @@ __device__ __forceinline__ void count_up(unsigned long long *p) {
 }
 
 __device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
+// value of lane `src` (wave-uniform src), known to the compiler to be uniform
+__device__ __forceinline__ unsigned long long lane64(unsigned long long v, int src) {
+    unsigned lo = __builtin_amdgcn_readlane((unsigned)v, src);
+    unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), src);
+    return ((unsigned long long)hi << 32) | lo;
+}
 
 // smallest i with u <= cdf[i], starting from the guide entry
 __device__ __forceinline__ int sample_guided(const float *cdf, const unsigned char *guide,
@@ -222,68 +240,171 @@ __device__ bool snapshot_help(const ThermParams &P, int lane) {
             st_rlx32(&P.snap_ready[k], 1u);
             st_rlx32(P.next_snap, k + 1u);
             st_rlx64(&P.snap_time[k], wall_clock64());
+            if (k + 1u == (unsigned)P.nepoch) atomicAdd(P.runs_done, 1u);
         }
     }
     return true;
 }
 
-// MS, MG: compile-time bounds on species and absorption groups (register caches)
-template <int MS, int MG>
-__global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
+// Wave-level helpers of the batched loop --------------------------------------
+
+// Stages run R's opacities and grid walls into this wave's LDS slice and
+// points g and the table pointers at them.
+__device__ __forceinline__ void stage_run(const ThermParams &R, float *wl, int lane, DevGrid &g,
+                                          float *&s_ka, float *&s_ks, float *&s_gka,
+                                          float *&s_g, float *&s_invdr) {
+    const int S = R.nspec, NL = R.nlam, G = R.ngroup;
+    const int NR = R.g.nr, NT = R.g.nt, NP = R.g.np;
+    s_ka = wl;
+    s_ks = s_ka + S * NL;
+    s_gka = s_ks + S * NL;
+    s_g = s_gka + G * NL;
+    float *rw = s_g + (R.hg ? S * NL : 0);
+    float *rw2 = rw + NR + 1;
+    float *tw = rw2 + NR + 1;
+    float *ct = tw + NT + 1;
+    float *st = ct + NT + 1;
+    float *pw = st + NT + 1;
+    float *cp = pw + NP + 1;
+    float *sp = cp + NP + 1;
+    s_invdr = sp + NP + 1;
+    for (int i = lane; i < S * NL; i += 64) {
+        s_ka[i] = R.ka[i];
+        s_ks[i] = R.ks[i];
+        if (R.hg) s_g[i] = R.gg[i];
+    }
+    for (int i = lane; i < G * NL; i += 64) s_gka[i] = R.gka[i] * R.ginvk[i / NL];
+    for (int i = lane; i <= NR; i += 64) {
+        rw[i] = R.g.rw[i];
+        rw2[i] = R.g.rw2[i];
+        if (i < NR) s_invdr[i] = R.invdr[i];
+    }
+    for (int i = lane; i <= NT; i += 64) {
+        tw[i] = R.g.tw[i];
+        ct[i] = R.g.ct[i];
+        st[i] = R.g.st[i];
+    }
+    for (int i = lane; i <= NP; i += 64) {
+        pw[i] = R.g.pw[i];
+        cp[i] = R.g.cp[i];
+        sp[i] = R.g.sp[i];
+    }
+    // the wave reads what other lanes of it wrote
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    g = R.g;
+    g.rw = rw;
+    g.rw2 = rw2;
+    g.tw = tw;
+    g.ct = ct;
+    g.st = st;
+    g.pw = pw;
+    g.cp = cp;
+    g.sp = sp;
+}
+
+// First run, from `start` on, whose next unclaimed epoch may start now (its
+// snapshot is published); -1 if none.  e_out: that epoch.
+template <bool BATCH>
+__device__ __forceinline__ const ThermParams &run_at(const ThermParams *__restrict__ runs,
+                                                     const ThermParams &P1, int r) {
+    return BATCH ? runs[r] : P1;
+}
+
+template <bool BATCH>
+__device__ __forceinline__ int select_run(const BatchArgs &A,
+                                          const ThermParams *__restrict__ runs,
+                                          const ThermParams &P1, int start, int &e_out) {
+    if (!BATCH) {
+        const unsigned e = uniform32((unsigned)ld_rlx64(P1.ep_cur));
+        if (e < (unsigned)P1.nepoch &&
+            (e < 2u || uniform32(ld_rlx32(&P1.snap_ready[e - 2])) != 0u)) {
+            e_out = (int)e;
+            return 0;
+        }
+        return -1;
+    }
+    int r = start;
+    for (int k = 0; k < A.nrun; ++k) {
+        const ThermParams &R = run_at<BATCH>(runs, P1, r);
+        const unsigned e = uniform32((unsigned)ld_rlx64(R.ep_cur));
+        if (e < (unsigned)R.nepoch &&
+            (e < 2u || uniform32(ld_rlx32(&R.snap_ready[e - 2])) != 0u)) {
+            e_out = (int)e;
+            return r;
+        }
+        r = r + 1 == A.nrun ? 0 : r + 1;
+    }
+    return -1;
+}
+
+// Adds a wave's counters for run R (one atomic per counter per wave) and
+// clears them.
+__device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uint32_t (&v)[8],
+                                             uint32_t &max_pkt, unsigned long long &ovf) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        unsigned long long x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+        if (lane == 0 && x) atomicAdd(&R.cnt[CNT_CROSS + k], x);
+        v[k] = 0;
+    }
+    unsigned long long mp = max_pkt;
+    for (int off = 32; off > 0; off >>= 1) mp = max(mp, (unsigned long long)__shfl_down(mp, off));
+    if (lane == 0 && mp) atomicMax(&R.cnt[CNT_MAXPKT], mp);
+    if (__any((ovf >> 62) != 0ull) && lane == 0) atomicOr(&R.cnt[CNT_OVERFLOW], 1ull);
+    max_pkt = 0;
+    ovf = 0;
+}
+
+// One persistent launch for a batch of runs (one model each).  Each wavefront
+// serves one run at a time: its LDS slice holds that run's opacities and
+// walls; when it has no packet left and the run has none to hand out (all
+// claimed, or the next epoch waits for its snapshot) it moves to the next run
+// that has.  MS, MG: compile-time bounds on species and absorption groups
+// (register caches).  BATCH: the parameter blocks are read from `runs` in
+// device memory; otherwise the single run's block P1 is a kernel argument.
+template <int MS, int MG, bool BATCH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 ? 4 : MS <= 4 ? 3 : 2)))
+k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     extern __shared__ float lds[];
-    const int S = P.nspec, NL = P.nlam, G = P.ngroup;
-    float *s_ka = lds;
-    float *s_ks = s_ka + S * NL;
-    float *s_gka = s_ks + S * NL;
-    float *s_g = s_gka + G * NL;  // only when P.hg
-    // grid walls in LDS: every step reads several of them
-    const int NR = P.g.nr, NT = P.g.nt, NP = P.g.np;
-    float *s_rw = s_g + (P.hg ? S * NL : 0);
-    float *s_rw2 = s_rw + NR + 1;
-    float *s_tw = s_rw2 + NR + 1;
-    float *s_ct = s_tw + NT + 1;
-    float *s_st = s_ct + NT + 1;
-    float *s_pw = s_st + NT + 1;
-    float *s_cp = s_pw + NP + 1;
-    float *s_sp = s_cp + NP + 1;
-    float *s_invdr = s_sp + NP + 1;
-    for (int i = threadIdx.x; i < S * NL; i += blockDim.x) {
-        s_ka[i] = P.ka[i];
-        s_ks[i] = P.ks[i];
-        if (P.hg) s_g[i] = P.gg[i];
-    }
-    for (int i = threadIdx.x; i < G * NL; i += blockDim.x) s_gka[i] = P.gka[i] * P.ginvk[i / NL];
-    for (int i = threadIdx.x; i <= NR; i += blockDim.x) {
-        s_rw[i] = P.g.rw[i];
-        s_rw2[i] = P.g.rw2[i];
-        if (i < NR) s_invdr[i] = P.invdr[i];
-    }
-    for (int i = threadIdx.x; i <= NT; i += blockDim.x) {
-        s_tw[i] = P.g.tw[i];
-        s_ct[i] = P.g.ct[i];
-        s_st[i] = P.g.st[i];
-    }
-    for (int i = threadIdx.x; i <= NP; i += blockDim.x) {
-        s_pw[i] = P.g.pw[i];
-        s_cp[i] = P.g.cp[i];
-        s_sp[i] = P.g.sp[i];
-    }
-    __syncthreads();
-
-    DevGrid g = P.g;
-    g.rw = s_rw;
-    g.rw2 = s_rw2;
-    g.tw = s_tw;
-    g.ct = s_ct;
-    g.st = s_st;
-    g.pw = s_pw;
-    g.cp = s_cp;
-    g.sp = s_sp;
-
     const int lane = __lane_id();
+    float *const wl = lds + (size_t)(threadIdx.x >> 6) * A.lds_wave;  // this wave's slice
     const uint64_t t_start = wall_clock64();
-    const int E = P.nepoch;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st_rlx64(&P.snap_time[RT_MAX_EPOCH], t_start);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int r = 0; r < A.nrun; ++r)
+            st_rlx64(&run_at<BATCH>(runs, P1, r).snap_time[RT_MAX_EPOCH], t_start);
+
+    // the run this wave serves (wave-uniform; -1: none yet) and its staged data
+    int m = -1;
+    const ThermParams *Pp = BATCH ? runs : &P1;
+#define P (*Pp)
+    int S = 0, NL = 0, G = 0, E = 0;
+    DevGrid g = P.g;  // LDS pointers set below, FP64 walls stay in global memory
+    g.rw = g.rw2 = g.tw = g.ct = g.st = g.pw = g.cp = g.sp = wl;
+    float *s_ka = wl, *s_ks = wl, *s_gka = wl, *s_g = wl, *s_invdr = wl;
+    float ginv[MG];  // 2^shift / kmax per group
+    int grpr[MS];    // absorption group of each species
+#pragma unroll
+    for (int q = 0; q < MG; ++q) ginv[q] = 0.f;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) grpr[s] = 0;
+    int scan = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (unsigned)A.nrun);
+    if (!BATCH) {
+        // one run: staged once, before the loop
+        m = 0;
+        scan = 0;
+        stage_run(P1, wl, lane, g, s_ka, s_ks, s_gka, s_g, s_invdr);
+        S = P1.nspec;
+        NL = P1.nlam;
+        G = P1.ngroup;
+        E = P1.nepoch;
+#pragma unroll
+        for (int q = 0; q < MG; ++q) ginv[q] = q < G ? P1.ginvk[q] : 0.f;
+#pragma unroll
+        for (int s = 0; s < MS; ++s) grpr[s] = s < S ? P1.grp[s] : 0;
+    }
 
     Pos p;
     uint64_t pid = 0;   // packet id (Philox counter words 1, 2)
@@ -296,7 +417,7 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
     int pe = 0;                                // epoch of this lane's packet
     int slot = 0;                              // its current deposit slot
     const float *xs = nullptr;                 // its temperature snapshot (null: T = 0)
-    unsigned long long *tb = P.tslot;          // its deposit slot's tallies
+    unsigned long long *tb = nullptr;          // its deposit slot's tallies
     int ccell = -1;                            // cell whose data sit in registers
     float aRc = -1.f;                          // its Rosseland extinction (-1: not yet)
     float rcache[MS];
@@ -306,19 +427,17 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 #pragma unroll
     for (int q = 0; q < MG; ++q) xcache[q] = -1.f;
     unsigned long long acc[MG], aold[MG];
-    unsigned long long ovf = 0;  // OR of tally values seen and of scaled adds (overflow check)
-    float ginv[MG];              // 2^shift / kmax per group
+    unsigned long long ovf = 0;  // OR of tally values seen and of adds (overflow check)
 #pragma unroll
     for (int q = 0; q < MG; ++q) {
         acc[q] = 0;
         aold[q] = 0;
-        ginv[q] = q < G ? P.ginvk[q] : 0.f;
     }
-    int grpr[MS];                // absorption group of each species
-#pragma unroll
-    for (int s = 0; s < MS; ++s) grpr[s] = s < S ? P.grp[s] : 0;
-    uint32_t n_cross = 0, n_abs = 0, n_scat = 0, n_esc = 0, n_kill = 0, n_ev = 0, n_mrw = 0;
-    uint32_t n_nudge = 0;
+    // per-run wave counters: crossings, absorptions, scatterings, escapes,
+    // kills, interactions, MRW steps, nudges
+    uint32_t cnt8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t &n_cross = cnt8[0], &n_abs = cnt8[1], &n_scat = cnt8[2], &n_esc = cnt8[3];
+    uint32_t &n_kill = cnt8[4], &n_ev = cnt8[5], &n_mrw = cnt8[6], &n_nudge = cnt8[7];
     uint32_t max_pkt = 0;
     int cur_e = 0;          // wave-uniform: epoch packets are claimed from
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
@@ -350,96 +469,25 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
 
     for (;; ++iter) {
         RT_PHASE(1);
+        // wave-uniform state, stated as such for the compiler (the loop is
+        // re-entered from lane-divergent paths)
+        m = (int)uniform32((unsigned)m);
+        cur_e = (int)uniform32((unsigned)cur_e);
+        epoch_ok = uniform32(epoch_ok ? 1u : 0u) != 0u;
+        scan = (int)uniform32((unsigned)scan);
+        S = (int)uniform32((unsigned)S);
+        NL = (int)uniform32((unsigned)NL);
+        G = (int)uniform32((unsigned)G);
+        E = (int)uniform32((unsigned)E);
+        Pp = BATCH ? runs + (m < 0 ? 0 : m) : &P1;
         const bool any0 = __any(alive);
-        // ---- help with the next temperature snapshot
-        if (!any0 || (iter & 63u) == 0u) snapshot_help(P, lane);
-        RT_PHASE(2);
-
-        // ---- refill dead lanes with packets of the current epoch
-        if (cur_e < E) {
-            if (!epoch_ok) {
-                epoch_ok = cur_e < 2 || uniform32(ld_rlx32(&P.snap_ready[cur_e - 2])) != 0u;
-                if (epoch_ok) order_acquire();
-            }
-            if (epoch_ok) {
-                uint64_t need = __ballot(!alive);
-                if (need) {
-                    int cnt = __popcll(need);
-                    int leader = __ffsll((unsigned long long)need) - 1;
-                    unsigned long long base = 0;
-                    if (lane == leader)
-                        base = atomicAdd(&P.ep_claim[cur_e], (unsigned long long)cnt);
-                    base = __shfl(base, leader);
-                    const unsigned long long e0 = P.ep_start[cur_e];
-                    const unsigned long long size = P.ep_start[cur_e + 1] - e0;
-                    if (!alive) {
-                        uint64_t idx = base + __popcll(need & ((1ull << lane) - 1ull));
-                        if (idx < size) {
-                            pid = e0 + idx;
-                            pe = cur_e;
-                            slot = pe;
-                            xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
-                            tb = P.tslot + (size_t)pe * P.slot;
-                            // emission (random event 0): star, frequency, direction, tau
-                            float u[8];
-                            rng8(P.seed, pid, 0u, u);
-                            krng = 1;
-                            int ks = 0;
-                            while (ks < P.nstar - 1 && u[0] > P.spick[ks]) ++ks;
-                            ifr = sample_guided(P.scdf + (size_t)ks * NL,
-                                                P.sguide + (size_t)ks * RT_NGUIDE, u[1]);
-                            iso_dir(u[2], u[3], p.dx, p.dy, p.dz);
-                            p.x = P.spos[3 * ks];
-                            p.y = P.spos[3 * ks + 1];
-                            p.z = P.spos[3 * ks + 2];
-                            locate(g, p);
-                            nev = 0;
-                            nstep = 0;
-                            nev_cell = 0;
-                            nzero = 0;
-                            slice_left = P.slice;
-                            ccell = -1;
-                            if (p.ir >= g.nr) {
-                                n_esc++;
-                                count_up(&P.slot_out[pe]);
-                            } else {
-                                alive = true;
-                                if (p.ir >= 0 && P.visits)
-                                    atomicAdd(&P.visits[cell_index(g, p)], 1u);
-                                tau = -__logf(u[4]);
-                            }
-                        }
-                    }
-                    {
-                        unsigned long long nst = base >= size ? 0ull : std::min((unsigned long long)cnt, size - base);
-                        if (lane == leader && nst) {
-                            order_release();
-                            atomicAdd(&P.slot_start[cur_e], nst);
-                        }
-                    }
-                    if (base + cnt >= size) {
-                        cur_e++;
-                        epoch_ok = false;
-                    }
-                }
-            }
-        }
-        RT_PHASE(0);
-        const bool any1 = __any(alive);
-        if (!any1) {
-            // nothing to step: all claimed and all snapshots published -> done
-            if (cur_e >= E && uniform32(ld_rlx32(P.next_snap)) >= (unsigned)E) break;
-            n_idle++;
-            __builtin_amdgcn_s_sleep(32);
-            RT_PHASE(3);
-        }
 
         // ---- watchdog (wave-uniform check)
         if ((iter & 1023u) == 0u) {
-            bool stop = (wall_clock64() - t_start) > P.max_ticks ||
-                        uniform32((unsigned)ld_rlx64(&P.cnt[CNT_ABORT])) != 0u;
+            bool stop = (wall_clock64() - t_start) > A.max_ticks ||
+                        uniform32((unsigned)ld_rlx64(A.abort)) != 0u;
             if (stop) {
-                if (lane == 0) atomicOr(&P.cnt[CNT_ABORT], 1ull);
+                if (lane == 0) atomicOr(A.abort, 1ull);
                 if (alive) {
                     if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, aold, ovf);
                     max_pkt = max(max_pkt, nev + nstep);
@@ -449,6 +497,133 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
                 break;
             }
         }
+
+        // ---- a wave without packets whose run has none to hand out moves on
+        if (!any0) {
+            bool can = m >= 0 && cur_e < E;
+            if (can && !epoch_ok) {
+                epoch_ok = cur_e < 2 || uniform32(ld_rlx32(&P.snap_ready[cur_e - 2])) != 0u;
+                if (epoch_ok) order_acquire();
+                can = epoch_ok;
+            }
+            if (!can) {
+                int e_new = 0;
+                const int r =
+                    (int)uniform32((unsigned)select_run<BATCH>(A, runs, P1, scan, e_new));
+                e_new = (int)uniform32((unsigned)e_new);
+                if (r >= 0) {
+                    order_acquire();
+                    if (BATCH && r != m) {
+                        if (m >= 0) {
+#pragma unroll
+                            for (int q = 0; q < MG; ++q) {
+                                ovf |= aold[q];
+                                aold[q] = 0;
+                            }
+                            flush_counts(P, lane, cnt8, max_pkt, ovf);
+                        }
+                        m = r;
+                        Pp = BATCH ? runs + m : &P1;
+                        stage_run(P, wl, lane, g, s_ka, s_ks, s_gka, s_g, s_invdr);
+                        S = P.nspec;
+                        NL = P.nlam;
+                        G = P.ngroup;
+                        E = P.nepoch;
+#pragma unroll
+                        for (int q = 0; q < MG; ++q) ginv[q] = q < G ? P.ginvk[q] : 0.f;
+#pragma unroll
+                        for (int s = 0; s < MS; ++s) grpr[s] = s < S ? P.grp[s] : 0;
+                        cur_e = e_new;
+                    } else {
+                        cur_e = max(cur_e, e_new);
+                    }
+                    epoch_ok = true;
+                    scan = r;
+                } else {
+                    // nothing to hand out anywhere: help a snapshot, finish or wait
+                    if (uniform32(ld_rlx32(A.runs_done)) >= (unsigned)A.nrun) break;
+                    snapshot_help(run_at<BATCH>(runs, P1, scan), lane);
+                    scan = scan + 1 == A.nrun ? 0 : scan + 1;
+                    n_idle++;
+                    __builtin_amdgcn_s_sleep(32);
+                    RT_PHASE(3);
+                    continue;
+                }
+            }
+        }
+
+        // ---- help with the run's next temperature snapshot
+        if ((iter & 63u) == 0u) snapshot_help(P, lane);
+        RT_PHASE(2);
+
+        // ---- refill dead lanes with packets of the current epoch
+        if (cur_e < E && !epoch_ok) {
+            epoch_ok = cur_e < 2 || uniform32(ld_rlx32(&P.snap_ready[cur_e - 2])) != 0u;
+            if (epoch_ok) order_acquire();
+        }
+        if (cur_e < E && epoch_ok) {
+            uint64_t need = __ballot(!alive);
+            if (need) {
+                int cnt = __popcll(need);
+                int leader = (int)uniform32((unsigned)(__ffsll((unsigned long long)need) - 1));
+                unsigned long long base = 0;
+                if (lane == leader) base = atomicAdd(&P.ep_claim[cur_e], (unsigned long long)cnt);
+                base = lane64(base, leader);
+                const unsigned long long e0 = P.ep_start[cur_e];
+                const unsigned long long size = P.ep_start[cur_e + 1] - e0;
+                if (!alive) {
+                    uint64_t idx = base + __popcll(need & ((1ull << lane) - 1ull));
+                    if (idx < size) {
+                        pid = e0 + idx;
+                        pe = cur_e;
+                        slot = pe;
+                        xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
+                        tb = P.tslot + (size_t)pe * P.slot;
+                        // emission (random event 0): star, frequency, direction, tau
+                        float u[8];
+                        rng8(P.seed, pid, 0u, u);
+                        krng = 1;
+                        int ks = 0;
+                        while (ks < P.nstar - 1 && u[0] > P.spick[ks]) ++ks;
+                        ifr = sample_guided(P.scdf + (size_t)ks * NL,
+                                            P.sguide + (size_t)ks * RT_NGUIDE, u[1]);
+                        iso_dir(u[2], u[3], p.dx, p.dy, p.dz);
+                        p.x = P.spos[3 * ks];
+                        p.y = P.spos[3 * ks + 1];
+                        p.z = P.spos[3 * ks + 2];
+                        locate(g, p);
+                        nev = 0;
+                        nstep = 0;
+                        nev_cell = 0;
+                        nzero = 0;
+                        slice_left = P.slice;
+                        ccell = -1;
+                        if (p.ir >= g.nr) {
+                            n_esc++;
+                            count_up(&P.slot_out[pe]);
+                        } else {
+                            alive = true;
+                            if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
+                            tau = -__logf(u[4]);
+                        }
+                    }
+                }
+                {
+                    unsigned long long nst =
+                        base >= size ? 0ull : std::min((unsigned long long)cnt, size - base);
+                    if (lane == leader && nst) {
+                        order_release();
+                        atomicAdd(&P.slot_start[cur_e], nst);
+                    }
+                }
+                if (base + cnt >= size) {
+                    cur_e++;
+                    epoch_ok = false;
+                    if (lane == leader) atomicMax(P.ep_cur, (unsigned long long)cur_e);
+                }
+            }
+        }
+        RT_PHASE(0);
         if (!alive) continue;
 
         // ---- one step: to the next wall or to the next interaction
@@ -654,33 +829,29 @@ __global__ void __launch_bounds__(256) k_thermal(ThermParams P) {
             if (p.ir >= 0 && P.visits) atomicAdd(&P.visits[cell_index(g, p)], 1u);
         }
     }
-    // ---- statistics: wave-reduce and one atomic per counter per wave
-    unsigned long long v[8] = {n_cross, n_abs, n_scat, n_esc, n_kill, n_ev, n_mrw, n_nudge};
+    // ---- statistics of the last run served, batch counters
+    if (m >= 0) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        unsigned long long x = v[k];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-        if (lane == 0 && x) atomicAdd(&P.cnt[CNT_CROSS + k], x);
+        for (int q = 0; q < MG; ++q) ovf |= aold[q];
+        flush_counts(P, lane, cnt8, max_pkt, ovf);
     }
-    unsigned long long mp = max_pkt;
-    for (int off = 32; off > 0; off >>= 1) mp = max(mp, (unsigned long long)__shfl_down(mp, off));
-    if (lane == 0 && mp) atomicMax(&P.cnt[CNT_MAXPKT], mp);
-#pragma unroll
-    for (int q = 0; q < MG; ++q) ovf |= aold[q];
-    if (__any((ovf >> 62) != 0ull) && lane == 0) atomicOr(&P.cnt[CNT_OVERFLOW], 1ull);
     if (lane == 0) {
-        atomicMax(&P.snap_time[RT_MAX_EPOCH + 1], (unsigned long long)wall_clock64());
-        atomicAdd(&P.cnt[CNT_WITER], (unsigned long long)iter + 1ull);
-        if (n_idle) atomicAdd(&P.cnt[CNT_IDLE], (unsigned long long)n_idle);
+        for (int r = 0; r < A.nrun; ++r)
+            atomicMax(&run_at<BATCH>(runs, P1, r).snap_time[RT_MAX_EPOCH + 1],
+                      (unsigned long long)wall_clock64());
+        const ThermParams &R0 = run_at<BATCH>(runs, P1, 0);
+        atomicAdd(&R0.cnt[CNT_WITER], (unsigned long long)iter + 1ull);
+        if (n_idle) atomicAdd(&R0.cnt[CNT_IDLE], (unsigned long long)n_idle);
     }
 #if RT_THERMAL_PROFILE
     for (int k = 0; k < RT_NPHASE; ++k) {
         unsigned long long x = tk[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-        if (lane == 0) atomicAdd(&P.cnt[CNT_TK0 + k], x);
+        if (lane == 0) atomicAdd(&run_at<BATCH>(runs, P1, 0).cnt[CNT_TK0 + k], x);
     }
 #endif
 #undef RT_PHASE
+#undef P
 }
 
 // final temperatures from the cumulative tallies
@@ -735,15 +906,44 @@ static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t fir
 }
 
 // pipeline counters and flags: u64 [ep_start E+1 | ep_claim | slot_start |
-// slot_in | slot_out | snap_time | t_begin, t_end] then u32 [snap_claim |
-// snap_done | snap_ready | next_snap]
-#define PIPE_U64 (6 * RT_MAX_EPOCH + 3)
+// slot_in | slot_out | snap_time | t_begin, t_end | ep_cur | abort] then u32
+// [snap_claim | snap_done | snap_ready | next_snap, runs_done]
+#define PIPE_U64 (6 * RT_MAX_EPOCH + 5)
 #define PIPE_U32 (3 * RT_MAX_EPOCH + 4)
+
+struct PipeView {
+    unsigned long long *ep_start, *ep_claim, *slot_start, *slot_in, *slot_out, *snap_time, *ep_cur,
+        *abort;
+    unsigned int *u32base, *snap_claim, *snap_done, *snap_ready, *next_snap, *runs_done;
+};
+
+static PipeView pipe_view(void *d_pipe) {
+    PipeView v;
+    v.ep_start = (unsigned long long *)d_pipe;
+    v.ep_claim = v.ep_start + RT_MAX_EPOCH + 1;
+    v.slot_start = v.ep_claim + RT_MAX_EPOCH;
+    v.slot_in = v.slot_start + RT_MAX_EPOCH;
+    v.slot_out = v.slot_in + RT_MAX_EPOCH;
+    v.snap_time = v.slot_out + RT_MAX_EPOCH;  // then t_begin, t_end
+    v.ep_cur = v.snap_time + RT_MAX_EPOCH + 2;
+    v.abort = v.ep_cur + 1;
+    v.u32base = (unsigned int *)(v.ep_start + PIPE_U64);
+    v.snap_claim = v.u32base;
+    v.snap_done = v.snap_claim + RT_MAX_EPOCH;
+    v.snap_ready = v.snap_done + RT_MAX_EPOCH;
+    v.next_snap = v.snap_ready + RT_MAX_EPOCH;
+    v.runs_done = v.next_snap + 1;
+    return v;
+}
 
 static int ensure_pipe_buffers(rt_model *m) {
     rt_ctx *ctx = m->ctx;
-    if (m->d_pipe) return 0;
-    HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * PIPE_U64 + sizeof(unsigned int) * PIPE_U32));
+    if (!m->d_pipe)
+        HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * PIPE_U64 +
+                                              sizeof(unsigned int) * PIPE_U32));
+    if (!m->d_run) HIPCHK(ctx, hipMalloc(&m->d_run, sizeof(ThermParams)));
+    if (!m->h_run) m->h_run = std::calloc(1, sizeof(ThermParams));
+    if (!m->h_run) return set_err(ctx, RT_E_NOMEM, "host allocation failed");
     return 0;
 }
 
@@ -774,10 +974,18 @@ __global__ void k_add_u64(unsigned long long *dst, const unsigned long long *src
     if (i < n) dst[i] += src[i];
 }
 
-int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
-                rt_thermal_stats *st) {
+// LDS floats one wavefront needs for run m's opacities and walls
+static int lds_floats(const rt_model *m) {
+    const int S = m->nspec, NL = m->nlam, G = m->tab.ngroup;
+    return S * NL * (m->has_g ? 3 : 2) + G * NL + 2 * (m->nr + 1) + m->nr + 3 * (m->nt + 1) +
+           3 * (m->np + 1);
+}
+
+// Validates the options, lays out the epochs, allocates the run's buffers and
+// fills its parameter block (host copy in m->h_run).
+int thermal_plan(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
+                 ThermPlan &rp) {
     rt_ctx *ctx = m->ctx;
-    hipStream_t sm = ctx->stream;
     rt_thermal_opts opt{};
     if (o) opt = *o;
     if (nphot == 0) return set_err(ctx, RT_E_ARG, "nphot must be > 0");
@@ -785,50 +993,20 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     const uint64_t max_events = opt.max_events ? opt.max_events : 1000000ull;
     uint64_t first = opt.epoch_first ? opt.epoch_first : std::max<uint64_t>(1024, nphot / 4096);
     double growth = opt.epoch_growth > 1.0 ? opt.epoch_growth : 1.25;
-    double max_s = opt.max_seconds > 0 ? opt.max_seconds : 60.0;
-    int bs = opt.block_size > 0 ? opt.block_size : 256;
-    if (bs % 64 || bs > 256) return set_err(ctx, RT_E_ARG, "block_size must be 64..256, multiple of 64");
     const int G = m->tab.ngroup;
     const size_t ncg = (size_t)m->ncell * G;
     const double eps = m->tab.lum / (double)nphot;
-    std::vector<unsigned long long> est = epoch_starts(nphot, first, growth);
-    const int E = (int)est.size() - 1;
+    m->h_est = epoch_starts(nphot, first, growth);
+    const int E = (int)m->h_est.size() - 1;
     if (E > RT_MAX_EPOCH)
         return set_err(ctx, RT_E_LIMIT, "%d epochs > %d: raise epoch_first or epoch_growth", E,
                        RT_MAX_EPOCH);
     if (ensure_pipe_buffers(m) || ensure_slot_buffers(m, E)) return RT_E_HIP;
-    const size_t sl = snap_slot(ncg);
     const uint64_t slice = opt.slice_events ? opt.slice_events : 10000ull;
+    PipeView pv = pipe_view(m->d_pipe);
 
-    // pipeline state layout in d_pipe
-    unsigned long long *ep_start = (unsigned long long *)m->d_pipe;
-    unsigned long long *ep_claim = ep_start + RT_MAX_EPOCH + 1;
-    unsigned long long *slot_start = ep_claim + RT_MAX_EPOCH;
-    unsigned long long *slot_in = slot_start + RT_MAX_EPOCH;
-    unsigned long long *slot_out = slot_in + RT_MAX_EPOCH;
-    unsigned long long *snap_time = slot_out + RT_MAX_EPOCH;  // then t_begin, t_end
-    unsigned int *u32base = (unsigned int *)(ep_start + PIPE_U64);
-    unsigned int *snap_claim = u32base;
-    unsigned int *snap_done = snap_claim + RT_MAX_EPOCH;
-    unsigned int *snap_ready = snap_done + RT_MAX_EPOCH;
-    unsigned int *next_snap = snap_ready + RT_MAX_EPOCH;
-
-    // reset tallies, counters, pipeline state (fill kernels: no memset nodes if captured)
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, ncg, 0ull);
-    const size_t nts = sl * (size_t)(E + 1);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nts, 256)), dim3(256), 0, sm, m->d_tally2, nts, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)CNT_N, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(5 * RT_MAX_EPOCH + 2, 256)), dim3(256), 0, sm, ep_claim,
-                       (size_t)(5 * RT_MAX_EPOCH + 2), 0ull);
-    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(PIPE_U32, 256)), dim3(256), 0, sm, u32base,
-                       (size_t)PIPE_U32, 0u);
-    HIPCHK(ctx, hipMemcpyAsync(ep_start, est.data(), sizeof(unsigned long long) * est.size(),
-                               hipMemcpyHostToDevice, sm));
-    if (opt.count_visits)
-        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, m->d_visits,
-                           (size_t)m->ncell, 0u);
-
-    ThermParams P{};
+    ThermParams &P = *(ThermParams *)m->h_run;
+    P = ThermParams{};
     P.g = dev_grid(m);
     P.nspec = m->nspec;
     P.ngroup = G;
@@ -858,7 +1036,6 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.seed = seed;
     P.max_events = max_events;
     P.max_steps = 100000000u;
-    P.max_ticks = (uint64_t)(max_s * (double)ctx->wallclock_khz * 1000.0);
     P.mrw = opt.mrw;
     P.mrw_count = opt.mrw_count_trigger > 0 ? opt.mrw_count_trigger : 100;
     P.mrw_tau = (float)(opt.mrw_tauthres > 0 ? opt.mrw_tauthres : 10.0);
@@ -866,17 +1043,19 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.nepoch = E;
     P.ncg = (int)ncg;
     P.nchunk = (int)((ncg + 63) / 64);
-    P.ep_start = ep_start;
-    P.ep_claim = ep_claim;
-    P.slot_start = slot_start;
-    P.slot_in = slot_in;
-    P.slot_out = slot_out;
+    P.ep_start = pv.ep_start;
+    P.ep_claim = pv.ep_claim;
+    P.slot_start = pv.slot_start;
+    P.slot_in = pv.slot_in;
+    P.slot_out = pv.slot_out;
     P.slice = (uint32_t)std::min<uint64_t>(slice, 0x7fffffffull);
-    P.snap_claim = snap_claim;
-    P.snap_done = snap_done;
-    P.snap_ready = snap_ready;
-    P.snap_time = snap_time;
-    P.next_snap = next_snap;
+    P.snap_claim = pv.snap_claim;
+    P.snap_done = pv.snap_done;
+    P.snap_ready = pv.snap_ready;
+    P.next_snap = pv.next_snap;
+    P.snap_time = pv.snap_time;
+    P.ep_cur = pv.ep_cur;
+    P.runs_done = pv.runs_done;
     P.tslot = m->d_tally2;
     P.cum = m->d_tally;
     P.xsnap = m->d_xring;
@@ -886,18 +1065,78 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     P.eps = eps;
     P.lnt0 = m->tab.lnt0;
     P.dlnt = m->tab.dlnt;
-    size_t lds = sizeof(float) * ((size_t)m->nspec * m->nlam * (P.hg ? 3 : 2) + (size_t)G * m->nlam +
-                                  2 * (size_t)(m->nr + 1) + (size_t)m->nr + 3 * (size_t)(m->nt + 1) +
-                                  3 * (size_t)(m->np + 1));
-    if (lds > 64 * 1024) return set_err(ctx, RT_E_LIMIT, "thermal kernel LDS %zu B > 64 KiB", lds);
-    // instantiation by species / group counts (register caches sized to fit)
-    void (*kern)(ThermParams);
-    if (m->nspec <= 2 && G <= 1)
-        kern = k_thermal<2, 1>;
-    else if (m->nspec <= 4 && G <= 2)
-        kern = k_thermal<4, 2>;
+
+    rp.E = E;
+    rp.eps = eps;
+    rp.count_visits = opt.count_visits != 0;
+    rp.lds_wave = lds_floats(m);
+    return RT_OK;
+}
+
+// zero the run's tallies, counters and pipeline state (fill kernels: no memset
+// nodes if the stream is being captured) and upload the epoch starts
+int thermal_enqueue_reset(rt_model *m, const ThermPlan &rp, hipStream_t sm, bool upload_est) {
+    rt_ctx *ctx = m->ctx;
+    const int G = m->tab.ngroup;
+    const size_t ncg = (size_t)m->ncell * G;
+    const size_t sl = snap_slot(ncg);
+    PipeView pv = pipe_view(m->d_pipe);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, ncg, 0ull);
+    const size_t nts = sl * (size_t)(rp.E + 1);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(nts, 256)), dim3(256), 0, sm, m->d_tally2, nts, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, m->d_cnt, (size_t)CNT_N, 0ull);
+    const size_t n64 = (size_t)(PIPE_U64 - (RT_MAX_EPOCH + 1));
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(n64, 256)), dim3(256), 0, sm, pv.ep_claim, n64, 0ull);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblk(PIPE_U32, 256)), dim3(256), 0, sm, pv.u32base,
+                       (size_t)PIPE_U32, 0u);
+    if (upload_est)
+        HIPCHK(ctx, hipMemcpyAsync(pv.ep_start, m->h_est.data(),
+                                   sizeof(unsigned long long) * m->h_est.size(),
+                                   hipMemcpyHostToDevice, sm));
+    if (rp.count_visits)
+        hipLaunchKernelGGL(k_fill_u32, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, m->d_visits,
+                           (size_t)m->ncell, 0u);
+    return RT_OK;
+}
+
+// final temperatures: cumulative tallies plus the last slot, rows, species
+void thermal_enqueue_finish(rt_model *m, const ThermPlan &rp, hipStream_t sm) {
+    const int G = m->tab.ngroup;
+    const size_t ncg = (size_t)m->ncell * G;
+    const size_t sl = snap_slot(ncg);
+    hipLaunchKernelGGL(k_add_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally,
+                       m->d_tally2 + sl * (size_t)rp.E, ncg);
+    hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, m->d_econv,
+                       rp.eps, m->d_lnq, m->ntemp, m->tab.lnt0, m->tab.dlnt, (int)ncg, G, m->d_xrow,
+                       m->d_Tg);
+    hipLaunchKernelGGL(k_species_T, dim3(nblk((size_t)m->nspec * m->ncell, 256)), dim3(256), 0, sm,
+                       m->d_Tg, m->d_rhopos, m->d_grp, m->nspec, G, m->ncell, m->d_Tout);
+}
+
+// The launch itself: kernel instance by the largest species / group count,
+// LDS slices per wavefront, blocks per CU from the occupancy.
+int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, int nrun,
+                           int lds_wave, int max_s, int max_g, const rt_thermal_opts *o,
+                           unsigned long long *d_abort, unsigned int *d_done, hipStream_t sm) {
+    rt_thermal_opts opt{};
+    if (o) opt = *o;
+    int bs = opt.block_size > 0 ? opt.block_size : 256;
+    if (bs % 64 || bs > 256) return set_err(ctx, RT_E_ARG, "block_size must be 64..256, multiple of 64");
+    double max_sec = opt.max_seconds > 0 ? opt.max_seconds : 60.0;
+    size_t lds = sizeof(float) * (size_t)lds_wave * (size_t)(bs / 64);
+    if (lds > 64 * 1024)
+        return set_err(ctx, RT_E_LIMIT, "thermal kernel LDS %zu B > 64 KiB (grid too large for "
+                                        "block_size %d)", lds, bs);
+    // one run: its block is a kernel argument (h_run1); several: device array
+    const bool batch = h_run1 == nullptr;
+    void (*kern)(BatchArgs, const ThermParams *, ThermParams);
+    if (max_s <= 2 && max_g <= 1)
+        kern = batch ? k_thermal<2, 1, true> : k_thermal<2, 1, false>;
+    else if (max_s <= 4 && max_g <= 2)
+        kern = batch ? k_thermal<4, 2, true> : k_thermal<4, 2, false>;
     else
-        kern = k_thermal<RT_MAX_SPEC, RT_MAX_GROUP>;
+        kern = batch ? k_thermal<RT_MAX_SPEC, RT_MAX_GROUP, true>
+                     : k_thermal<RT_MAX_SPEC, RT_MAX_GROUP, false>;
     int bpc = opt.blocks_per_cu;
     if (bpc <= 0) {
         int occ = 0;
@@ -905,49 +1144,44 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
             occ = 1;
         bpc = occ;
     }
-    unsigned grid = (unsigned)(ctx->n_cu * bpc);
-
-    hipEvent_t e_begin, e_k0, e_k1, e_end;
-    (void)hipEventCreate(&e_begin);
-    (void)hipEventCreate(&e_k0);
-    (void)hipEventCreate(&e_k1);
-    (void)hipEventCreate(&e_end);
-    HIPCHK(ctx, hipEventRecord(e_begin, sm));
-    HIPCHK(ctx, hipEventRecord(e_k0, sm));
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(bs), lds, sm, P);
+    BatchArgs A{};
+    A.nrun = nrun;
+    A.lds_wave = lds_wave;
+    A.max_ticks = (uint64_t)(max_sec * (double)ctx->wallclock_khz * 1000.0);
+    A.abort = d_abort;
+    A.runs_done = d_done;
+    ThermParams P1{};
+    if (!batch) P1 = *(const ThermParams *)h_run1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(ctx->n_cu * bpc)), dim3(bs), lds, sm, A,
+                       (const ThermParams *)d_runs, P1);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipEventRecord(e_k1, sm));
-    hipLaunchKernelGGL(k_add_u64, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally,
-                       m->d_tally2 + sl * (size_t)E, ncg);
-    hipLaunchKernelGGL(k_snapshot, dim3(nblk(ncg, 256)), dim3(256), 0, sm, m->d_tally, m->d_econv, eps,
-                       m->d_lnq, m->ntemp, m->tab.lnt0, m->tab.dlnt, (int)ncg, G, m->d_xrow, m->d_Tg);
-    hipLaunchKernelGGL(k_species_T, dim3(nblk((size_t)m->nspec * m->ncell, 256)), dim3(256), 0, sm,
-                       m->d_Tg, m->d_rhopos, m->d_grp, m->nspec, G, m->ncell, m->d_Tout);
-    HIPCHK(ctx, hipEventRecord(e_end, sm));
-    HIPCHK(ctx, hipEventSynchronize(e_end));
+    return RT_OK;
+}
+
+// After the stream has drained: statistics, watchdog / overflow / pipeline
+// checks and the timeline of run m.
+int thermal_collect(rt_model *m, const ThermPlan &rp, bool aborted, float kms, float tot,
+                    rt_thermal_stats *st) {
+    rt_ctx *ctx = m->ctx;
+    PipeView pv = pipe_view(m->d_pipe);
     unsigned long long cnt[CNT_N];
     HIPCHK(ctx, hipMemcpy(cnt, m->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost));
     unsigned int nsnap = 0;
-    HIPCHK(ctx, hipMemcpy(&nsnap, next_snap, sizeof(nsnap), hipMemcpyDeviceToHost));
-    float tot = 0, kms = 0;
-    (void)hipEventElapsedTime(&tot, e_begin, e_end);
-    (void)hipEventElapsedTime(&kms, e_k0, e_k1);
-    (void)hipEventDestroy(e_begin);
-    (void)hipEventDestroy(e_k0);
-    (void)hipEventDestroy(e_k1);
-    (void)hipEventDestroy(e_end);
+    HIPCHK(ctx, hipMemcpy(&nsnap, pv.next_snap, sizeof(nsnap), hipMemcpyDeviceToHost));
+    const int E = rp.E;
     {
         std::vector<unsigned long long> tl(RT_MAX_EPOCH + 2);
-        HIPCHK(ctx, hipMemcpy(tl.data(), snap_time, sizeof(unsigned long long) * tl.size(),
+        HIPCHK(ctx, hipMemcpy(tl.data(), pv.snap_time, sizeof(unsigned long long) * tl.size(),
                               hipMemcpyDeviceToHost));
         const double tick_ms = 1.0 / (double)ctx->wallclock_khz;
         const unsigned long long t0 = tl[RT_MAX_EPOCH];
         m->timeline.assign(E + 1, -1.0);
         for (int k = 0; k < E; ++k)
             if (tl[k]) m->timeline[k] = (double)(long long)(tl[k] - t0) * tick_ms;
-        if (tl[RT_MAX_EPOCH + 1]) m->timeline[E] = (double)(long long)(tl[RT_MAX_EPOCH + 1] - t0) * tick_ms;
+        if (tl[RT_MAX_EPOCH + 1])
+            m->timeline[E] = (double)(long long)(tl[RT_MAX_EPOCH + 1] - t0) * tick_ms;
     }
-    m->eps_last = eps;
+    m->eps_last = rp.eps;
     m->have_T = true;
     if (st) {
         st->n_cross = cnt[CNT_CROSS];
@@ -957,10 +1191,10 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->n_killed = cnt[CNT_KILLED];
         st->n_events = cnt[CNT_EVENTS];
         st->n_mrw = cnt[CNT_MRW];
-        st->eps = eps;
+        st->eps = rp.eps;
         st->lum = m->tab.lum;
         st->n_epochs = E;
-        st->watchdog_fired = cnt[CNT_ABORT] ? 1 : 0;
+        st->watchdog_fired = aborted ? 1 : 0;
         st->kernel_ms = kms;
         st->total_ms = tot;
         st->max_epoch_ms = kms;
@@ -970,11 +1204,69 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
         st->idle_iters = cnt[CNT_IDLE];
         for (int k = 0; k < 8; ++k) st->phase_clocks[k] = cnt[CNT_TK0 + k];
     }
-    if (cnt[CNT_ABORT])
-        return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded %.1f s; packets were killed",
-                       max_s);
+    if (aborted)
+        return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded its time budget; packets were killed");
     if (cnt[CNT_OVERFLOW]) return set_err(ctx, RT_E_OVERFLOW, "fixed-point tally overflow");
     if (nsnap != (unsigned)E)
         return set_err(ctx, RT_E_HIP, "thermal pipeline ended after %u of %d epochs", nsnap, E);
     return RT_OK;
+}
+
+int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
+                rt_thermal_stats *st) {
+    rt_ctx *ctx = m->ctx;
+    hipStream_t sm = ctx->stream;
+    ThermPlan rp{};
+    int rc = thermal_plan(m, nphot, seed, o, rp);
+    if (rc) return rc;
+    rc = thermal_enqueue_reset(m, rp, sm, true);
+    if (rc) return rc;
+    PipeView pv = pipe_view(m->d_pipe);
+    hipEvent_t e_begin, e_k0, e_k1, e_end;
+    (void)hipEventCreate(&e_begin);
+    (void)hipEventCreate(&e_k0);
+    (void)hipEventCreate(&e_k1);
+    (void)hipEventCreate(&e_end);
+    HIPCHK(ctx, hipEventRecord(e_begin, sm));
+    HIPCHK(ctx, hipEventRecord(e_k0, sm));
+    rc = thermal_enqueue_launch(ctx, m->d_run, m->h_run, 1, rp.lds_wave, m->nspec, m->tab.ngroup,
+                                o, pv.abort, pv.runs_done, sm);
+    if (rc) return rc;
+    HIPCHK(ctx, hipEventRecord(e_k1, sm));
+    thermal_enqueue_finish(m, rp, sm);
+    HIPCHK(ctx, hipEventRecord(e_end, sm));
+    HIPCHK(ctx, hipEventSynchronize(e_end));
+    float tot = 0, kms = 0;
+    (void)hipEventElapsedTime(&tot, e_begin, e_end);
+    (void)hipEventElapsedTime(&kms, e_k0, e_k1);
+    (void)hipEventDestroy(e_begin);
+    (void)hipEventDestroy(e_k0);
+    (void)hipEventDestroy(e_k1);
+    (void)hipEventDestroy(e_end);
+    unsigned long long ab = 0;
+    HIPCHK(ctx, hipMemcpy(&ab, pv.abort, sizeof(ab), hipMemcpyDeviceToHost));
+    return thermal_collect(m, rp, ab != 0, kms, tot, st);
+}
+
+size_t thermal_params_size() { return sizeof(ThermParams); }
+
+// copies run m's planned parameter block (host) into dst_host
+int thermal_params_copy(const rt_model *m, void *dst_host) {
+    if (!m->h_run) return RT_E_ARG;
+    std::memcpy(dst_host, m->h_run, sizeof(ThermParams));
+    return RT_OK;
+}
+
+// points run m's parameter block at the batch's completion counter
+void thermal_params_set_done(rt_model *m, unsigned int *d_done) {
+    ((ThermParams *)m->h_run)->runs_done = d_done;
+}
+
+// where run m's epoch starts go (the reset of a batch step does not upload them)
+PipeUpload thermal_epoch_starts(rt_model *m) {
+    PipeUpload u;
+    u.dst = pipe_view(m->d_pipe).ep_start;
+    u.src = m->h_est.data();
+    u.bytes = sizeof(unsigned long long) * m->h_est.size();
+    return u;
 }

@@ -185,17 +185,19 @@ int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx,
     return RT_OK;
 }
 
+size_t chisq_workspace(int K) { return (size_t)((K + CHI_BS - 1) / CHI_BS); }
+
 int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, const float *mre,
-              const float *mim, int K, double *out, float *gre, float *gim) {
+              const float *mim, int K, double *out, float *gre, float *gim, double *ws) {
     if (K < 1 || !out) return set_err(ctx, RT_E_ARG, "chisq: invalid arguments");
     if ((gre == nullptr) != (gim == nullptr)) return set_err(ctx, RT_E_ARG, "chisq: gre/gim both or neither");
-    int nb = (K + CHI_BS - 1) / CHI_BS;
-    double *partial = nullptr;
-    HIPCHK(ctx, hipMallocAsync((void **)&partial, sizeof(double) * nb, ctx->stream));
+    int nb = (int)chisq_workspace(K);
+    double *partial = ws;
+    if (!partial) HIPCHK(ctx, hipMallocAsync((void **)&partial, sizeof(double) * nb, ctx->stream));
     hipLaunchKernelGGL(k_chisq, dim3(nb), dim3(CHI_BS), 0, ctx->stream, dre, dim, w, mre, mim, K,
                        partial, gre, gim);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(CHI_BS), 0, ctx->stream, partial, nb, out);
-    HIPCHK(ctx, hipFreeAsync(partial, ctx->stream));
+    if (!ws) HIPCHK(ctx, hipFreeAsync(partial, ctx->stream));
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
 }

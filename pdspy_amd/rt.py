@@ -142,13 +142,7 @@ class DeviceModel:
         self._rho_dev = None
 
     def thermal(self, nphot, seed=0, **opts):
-        o = _lib.ThermalOpts()
-        for k, v in opts.items():
-            if v is None:
-                continue
-            if not hasattr(o, k):
-                raise TypeError("unknown thermal option %r" % k)
-            setattr(o, k, v)
+        o = _thermal_opts(opts)
         st = _lib.ThermalStats()
         self.ctx.sync_stream()
         self.ctx.check(self.ctx.L.rt_thermal(self.h, int(nphot), int(seed), C.byref(o), C.byref(st)))
@@ -214,6 +208,82 @@ class DeviceModel:
             None if rho_dev is None else C.c_void_p(rho_dev.contiguous().data_ptr()),
             C.c_void_p(dimg.data_ptr()), C.c_void_p(g_lr.data_ptr()), C.c_void_p(g_T.data_ptr())))
         return g_lr, g_T
+
+
+def _thermal_opts(opts):
+    o = _lib.ThermalOpts()
+    for k, v in opts.items():
+        if v is None:
+            continue
+        if not hasattr(o, k):
+            raise TypeError("unknown thermal option %r" % k)
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    """Several ``DeviceModel``s run as one sampler step (``rt_batch``): the
+    thermal Monte Carlo of all of them in one persistent launch, then per model
+    image -> visibilities -> chi^2 -> adjoint, optionally replayed from a
+    captured hipGraph.  This is the per-step work of pdspy's fitting loop
+    (run_disk_model.py:31-511) for a whole batch of walkers."""
+
+    def __init__(self, models, device=0):
+        self.ctx = _lib.context(device)
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m.h.value for m in self.models])
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.L.rt_batch_create(self.ctx.h, arr, len(self.models), C.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.ctx.L.rt_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _arrays(self, nphot, seeds):
+        n = len(self.models)
+        nphot = np.ascontiguousarray(np.broadcast_to(np.asarray(nphot, dtype=np.uint64), (n,)))
+        seeds = np.ascontiguousarray(np.broadcast_to(np.asarray(seeds, dtype=np.uint64), (n,)))
+        return nphot, seeds
+
+    def thermal(self, nphot, seeds, **opts):
+        """One launch for all models; returns one stats dict per model."""
+        nphot, seeds = self._arrays(nphot, seeds)
+        o = _thermal_opts(opts)
+        st = (_lib.ThermalStats * len(self.models))()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_batch_thermal(self.h, nphot.ctypes.data, seeds.ctypes.data,
+                                                   C.byref(o), st))
+        return [x.as_dict() for x in st]
+
+    @staticmethod
+    def item(cam, dxy, u, v, re, im, w, img, chi2, g_logrho=None, g_T=None, scale=1.0,
+             dra=0.0, ddec=0.0):
+        """A step item from device tensors (u, v float64; re, im, w, img,
+        g_logrho, g_T float32; chi2 float64 [1])."""
+        P = lambda t: None if t is None else t.data_ptr()
+        return _lib.StepItem(cam, float(dxy), float(scale), P(u), P(v), int(u.numel()),
+                             float(dra), float(ddec), P(re), P(im), P(w), P(img), P(chi2),
+                             P(g_logrho), P(g_T))
+
+    def step(self, items, nphot, seeds, use_graph=True, **opts):
+        """One sampler step; returns (stats per model, device ms of the step)."""
+        nphot, seeds = self._arrays(nphot, seeds)
+        o = _thermal_opts(opts)
+        arr = (_lib.StepItem * len(self.models))(*items)
+        self._keep = arr
+        st = (_lib.ThermalStats * len(self.models))()
+        ms = C.c_double()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_batch_step(self.h, nphot.ctypes.data, seeds.ctypes.data,
+                                                C.byref(o), arr, 1 if use_graph else 0, st,
+                                                C.byref(ms)))
+        return [x.as_dict() for x in st], ms.value
 
 
 def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0, device=0):

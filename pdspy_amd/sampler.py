@@ -1,0 +1,107 @@
+"""One batched sampler step: the work pdspy's fitting loop does for every walker
+of an MCMC step (pdspy/modeling/run_disk_model.py:31-511 with
+utils/emcee.py:183-203: thermal run, 1.3 mm image, visibilities at the data's
+(u,v), chi^2), here for a whole batch of models at once, plus the adjoint
+gradient of chi^2 with respect to per-cell log density and temperature.
+
+``SamplerStep`` builds the models of a parameter draw (``configs.sampler_batch``:
+uneven grids and photon counts), synthetic data for each (its own visibilities
+at a reference seed plus Gaussian noise of 1 % of the peak), and runs
+
+* ``step()``       -- the batched step (one thermal launch for all models,
+  optionally replayed from a captured hipGraph), and
+* ``sequential()`` -- the same work model by model with the single-model API.
+"""
+import time
+
+import numpy as np
+
+from . import configs, rt
+from .inputs import AU, arcsec, c as c_light, pc
+
+LAM_UM = 1300.0
+DPC = 140.0
+JY = 1e-23
+
+
+class SamplerStep:
+    def __init__(self, params, npix=256, field_arcsec=2.0, K=5000, seed=0, max_events=1000000,
+                 mrw=1, device=0, data_seed=12345):
+        import torch
+        self.params = list(params)
+        self.n = len(self.params)
+        self.npix = npix
+        self.K = K
+        self.opts = dict(max_events=int(max_events), mrw=int(mrw))
+        self.nphot = np.array([int(p["nphot"]) for p in self.params], dtype=np.uint64)
+        self.models = [rt.DeviceModel(rt.model_arrays(configs.batch_model(p).grid), device=device)
+                       for p in self.params]
+        self.batch = rt.Batch(self.models, device=device)
+        pixel = field_arcsec / npix                      # arcsec
+        self.pix_cm = pixel * DPC * AU
+        self.dxy = pixel * arcsec                        # rad
+        # intensity (erg/s/cm^2/Hz/sr) -> Jy per pixel
+        self.scale = (pixel * arcsec) ** 2 / JY
+        u, v = configs.uv_points(K, lam_m=LAM_UM * 1e-6, seed=seed)
+        dev = "cuda"
+        self.u = torch.tensor(u, dtype=torch.float64, device=dev)
+        self.v = torch.tensor(v, dtype=torch.float64, device=dev)
+        self.cams = [rt.DeviceModel.camera(npix, self.pix_cm, LAM_UM, incl=p["incl"])
+                     for p in self.params]
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.img = [torch.zeros((npix, npix), **f32) for _ in range(self.n)]
+        self.chi2 = [torch.zeros(1, dtype=torch.float64, device=dev) for _ in range(self.n)]
+        self.g_logrho = [torch.zeros((m.nspec, m.ncell), **f32) for m in self.models]
+        self.g_T = [torch.zeros(m.ncell, **f32) for m in self.models]
+        self.w = [torch.ones(K, **f32) for _ in range(self.n)]
+        self.re = [torch.zeros(K, **f32) for _ in range(self.n)]
+        self.im = [torch.zeros(K, **f32) for _ in range(self.n)]
+        # synthetic data: each model's own visibilities at a reference seed + noise
+        self._items = None
+        self.step(seed=data_seed, use_graph=False)
+        rng = np.random.default_rng(data_seed)
+        for i in range(self.n):
+            re, im = rt.vis_dft(self.img[i] * self.scale, self.dxy, self.u, self.v)
+            sig = 0.01 * float(torch.sqrt(re * re + im * im).max())
+            sig = sig if sig > 0 else 1.0
+            self.re[i] = (re + torch.tensor(rng.normal(0, sig, K), **f32)).contiguous()
+            self.im[i] = (im + torch.tensor(rng.normal(0, sig, K), **f32)).contiguous()
+            self.w[i] = torch.full((K,), 1.0 / sig ** 2, **f32)
+        self._items = None
+
+    def items(self):
+        if self._items is None:
+            self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u, self.v, self.re[i],
+                                         self.im[i], self.w[i], self.img[i], self.chi2[i],
+                                         self.g_logrho[i], self.g_T[i], scale=self.scale)
+                           for i in range(self.n)]
+        return self._items
+
+    def step(self, seed=0, use_graph=True):
+        """The batched step; returns (stats per model, device ms)."""
+        seeds = np.arange(self.n, dtype=np.uint64) + np.uint64(seed) * np.uint64(1000)
+        return self.batch.step(self.items(), self.nphot, seeds, use_graph=use_graph, **self.opts)
+
+    def sequential(self, seed=0):
+        """The same work, model by model through the single-model calls; returns
+        wall seconds (device synchronised) and per-model chi^2."""
+        import torch
+        seeds = np.arange(self.n, dtype=np.uint64) + np.uint64(seed) * np.uint64(1000)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chis = []
+        for i, m in enumerate(self.models):
+            m.thermal(int(self.nphot[i]), int(seeds[i]), **self.opts)
+            T = m.temperature_device()
+            rho = m.rho_device()
+            dens = torch.where(rho == 0, torch.full_like(rho, 1e-30), rho)
+            Timg = ((dens * T).sum(0) / dens.sum(0)).contiguous()
+            img = m.image(self.cams[i], Timg)
+            re, im = rt.vis_dft(img * self.scale, self.dxy, self.u, self.v)
+            c2, gre, gim = rt.chisq(self.re[i], self.im[i], self.w[i], re, im, with_grad=True)
+            gimg = rt.vis_dft_adjoint(gre, gim, self.npix, self.npix, self.dxy, self.u,
+                                      self.v) * self.scale
+            m.image_adjoint(self.cams[i], Timg, None, gimg)
+            chis.append(c2)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, chis
