@@ -11,6 +11,13 @@ back); the image part renders the 256 x 256 1.3 mm image at 60 deg (2" field at
 140 pc) of the resulting temperatures and its adjoint.
 
 Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
+
+--batch: instead, the batched sampler step -- 64 disk+envelope models drawn as
+configs.sampler_batch (uneven grids 64-256 x 64-128, 1e6-1e7 packets each, MRW
+on), each step = one thermal launch for all of them, then per model the 1.3 mm
+256x256 image, visibilities at 5e4 (u,v) points, chi^2 and the adjoint
+gradient, replayed from a hipGraph -- timed against the same work done model by
+model through the single-model calls.
 """
 import argparse
 import hashlib
@@ -41,7 +48,56 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-image", action="store_true")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "thermal_traffic.json"))
+    p.add_argument("--batch", action="store_true", help="batched sampler step instead")
+    p.add_argument("--batch-models", type=int, default=64)
+    p.add_argument("--batch-k", type=int, default=50000)
+    p.add_argument("--no-sequential", action="store_true")
     return p.parse_args()
+
+
+def batch_main(a):
+    import torch
+    from pdspy_amd import configs
+    from pdspy_amd.sampler import SamplerStep
+
+    params = configs.sampler_batch(a.batch_models, seed=0)
+    t0 = time.perf_counter()
+    s = SamplerStep(params, npix=256, K=a.batch_k, max_events=int(a.max_events), mrw=a.mrw)
+    t_build = time.perf_counter() - t0
+    tuned = s.autotune()
+    print("batch: built and tuned", json.dumps(tuned), file=sys.stderr, flush=True)
+    for w in range(max(1, a.warmup)):
+        s.step(seed=100 + w, use_graph=True)       # first one captures the graph
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        stats, ms = s.step(seed=k, use_graph=True)
+        dev_ms.append(ms)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / a.steps
+    print("batch: %d steps, %.2f s each" % (a.steps, wall), file=sys.stderr, flush=True)
+    seq = None
+    if not a.no_sequential:
+        seq, _ = s.sequential(seed=0)
+        print("batch: sequential %.2f s" % seq, file=sys.stderr, flush=True)
+    npk = int(s.nphot.sum())
+    out = {
+        "metric": "batched sampler step (thermal MC + 1.3 mm image + visibilities + chi^2 "
+                  "gradient), %d models" % s.n,
+        "value": s.n / wall, "unit": "models/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": max(1, a.warmup), "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "configs.sampler_batch(%d, seed=0): grids nr in {64,128,256} x "
+                               "ntheta in {64,128}, nphot 1e6-1e7 log-uniform, MRW %s; 256x256 "
+                               "images at 1.3 mm, K=%d" % (s.n, "on" if a.mrw else "off", s.K),
+                   "models": s.n, "packets_per_step": npk, "cells": [m.ncell for m in s.models]},
+        "step_device_ms": dev_ms, "graph": True, "tuned": tuned,
+        "packets_per_s": npk / wall, "build_s": t_build,
+        "sequential_s": seq, "speedup_vs_sequential": (seq / wall) if seq else None,
+        "killed": int(sum(x["n_killed"] for x in stats)),
+    }
+    print(json.dumps(out), flush=True)
 
 
 def dT_stats(T_ref, T, visits, min_visits=100):
@@ -82,6 +138,9 @@ def main():
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if a.batch:
+        batch_main(a)
+        return
     import torch
     from pdspy_amd import configs, inputs, rt
 

@@ -119,6 +119,8 @@ typedef struct {
     int mode;           /* 0 intensity erg/s/cm^2/Hz/sr, 1 optical depth (tracetau) */
     int incl_star;      /* 1: add stars as point sources (pdspy nostar=False) */
     double tau_stop;    /* stop a ray once tau exceeds this (0 = 40) */
+    int tile_w, tile_h; /* ray tile of one thread block, a multiple of 64 pixels
+                           up to 256 (0, 0 = 64 x 1); results do not depend on it */
 } rt_camera;
 
 int rt_create(int device, rt_ctx **out);

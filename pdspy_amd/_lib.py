@@ -69,7 +69,7 @@ class Camera(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsub", C.c_int), ("pix_cm", C.c_double),
                 ("incl_deg", C.c_double), ("posang_deg", C.c_double), ("phi_deg", C.c_double),
                 ("lam_um", C.c_double), ("mode", C.c_int), ("incl_star", C.c_int),
-                ("tau_stop", C.c_double)]
+                ("tau_stop", C.c_double), ("tile_w", C.c_int), ("tile_h", C.c_int)]
 
 
 class StepItem(C.Structure):

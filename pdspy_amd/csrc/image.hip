@@ -25,6 +25,7 @@ struct CamParams {
     float cpa, spa;
     float pix;       // au
     int nx, ny, nsub, mode;
+    int tw, th, ntx;  // ray tile of a block: tw x th pixels, ntx tiles per image row
     float tau_stop;
     const float *alpha;  // [cell] per au
     const float *src;    // [cell]
@@ -73,10 +74,20 @@ __device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
     return C.mode == 0 ? I : tau;
 }
 
-__global__ void __launch_bounds__(64) k_image(CamParams C, float *img) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= C.nx * C.ny) return;
-    int j = pix / C.nx, k = pix - j * C.nx;
+// pixel of this lane: blocks cover tw x th tiles of the image, so the rays of
+// a wavefront stay close together (they cross the same cells)
+__device__ __forceinline__ bool tile_pixel(const CamParams &C, int &j, int &k) {
+    const int tile = blockIdx.x, ty = tile / C.ntx, tx = tile - ty * C.ntx;
+    const int t = threadIdx.x, dy = t / C.tw, dx = t - dy * C.tw;
+    k = tx * C.tw + dx;
+    j = ty * C.th + dy;
+    return k < C.nx && j < C.ny;
+}
+
+__global__ void __launch_bounds__(256) k_image(CamParams C, float *img) {
+    int j, k;
+    if (!tile_pixel(C, j, k)) return;
+    const int pix = j * C.nx + k;
     float acc = 0.f;
     for (int a = 0; a < C.nsub; ++a)
         for (int b = 0; b < C.nsub; ++b) {
@@ -88,13 +99,13 @@ __global__ void __launch_bounds__(64) k_image(CamParams C, float *img) {
     img[pix] = acc / (float)(C.nsub * C.nsub);
 }
 
-__global__ void __launch_bounds__(64) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
-                                                  double *g_src) {
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pix >= C.nx * C.ny) return;
+__global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
+                                                   double *g_src) {
+    int j, k;
+    if (!tile_pixel(C, j, k)) return;
+    const int pix = j * C.nx + k;
     float gI = dimg[pix] / (float)(C.nsub * C.nsub);
     if (gI == 0.f) return;
-    int j = pix / C.nx, k = pix - j * C.nx;
     for (int a = 0; a < C.nsub; ++a)
         for (int b = 0; b < C.nsub; ++b) {
             float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
@@ -280,6 +291,14 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
     C.ny = cam->ny;
     C.nsub = cam->nsub > 0 ? cam->nsub : 1;
     C.mode = cam->mode;
+    // ray tile: tile_w x tile_h pixels per block (default 64 x 1: one row run)
+    C.tw = cam->tile_w > 0 ? cam->tile_w : 64;
+    C.th = cam->tile_h > 0 ? cam->tile_h : 64 / C.tw;
+    if (C.th < 1) C.th = 1;
+    if (C.tw * C.th > 256 || (C.tw * C.th) % 64)
+        return set_err(ctx, RT_E_ARG, "ray tile %d x %d: need a multiple of 64 pixels, at most 256",
+                       C.tw, C.th);
+    C.ntx = (C.nx + C.tw - 1) / C.tw;
     C.tau_stop = (float)(cam->tau_stop > 0 ? cam->tau_stop : 40.0);
     for (int s = 0; s < RT_MAX_SPEC; ++s)
         kap.k[s] = s < m->nspec ? (float)kappa_at(m, m->kabs, s, cam->lam_um) : 0.f;
@@ -318,7 +337,8 @@ int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *
                        m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->mode, m->d_alpha,
                        m->d_src);
     int npx = cam->nx * cam->ny;
-    hipLaunchKernelGGL(k_image, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_img);
+    const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
+    hipLaunchKernelGGL(k_image, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_img);
     if (cam->incl_star && cam->mode == 0 && m->nstar > 0) {
         StarArg fl;
         star_fluxes(m, cam, fl);
@@ -347,7 +367,8 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, nc);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
     int npx = cam->nx * cam->ny;
-    hipLaunchKernelGGL(k_image_adj, dim3(nblk(npx, 64)), dim3(64), 0, sm, C, d_dimg, m->d_galpha,
+    const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
+    hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_dimg, m->d_galpha,
                        m->d_gsrc);
     if (cam->incl_star && m->nstar > 0) {
         StarArg fl;

@@ -180,10 +180,11 @@ class DeviceModel:
 
     @staticmethod
     def camera(npix, pix_cm, lam_um, incl=0.0, pa=0.0, phi=0.0, mode=0, incl_star=True,
-               nsub=1, tau_stop=40.0, ny=None):
+               nsub=1, tau_stop=40.0, ny=None, tile=(0, 0)):
         return _lib.Camera(int(npix), int(ny if ny is not None else npix), int(nsub),
                            float(pix_cm), float(incl), float(pa), float(phi), float(lam_um),
-                           int(mode), 1 if incl_star else 0, float(tau_stop))
+                           int(mode), 1 if incl_star else 0, float(tau_stop), int(tile[0]),
+                           int(tile[1]))
 
     def image(self, cam, T_dev, rho_dev=None):
         """Intensity image (erg/s/cm^2/Hz/sr, or tau) as a float32 torch tensor [ny, nx]."""

@@ -69,6 +69,35 @@ class SamplerStep:
             self.w[i] = torch.full((K,), 1.0 / sig ** 2, **f32)
         self._items = None
 
+    def autotune(self, frac=0.1):
+        """Picks the thermal launch shape (timed on `frac` of each model's packets)
+        and the ray tile (timed on the largest model's image + adjoint); returns
+        the choices.  The results of a step do not depend on them."""
+        import torch
+        from . import tune
+        nph = np.maximum(1024, (self.nphot * frac).astype(np.uint64))
+        seeds = np.arange(self.n, dtype=np.uint64)
+        key = ("batch", tuple(int(x) for x in self.nphot),
+               tuple((m.nr, m.nt, m.np) for m in self.models))
+        best = tune.tune_thermal(key, lambda **c: self.batch.thermal(nph, seeds, **self.opts, **c))
+        self.opts.update(best)
+        big = max(range(self.n), key=lambda i: self.models[i].ncell)
+        m = self.models[big]
+        T = torch.ones(m.ncell, dtype=torch.float32, device="cuda") * 20.0
+        dimg = torch.ones((self.npix, self.npix), dtype=torch.float32, device="cuda")
+
+        def render(tile):
+            cam = rt.DeviceModel.camera(self.npix, self.pix_cm, LAM_UM,
+                                        incl=self.params[big]["incl"], tile=tile)
+            m.image(cam, T)
+            m.image_adjoint(cam, T, None, dimg)
+
+        tile = tune.tune_tiles(("tiles", self.npix, m.nr, m.nt, m.np), render)
+        self.cams = [rt.DeviceModel.camera(self.npix, self.pix_cm, LAM_UM, incl=p["incl"],
+                                           tile=tile) for p in self.params]
+        self._items = None
+        return dict(thermal=best, tile=tile)
+
     def items(self):
         if self._items is None:
             self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u, self.v, self.re[i],

@@ -7,6 +7,7 @@ import torch
 import oracle
 from pdspy_amd import autograd as rtag
 from pdspy_amd import configs, inputs, rt
+from pdspy_amd._lib import RTError
 from tests.helpers import oracle_arrays
 from tests.test_oracle import C_LIGHT, _uniform_sphere, np_planck
 
@@ -63,6 +64,29 @@ def test_image_matches_oracle(incl, pa):
     assert err < 2e-3
     # FP32 vs FP64 traversal: total flux to 1e-4
     assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+
+
+def test_ray_tiles_do_not_change_results():
+    # the launch geometry (pixels per block) must not change any ray
+    arr, T = _disk_setup()
+    dm = _dev(arr)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    npix, pix = 72, 4.0 * AU  # not a multiple of the tile sizes: edge tiles
+    ref = None
+    dimg = torch.rand((npix, npix), dtype=torch.float32, device="cuda")
+    for tile in [(0, 0), (64, 1), (8, 8), (16, 4), (16, 16), (32, 8)]:
+        cam = dm.camera(npix, pix, 1300.0, incl=50.0, pa=10.0, tile=tile)
+        img = dm.image(cam, Td).cpu().numpy()
+        glr, gT = dm.image_adjoint(cam, Td, None, dimg)
+        if ref is None:
+            ref = (img, glr.cpu().numpy(), gT.cpu().numpy())
+            continue
+        assert np.array_equal(img, ref[0]), tile
+        # adjoint accumulates with FP64 atomics: order may differ
+        assert np.allclose(gT.cpu().numpy(), ref[2], rtol=1e-5, atol=1e-6 * np.abs(ref[2]).max())
+        assert np.allclose(glr.cpu().numpy(), ref[1], rtol=1e-5, atol=1e-6 * np.abs(ref[1]).max())
+    with pytest.raises(RTError):
+        dm.image(dm.camera(npix, pix, 1300.0, tile=(10, 3)), Td)
 
 
 def test_image_3d_and_tau_mode_match_oracle():
