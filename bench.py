@@ -79,7 +79,8 @@ def batch_main(a):
     print("batch: %d steps, %.2f s each" % (a.steps, wall), file=sys.stderr, flush=True)
     seq = None
     if not a.no_sequential:
-        seq, _ = s.sequential(seed=0)
+        seq, _ = s.sequential(seed=0, progress=lambda i, t: print(
+            "batch: sequential model %d done at %.1f s" % (i, t), file=sys.stderr, flush=True))
         print("batch: sequential %.2f s" % seq, file=sys.stderr, flush=True)
     npk = int(s.nphot.sum())
     out = {

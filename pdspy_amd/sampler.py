@@ -111,9 +111,10 @@ class SamplerStep:
         seeds = np.arange(self.n, dtype=np.uint64) + np.uint64(seed) * np.uint64(1000)
         return self.batch.step(self.items(), self.nphot, seeds, use_graph=use_graph, **self.opts)
 
-    def sequential(self, seed=0):
+    def sequential(self, seed=0, progress=None):
         """The same work, model by model through the single-model calls; returns
-        wall seconds (device synchronised) and per-model chi^2."""
+        wall seconds (device synchronised) and per-model chi^2.  progress(i, s):
+        called after model i with the seconds so far."""
         import torch
         seeds = np.arange(self.n, dtype=np.uint64) + np.uint64(seed) * np.uint64(1000)
         torch.cuda.synchronize()
@@ -132,5 +133,8 @@ class SamplerStep:
                                       self.v) * self.scale
             m.image_adjoint(self.cams[i], Timg, None, gimg)
             chis.append(c2)
+            if progress is not None:
+                torch.cuda.synchronize()
+                progress(i, time.perf_counter() - t0)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, chis
