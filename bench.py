@@ -224,9 +224,17 @@ def main():
         # 1.3 mm, 256x256, 2" field at 140 pc, 60 deg; forward and adjoint
         T = torch.tensor(rt.imaging_temperature(_with_T(model, temps[0], dm)),
                          dtype=torch.float32, device="cuda")
+        from pdspy_amd import tune
         pix_cm = 2.0 / 256 * 140 * inputs.AU
-        cam = dm.camera(256, pix_cm, 1300.0, incl=60.0)
         dimg = torch.ones((256, 256), dtype=torch.float32, device="cuda")
+
+        def render(tile):
+            c = dm.camera(256, pix_cm, 1300.0, incl=60.0, tile=tile)
+            dm.image(c, T)
+            dm.image_adjoint(c, T, None, dimg)
+
+        tile = tune.tune_tiles(("bench", 256, dm.nr, dm.nt, dm.np), render)
+        cam = dm.camera(256, pix_cm, 1300.0, incl=60.0, tile=tile)
         for _ in range(2):
             dm.image(cam, T)
             dm.image_adjoint(cam, T, None, dimg)
@@ -245,7 +253,8 @@ def main():
         t_fa = (time.perf_counter() - t0) / n_rep
         out["image"] = {"rays_per_s_fwd": 65536 / t_f, "rays_per_s_fwd_adj": 65536 / t_fa,
                         "ms_fwd": t_f * 1e3, "ms_fwd_adj": t_fa * 1e3,
-                        "config": "256x256 px, 1.3 mm, incl 60, 2 arcsec field at 140 pc"}
+                        "config": "256x256 px, 1.3 mm, incl 60, 2 arcsec field at 140 pc",
+                        "ray_tile": list(tile)}
     if not a.no_cpu:
         cb = cpu_baseline(arr, a.max_events, a.cpu_seconds, a.mrw)
         out["cpu_baseline"] = cb

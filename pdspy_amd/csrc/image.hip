@@ -26,6 +26,7 @@ struct CamParams {
     float pix;       // au
     int nx, ny, nsub, mode;
     int tw, th, ntx;  // ray tile of a block: tw x th pixels, ntx tiles per image row
+    unsigned hbits;   // adjoint LDS table: 2^hbits entries (>= 4 per ray of a block)
     float tau_stop;
     const float *alpha;  // [cell] per au
     const float *src;    // [cell]
@@ -99,14 +100,56 @@ __global__ void __launch_bounds__(256) k_image(CamParams C, float *img) {
     img[pix] = acc / (float)(C.nsub * C.nsub);
 }
 
+// The adjoint's per-cell sums are first gathered in a small open-addressing
+// table in LDS: the rays of one block are neighbours on the sky and cross
+// mostly the same cells, so each cell then costs one global FP64 atomic per
+// block instead of one per ray.  Entries: 2^hbits keys (cell index, -1 =
+// empty) and two FP64 sums; a contribution that finds no slot in 8 probes
+// goes straight to global memory.
+__device__ __forceinline__ void adj_add(int c, double va, double vs, int *key, double *ga,
+                                        double *gs, unsigned hbits, double *g_alpha,
+                                        double *g_src) {
+    const unsigned mask = (1u << hbits) - 1u;
+    unsigned slot = ((unsigned)c * 2654435761u) >> (32u - hbits);
+    for (int probe = 0; probe < 8; ++probe) {
+        int cur = __hip_atomic_load(&key[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == -1) {
+            int exp = -1;
+            __hip_atomic_compare_exchange_strong(&key[slot], &exp, c, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            cur = exp == -1 ? c : exp;
+        }
+        if (cur == c) {
+            __hip_atomic_fetch_add(&ga[slot], va, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&gs[slot], vs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            return;
+        }
+        slot = (slot + 1u) & mask;
+    }
+    unsafeAtomicAdd(&g_alpha[c], va);
+    unsafeAtomicAdd(&g_src[c], vs);
+}
+
 __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
                                                    double *g_src) {
+    extern __shared__ double adj_lds[];
+    const unsigned H = 1u << C.hbits;
+    double *ga = adj_lds, *gs = adj_lds + H;
+    int *key = (int *)(adj_lds + 2 * H);
+    for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
+        key[i] = -1;
+        ga[i] = 0.0;
+        gs[i] = 0.0;
+    }
+    __syncthreads();
     int j, k;
-    if (!tile_pixel(C, j, k)) return;
-    const int pix = j * C.nx + k;
-    float gI = dimg[pix] / (float)(C.nsub * C.nsub);
-    if (gI == 0.f) return;
-    for (int a = 0; a < C.nsub; ++a)
+    bool act = tile_pixel(C, j, k);
+    float gI = 0.f;
+    if (act) {
+        gI = dimg[j * C.nx + k] / (float)(C.nsub * C.nsub);
+        act = gI != 0.f;
+    }
+    for (int a = 0; act && a < C.nsub; ++a)
         for (int b = 0; b < C.nsub; ++b) {
             float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
             float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
@@ -130,8 +173,8 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
                     cum = fmaf(att * om, S, cum);
                     float Rk = I - cum;
                     float dI_ddtau = att * (1.f - om) * S - Rk;
-                    unsafeAtomicAdd(&g_alpha[c], (double)(gI * dI_ddtau * tw));
-                    unsafeAtomicAdd(&g_src[c], (double)(gI * att * om));
+                    adj_add(c, (double)(gI * dI_ddtau * tw), (double)(gI * att * om), key, ga, gs,
+                            C.hbits, g_alpha, g_src);
                     tau += dtau;
                     if (tau > C.tau_stop) break;
                 }
@@ -139,6 +182,14 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
                 if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
             }
         }
+    __syncthreads();
+    for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
+        const int c = key[i];
+        if (c >= 0) {
+            unsafeAtomicAdd(&g_alpha[c], ga[i]);
+            unsafeAtomicAdd(&g_src[c], gs[i]);
+        }
+    }
 }
 
 // stars: one thread per star; adds F exp(-tau) to the pixel holding the star,
@@ -299,6 +350,8 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
         return set_err(ctx, RT_E_ARG, "ray tile %d x %d: need a multiple of 64 pixels, at most 256",
                        C.tw, C.th);
     C.ntx = (C.nx + C.tw - 1) / C.tw;
+    C.hbits = 8;
+    while ((1u << C.hbits) < 4u * (unsigned)(C.tw * C.th)) ++C.hbits;
     C.tau_stop = (float)(cam->tau_stop > 0 ? cam->tau_stop : 40.0);
     for (int s = 0; s < RT_MAX_SPEC; ++s)
         kap.k[s] = s < m->nspec ? (float)kappa_at(m, m->kabs, s, cam->lam_um) : 0.f;
@@ -368,7 +421,8 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
     int npx = cam->nx * cam->ny;
     const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
-    hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_dimg, m->d_galpha,
+    const size_t adj_lds = (size_t)(1u << C.hbits) * (2 * sizeof(double) + sizeof(int));
+    hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), adj_lds, sm, C, d_dimg, m->d_galpha,
                        m->d_gsrc);
     if (cam->incl_star && m->nstar > 0) {
         StarArg fl;
