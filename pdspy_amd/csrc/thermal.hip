@@ -680,7 +680,12 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             // u2 temperature row, u3 frequency, u4 u5 direction, u6 next tau
             float u[8];
             rng8(P.seed, pid, krng++, u);
-            if (u[0] * aext < asca) {
+            // scattering (Henyey-Greenstein if the model has g) or absorption and
+            // re-emission; the new direction is drawn once after the branch
+            const bool scat = u[0] * aext < asca;
+            bool use_hg = false;
+            float ghg = 0.f;
+            if (scat) {
                 n_scat++;
                 if (P.hg) {
                     float us = u[1] * asca, a = 0.f;
@@ -692,9 +697,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                             break;
                         }
                     }
-                    hg_dir(u[4], u[5], s_g[sp * NL + ifr], p.dx, p.dy, p.dz);
-                } else {
-                    iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
+                    ghg = s_g[sp * NL + ifr];
+                    use_hg = true;
                 }
             } else {
                 n_abs++;
@@ -714,8 +718,11 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 int row = lo + (u[2] < w ? 1 : 0);
                 size_t ro = (size_t)q * P.ntemp + row;
                 ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, u[3]);
-                iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
             }
+            if (use_hg)
+                hg_dir(u[4], u[5], ghg, p.dx, p.dy, p.dz);
+            else
+                iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
             tau = -__logf(u[6]);
             RT_PHASE(6);
             if (P.max_events && nev >= P.max_events) {
