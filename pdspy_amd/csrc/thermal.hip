@@ -443,6 +443,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
     uint64_t n_idle = 0;
     uint32_t iter = 0;
+    int prio = 0;  // wave priority set with s_setprio
 #if RT_THERMAL_PROFILE
     // wave time by phase: refill, loop overhead, snapshot, idle, geometry,
     // crossing, interaction, modified random walk.  The wave's last time stamp
@@ -553,7 +554,22 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
         }
 
         // ---- help with the run's next temperature snapshot
-        if ((iter & 63u) == 0u) snapshot_help(P, lane);
+        if ((iter & 63u) == 0u) {
+            snapshot_help(P, lane);
+            // a wave holding a long random walk is issued first on its SIMD:
+            // those walks are what the end of a run waits for
+            const bool l2 = __any(alive && nev > 200000u), l1 = __any(alive && nev > 20000u);
+            const int want = l2 ? 2 : (l1 ? 1 : 0);
+            if (want != prio) {
+                if (want == 2)
+                    __builtin_amdgcn_s_setprio(2);
+                else if (want == 1)
+                    __builtin_amdgcn_s_setprio(1);
+                else
+                    __builtin_amdgcn_s_setprio(0);
+                prio = want;
+            }
+        }
         RT_PHASE(2);
 
         // ---- refill dead lanes with packets of the current epoch
