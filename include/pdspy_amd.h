@@ -84,6 +84,11 @@ typedef struct {
     int blocks_per_cu;     /* persistent blocks per CU (0 = auto)                 */
     uint64_t slice_events; /* a packet of epoch e deposits into temperature slot
                               e + floor(interactions / slice_events) (default 1e4) */
+    int no_park;           /* batches: 1 keeps every packet on the wavefront that
+                              started it (default 0: a wavefront left with a few
+                              packets whose model has none to hand out passes them
+                              to another wavefront of that model); the results
+                              are the same either way */
 } rt_thermal_opts;
 
 typedef struct {
@@ -105,6 +110,8 @@ typedef struct {
                                  loop overhead, snapshot, idle, geometry, crossing,
                                  interaction, modified random walk; zero unless the
                                  library is built with RT_THERMAL_PROFILE              */
+    uint64_t n_parked;    /* batches: packets handed from a wavefront left with a few
+                             of them to another wavefront of the same model        */
 } rt_thermal_stats;
 
 /* Camera (RADMC-3D `image` options pdspy passes, Model.py:515-534):

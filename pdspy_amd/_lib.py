@@ -46,7 +46,7 @@ class ThermalOpts(C.Structure):
                 ("mrw_tauthres", C.c_double), ("mrw_count_trigger", C.c_int),
                 ("mrw_gamma", C.c_double), ("max_seconds", C.c_double),
                 ("block_size", C.c_int), ("blocks_per_cu", C.c_int),
-                ("slice_events", C.c_uint64)]
+                ("slice_events", C.c_uint64), ("no_park", C.c_int)]
 
 
 class ThermalStats(C.Structure):
@@ -57,7 +57,7 @@ class ThermalStats(C.Structure):
                 ("total_ms", C.c_double), ("max_epoch_ms", C.c_double),
                 ("max_packet_steps", C.c_uint64), ("wave_iters", C.c_uint64),
                 ("idle_iters", C.c_uint64), ("n_nudged", C.c_uint64),
-                ("phase_clocks", C.c_uint64 * 8)]
+                ("phase_clocks", C.c_uint64 * 8), ("n_parked", C.c_uint64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -28,6 +28,10 @@ struct rt_batch {
     std::vector<ThermPlan> plans;
     unsigned long long *d_abort = nullptr;  // [1]
     unsigned int *d_done = nullptr;         // [1]
+    int *d_parked = nullptr;                // [1] packets in the pools
+    // per model: a pool of parked packets (ring of park_cap states) and flags
+    unsigned int *d_park = nullptr, *d_park_flag = nullptr;
+    uint32_t park_cap = 8192;
     int lds_wave = 0, max_s = 0, max_g = 0;
     std::vector<uint64_t> plan_nphot;       // nphot the plans were made for
     rt_thermal_opts plan_opts{};
@@ -67,11 +71,15 @@ __global__ void k_scale_copy(const float *src, float a, size_t n, float *dst) {
     if (i < n) dst[i] = a * src[i];
 }
 
-__global__ void k_fill_flags(unsigned long long *abort, unsigned int *done) {
-    if (threadIdx.x == 0) {
+__global__ void k_fill_flags(unsigned long long *abort, unsigned int *done, int *parked,
+                             unsigned int *park_flag, size_t nflag) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
         abort[0] = 0ull;
         done[0] = 0u;
+        parked[0] = 0;
     }
+    if (i < nflag) park_flag[i] = 0u;
 }
 
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -103,6 +111,11 @@ int batch_create(rt_ctx *ctx, rt_model *const *models, int n, rt_batch **out) {
     hipError_t e = hipMalloc(&b->d_runs, psz * (size_t)n);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_abort, sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_done, sizeof(unsigned int));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_parked, sizeof(int));
+    const size_t npark = (size_t)b->park_cap * (size_t)n;
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&b->d_park, sizeof(unsigned int) * thermal_park_words() * npark);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_park_flag, sizeof(unsigned int) * npark);
     if (e == hipSuccess) e = hipEventCreate(&b->e0);
     if (e == hipSuccess) e = hipEventCreate(&b->e1);
     if (e == hipSuccess) e = hipEventCreate(&b->e2);
@@ -133,6 +146,9 @@ int batch_destroy(rt_batch *b) {
     (void)hipFree(b->d_runs);
     (void)hipFree(b->d_abort);
     (void)hipFree(b->d_done);
+    (void)hipFree(b->d_parked);
+    (void)hipFree(b->d_park);
+    (void)hipFree(b->d_park_flag);
     for (size_t i = 0; i < b->models.size(); ++i) {
         (void)hipFree(b->d_Timg[i]);
         (void)hipFree(b->d_imgs[i]);
@@ -181,6 +197,10 @@ static int batch_plan(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
         int rc = thermal_plan(m, nphot[i], seeds[i], &opt, rp);
         if (rc) return rc;
         thermal_params_set_done(m, b->d_done);
+        if (!opt.no_park)
+            thermal_params_set_pool(
+                m, b->d_park + thermal_park_words() * (size_t)b->park_cap * (size_t)i,
+                b->d_park_flag + (size_t)b->park_cap * (size_t)i, b->park_cap);
         thermal_params_copy(m, b->h_runs.data() + psz * (size_t)i);
         // the epoch starts every time: a single run of the model in between
         // may have left its own there
@@ -208,10 +228,12 @@ static int batch_enqueue_thermal(rt_batch *b, hipStream_t sm, bool timed) {
         int rc = thermal_enqueue_reset(b->models[i], b->plans[i], sm, false);
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_fill_flags, dim3(1), dim3(64), 0, sm, b->d_abort, b->d_done);
+    const size_t nflag = (size_t)b->park_cap * (size_t)n;
+    hipLaunchKernelGGL(k_fill_flags, dim3(nblk(nflag, 256)), dim3(256), 0, sm, b->d_abort,
+                       b->d_done, b->d_parked, b->d_park_flag, nflag);
     if (timed) HIPCHK(ctx, hipEventRecord(b->e0, sm));
     int rc = thermal_enqueue_launch(ctx, b->d_runs, nullptr, n, b->lds_wave, b->max_s, b->max_g,
-                                    &b->plan_opts, b->d_abort, b->d_done, sm);
+                                    &b->plan_opts, b->d_abort, b->d_done, b->d_parked, sm);
     if (rc) return rc;
     if (timed) HIPCHK(ctx, hipEventRecord(b->e1, sm));
     for (int i = 0; i < n; ++i) thermal_enqueue_finish(b->models[i], b->plans[i], sm);

@@ -42,6 +42,7 @@ enum {
     CNT_MAXPKT,
     CNT_WITER,     // loop iterations summed over wavefronts
     CNT_IDLE,      // of which with no packet to step
+    CNT_PARKED,    // packets passed from a sparse wavefront to another (batch)
     CNT_TK0,       // 8 phase clocks (RT_THERMAL_PROFILE builds only)
     CNT_N = CNT_TK0 + 8
 };
@@ -149,12 +150,15 @@ void thermal_enqueue_finish(rt_model *m, const ThermPlan &rp, hipStream_t sm);
 // single run's host block passed as a kernel argument
 int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, int nrun,
                            int lds_wave, int max_s, int max_g, const rt_thermal_opts *o,
-                           unsigned long long *d_abort, unsigned int *d_done, hipStream_t sm);
+                           unsigned long long *d_abort, unsigned int *d_done, int *d_parked,
+                           hipStream_t sm);
 int thermal_collect(rt_model *m, const ThermPlan &rp, bool aborted, float kms, float tot,
                     rt_thermal_stats *st);
 size_t thermal_params_size();
 int thermal_params_copy(const rt_model *m, void *dst_host);
 void thermal_params_set_done(rt_model *m, unsigned int *d_done);
+void thermal_params_set_pool(rt_model *m, unsigned int *park, unsigned int *flag, uint32_t cap);
+size_t thermal_park_words();
 struct PipeUpload {
     void *dst;
     const void *src;

@@ -98,6 +98,12 @@ struct ThermParams {
     unsigned long long *snap_time;       // [E] clock at each snapshot, then t_begin, t_end
     unsigned long long *ep_cur;          // [1] first epoch with unclaimed packets
     unsigned int *runs_done;             // [1] of the batch: runs with all snapshots out
+    // batch only: pool of parked packets (a ring of park_cap states of
+    // RT_PARK_WORDS words; flag = index + 1 once a state is written)
+    unsigned int *park;
+    unsigned int *park_flag;
+    unsigned long long *park_head, *park_tail;
+    uint32_t park_cap;
     unsigned long long *tslot;           // [E+1][slot] tallies by deposit slot (last: all later)
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
     float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
@@ -114,7 +120,11 @@ struct BatchArgs {
     uint64_t max_ticks;            // watchdog (wall-clock ticks)
     unsigned long long *abort;     // [1] watchdog flag
     unsigned int *runs_done;       // [1] runs whose last snapshot is published
+    int *parked;                   // [1] packets parked in the runs' pools (batch)
 };
+
+#define RT_PARK_WORDS 22   // words of a parked packet's state
+#define RT_PARK_MAXLIVE 8  // a wave with at most this many live packets may park them
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ unsigned ld_rlx32(const unsigned *p) {
@@ -129,9 +139,17 @@ __device__ __forceinline__ void st_rlx64(unsigned long long *p, unsigned long lo
 __device__ __forceinline__ void st_rlx32(unsigned *p, unsigned v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// program-order fences that wait for outstanding memory operations of this
-// wave without touching the L2 (the data they order is device-coherent)
-__device__ __forceinline__ void order_release() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); }
+// Ordering of the wave's device-coherent (agent-scope atomic) memory operations
+// without L2 write-back or invalidation, which the data they order does not
+// need.  Release: every earlier load, store and atomic of the wave has
+// completed before any later one is issued -- a workgroup-scope fence alone is
+// only a compiler barrier on gfx950 and emits no wait, so the wait is explicit.
+// Acquire: a compiler barrier; the loads it orders are issued after a branch
+// on the value that was acquired, so they follow it in hardware too.
+__device__ __forceinline__ void order_release() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
 __device__ __forceinline__ void order_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 __device__ __forceinline__ void count_up(unsigned long long *p) {
     order_release();
@@ -313,9 +331,37 @@ __device__ __forceinline__ const ThermParams &run_at(const ThermParams *__restri
 }
 
 template <bool BATCH>
+__device__ __forceinline__ bool pool_nonempty(const ThermParams &R) {
+    if (!BATCH || !R.park) return false;
+    return uniform32((unsigned)(ld_rlx64(R.park_tail) > ld_rlx64(R.park_head))) != 0u;
+}
+
+// whether run R's next unclaimed epoch may start now
+__device__ __forceinline__ bool epoch_claimable(const ThermParams &R, unsigned &e) {
+    e = uniform32((unsigned)ld_rlx64(R.ep_cur));
+    return e < (unsigned)R.nepoch && (e < 2u || uniform32(ld_rlx32(&R.snap_ready[e - 2])) != 0u);
+}
+
+// another run (not `self`) with an epoch to hand out
+template <bool BATCH>
+__device__ __forceinline__ bool other_epoch_work(const BatchArgs &A,
+                                                 const ThermParams *__restrict__ runs,
+                                                 const ThermParams &P1, int self) {
+    if (!BATCH) return false;
+    for (int r = 0; r < A.nrun; ++r) {
+        unsigned e;
+        if (r != self && epoch_claimable(runs[r], e)) return true;
+    }
+    return false;
+}
+
+// First run, from `start` on, with work to hand out: an epoch that may start
+// now or (batch) parked packets -- except the pool of run `skip`, whose packets
+// the calling wave has just parked; -1 if none.  e_out: the run's next epoch.
+template <bool BATCH>
 __device__ __forceinline__ int select_run(const BatchArgs &A,
                                           const ThermParams *__restrict__ runs,
-                                          const ThermParams &P1, int start, int &e_out) {
+                                          const ThermParams &P1, int start, int skip, int &e_out) {
     if (!BATCH) {
         const unsigned e = uniform32((unsigned)ld_rlx64(P1.ep_cur));
         if (e < (unsigned)P1.nepoch &&
@@ -328,15 +374,23 @@ __device__ __forceinline__ int select_run(const BatchArgs &A,
     int r = start;
     for (int k = 0; k < A.nrun; ++k) {
         const ThermParams &R = run_at<BATCH>(runs, P1, r);
-        const unsigned e = uniform32((unsigned)ld_rlx64(R.ep_cur));
-        if (e < (unsigned)R.nepoch &&
-            (e < 2u || uniform32(ld_rlx32(&R.snap_ready[e - 2])) != 0u)) {
+        unsigned e;
+        const bool ok = epoch_claimable(R, e);
+        if (ok || (r != skip && pool_nonempty<BATCH>(R))) {
             e_out = (int)e;
             return r;
         }
         r = r + 1 == A.nrun ? 0 : r + 1;
     }
     return -1;
+}
+
+// parked packet state: device-coherent word stores and loads
+__device__ __forceinline__ void park_put(unsigned *w, int k, unsigned v) {
+    __hip_atomic_store(w + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned park_get(const unsigned *w, int k) {
+    return __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Adds a wave's counters for run R (one atomic per counter per wave) and
@@ -443,6 +497,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
     uint64_t n_idle = 0;
     uint32_t iter = 0;
+    uint32_t last_adopt = 0;  // iteration of the wave's last adoption of parked packets
+    int parked_from = -1;     // run whose pool the wave has just filled (not to be emptied by it)
     int prio = 0;  // wave priority set with s_setprio
 #if RT_THERMAL_PROFILE
     // wave time by phase: refill, loop overhead, snapshot, idle, geometry,
@@ -507,10 +563,11 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 if (epoch_ok) order_acquire();
                 can = epoch_ok;
             }
+            if (!can && m >= 0 && m != parked_from) can = pool_nonempty<BATCH>(P);
             if (!can) {
                 int e_new = 0;
-                const int r =
-                    (int)uniform32((unsigned)select_run<BATCH>(A, runs, P1, scan, e_new));
+                const int r = (int)uniform32(
+                    (unsigned)select_run<BATCH>(A, runs, P1, scan, parked_from, e_new));
                 e_new = (int)uniform32((unsigned)e_new);
                 if (r >= 0) {
                     order_acquire();
@@ -538,13 +595,20 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                     } else {
                         cur_e = max(cur_e, e_new);
                     }
-                    epoch_ok = true;
+                    // one run: select_run found its epoch claimable; batch: the run may
+                    // have been picked for its pool, checked again before the next claim
+                    epoch_ok = !BATCH;
                     scan = r;
+                    parked_from = -1;
                 } else {
                     // nothing to hand out anywhere: help a snapshot, finish or wait
-                    if (uniform32(ld_rlx32(A.runs_done)) >= (unsigned)A.nrun) break;
+                    if (uniform32(ld_rlx32(A.runs_done)) >= (unsigned)A.nrun &&
+                        (!BATCH || uniform32((unsigned)__hip_atomic_load(
+                                       A.parked, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0u))
+                        break;
                     snapshot_help(run_at<BATCH>(runs, P1, scan), lane);
                     scan = scan + 1 == A.nrun ? 0 : scan + 1;
+                    parked_from = -1;  // no other run has work: its own pool is fair game
                     n_idle++;
                     __builtin_amdgcn_s_sleep(32);
                     RT_PHASE(3);
@@ -572,10 +636,170 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
         }
         RT_PHASE(2);
 
+        // ---- batch: a wave left with a few live packets and nothing to hand out
+        // in its run parks them in the run's pool and moves on to another run
+        // (a sparse wave costs as many issue slots as a full one)
+        if (BATCH && (iter & 15u) == 0u && m >= 0 && P.park &&
+            iter - last_adopt > 2048u) {
+            const uint64_t live = __ballot(alive);
+            const int n = __popcll(live);
+            unsigned e_chk;
+            if (n > 0 && n <= RT_PARK_MAXLIVE && !epoch_claimable(P, e_chk) &&
+                !pool_nonempty<BATCH>(P) && other_epoch_work<BATCH>(A, runs, P1, m)) {
+                const int leader = (int)uniform32((unsigned)(__ffsll((unsigned long long)live) - 1));
+                unsigned long long base = 0, ok = 0;
+                if (lane == leader) {
+                    // counted first, so the count never misses a parked packet
+                    __hip_atomic_fetch_add(A.parked, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int t = 0; t < 8 && !ok; ++t) {
+                        unsigned long long tl = ld_rlx64(P.park_tail), hd = ld_rlx64(P.park_head);
+                        if (tl + (unsigned long long)n - hd > P.park_cap) break;
+                        unsigned long long ex = tl;
+                        if (__hip_atomic_compare_exchange_strong(P.park_tail, &ex, tl + n,
+                                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+                            base = tl;
+                            ok = 1;
+                        }
+                    }
+                    if (!ok)
+                        __hip_atomic_fetch_add(A.parked, -n, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        atomicAdd(&P.cnt[CNT_PARKED], (unsigned long long)n);
+                }
+                ok = lane64(ok, leader);
+                base = lane64(base, leader);
+                if (ok && alive) {
+                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, aold, ovf);
+                    const unsigned long long idx = base + __popcll(live & ((1ull << lane) - 1ull));
+                    const size_t s0 = (size_t)(idx % P.park_cap);
+                    unsigned *w = P.park + s0 * RT_PARK_WORDS;
+                    // the state last held here (a lap of the ring earlier) has been
+                    // taken; its reader clears the flag once it has read it
+                    bool free_ = false;
+                    for (int wt = 0; wt < (1 << 22) && !free_; ++wt) {
+                        free_ = ld_rlx32(&P.park_flag[s0]) == 0u;
+                        if (!free_) __builtin_amdgcn_s_sleep(2);
+                    }
+                    if (!free_) atomicOr(A.abort, 1ull);  // (bounded) the run is invalid
+                    park_put(w, 0, __float_as_uint(p.x));
+                    park_put(w, 1, __float_as_uint(p.y));
+                    park_put(w, 2, __float_as_uint(p.z));
+                    park_put(w, 3, __float_as_uint(p.dx));
+                    park_put(w, 4, __float_as_uint(p.dy));
+                    park_put(w, 5, __float_as_uint(p.dz));
+                    park_put(w, 6, (unsigned)p.ir);
+                    park_put(w, 7, (unsigned)p.it);
+                    park_put(w, 8, (unsigned)p.ip);
+                    park_put(w, 9, (unsigned)p.hemi);
+                    park_put(w, 10, (unsigned)pid);
+                    park_put(w, 11, (unsigned)(pid >> 32));
+                    park_put(w, 12, krng);
+                    park_put(w, 13, (unsigned)ifr);
+                    park_put(w, 14, __float_as_uint(tau));
+                    park_put(w, 15, nev);
+                    park_put(w, 16, nstep);
+                    park_put(w, 17, nev_cell);
+                    park_put(w, 18, nzero);
+                    park_put(w, 19, slice_left);
+                    park_put(w, 20, (unsigned)pe);
+                    park_put(w, 21, (unsigned)slot);
+                    order_release();
+                    st_rlx32(&P.park_flag[s0], (unsigned)(idx + 1ull));
+                    alive = false;
+                }
+                if (ok) {
+                    parked_from = m;
+                    RT_PHASE(0);
+                    continue;  // empty now: the next iteration moves to another run
+                }
+            }
+        }
+
         // ---- refill dead lanes with packets of the current epoch
         if (cur_e < E && !epoch_ok) {
             epoch_ok = cur_e < 2 || uniform32(ld_rlx32(&P.snap_ready[cur_e - 2])) != 0u;
             if (epoch_ok) order_acquire();
+        }
+        if (BATCH && !(cur_e < E && epoch_ok) && m >= 0 && m != parked_from && P.park &&
+            pool_nonempty<BATCH>(P)) {
+            // nothing to claim: take parked packets of this run into dead lanes
+            const uint64_t need = __ballot(!alive);
+            if (need) {
+                const int cnt = __popcll(need);
+                const int leader = (int)uniform32((unsigned)(__ffsll((unsigned long long)need) - 1));
+                unsigned long long h = 0, got = 0;
+                if (lane == leader) {
+                    for (int t = 0; t < 8 && !got; ++t) {
+                        unsigned long long hd = ld_rlx64(P.park_head), tl = ld_rlx64(P.park_tail);
+                        if (tl <= hd) break;
+                        unsigned long long want = std::min((unsigned long long)cnt, tl - hd);
+                        unsigned long long ex = hd;
+                        if (__hip_atomic_compare_exchange_strong(P.park_head, &ex, hd + want,
+                                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT)) {
+                            h = hd;
+                            got = want;
+                        }
+                    }
+                }
+                h = lane64(h, leader);
+                got = lane64(got, leader);
+                if (got) {
+                    last_adopt = iter;
+                    const unsigned long long r = __popcll(need & ((1ull << lane) - 1ull));
+                    if (!alive && r < got) {
+                        const unsigned long long idx = h + r;
+                        const size_t s0 = (size_t)(idx % P.park_cap);
+                        // the parker has reserved this state and writes it without
+                        // waiting on anything: the wait ends (bounded all the same)
+                        bool ready = false;
+                        for (int wt = 0; wt < (1 << 22) && !ready; ++wt) {
+                            ready = ld_rlx32(&P.park_flag[s0]) == (unsigned)(idx + 1ull);
+                            if (!ready) __builtin_amdgcn_s_sleep(2);
+                        }
+                        order_acquire();
+                        if (ready) {
+                            const unsigned *w = P.park + s0 * RT_PARK_WORDS;
+                            p.x = __uint_as_float(park_get(w, 0));
+                            p.y = __uint_as_float(park_get(w, 1));
+                            p.z = __uint_as_float(park_get(w, 2));
+                            p.dx = __uint_as_float(park_get(w, 3));
+                            p.dy = __uint_as_float(park_get(w, 4));
+                            p.dz = __uint_as_float(park_get(w, 5));
+                            p.ir = (int)park_get(w, 6);
+                            p.it = (int)park_get(w, 7);
+                            p.ip = (int)park_get(w, 8);
+                            p.hemi = (int)park_get(w, 9);
+                            pid = (unsigned long long)park_get(w, 10) |
+                                  ((unsigned long long)park_get(w, 11) << 32);
+                            krng = park_get(w, 12);
+                            ifr = (int)park_get(w, 13);
+                            tau = __uint_as_float(park_get(w, 14));
+                            nev = park_get(w, 15);
+                            nstep = park_get(w, 16);
+                            nev_cell = park_get(w, 17);
+                            nzero = park_get(w, 18);
+                            slice_left = park_get(w, 19);
+                            pe = (int)park_get(w, 20);
+                            slot = (int)park_get(w, 21);
+                            order_release();  // read before the slot is handed back
+                            st_rlx32(&P.park_flag[s0], 0u);
+                            xs = pe >= 2 ? P.xsnap + (size_t)(pe - 2) * P.slot : nullptr;
+                            tb = P.tslot + (size_t)slot * P.slot;
+                            ccell = -1;
+                            aRc = -1.f;
+                            alive = true;
+                        } else {
+                            atomicOr(A.abort, 1ull);  // a lost packet: the run is invalid
+                        }
+                    }
+                    if (lane == leader)
+                        __hip_atomic_fetch_add(A.parked, -(int)got, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
         if (cur_e < E && epoch_ok) {
             uint64_t need = __ballot(!alive);
@@ -929,14 +1153,14 @@ static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t fir
 }
 
 // pipeline counters and flags: u64 [ep_start E+1 | ep_claim | slot_start |
-// slot_in | slot_out | snap_time | t_begin, t_end | ep_cur | abort] then u32
-// [snap_claim | snap_done | snap_ready | next_snap, runs_done]
-#define PIPE_U64 (6 * RT_MAX_EPOCH + 5)
+// slot_in | slot_out | snap_time | t_begin, t_end | ep_cur | abort | park_head,
+// park_tail] then u32 [snap_claim | snap_done | snap_ready | next_snap, runs_done]
+#define PIPE_U64 (6 * RT_MAX_EPOCH + 7)
 #define PIPE_U32 (3 * RT_MAX_EPOCH + 4)
 
 struct PipeView {
     unsigned long long *ep_start, *ep_claim, *slot_start, *slot_in, *slot_out, *snap_time, *ep_cur,
-        *abort;
+        *abort, *park_head, *park_tail;
     unsigned int *u32base, *snap_claim, *snap_done, *snap_ready, *next_snap, *runs_done;
 };
 
@@ -950,6 +1174,8 @@ static PipeView pipe_view(void *d_pipe) {
     v.snap_time = v.slot_out + RT_MAX_EPOCH;  // then t_begin, t_end
     v.ep_cur = v.snap_time + RT_MAX_EPOCH + 2;
     v.abort = v.ep_cur + 1;
+    v.park_head = v.abort + 1;
+    v.park_tail = v.park_head + 1;
     v.u32base = (unsigned int *)(v.ep_start + PIPE_U64);
     v.snap_claim = v.u32base;
     v.snap_done = v.snap_claim + RT_MAX_EPOCH;
@@ -1079,6 +1305,8 @@ int thermal_plan(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_op
     P.snap_time = pv.snap_time;
     P.ep_cur = pv.ep_cur;
     P.runs_done = pv.runs_done;
+    P.park_head = pv.park_head;  // the pool itself is a batch's (P.park stays null)
+    P.park_tail = pv.park_tail;
     P.tslot = m->d_tally2;
     P.cum = m->d_tally;
     P.xsnap = m->d_xring;
@@ -1140,7 +1368,8 @@ void thermal_enqueue_finish(rt_model *m, const ThermPlan &rp, hipStream_t sm) {
 // LDS slices per wavefront, blocks per CU from the occupancy.
 int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, int nrun,
                            int lds_wave, int max_s, int max_g, const rt_thermal_opts *o,
-                           unsigned long long *d_abort, unsigned int *d_done, hipStream_t sm) {
+                           unsigned long long *d_abort, unsigned int *d_done, int *d_parked,
+                           hipStream_t sm) {
     rt_thermal_opts opt{};
     if (o) opt = *o;
     int bs = opt.block_size > 0 ? opt.block_size : 256;
@@ -1173,6 +1402,8 @@ int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, 
     A.max_ticks = (uint64_t)(max_sec * (double)ctx->wallclock_khz * 1000.0);
     A.abort = d_abort;
     A.runs_done = d_done;
+    A.parked = d_parked;
+    if (batch && !d_parked) return set_err(ctx, RT_E_ARG, "batch launch without a parked-packet counter");
     ThermParams P1{};
     if (!batch) P1 = *(const ThermParams *)h_run1;
     hipLaunchKernelGGL(kern, dim3((unsigned)(ctx->n_cu * bpc)), dim3(bs), lds, sm, A,
@@ -1225,6 +1456,7 @@ int thermal_collect(rt_model *m, const ThermPlan &rp, bool aborted, float kms, f
         st->wave_iters = cnt[CNT_WITER];
         st->n_nudged = cnt[CNT_NUDGE];
         st->idle_iters = cnt[CNT_IDLE];
+        st->n_parked = cnt[CNT_PARKED];
         for (int k = 0; k < 8; ++k) st->phase_clocks[k] = cnt[CNT_TK0 + k];
     }
     if (aborted)
@@ -1253,7 +1485,7 @@ int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opt
     HIPCHK(ctx, hipEventRecord(e_begin, sm));
     HIPCHK(ctx, hipEventRecord(e_k0, sm));
     rc = thermal_enqueue_launch(ctx, m->d_run, m->h_run, 1, rp.lds_wave, m->nspec, m->tab.ngroup,
-                                o, pv.abort, pv.runs_done, sm);
+                                o, pv.abort, pv.runs_done, nullptr, sm);
     if (rc) return rc;
     HIPCHK(ctx, hipEventRecord(e_k1, sm));
     thermal_enqueue_finish(m, rp, sm);
@@ -1284,6 +1516,17 @@ int thermal_params_copy(const rt_model *m, void *dst_host) {
 void thermal_params_set_done(rt_model *m, unsigned int *d_done) {
     ((ThermParams *)m->h_run)->runs_done = d_done;
 }
+
+// gives run m's parameter block a pool for parked packets: `cap` states of
+// RT_PARK_WORDS words and one flag word each (flags zero at the launch)
+void thermal_params_set_pool(rt_model *m, unsigned int *park, unsigned int *flag, uint32_t cap) {
+    ThermParams &P = *(ThermParams *)m->h_run;
+    P.park = park;
+    P.park_flag = flag;
+    P.park_cap = cap;
+}
+
+size_t thermal_park_words() { return RT_PARK_WORDS; }
 
 // where run m's epoch starts go (the reset of a batch step does not upload them)
 PipeUpload thermal_epoch_starts(rt_model *m) {

@@ -42,6 +42,32 @@ def test_batch_thermal_equals_single_runs_bitwise():
         assert np.array_equal(m.temperature(), single[i][0])
 
 
+def test_batch_parking_keeps_results_bitwise():
+    # many small models with uneven packet counts: waves run out of work in one
+    # model while others still hand out epochs, so packets get parked
+    print("case: batch of 12 models, parking on vs off", flush=True)
+    ps = configs.sampler_batch(12, seed=3)
+    for k, p in enumerate(ps):
+        p.update(nr=16 + 8 * (k % 3), nt=16, nphot=(20000, 60000, 150000)[k % 3])
+    models = [rt.DeviceModel(rt.model_arrays(configs.batch_model(p).grid)) for p in ps]
+    nph = [p["nphot"] for p in ps]
+    seeds = list(range(100, 112))
+    b = rt.Batch(models)
+    opts = dict(max_events=100000, mrw=1)
+    st0 = b.thermal(nph, seeds, no_park=1, **opts)
+    T0 = [m.temperature() for m in models]
+    assert sum(x["n_parked"] for x in st0) == 0
+    parked = 0
+    for rep in range(2):
+        st1 = b.thermal(nph, seeds, **opts)
+        parked += sum(x["n_parked"] for x in st1)
+        for i, m in enumerate(models):
+            assert np.array_equal(m.temperature(), T0[i]), (rep, i)
+            for k in ("n_cross", "n_abs", "n_scat", "n_escape", "n_killed", "n_events", "n_mrw"):
+                assert st1[i][k] == st0[i][k], (rep, i, k)
+    print("parked packets over 2 launches: %d" % parked, flush=True)
+
+
 def _snapshot(s):
     return ([m.temperature() for m in s.models], [float(c.item()) for c in s.chi2],
             [x.cpu().numpy().copy() for x in s.img], [g.cpu().numpy().copy() for g in s.g_T],
