@@ -77,6 +77,24 @@ def batch_main(a):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / a.steps
     print("batch: %d steps, %.2f s each" % (a.steps, wall), file=sys.stderr, flush=True)
+    # a new parameter draw for the same grids: densities rebuilt on the device,
+    # then a graph step with them; the host builders timed on 8 of the models
+    new = configs.sampler_batch(a.batch_models, seed=1)
+    for p, q in zip(new, params):
+        p.update(nr=q["nr"], nt=q["nt"], nphot=q["nphot"])
+    t_gen = s.set_params(new)
+    t0 = time.perf_counter()
+    stats_new, _ = s.step(seed=7, use_graph=True)
+    torch.cuda.synchronize()
+    t_new = time.perf_counter() - t0
+    chi_new = [float(c.item()) for c in s.chi2]
+    t0 = time.perf_counter()
+    for p in new[:8]:
+        configs.batch_model(p)
+    t_host_gen = (time.perf_counter() - t0) / 8
+    print("batch: new parameters: device densities %.1f ms for %d models, step %.2f s; host "
+          "builders %.2f s per model" % (t_gen * 1e3, s.n, t_new, t_host_gen),
+          file=sys.stderr, flush=True)
     seq = None
     if not a.no_sequential:
         seq, _ = s.sequential(seed=0, progress=lambda i, t: print(
@@ -97,6 +115,10 @@ def batch_main(a):
         "packets_per_s": npk / wall, "build_s": t_build,
         "sequential_s": seq, "speedup_vs_sequential": (seq / wall) if seq else None,
         "killed": int(sum(x["n_killed"] for x in stats)),
+        "new_params": {"device_density_ms": t_gen * 1e3, "step_s": t_new,
+                       "host_builder_s_per_model": t_host_gen,
+                       "chi2_finite": bool(np.all(np.isfinite(chi_new))),
+                       "killed": int(sum(x["n_killed"] for x in stats_new))},
     }
     print(json.dumps(out), flush=True)
 

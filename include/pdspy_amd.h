@@ -146,6 +146,33 @@ int rt_model_ncells(const rt_model *m);
 /* Replace the model's densities (host, same layout as rt_model_desc.h_rho). */
 int rt_model_set_density(rt_model *m, const double *h_rho);
 
+/* Density components generated on the device (pdspy's model builders, which
+ * run on the host there):
+ *   RT_COMP_DISK        Disk.density, pdspy/modeling/Disk.py:48-135
+ *                       (mass, rmin, rmax, plrho, h0, plh)
+ *   RT_COMP_ULRICH_EXT  UlrichEnvelopeExtended.density,
+ *                       pdspy/modeling/UlrichEnvelopeExtended.py:37-110
+ *                       (mass, rmin, rmax, rcent, cavpl, cavrfact, theta_open,
+ *                       zoffset)
+ * Masses in Msun, lengths in au, angles in degrees, as pdspy's add_disk /
+ * add_ulrichextended_envelope take them.  Each component adds into the
+ * density of `species`; species no component names get zero.           */
+enum { RT_COMP_DISK = 1, RT_COMP_ULRICH_EXT = 2 };
+typedef struct {
+    int kind;
+    int species;
+    double mass, rmin, rmax;
+    double plrho, h0, plh;                               /* disk */
+    double rcent, cavpl, cavrfact, theta_open, zoffset;  /* envelope */
+} rt_density_comp;
+/* Replaces the model's densities with the sum of up to 8 components computed
+ * on the device (FP64; the envelope's mu0 roots by the same Brent iteration
+ * as the host builder).  Synchronous; the host copy of the densities is not
+ * updated (rt_get_density_device reads the result). */
+int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp);
+/* Device copy of the model's densities as float [nspec][cells]. */
+int rt_get_density_device(rt_model *m, float *d_rho);
+
 /* Thermal MC (RADMC-3D `mctherm`, run.py:9-38): nphot packets, Philox key =
  * seed.  Temperatures stay on the device; bit-reproducible for a given seed. */
 int rt_thermal(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *opts,

@@ -14,6 +14,7 @@ LIB_PATH = os.environ.get("PDSPY_AMD_LIB") or os.path.join(HERE, "lib", "libpdsp
 EXPORTS = [
     "rt_create", "rt_destroy", "rt_set_stream", "rt_last_error", "rt_device_info",
     "rt_model_create", "rt_model_destroy", "rt_model_ncells", "rt_model_set_density",
+    "rt_model_generate_density", "rt_get_density_device",
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_chisq",
@@ -38,6 +39,17 @@ class ModelDesc(C.Structure):
                 ("h_ksca", C.c_void_p), ("h_g", C.c_void_p), ("nstar", C.c_int),
                 ("h_stars", C.c_void_p), ("ntemp", C.c_int), ("temp0", C.c_double),
                 ("temp1", C.c_double)]
+
+
+RT_COMP_DISK, RT_COMP_ULRICH_EXT = 1, 2
+
+
+class DensityComp(C.Structure):
+    _fields_ = [("kind", C.c_int), ("species", C.c_int), ("mass", C.c_double),
+                ("rmin", C.c_double), ("rmax", C.c_double), ("plrho", C.c_double),
+                ("h0", C.c_double), ("plh", C.c_double), ("rcent", C.c_double),
+                ("cavpl", C.c_double), ("cavrfact", C.c_double), ("theta_open", C.c_double),
+                ("zoffset", C.c_double)]
 
 
 class ThermalOpts(C.Structure):
@@ -103,6 +115,8 @@ def load():
         "rt_model_destroy": ([P], I),
         "rt_model_ncells": ([P], I),
         "rt_model_set_density": ([P, P], I),
+        "rt_model_generate_density": ([P, C.POINTER(DensityComp), I], I),
+        "rt_get_density_device": ([P, P], I),
         "rt_thermal": ([P, U64, U64, C.POINTER(ThermalOpts), C.POINTER(ThermalStats)], I),
         "rt_get_temperature": ([P, P, P], I),
         "rt_thermal_timeline": ([P, P, I], I),

@@ -94,5 +94,22 @@ def batch_model(p):
                          plrho=p["gamma"] + p["plh"], plh=p["plh"], menv=p["menv"])
 
 
+def disk_envelope_components(mdisk=1e-3, plrho=1.75, plh=1.25, menv=0.1, rcent=50.,
+                             envelope=True):
+    """The density components of ``disk_envelope`` for the device generator
+    (``DeviceModel.generate_density``): the disk in species 0, the envelope in 1."""
+    comps = [dict(kind="disk", species=0, mass=mdisk, rmin=0.1, rmax=100., plrho=plrho,
+                  h0=0.1, plh=plh)]
+    if envelope:
+        comps.append(dict(kind="ulrich_ext", species=1, mass=menv, rmin=0.1, rmax=5000.,
+                          rcent=rcent, cavpl=1.0, cavrfact=0.01, theta_open=20., zoffset=1.))
+    return comps
+
+
+def batch_components(p):
+    return disk_envelope_components(mdisk=p["mdisk"], plrho=p["gamma"] + p["plh"], plh=p["plh"],
+                                    menv=p["menv"])
+
+
 __all__ = ["powerlaw_dust", "disk_c1", "disk_envelope", "spiral_3d", "uv_points",
-           "sampler_batch", "batch_model", "AU"]
+           "sampler_batch", "batch_model", "disk_envelope_components", "batch_components", "AU"]

@@ -65,7 +65,8 @@ static void free_model(rt_model *m) {
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
-                    m->d_xring, m->d_run};
+                    m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
+                    m->d_dens_fac, m->d_comps};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     std::free(m->h_run);
@@ -338,7 +339,21 @@ int rt_model_set_density(rt_model *m, const double *h_rho) {
         if (h_rho[i] < 0) return set_err(m->ctx, RT_E_ARG, "negative density");
     m->rho.assign(h_rho, h_rho + n);
     m->have_T = false;
+    m->rho_host_stale = false;
     return upload_density(m);
+}
+
+int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp) {
+    if (!m) return RT_E_ARG;
+    return density_generate(m, h_comps, ncomp);
+}
+
+int rt_get_density_device(rt_model *m, float *d_rho) {
+    if (!m || !d_rho) return RT_E_ARG;
+    rt_ctx *ctx = m->ctx;
+    HIPCHK(ctx, hipMemcpyAsync(d_rho, m->d_rho, sizeof(float) * m->nspec * m->ncell,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    return RT_OK;
 }
 
 int rt_thermal(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *opts,

@@ -12,6 +12,7 @@
 #define RT_MAX_GROUP 4   // distinct absorption opacities per model
 #define RT_MAX_LAM 255   // wavelength bins (guide tables are uint8)
 #define RT_MAX_STAR 16   // point stars per model
+#define RT_MAX_COMP 8    // density components of the device generator
 #define RT_NGUIDE 128    // guide-table entries per CDF row
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
@@ -115,6 +116,14 @@ struct rt_model {
     double *d_galpha = nullptr, *d_gsrc = nullptr;
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
+    // device density generator: cell centres, per-component scratch
+    double *d_rc = nullptr, *d_tc = nullptr;      // r (cm), theta centres
+    double *d_dens_w = nullptr;                   // [RT_MAX_COMP][nt][nr]
+    double *d_dens_rows = nullptr;                // [RT_MAX_COMP][nr]
+    double *d_dens_fac = nullptr;                 // [2][RT_MAX_COMP]
+    void *d_comps = nullptr;                      // [RT_MAX_COMP] rt_density_comp
+    double dens_solid = 0;                        // 4 pi (2 pi for full-sphere grids)
+    bool rho_host_stale = false;                  // densities generated on the device
     void *h_run = nullptr;        // host parameter block of the thermal kernel (ThermParams)
     void *d_run = nullptr;        // its device copy
     std::vector<unsigned long long> h_est;  // epoch starts of the last planned run
@@ -175,6 +184,9 @@ int batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
                const rt_thermal_opts *o, const rt_step_item *items, int use_graph,
                rt_thermal_stats *stats, double *step_ms);
 rt::DevGrid dev_grid(const rt_model *m);
+int density_prepare(rt_model *m, const rt_density_comp *h_comps, int ncomp);
+void density_enqueue(rt_model *m, const rt_density_comp *d_comps, int ncomp, hipStream_t sm);
+int density_generate(rt_model *m, const rt_density_comp *h_comps, int ncomp);
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                 rt_thermal_stats *st);
 int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,

@@ -496,6 +496,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     int cur_e = 0;          // wave-uniform: epoch packets are claimed from
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
     uint64_t n_idle = 0;
+    uint32_t idle_streak = 0;  // consecutive idle polls (back-off)
     uint32_t iter = 0;
     uint32_t last_adopt = 0;  // iteration of the wave's last adoption of parked packets
     int parked_from = -1;     // run whose pool the wave has just filled (not to be emptied by it)
@@ -600,17 +601,26 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                     epoch_ok = !BATCH;
                     scan = r;
                     parked_from = -1;
+                    idle_streak = 0;
                 } else {
                     // nothing to hand out anywhere: help a snapshot, finish or wait
                     if (uniform32(ld_rlx32(A.runs_done)) >= (unsigned)A.nrun &&
                         (!BATCH || uniform32((unsigned)__hip_atomic_load(
                                        A.parked, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0u))
                         break;
-                    snapshot_help(run_at<BATCH>(runs, P1, scan), lane);
+                    const bool helped = snapshot_help(run_at<BATCH>(runs, P1, scan), lane);
                     scan = scan + 1 == A.nrun ? 0 : scan + 1;
                     parked_from = -1;  // no other run has work: its own pool is fair game
                     n_idle++;
-                    __builtin_amdgcn_s_sleep(32);
+                    // back off while nothing turns up: every idle poll reads all
+                    // runs' counters, load the remaining walks share
+                    idle_streak = helped ? 0u : idle_streak + 1u;
+                    if (idle_streak < 8u) {
+                        __builtin_amdgcn_s_sleep(32);
+                    } else {
+                        const int naps = idle_streak < 64u ? 1 : 4;
+                        for (int z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+                    }
                     RT_PHASE(3);
                     continue;
                 }

@@ -173,10 +173,34 @@ class DeviceModel:
         return out
 
     def rho_device(self):
+        """The model's densities on the device, float [nspec, cells]."""
         import torch
         if self._rho_dev is None:
-            self._rho_dev = torch.tensor(self.rho_host, dtype=torch.float32, device="cuda")
+            out = torch.empty((self.nspec, self.ncell), dtype=torch.float32, device="cuda")
+            self.ctx.sync_stream()
+            self.ctx.check(self.ctx.L.rt_get_density_device(self.h, C.c_void_p(out.data_ptr())))
+            self.ctx.sync_stream()
+            self._rho_dev = out
         return self._rho_dev
+
+    def generate_density(self, components):
+        """Replaces the densities with components computed on the device:
+        dicts with ``kind`` ("disk" or "ulrich_ext"), ``species`` and the
+        parameters of pdspy's add_disk / add_ulrichextended_envelope (Msun,
+        au, degrees).  See ``density_components``."""
+        arr = (_lib.DensityComp * len(components))()
+        for k, c in enumerate(components):
+            c = dict(c)
+            kind = {"disk": _lib.RT_COMP_DISK, "ulrich_ext": _lib.RT_COMP_ULRICH_EXT}[c.pop("kind")]
+            arr[k].kind = kind
+            for key, v in c.items():
+                if not hasattr(arr[k], key):
+                    raise TypeError("unknown density component parameter %r" % key)
+                setattr(arr[k], key, v)
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_model_generate_density(self.h, arr, len(components)))
+        self.rho_host = None
+        self._rho_dev = None
 
     @staticmethod
     def camera(npix, pix_cm, lam_um, incl=0.0, pa=0.0, phi=0.0, mode=0, incl_star=True,

@@ -98,6 +98,31 @@ class SamplerStep:
         self._items = None
         return dict(thermal=best, tile=tile)
 
+    def set_params(self, params):
+        """A new parameter draw for the same models (grids and photon counts
+        stay): densities are rebuilt on the device (disk + Ulrich envelope
+        generator) and the cameras take the new inclinations.  Returns the
+        wall seconds of the density generation."""
+        import torch
+        params = list(params)
+        if len(params) != self.n:
+            raise ValueError("expected %d parameter sets" % self.n)
+        for p, q in zip(params, self.params):
+            if (p["nr"], p["nt"], int(p["nphot"])) != (q["nr"], q["nt"], int(q["nphot"])):
+                raise ValueError("set_params keeps each model's grid and photon count")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for m, p in zip(self.models, params):
+            m.generate_density(configs.batch_components(p))
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        self.params = params
+        tile = (self.cams[0].tile_w, self.cams[0].tile_h)
+        self.cams = [rt.DeviceModel.camera(self.npix, self.pix_cm, LAM_UM, incl=p["incl"],
+                                           tile=tile) for p in self.params]
+        self._items = None
+        return dt
+
     def items(self):
         if self._items is None:
             self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u, self.v, self.re[i],

@@ -148,3 +148,27 @@ def test_invalid_inputs_rejected():
     bad["rw"] = arr["rw"][::-1].copy()
     with pytest.raises(RTError):
         rt.DeviceModel(bad)
+
+
+def test_device_density_generator_matches_host_builders():
+    # the disk + Ulrich envelope builders on the device against the host ones
+    # (scipy brenth for mu0, numpy trapezoids); includes r < Rc cells where the
+    # cubic for mu0 has three real roots
+    print("case: device density generator vs host builders", flush=True)
+    cases = [dict(nr=256, nt=128, mdisk=1e-3, plrho=1.75, plh=1.25, menv=0.1),
+             dict(nr=64, nt=64, mdisk=3e-5, plrho=1.2, plh=1.05, menv=2e-3, rcent=20.0),
+             dict(nr=128, nt=64, mdisk=8e-3, plrho=2.6, plh=1.3, menv=0.25, rcent=120.0)]
+    for c in cases:
+        host = configs.disk_envelope(**c)
+        m = rt.DeviceModel(rt.model_arrays(configs.disk_envelope(nr=c["nr"], nt=c["nt"]).grid))
+        comps = configs.disk_envelope_components(
+            mdisk=c["mdisk"], plrho=c["plrho"], plh=c["plh"], menv=c["menv"],
+            rcent=c.get("rcent", 50.0))
+        m.generate_density(comps)
+        got = m.rho_device().cpu().numpy().astype(np.float64)
+        want = rt.model_arrays(host.grid)["rho"].reshape(m.nspec, m.ncell).astype(np.float32)
+        want = want.astype(np.float64)
+        assert got.shape == want.shape
+        assert np.array_equal(got > 0, want > 0), c
+        rel = np.abs(got - want) / np.maximum(want, 1e-300)
+        assert rel.max() <= 2e-7, (c, rel.max())
