@@ -66,6 +66,12 @@ class _ThermStats(C.Structure):
                 ("n_nudged", C.c_uint64)]
 
 
+class _ScatStats(C.Structure):
+    _fields_ = [("n_escape", C.c_uint64), ("n_abs", C.c_uint64), ("n_scat", C.c_uint64),
+                ("n_cross", C.c_uint64), ("n_killed", C.c_uint64), ("lum_star", C.c_double),
+                ("lum_dust", C.c_double), ("eps", C.c_double)]
+
+
 class _Camera(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsub", C.c_int),
                 ("pix_cm", C.c_double), ("incl", C.c_double), ("posang", C.c_double),
@@ -80,6 +86,11 @@ def _setup(L):
     L.or_image.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P]
     L.or_image_adjoint.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P, P, P]
     L.or_trace_ray.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P]
+    L.or_scat_source.argtypes = [C.POINTER(_Model), P, P, P, C.c_double, C.c_uint64,
+                                 C.c_uint64, C.c_uint64, C.c_int, P, C.POINTER(_ScatStats)]
+    L.or_image_scat.argtypes = [C.POINTER(_Model), P, P, P, P, C.POINTER(_Camera), P]
+    L.or_image_adjoint_scat.argtypes = [C.POINTER(_Model), P, P, P, P, C.POINTER(_Camera), P,
+                                        P, P]
     L.or_vis_dft.argtypes = [P, C.c_int, C.c_int, C.c_double, P, P, C.c_int,
                              C.c_double, C.c_double, P, P]
     L.or_vis_dft_adjoint.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, P,
@@ -164,23 +175,46 @@ def _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_
                    int(incl_star), float(tau_stop))
 
 
+def scat_source(m, T, kap_abs, kap_sca, lam_um, nphot, seed=0, max_events=1000000,
+                nthreads=0):
+    """Mean intensity J[cells] at lam_um from the monochromatic scattering
+    Monte Carlo; returns (J, stats dict)."""
+    pm = _Pinned(m)
+    T = _f64(T).reshape(-1)
+    ka = _f64(kap_abs).reshape(-1)
+    ks = _f64(kap_sca).reshape(-1)
+    J = np.zeros(pm.ncell)
+    st = _ScatStats()
+    rc = lib().or_scat_source(C.byref(pm.s), T.ctypes.data, ka.ctypes.data, ks.ctypes.data,
+                              float(lam_um), int(nphot), int(seed), int(max_events),
+                              int(nthreads), J.ctypes.data, C.byref(st))
+    if rc != 0:
+        raise RuntimeError("or_scat_source failed: %d" % rc)
+    return J, {k: getattr(st, k) for k, _ in _ScatStats._fields_}
+
+
 def image(m, T, kap, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0, lam_um=1300.0,
-          mode=0, incl_star=False, nsub=1, tau_stop=40.0):
-    """Returns img[ny][nx] (erg/s/cm^2/Hz/sr, or tau for mode=1)."""
+          mode=0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None):
+    """Returns img[ny][nx] (erg/s/cm^2/Hz/sr, or tau for mode=1).  With ksca
+    and J (from scat_source): isotropic scattering source."""
     pm = _Pinned(m)
     T = _f64(T).reshape(-1)
     kap = _f64(kap).reshape(-1)
     cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop)
     img = np.zeros((cam.ny, cam.nx))
-    rc = lib().or_image(C.byref(pm.s), T.ctypes.data, kap.ctypes.data, C.byref(cam),
-                        img.ctypes.data)
+    ks = None if ksca is None else _f64(ksca).reshape(-1)
+    Jv = None if J is None else _f64(J).reshape(-1)
+    rc = lib().or_image_scat(C.byref(pm.s), T.ctypes.data, kap.ctypes.data,
+                             None if ks is None else ks.ctypes.data,
+                             None if Jv is None else Jv.ctypes.data, C.byref(cam),
+                             img.ctypes.data)
     if rc != 0:
         raise RuntimeError("or_image failed: %d" % rc)
     return img
 
 
 def image_adjoint(m, T, kap, dimg, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0,
-                  lam_um=1300.0, incl_star=False, nsub=1, tau_stop=40.0):
+                  lam_um=1300.0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None):
     pm = _Pinned(m)
     T = _f64(T).reshape(-1)
     kap = _f64(kap).reshape(-1)
@@ -188,9 +222,13 @@ def image_adjoint(m, T, kap, dimg, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0,
     cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, 0, incl_star, nsub, tau_stop)
     g_lr = np.zeros((pm.nspec, pm.ncell))
     g_T = np.zeros(pm.ncell)
-    rc = lib().or_image_adjoint(C.byref(pm.s), T.ctypes.data, kap.ctypes.data,
-                                C.byref(cam), dimg.ctypes.data, g_lr.ctypes.data,
-                                g_T.ctypes.data)
+    ks = None if ksca is None else _f64(ksca).reshape(-1)
+    Jv = None if J is None else _f64(J).reshape(-1)
+    rc = lib().or_image_adjoint_scat(C.byref(pm.s), T.ctypes.data, kap.ctypes.data,
+                                     None if ks is None else ks.ctypes.data,
+                                     None if Jv is None else Jv.ctypes.data,
+                                     C.byref(cam), dimg.ctypes.data, g_lr.ctypes.data,
+                                     g_T.ctypes.data)
     if rc != 0:
         raise RuntimeError("or_image_adjoint failed: %d" % rc)
     return g_lr, g_T

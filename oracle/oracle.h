@@ -80,6 +80,19 @@ typedef struct {
 int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
                uint64_t *visits, or_thermal_stats *st);
 
+/* Scattering source: mean intensity J_nu[cells] (erg/s/cm^2/Hz/sr) at lam_um
+ * from a monochromatic Monte Carlo with isotropic scattering (sources: stars
+ * and the dust's thermal emission at the imaging temperature T[cells]).
+ * kap_abs / kap_sca[nspec]: opacities at lam_um (cm^2/g).                    */
+typedef struct {
+    uint64_t n_escape, n_abs, n_scat, n_cross, n_killed;
+    double lum_star, lum_dust; /* source luminosities L_nu (erg/s/Hz)      */
+    double eps;                /* packet energy (L_star + L_dust) / nphot   */
+} or_scat_stats;
+int or_scat_source(const or_model *m, const double *T, const double *kap_abs,
+                   const double *kap_sca, double lam_um, uint64_t nphot, uint64_t seed,
+                   uint64_t max_events, int nthreads, double *J, or_scat_stats *st);
+
 /* Camera.  Parallel rays; pixel (j,k) centre at ((k - nx/2)*pix, (j - ny/2)*pix)
  * cm in the image plane (integer division, the centring pdspy's zoomau
  * produces, Model.py:522-527), rotated by posang.  incl/posang/phi in degrees.
@@ -106,6 +119,15 @@ int or_image(const or_model *m, const double *T, const double *kap,
 int or_image_adjoint(const or_model *m, const double *T, const double *kap,
                      const or_camera *cam, const double *dimg,
                      double *g_logrho, double *g_T);
+
+/* The same with a scattering source: extinction rho (kabs + ksca), source
+ * (rho kabs B + rho ksca J) / (rho (kabs + ksca)); J[cells] from
+ * or_scat_source, held fixed in the gradient.  ksca/J NULL = emission only. */
+int or_image_scat(const or_model *m, const double *T, const double *kap,
+                  const double *ksca, const double *J, const or_camera *cam, double *img);
+int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
+                          const double *ksca, const double *J, const or_camera *cam,
+                          const double *dimg, double *g_logrho, double *g_T);
 
 /* Ray segments through the grid: origin o[3], direction d[3] (unit); writes up
  * to maxseg (cell, length) pairs in crossing order, cell = -1 for the inner

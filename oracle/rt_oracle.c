@@ -1389,6 +1389,200 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
     return ovf ? -20 : 0;
 }
 
+/* ------------------------------------------------ scattering source MC */
+/* Monochromatic Monte Carlo for the mean intensity J_nu per cell at the image
+ * wavelength, which RADMC-3D computes before each image when the opacities
+ * carry scattering (nphot_scat packets, pdspy/radmc3d/write.py:34 and
+ * Model.py:520; run.py:157-289), for isotropic scattering.  Sources: the
+ * stars (L_nu = 4 pi^2 R^2 B_nu(T*)) and the thermal emission of the dust at
+ * the imaging temperature (L_nu,c = 4 pi V_c B_nu(T_c) sum_s rho_s kabs_s).
+ * Packets of energy L/N travel with extinction rho (kabs + ksca); at each
+ * interaction they scatter isotropically with probability ksca / kext and
+ * end otherwise.  Path-length estimator: J_c = (L/N) sum l / (4 pi V_c).
+ * Random events per packet (Philox key = seed, counter (k, packet)):
+ * k = 0: u0 source, u1 u2 u3 position in the cell (r^3, cos theta, phi
+ * uniform), u4 u5 direction, u6 first tau, u7 hemisphere (mirrored grids);
+ * k >= 1 at the k-th interaction: u0 scatter or end, u1 u2 new direction,
+ * u3 next tau.                                                           */
+static int sample_cum(const double *cum, int n, double x) {
+    /* first i with cum[i] > x (cum ascending, cum[n-1] = total) */
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (cum[mid] > x)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+int or_scat_source(const or_model *m, const double *T, const double *kap_abs,
+                   const double *kap_sca, double lam_um, uint64_t nphot, uint64_t seed,
+                   uint64_t max_events, int nthreads, double *J, or_scat_stats *st) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    const int nc = g.ncell, ns = m->nspec;
+    const double nu = C_C / (lam_um * 1e-4);
+    /* source luminosities: stars first, then cells (cumulative) */
+    double *cum = malloc(sizeof(double) * (size_t)(m->nstar + nc));
+    double acc = 0;
+    for (int k = 0; k < m->nstar; ++k) {
+        double R = m->star[5 * k];
+        acc += 4.0 * PI_D * PI_D * R * R * or_planck(nu, m->star[5 * k + 1]);
+        cum[k] = acc;
+    }
+    const double lstar = acc;
+    for (int c = 0; c < nc; ++c) {
+        double a = 0;
+        for (int sp = 0; sp < ns; ++sp) a += m->rho[(size_t)sp * nc + c] * kap_abs[sp];
+        acc += 4.0 * PI_D * g.vol[c] * a * or_planck(nu, T[c]);
+        cum[m->nstar + c] = acc;
+    }
+    const double ltot = acc;
+    memset(J, 0, sizeof(double) * nc);
+    or_scat_stats tot;
+    memset(&tot, 0, sizeof(tot));
+    tot.lum_star = lstar;
+    tot.lum_dust = ltot - lstar;
+    if (!(ltot > 0) || nphot == 0) {
+        free(cum);
+        geom_free(&g);
+        if (st) *st = tot;
+        return 0;
+    }
+    int nth = nthreads > 0 ? nthreads : 1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nth = omp_get_max_threads();
+#else
+    nth = 1;
+#endif
+    double *part = calloc((size_t)nth * nc, sizeof(double));
+#pragma omp parallel num_threads(nth)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *Jt = part + (size_t)tid * nc;
+        uint64_t c_esc = 0, c_abs = 0, c_sca = 0, c_cross = 0, c_kill = 0;
+#pragma omp for schedule(dynamic, 256)
+        for (long long ip = 0; ip < (long long)nphot; ++ip) {
+            const uint64_t pid = (uint64_t)ip;
+            double u[8];
+            rng8(seed, pid, 0, u);
+            pk_t p;
+            memset(&p, 0, sizeof(p));
+            const int src = sample_cum(cum, m->nstar + nc, u[0] * ltot);
+            if (src < m->nstar) {
+                p.x = m->star[5 * src + 2];
+                p.y = m->star[5 * src + 3];
+                p.z = m->star[5 * src + 4];
+            } else {
+                const int c = src - m->nstar;
+                const int ir = c % g.nr, it = (c / g.nr) % g.nt, iq = c / (g.nr * g.nt);
+                const double r1 = m->rw[ir], r2 = m->rw[ir + 1];
+                const double r = cbrt(r1 * r1 * r1 + u[1] * (r2 * r2 * r2 - r1 * r1 * r1));
+                const double ct = g.ct[it] + u[2] * (g.ct[it + 1] - g.ct[it]);
+                const double sth = sqrt(fmax(0.0, 1.0 - ct * ct));
+                const double ph = m->pw[iq] + u[3] * (m->pw[iq + 1] - m->pw[iq]);
+                p.x = r * sth * cos(ph);
+                p.y = r * sth * sin(ph);
+                p.z = r * ct * ((g.mirror && u[7] < 0.5) ? -1.0 : 1.0);
+            }
+            double d[3];
+            iso_dir(u[4], u[5], d);
+            p.dx = d[0];
+            p.dy = d[1];
+            p.dz = d[2];
+            locate(&g, &p);
+            if (p.ir >= g.nr) {
+                c_esc++;
+                continue;
+            }
+            double tau = -log(u[6]);
+            uint32_t k = 1;
+            int nzero = 0;
+            uint64_t nev = 0;
+            for (;;) {
+                int kind;
+                double tw = next_wall(&g, &p, &kind);
+                double aabs = 0, asca = 0;
+                int c = -1;
+                if (p.ir >= 0) {
+                    c = cell_of(&g, &p);
+                    for (int sp = 0; sp < ns; ++sp) {
+                        double rho = m->rho[(size_t)sp * nc + c];
+                        aabs += rho * kap_abs[sp];
+                        asca += rho * kap_sca[sp];
+                    }
+                }
+                double aext = aabs + asca;
+                if (aext * tw > tau) {
+                    double sseg = tau / aext;
+                    Jt[c] += sseg;
+                    p.x += sseg * p.dx;
+                    p.y += sseg * p.dy;
+                    p.z += sseg * p.dz;
+                    double v[8];
+                    rng8(seed, pid, k++, v);
+                    nev++;
+                    if (v[0] * aext < asca) {
+                        c_sca++;
+                        iso_dir(v[1], v[2], d);
+                        p.dx = d[0];
+                        p.dy = d[1];
+                        p.dz = d[2];
+                        tau = -log(v[3]);
+                        nzero = 0;
+                        if (max_events && nev >= max_events) {
+                            c_kill++;
+                            break;
+                        }
+                    } else {
+                        c_abs++;
+                        break;
+                    }
+                } else {
+                    if (c >= 0) Jt[c] += tw;
+                    tau -= aext * tw;
+                    c_cross++;
+                    double rcell = g.rw[p.ir + 1];
+                    if (cross_wall(&g, &p, tw, kind)) {
+                        c_esc++;
+                        break;
+                    }
+                    if (unstick(&g, &p, tw, rcell, &nzero) && p.ir >= g.nr) {
+                        c_esc++;
+                        break;
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        {
+            tot.n_escape += c_esc;
+            tot.n_abs += c_abs;
+            tot.n_scat += c_sca;
+            tot.n_cross += c_cross;
+            tot.n_killed += c_kill;
+        }
+    }
+    const double eps = ltot / (double)nphot;
+    for (int c = 0; c < nc; ++c) {
+        double sum = 0;
+        for (int t = 0; t < nth; ++t) sum += part[(size_t)t * nc + c];
+        J[c] = eps * sum / (4.0 * PI_D * g.vol[c]);
+    }
+    tot.eps = eps;
+    if (st) *st = tot;
+    free(part);
+    free(cum);
+    geom_free(&g);
+    return 0;
+}
+
 /* ---------------------------------------------------------------- camera */
 typedef struct {
     double n[3], ex[3], ey[3];
@@ -1421,9 +1615,9 @@ typedef struct {
 /* Integrate one ray entering from the observer side.  Collects segments when
  * segs != NULL.  Returns intensity (mode 0) or total tau (mode 1).         */
 static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
-                            const double *kap, const or_camera *cam,
-                            const cam_frame_t *f, double X, double Y, seg_t *segs,
-                            int maxseg, int *nseg) {
+                            const double *kap, const double *ksca, const double *Jsc,
+                            const or_camera *cam, const cam_frame_t *f, double X, double Y,
+                            seg_t *segs, int maxseg, int *nseg) {
     /* sky (X,Y) -> model image plane (x', y') rotated by -posang */
     double xp = X * f->cpa + Y * f->spa;
     double yp = -X * f->spa + Y * f->cpa;
@@ -1460,10 +1654,17 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
         double rcell = g->rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_of(g, &p);
-            double a = 0;
-            for (int s = 0; s < ns; ++s) a += m->rho[(size_t)s * g->ncell + c] * kap[s];
+            /* extinction and source function; with a scattering source
+             * (isotropic): S = (rho kabs B + rho ksca J) / (rho (kabs + ksca)) */
+            double aa = 0, as = 0;
+            for (int s = 0; s < ns; ++s) {
+                aa += m->rho[(size_t)s * g->ncell + c] * kap[s];
+                if (Jsc) as += m->rho[(size_t)s * g->ncell + c] * ksca[s];
+            }
+            double a = aa + as;
             double dtau = a * tw;
-            double S = or_planck(nu, T[c]);
+            double B = or_planck(nu, T[c]);
+            double S = Jsc ? (a > 0 ? (aa * B + as * Jsc[c]) / a : 0.0) : B;
             if (cam->mode == 0) I += exp(-tau) * (-expm1(-dtau)) * S;
             if (segs && n < maxseg) {
                 segs[n].cell = c;
@@ -1485,8 +1686,8 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
 
 /* optical depth from point q to the observer along +n */
 static double star_tau(const or_model *m, const geom_t *g, const double *kap,
-                       const double *q, const double *n, seg_t *segs, int maxseg,
-                       int *nseg) {
+                       const double *ksca, const double *q, const double *n, seg_t *segs,
+                       int maxseg, int *nseg) {
     pk_t p;
     memset(&p, 0, sizeof(p));
     p.x = q[0];
@@ -1505,7 +1706,8 @@ static double star_tau(const or_model *m, const geom_t *g, const double *kap,
         if (p.ir >= 0) {
             int c = cell_of(g, &p);
             double a = 0;
-            for (int s = 0; s < m->nspec; ++s) a += m->rho[(size_t)s * g->ncell + c] * kap[s];
+            for (int s = 0; s < m->nspec; ++s)
+                a += m->rho[(size_t)s * g->ncell + c] * (kap[s] + (ksca ? ksca[s] : 0.0));
             if (segs && k < maxseg) {
                 segs[k].cell = c;
                 segs[k].s = tw;
@@ -1538,6 +1740,11 @@ static int star_pixel(const or_camera *cam, const cam_frame_t *f, const double *
 
 int or_image(const or_model *m, const double *T, const double *kap,
              const or_camera *cam, double *img) {
+    return or_image_scat(m, T, kap, NULL, NULL, cam, img);
+}
+
+int or_image_scat(const or_model *m, const double *T, const double *kap,
+                  const double *ksca, const double *Jsc, const or_camera *cam, double *img) {
     geom_t g;
     int rc = geom_init(&g, m);
     if (rc) return rc;
@@ -1552,7 +1759,8 @@ int or_image(const or_model *m, const double *T, const double *kap,
                 for (int b = 0; b < nsub; ++b) {
                     double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
                     double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
-                    acc += trace_cam_ray(m, &g, T, kap, cam, &f, X, Y, NULL, 0, NULL);
+                    acc += trace_cam_ray(m, &g, T, kap, Jsc ? ksca : NULL, Jsc, cam, &f, X, Y,
+                                         NULL, 0, NULL);
                 }
             img[(size_t)j * cam->nx + k] = acc / (nsub * nsub);
         }
@@ -1564,7 +1772,8 @@ int or_image(const or_model *m, const double *T, const double *kap,
             if (!star_pixel(cam, &f, q, &jj, &kk)) continue;
             double R = m->star[5 * s], Ts = m->star[5 * s + 1];
             double F = PI_D * R * R * or_planck(nu, Ts) / (cam->pix_cm * cam->pix_cm);
-            img[(size_t)jj * cam->nx + kk] += F * exp(-star_tau(m, &g, kap, q, f.n, NULL, 0, NULL));
+            img[(size_t)jj * cam->nx + kk] +=
+                F * exp(-star_tau(m, &g, kap, Jsc ? ksca : NULL, q, f.n, NULL, 0, NULL));
         }
     }
     geom_free(&g);
@@ -1574,6 +1783,18 @@ int or_image(const or_model *m, const double *T, const double *kap,
 int or_image_adjoint(const or_model *m, const double *T, const double *kap,
                      const or_camera *cam, const double *dimg, double *g_logrho,
                      double *g_T) {
+    return or_image_adjoint_scat(m, T, kap, NULL, NULL, cam, dimg, g_logrho, g_T);
+}
+
+/* With a scattering source J (held fixed): dtau = a s with a = sum rho (kabs +
+ * ksca), S = (sum rho kabs B + sum rho ksca J) / a, so
+ * dI/dlog rho_s = dI/ddtau s rho_s (kabs_s + ksca_s)
+ *               + w (rho_s kabs_s B + rho_s ksca_s J - S rho_s (kabs_s + ksca_s)) / a
+ * and dI/dT = w (sum rho kabs / a) dB/dT, w = e^-tau0 (1 - e^-dtau).  Without
+ * J this reduces to the emission-only gradient.                            */
+int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
+                          const double *ksca, const double *Jsc, const or_camera *cam,
+                          const double *dimg, double *g_logrho, double *g_T) {
     geom_t g;
     int rc = geom_init(&g, m);
     if (rc) return rc;
@@ -1595,7 +1816,8 @@ int or_image_adjoint(const or_model *m, const double *T, const double *kap,
                     double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
                     double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
                     int n;
-                    double I = trace_cam_ray(m, &g, T, kap, cam, &f, X, Y, segs, maxseg, &n);
+                    double I = trace_cam_ray(m, &g, T, kap, Jsc ? ksca : NULL, Jsc, cam, &f, X,
+                                             Y, segs, maxseg, &n);
                     if (n > maxseg) {
                         free(segs);
                         geom_free(&g);
@@ -1609,13 +1831,26 @@ int or_image_adjoint(const or_model *m, const double *T, const double *kap,
                         cum += contrib;
                         double Rk = I - cum;
                         double dI_ddtau = att * exp(-sg->dtau) * sg->S - Rk;
+                        double w = att * (-expm1(-sg->dtau));
+                        double a = 0, aa = 0;
                         for (int s = 0; s < ns; ++s) {
                             double rho = m->rho[(size_t)s * nc + sg->cell];
-                            g_logrho[(size_t)s * nc + sg->cell] +=
-                                gI * dI_ddtau * sg->s * rho * kap[s];
+                            aa += rho * kap[s];
+                            a += rho * (kap[s] + (Jsc ? ksca[s] : 0.0));
                         }
-                        g_T[sg->cell] += gI * att * (-expm1(-sg->dtau)) *
-                                         or_dplanck_dT(nu, T[sg->cell]);
+                        double B = or_planck(nu, T[sg->cell]);
+                        for (int s = 0; s < ns; ++s) {
+                            double rho = m->rho[(size_t)s * nc + sg->cell];
+                            double ks = Jsc ? ksca[s] : 0.0;
+                            double gS = Jsc && a > 0
+                                            ? (rho * kap[s] * B + rho * ks * Jsc[sg->cell] -
+                                               sg->S * rho * (kap[s] + ks)) / a
+                                            : 0.0;
+                            g_logrho[(size_t)s * nc + sg->cell] +=
+                                gI * (dI_ddtau * sg->s * rho * (kap[s] + ks) + w * gS);
+                        }
+                        double fa = Jsc ? (a > 0 ? aa / a : 0.0) : 1.0;
+                        g_T[sg->cell] += gI * w * fa * or_dplanck_dT(nu, T[sg->cell]);
                     }
                 }
         }
@@ -1628,13 +1863,13 @@ int or_image_adjoint(const or_model *m, const double *T, const double *kap,
             double R = m->star[5 * s], Ts = m->star[5 * s + 1];
             double F = PI_D * R * R * or_planck(nu, Ts) / (cam->pix_cm * cam->pix_cm);
             int n;
-            double tau = star_tau(m, &g, kap, q, f.n, segs, maxseg, &n);
+            double tau = star_tau(m, &g, kap, Jsc ? ksca : NULL, q, f.n, segs, maxseg, &n);
             double Is = F * exp(-tau);
             for (int q2 = 0; q2 < n && q2 < maxseg; ++q2)
                 for (int sp = 0; sp < ns; ++sp) {
                     double rho = m->rho[(size_t)sp * nc + segs[q2].cell];
                     g_logrho[(size_t)sp * nc + segs[q2].cell] +=
-                        gI * (-Is) * segs[q2].s * rho * kap[sp];
+                        gI * (-Is) * segs[q2].s * rho * (kap[sp] + (Jsc ? ksca[sp] : 0.0));
                 }
         }
     }

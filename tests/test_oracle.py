@@ -374,3 +374,122 @@ def test_mrw_temperatures_agree_with_plain_walk():
     assert sm["n_events"] < 0.5 * s0["n_events"]
     assert mrw[0] <= 1.5 * noise[0] + 1e-3, (mrw, noise)
     assert mrw[1] <= 2.5 * noise[1], (mrw, noise)
+
+
+# ---------------------------------------------------------------- scattering source
+def _shell_avg_inv_r2(r1, r2):
+    # volume average of 1 / r^2 over the shell [r1, r2]
+    return 3.0 * (r2 - r1) / (r2 ** 3 - r1 ** 3)
+
+
+def test_scat_source_optically_thin_star_dilution():
+    """Point star in an optically thin medium, dust emission off (T = 0): the
+    mean intensity is the diluted stellar one, J = L_nu / (16 pi^2 r^2),
+    averaged over each shell."""
+    m, _ = _uniform_sphere(rho=1e-30, nr=8, nt=3)
+    T = np.zeros(8 * 3)
+    lam = 1.0
+    J, st = oracle.scat_source(m, T, [1.0], [1.0], lam, 200000, seed=4)
+    R, Ts = m["stars"][0, 0], m["stars"][0, 1]
+    Lnu = 4 * np.pi ** 2 * R ** 2 * np_planck(C_LIGHT / (lam * 1e-4), Ts)
+    assert st["lum_star"] == pytest.approx(Lnu, rel=1e-12) and st["lum_dust"] == 0.0
+    assert st["n_escape"] == 200000
+    rw = m["rw"]
+    want = Lnu / (16 * np.pi ** 2) * _shell_avg_inv_r2(rw[:-1], rw[1:])
+    got = J.reshape(3, 8)
+    for it in range(3):
+        assert np.allclose(got[it], want, rtol=0.03), (it, got[it] / want)
+
+
+def test_scat_source_absorbing_sphere_attenuation():
+    """Pure absorber (albedo 0) around a central star: J = L e^{-alpha r} /
+    (16 pi^2 r^2) (shell averages in closed form), and the absorbed fraction
+    is 1 - e^{-alpha (r_out - r_in)}."""
+    rho, kap = 2e-15, 1.0
+    m, _ = _uniform_sphere(rho=rho, kappa=kap, nr=6, nt=2, rin=0.5, rout=40.0)
+    T = np.zeros(6 * 2)
+    n = 200000
+    J, st = oracle.scat_source(m, T, [kap], [0.0], 1.0, n, seed=5)
+    rw = m["rw"]
+    a = rho * kap
+    L = st["lum_star"]
+    r1, r2 = rw[:-1], rw[1:]
+    want = L / (4 * np.pi) * (np.exp(-a * r1) - np.exp(-a * r2)) / a / \
+        (4 * np.pi / 3 * (r2 ** 3 - r1 ** 3))
+    got = J.reshape(2, 6)
+    for it in range(2):
+        assert np.allclose(got[it], want, rtol=0.04), (it, got[it] / want)
+    p_abs = 1 - np.exp(-a * (rw[-1] - rw[0]))
+    assert st["n_abs"] / n == pytest.approx(p_abs, abs=4 * np.sqrt(p_abs * (1 - p_abs) / n))
+    assert st["n_abs"] + st["n_escape"] == n and st["n_scat"] == 0
+
+
+def test_scat_source_thin_isothermal_dust_emission():
+    """No star, optically thin isothermal sphere: volume-averaged J = j 3R/4
+    (mean distance from an interior point to the surface), j = rho kappa B."""
+    rho, kap, Td = 1e-19, 1.0, 40.0
+    m, _ = _uniform_sphere(rho=rho, kappa=kap, nr=10, nt=4, rin=0.01, rout=40.0)
+    m["stars"] = np.array([[1.0, 1.0, 0, 0, 0]])   # negligible
+    T = np.full(40, Td)
+    lam = 300.0
+    J, st = oracle.scat_source(m, T, [kap], [kap], lam, 400000, seed=6)
+    assert st["lum_star"] < 1e-30 * st["lum_dust"]
+    rw, tw = m["rw"], m["tw"]
+    r1, r2 = rw[:-1], rw[1:]
+    vol = (np.outer(np.cos(tw[:-1]) - np.cos(tw[1:]), (r2 ** 3 - r1 ** 3) / 3) * 2 * np.pi * 2).ravel()
+    j = rho * kap * np_planck(C_LIGHT / (lam * 1e-4), Td)
+    R = rw[-1]
+    assert (J * vol).sum() / vol.sum() == pytest.approx(j * 0.75 * R, rel=0.02)
+    # each shell: J(r) = j/2 (R^2 - r^2)/(2r) ... use the exact thin-sphere law
+    # J(r) = (j / 2) [R + (R^2 - r^2) / (2 r) ln((R + r) / (R - r))]
+    rc = 0.5 * (r1 + r2)
+    Jr = 0.5 * j * (R + (R ** 2 - rc ** 2) / (2 * rc) * np.log((R + rc) / (R - rc)))
+    shell = (J.reshape(4, 10) * vol.reshape(4, 10)).sum(0) / vol.reshape(4, 10).sum(0)
+    assert np.allclose(shell[2:-1], Jr[2:-1], rtol=0.05)
+
+
+def test_image_with_scattering_source_identities():
+    """With J = B(T) the source function is B whatever the albedo, so the
+    image equals the emission-only image with extinction kabs + ksca; with
+    ksca = 0 it equals the plain image whatever J."""
+    m, T = _uniform_sphere(rho=3e-16, nr=5, nt=4)
+    lam, npix, pix = 1300.0, 17, 5.0 * AU
+    B = np_planck(C_LIGHT / (lam * 1e-4), 50.0)
+    J = np.full(T.size, B)
+    a = oracle.image(m, T, [1.0], npix, pix, incl=40.0, lam_um=lam, ksca=[2.5], J=J,
+                     tau_stop=1e30)
+    b = oracle.image(m, T, [3.5], npix, pix, incl=40.0, lam_um=lam, tau_stop=1e30)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+    c = oracle.image(m, T, [1.0], npix, pix, incl=40.0, lam_um=lam, ksca=[0.0], J=J * 7.0,
+                     tau_stop=1e30)
+    d = oracle.image(m, T, [1.0], npix, pix, incl=40.0, lam_um=lam, tau_stop=1e30)
+    assert np.allclose(c, d, rtol=1e-12, atol=0)
+
+
+def test_image_adjoint_with_scattering_matches_finite_differences():
+    m, T = _uniform_sphere(rho=3e-16, nr=4, nt=4)
+    rng = np.random.default_rng(3)
+    m["rho"] = m["rho"] * rng.uniform(0.5, 2.0, m["rho"].shape)
+    T = T * rng.uniform(0.7, 1.3, T.size)
+    lam, npix, pix = 1300.0, 9, 9.0 * AU
+    B = np_planck(C_LIGHT / (lam * 1e-4), 50.0)
+    J = B * rng.uniform(0.1, 2.0, T.size)
+    kw = dict(incl=30.0, lam_um=lam, ksca=[1.7], J=J, tau_stop=1e30)
+    dimg = rng.normal(size=(npix, npix))
+    g_lr, g_T = oracle.image_adjoint(m, T, [1.0], dimg, npix, pix, **kw)
+    L = lambda mm, TT: float((oracle.image(mm, TT, [1.0], npix, pix, **kw) * dimg).sum())
+    for c in rng.choice(T.size, 8, replace=False):
+        h = 1e-4
+        mp, mm_ = dict(m), dict(m)
+        rp, rm = m["rho"].copy(), m["rho"].copy()
+        rp.reshape(-1)[c] *= np.exp(h)
+        rm.reshape(-1)[c] *= np.exp(-h)
+        mp["rho"], mm_["rho"] = rp, rm
+        fd = (L(mp, T) - L(mm_, T)) / (2 * h)
+        assert g_lr[0, c] == pytest.approx(fd, rel=1e-4, abs=1e-12 * abs(fd) + 1e-40)
+        Tp, Tm = T.copy(), T.copy()
+        dT = 1e-4 * T[c]
+        Tp[c] += dT
+        Tm[c] -= dT
+        fdT = (L(m, Tp) - L(m, Tm)) / (2 * dT)
+        assert g_T[c] == pytest.approx(fdT, rel=1e-4, abs=1e-40)
