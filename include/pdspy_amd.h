@@ -128,7 +128,29 @@ typedef struct {
     double tau_stop;    /* stop a ray once tau exceeds this (0 = 40) */
     int tile_w, tile_h; /* ray tile of one thread block, a multiple of 64 pixels
                            up to 256 (0, 0 = 64 x 1); results do not depend on it */
+    const float *d_jscat; /* scattering source: mean intensity J_nu[cells] at lam_um
+                             from rt_scattering_source; the extinction is then
+                             rho (kabs + ksca) and the source function
+                             (rho kabs B + rho ksca J) / (rho (kabs + ksca))
+                             (isotropic scattering).  NULL: emission only, as
+                             RADMC-3D with scattering_mode_max = 0.  The
+                             adjoint holds J fixed.                           */
 } rt_camera;
+
+/* Scattering source (RADMC-3D's monochromatic Monte Carlo before an image,
+ * nphot_scat packets: run.py:157-289 with write.py:34): nphot packets from
+ * the stars and from the dust's thermal emission at the imaging temperature
+ * d_T[cells], isotropic scattering, absorption ends a packet; d_J[cells] =
+ * mean intensity (erg/s/cm^2/Hz/sr) at lam_um.  Philox key = seed;
+ * bit-reproducible.  Synchronous.                                            */
+typedef struct {
+    uint64_t n_escape, n_abs, n_scat, n_cross, n_killed;
+    double lum_star, lum_dust; /* source luminosities L_nu (erg/s/Hz) */
+    double eps;                /* packet energy (erg/s/Hz)            */
+    double kernel_ms;
+} rt_scat_stats;
+int rt_scattering_source(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
+                         const float *d_T, float *d_J, rt_scat_stats *stats);
 
 int rt_create(int device, rt_ctx **out);
 int rt_destroy(rt_ctx *ctx);

@@ -14,7 +14,7 @@ LIB_PATH = os.environ.get("PDSPY_AMD_LIB") or os.path.join(HERE, "lib", "libpdsp
 EXPORTS = [
     "rt_create", "rt_destroy", "rt_set_stream", "rt_last_error", "rt_device_info",
     "rt_model_create", "rt_model_destroy", "rt_model_ncells", "rt_model_set_density",
-    "rt_model_generate_density", "rt_get_density_device",
+    "rt_model_generate_density", "rt_get_density_device", "rt_scattering_source",
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_chisq",
@@ -81,7 +81,14 @@ class Camera(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsub", C.c_int), ("pix_cm", C.c_double),
                 ("incl_deg", C.c_double), ("posang_deg", C.c_double), ("phi_deg", C.c_double),
                 ("lam_um", C.c_double), ("mode", C.c_int), ("incl_star", C.c_int),
-                ("tau_stop", C.c_double), ("tile_w", C.c_int), ("tile_h", C.c_int)]
+                ("tau_stop", C.c_double), ("tile_w", C.c_int), ("tile_h", C.c_int),
+                ("d_jscat", C.c_void_p)]
+
+
+class ScatStats(C.Structure):
+    _fields_ = [("n_escape", C.c_uint64), ("n_abs", C.c_uint64), ("n_scat", C.c_uint64),
+                ("n_cross", C.c_uint64), ("n_killed", C.c_uint64), ("lum_star", C.c_double),
+                ("lum_dust", C.c_double), ("eps", C.c_double), ("kernel_ms", C.c_double)]
 
 
 class StepItem(C.Structure):
@@ -117,6 +124,7 @@ def load():
         "rt_model_set_density": ([P, P], I),
         "rt_model_generate_density": ([P, C.POINTER(DensityComp), I], I),
         "rt_get_density_device": ([P, P], I),
+        "rt_scattering_source": ([P, D, U64, U64, P, P, C.POINTER(ScatStats)], I),
         "rt_thermal": ([P, U64, U64, C.POINTER(ThermalOpts), C.POINTER(ThermalStats)], I),
         "rt_get_temperature": ([P, P, P], I),
         "rt_thermal_timeline": ([P, P, I], I),

@@ -66,7 +66,7 @@ static void free_model(rt_model *m) {
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
                     m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
-                    m->d_dens_fac, m->d_comps};
+                    m->d_dens_fac, m->d_comps, m->d_vol, m->d_jtally, m->d_scum, m->d_scnt};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     std::free(m->h_run);
@@ -346,6 +346,12 @@ int rt_model_set_density(rt_model *m, const double *h_rho) {
 int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp) {
     if (!m) return RT_E_ARG;
     return density_generate(m, h_comps, ncomp);
+}
+
+int rt_scattering_source(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
+                         const float *d_T, float *d_J, rt_scat_stats *stats) {
+    if (!m) return RT_E_ARG;
+    return scat_source_run(m, lam_um, nphot, seed, d_T, d_J, stats);
 }
 
 int rt_get_density_device(rt_model *m, float *d_rho) {

@@ -250,30 +250,66 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
 }
 
 struct KapArg {
-    float k[RT_MAX_SPEC];
+    float k[RT_MAX_SPEC];   // absorption opacity at the camera wavelength
+    float ks[RT_MAX_SPEC];  // scattering opacity (0 without a scattering source)
 };
 
-// alpha (per au) and S = B_nu(T) per cell
+// alpha (per au) and the source function per cell: B_nu(T), or with a
+// scattering source J (isotropic) (rho kabs B + rho ksca J) / (rho kext)
 __global__ void k_cam_prep(const float *rho, int S, int ncell, KapArg kap, float lenu, const float *T,
-                           float pre, float hnu_k, int mode, float *alpha, float *src) {
+                           float pre, float hnu_k, int mode, const float *J, float *alpha,
+                           float *src) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    float a = 0.f;
-    for (int s = 0; s < S; ++s) a = fmaf(rho[(size_t)s * ncell + c], kap.k[s], a);
+    float aa = 0.f, as = 0.f;
+    for (int s = 0; s < S; ++s) {
+        const float r = rho[(size_t)s * ncell + c];
+        aa = fmaf(r, kap.k[s], aa);
+        as = fmaf(r, kap.ks[s], as);
+    }
+    const float a = aa + as;
     alpha[c] = a * lenu;
-    src[c] = mode == 0 ? planck_f(pre, hnu_k, T[c]) : 0.f;
+    float S_ = 0.f;
+    if (mode == 0) {
+        const float B = planck_f(pre, hnu_k, T[c]);
+        S_ = J ? (a > 0.f ? (aa * B + as * J[c]) / a : 0.f) : B;
+    }
+    src[c] = S_;
 }
 
+// chain rule to log rho_s and T.  With a scattering source (J held fixed):
+// d alpha / d log rho_s = rho_s kext_s, dS / d log rho_s = rho_s (kabs_s B +
+// ksca_s J - S kext_s) / (rho kext), dS / dT = (rho kabs / rho kext) dB/dT.
 __global__ void k_cam_post(const double *g_alpha, const double *g_src, const float *rho, int S,
                            int ncell, KapArg kap, float lenu, const float *T, float pre,
-                           float hnu_k, float *g_logrho, float *g_T) {
+                           float hnu_k, const float *J, const float *src, float *g_logrho,
+                           float *g_T) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     double ga = g_alpha[c];
-    for (int s = 0; s < S; ++s)
-        g_logrho[(size_t)s * ncell + c] =
-            (float)(ga * (double)rho[(size_t)s * ncell + c] * (double)kap.k[s] * (double)lenu);
-    g_T[c] = (float)(g_src[c] * (double)dplanck_f(pre, hnu_k, T[c]));
+    if (!J) {
+        for (int s = 0; s < S; ++s)
+            g_logrho[(size_t)s * ncell + c] =
+                (float)(ga * (double)rho[(size_t)s * ncell + c] * (double)kap.k[s] * (double)lenu);
+        g_T[c] = (float)(g_src[c] * (double)dplanck_f(pre, hnu_k, T[c]));
+        return;
+    }
+    const double gs = g_src[c];
+    double aa = 0.0, a = 0.0;
+    for (int s = 0; s < S; ++s) {
+        const double r = rho[(size_t)s * ncell + c];
+        aa += r * (double)kap.k[s];
+        a += r * ((double)kap.k[s] + (double)kap.ks[s]);
+    }
+    const double B = planck_f(pre, hnu_k, T[c]), Jc = J[c], Sc = src[c];
+    for (int s = 0; s < S; ++s) {
+        const double r = rho[(size_t)s * ncell + c];
+        const double ke = (double)kap.k[s] + (double)kap.ks[s];
+        const double dS = a > 0.0 ? r * ((double)kap.k[s] * B + (double)kap.ks[s] * Jc - Sc * ke) / a
+                                  : 0.0;
+        g_logrho[(size_t)s * ncell + c] = (float)(ga * r * ke * (double)lenu + gs * dS);
+    }
+    g_T[c] = (float)(gs * (a > 0.0 ? aa / a : 0.0) * (double)dplanck_f(pre, hnu_k, T[c]));
 }
 
 __global__ void k_zero_f64(double *p, size_t n) {
@@ -285,6 +321,12 @@ static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b
 
 // opacity at the camera wavelength: log-log interpolation of the model's table
 // (power-law extrapolation beyond the ends)
+static double kappa_at(const rt_model *m, const std::vector<double> &tab, int s, double lam);
+
+double kappa_at_lambda(const rt_model *m, const std::vector<double> &tab, int s, double lam) {
+    return kappa_at(m, tab, s, lam);
+}
+
 static double kappa_at(const rt_model *m, const std::vector<double> &tab, int s, double lam) {
     const int nl = m->nlam;
     const double *k = &tab[(size_t)s * nl];
@@ -353,8 +395,11 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
     C.hbits = 8;
     while ((1u << C.hbits) < 4u * (unsigned)(C.tw * C.th)) ++C.hbits;
     C.tau_stop = (float)(cam->tau_stop > 0 ? cam->tau_stop : 40.0);
-    for (int s = 0; s < RT_MAX_SPEC; ++s)
+    for (int s = 0; s < RT_MAX_SPEC; ++s) {
         kap.k[s] = s < m->nspec ? (float)kappa_at(m, m->kabs, s, cam->lam_um) : 0.f;
+        kap.ks[s] = (s < m->nspec && cam->d_jscat) ? (float)kappa_at(m, m->ksca, s, cam->lam_um)
+                                                   : 0.f;
+    }
     double nu = 2.99792458e10 / (cam->lam_um * 1e-4);
     pre = (float)(2.0 * 6.6260755e-27 * nu * nu * nu / (2.99792458e10 * 2.99792458e10));
     hnu_k = (float)(6.6260755e-27 * nu / 1.380658e-16);
@@ -387,8 +432,8 @@ int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *
     const float *rho = d_rho ? d_rho : m->d_rho;
     hipStream_t sm = ctx->stream;
     hipLaunchKernelGGL(k_cam_prep, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, rho, m->nspec,
-                       m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->mode, m->d_alpha,
-                       m->d_src);
+                       m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->mode,
+                       cam->d_jscat, m->d_alpha, m->d_src);
     int npx = cam->nx * cam->ny;
     const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
     hipLaunchKernelGGL(k_image, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_img);
@@ -416,7 +461,8 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     hipStream_t sm = ctx->stream;
     size_t nc = (size_t)m->ncell;
     hipLaunchKernelGGL(k_cam_prep, dim3(nblk(nc, 256)), dim3(256), 0, sm, rho, m->nspec, m->ncell,
-                       kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, 0, m->d_alpha, m->d_src);
+                       kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, 0, cam->d_jscat, m->d_alpha,
+                       m->d_src);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, nc);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
     int npx = cam->nx * cam->ny;
@@ -432,7 +478,7 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     }
     hipLaunchKernelGGL(k_cam_post, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, m->d_gsrc,
                        rho, m->nspec, m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k,
-                       d_g_logrho, d_g_T);
+                       cam->d_jscat, m->d_src, d_g_logrho, d_g_T);
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
 }

@@ -124,6 +124,11 @@ struct rt_model {
     void *d_comps = nullptr;                      // [RT_MAX_COMP] rt_density_comp
     double dens_solid = 0;                        // 4 pi (2 pi for full-sphere grids)
     bool rho_host_stale = false;                  // densities generated on the device
+    // scattering source Monte Carlo
+    double *d_vol = nullptr;                      // [cells] volumes (cm^3)
+    unsigned long long *d_jtally = nullptr;       // [cells] path-length tallies
+    double *d_scum = nullptr;                     // [cells] cumulative source luminosity
+    unsigned long long *d_scnt = nullptr;         // [8] counters
     void *h_run = nullptr;        // host parameter block of the thermal kernel (ThermParams)
     void *d_run = nullptr;        // its device copy
     std::vector<unsigned long long> h_est;  // epoch starts of the last planned run
@@ -187,6 +192,8 @@ rt::DevGrid dev_grid(const rt_model *m);
 int density_prepare(rt_model *m, const rt_density_comp *h_comps, int ncomp);
 void density_enqueue(rt_model *m, const rt_density_comp *d_comps, int ncomp, hipStream_t sm);
 int density_generate(rt_model *m, const rt_density_comp *h_comps, int ncomp);
+int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
+                    const float *d_T, float *d_J, rt_scat_stats *st);
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                 rt_thermal_stats *st);
 int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,

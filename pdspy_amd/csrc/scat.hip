@@ -1,0 +1,333 @@
+// Scattering source for images: the monochromatic Monte Carlo RADMC-3D runs
+// at the image wavelength before ray tracing when the opacities scatter
+// (nphot_scat packets; pdspy passes nphot to run_image, Model.py:515-534,
+// write.py:34; run.py:157-289), for isotropic scattering.
+//
+// Sources: the point stars (L_nu = 4 pi^2 R^2 B_nu(T*)) and the thermal
+// emission of the dust at the imaging temperature, L_nu,c = 4 pi V_c
+// B_nu(T_c) sum_s rho_s kabs_s.  A packet of energy L/N starts at a star or
+// uniformly inside a cell (r^3, cos theta, phi uniform; either hemisphere of a
+// mirrored grid), travels with extinction rho (kabs + ksca), scatters
+// isotropically with probability ksca / kext at each interaction and ends
+// otherwise.  The mean intensity is the path-length estimate
+// J_c = (L/N) sum l / (4 pi V_c), summed as u64 fixed point (l in units of
+// 2^-24 of the cell's radial width) so the result does not depend on the
+// order of the atomics.
+//
+// One packet per lane, grid-stride over packets: an albedo-weighted walk ends
+// after a few interactions, so the lanes of a wave stay close to each other
+// without the thermal kernel's refill machinery.  Random events (Philox key =
+// seed, counter (k, packet)): k = 0 source (u0), position (u1 u2 u3),
+// direction (u4 u5), tau (u6), hemisphere (u7); k >= 1: scatter or end (u0),
+// direction (u1 u2), next tau (u3).
+#include "rt_internal.h"
+
+#include <cmath>
+#include <vector>
+
+using namespace rt;
+
+namespace {
+
+constexpr double PI_D = 3.141592653589793;
+constexpr float TALLY_SCALE = 16777216.0f;  // 2^24
+
+struct ScatKap {
+    float a[RT_MAX_SPEC], s[RT_MAX_SPEC];  // kabs, ksca at the wavelength (cm^2/g)
+};
+
+struct ScatStars {
+    double cum[RT_MAX_STAR];  // cumulative luminosity of the stars
+    float pos[RT_MAX_STAR][3];  // au
+};
+
+struct ScatParams {
+    DevGrid g;
+    const float *rhoL;        // [cells][S] rho * au
+    int S, nstar;
+    ScatKap kap;
+    ScatStars st;
+    const double *cum;        // [cells] cumulative dust luminosity (after the stars)
+    double ltot;              // stars + dust
+    uint64_t seed, nphot;
+    uint32_t max_events;
+    const float *invdr;       // [nr] 1/dr (au^-1)
+    unsigned long long *tally;  // [cells]
+    unsigned long long *cnt;    // escape, absorbed, scattered, crossings, killed
+};
+
+__device__ __forceinline__ double planck_d(double nu, double T) {
+    if (!(T > 0.0)) return 0.0;
+    const double x = RT_H * nu / (RT_K * T);
+    return 2.0 * RT_H * nu * nu * nu / (RT_C * RT_C) / expm1(x);
+}
+
+// dust luminosity of each cell at frequency nu (erg/s/Hz)
+__global__ void k_scat_lum(const float *__restrict__ rho, int S, int nc, ScatKap kap,
+                           const float *__restrict__ T, double nu, const double *__restrict__ vol,
+                           double *__restrict__ lum) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double a = 0.0;
+    for (int s = 0; s < S; ++s) a += (double)rho[(size_t)s * nc + c] * (double)kap.a[s];
+    lum[c] = 4.0 * PI_D * vol[c] * a * planck_d(nu, (double)T[c]);
+}
+
+// In-place inclusive prefix sum of n doubles, plus `base`, by one block:
+// each thread sums a contiguous chunk, the chunk totals are scanned in LDS,
+// then each thread rewrites its chunk.  Fixed order: deterministic.
+__global__ void __launch_bounds__(1024) k_scan_f64(double *__restrict__ x, int n, double base) {
+    __shared__ double part[1024];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int chunk = (n + nt - 1) / nt;
+    const int i0 = min(n, t * chunk), i1 = min(n, i0 + chunk);
+    double s = 0.0;
+    for (int i = i0; i < i1; ++i) s += x[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        double acc = base;
+        for (int k = 0; k < nt; ++k) {
+            const double v = part[k];
+            part[k] = acc;  // exclusive prefix of chunk k
+            acc += v;
+        }
+    }
+    __syncthreads();
+    double acc = part[t];
+    for (int i = i0; i < i1; ++i) {
+        acc += x[i];
+        x[i] = acc;
+    }
+}
+
+// first index i with cum[i] > x (cum ascending); n - 1 if none
+__device__ __forceinline__ int first_above(const double *cum, int n, double x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cum[mid] > x)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void tally_path(const ScatParams &P, int c, int ir, float len) {
+    const unsigned long long v =
+        (unsigned long long)rintf(len * P.invdr[ir] * TALLY_SCALE);
+    if (v) atomicAdd(&P.tally[c], v);
+}
+
+__global__ void __launch_bounds__(256) k_scat_mc(ScatParams P) {
+    const DevGrid &g = P.g;
+    uint32_t c_esc = 0, c_abs = 0, c_sca = 0, c_cross = 0, c_kill = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < P.nphot;
+         pid += stride) {
+        float u[8];
+        rng8(P.seed, pid, 0u, u);
+        Pos p;
+        const double x = (double)u[0] * P.ltot;
+        int ks = -1;
+        for (int k = 0; k < P.nstar; ++k)
+            if (ks < 0 && x < P.st.cum[k]) ks = k;
+        if (ks >= 0) {
+            p.x = P.st.pos[ks][0];
+            p.y = P.st.pos[ks][1];
+            p.z = P.st.pos[ks][2];
+        } else {
+            const int c = first_above(P.cum, g.ncell, x);
+            const int ir = c % g.nr, it = (c / g.nr) % g.nt, ip = c / (g.nr * g.nt);
+            const double r1 = g.rwd[ir], r2 = g.rwd[ir + 1];
+            const double r = cbrt(r1 * r1 * r1 + (double)u[1] * (r2 * r2 * r2 - r1 * r1 * r1));
+            const double ct = g.ctd[it] + (double)u[2] * (g.ctd[it + 1] - g.ctd[it]);
+            const double sth = sqrt(fmax(0.0, 1.0 - ct * ct));
+            const double ph = (double)g.pw[ip] + (double)u[3] * (double)(g.pw[ip + 1] - g.pw[ip]);
+            double sp, cp;
+            sincos(ph, &sp, &cp);
+            p.x = (float)(r * sth * cp);
+            p.y = (float)(r * sth * sp);
+            p.z = (float)(r * ct * ((g.mirror && u[7] < 0.5f) ? -1.0 : 1.0));
+        }
+        iso_dir(u[4], u[5], p.dx, p.dy, p.dz);
+        locate(g, p);
+        if (p.ir >= g.nr) {
+            c_esc++;
+            continue;
+        }
+        float tau = -__logf(u[6]);
+        uint32_t k = 1, nev = 0, nzero = 0, nstep = 0;
+        for (;;) {
+            int kind;
+            const float tw = next_wall(g, p, kind);
+            float aabs = 0.f, asca = 0.f;
+            int c = -1;
+            if (p.ir >= 0) {
+                c = cell_index(g, p);
+                for (int s = 0; s < P.S; ++s) {
+                    const float rl = P.rhoL[(size_t)c * P.S + s];
+                    aabs = fmaf(rl, P.kap.a[s], aabs);
+                    asca = fmaf(rl, P.kap.s[s], asca);
+                }
+            }
+            const float aext = aabs + asca;
+            if (aext * tw > tau) {
+                const float sseg = tau / aext;
+                tally_path(P, c, p.ir, sseg);
+                p.x = fmaf(sseg, p.dx, p.x);
+                p.y = fmaf(sseg, p.dy, p.y);
+                p.z = fmaf(sseg, p.dz, p.z);
+                float v[8];
+                rng8(P.seed, pid, k++, v);
+                nev++;
+                if (v[0] * aext < asca) {
+                    c_sca++;
+                    iso_dir(v[1], v[2], p.dx, p.dy, p.dz);
+                    tau = -__logf(v[3]);
+                    nzero = 0;
+                    if (P.max_events && nev >= P.max_events) {
+                        c_kill++;
+                        break;
+                    }
+                } else {
+                    c_abs++;
+                    break;
+                }
+            } else {
+                if (c >= 0) tally_path(P, c, p.ir, tw);
+                tau -= aext * tw;
+                c_cross++;
+                const float rcell = g.rw[p.ir + 1];
+                if (++nstep > 100000000u) {  // last-resort cap on crossings
+                    c_kill++;
+                    break;
+                }
+                if (cross_wall(g, p, tw, kind)) {
+                    c_esc++;
+                    break;
+                }
+                if (unstick(g, p, tw, rcell, nzero) && p.ir >= g.nr) {
+                    c_esc++;
+                    break;
+                }
+            }
+        }
+    }
+    // one atomic per counter per wave
+    const uint32_t vals[5] = {c_esc, c_abs, c_sca, c_cross, c_kill};
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        unsigned long long v = vals[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (__lane_id() == 0 && v) atomicAdd(&P.cnt[q], v);
+    }
+}
+
+// J_c = eps * l_c / (4 pi V_c), l in cm
+__global__ void k_scat_J(const unsigned long long *__restrict__ tally,
+                         const float *__restrict__ invdr, const double *__restrict__ vol, int nr,
+                         int nc, double eps, float *__restrict__ J) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double len_cm =
+        (double)tally[c] / (double)TALLY_SCALE / (double)invdr[c % nr] * RT_LEN_UNIT;
+    J[c] = (float)(eps * len_cm / (4.0 * PI_D * vol[c]));
+}
+
+__global__ void k_zero_u64(unsigned long long *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// opacity at the wavelength: the camera's log-log interpolation
+double kappa_at_lambda(const rt_model *m, const std::vector<double> &tab, int s, double lam);
+
+int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
+                    const float *d_T, float *d_J, rt_scat_stats *st) {
+    rt_ctx *ctx = m->ctx;
+    if (!(lam_um > 0)) return set_err(ctx, RT_E_ARG, "wavelength must be > 0");
+    if (!d_T || !d_J) return set_err(ctx, RT_E_ARG, "NULL temperature or output pointer");
+    if (m->vol.size() != (size_t)m->ncell) return set_err(ctx, RT_E_ARG, "model has no cell volumes");
+    const int nc = m->ncell;
+    if (!m->d_vol) {
+        HIPCHK(ctx, hipMalloc((void **)&m->d_vol, sizeof(double) * nc));
+        HIPCHK(ctx, hipMemcpy(m->d_vol, m->vol.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMalloc((void **)&m->d_jtally, sizeof(unsigned long long) * nc));
+        HIPCHK(ctx, hipMalloc((void **)&m->d_scum, sizeof(double) * nc));
+        HIPCHK(ctx, hipMalloc((void **)&m->d_scnt, sizeof(unsigned long long) * 8));
+    }
+    hipStream_t sm = ctx->stream;
+    ScatParams P{};
+    P.g = dev_grid(m);
+    P.rhoL = m->d_rhoL;
+    P.S = m->nspec;
+    for (int s = 0; s < RT_MAX_SPEC; ++s) {
+        P.kap.a[s] = s < m->nspec ? (float)kappa_at_lambda(m, m->kabs, s, lam_um) : 0.f;
+        P.kap.s[s] = s < m->nspec ? (float)kappa_at_lambda(m, m->ksca, s, lam_um) : 0.f;
+    }
+    const double nu = RT_C / (lam_um * 1e-4);
+    double lstar = 0.0;
+    P.nstar = std::min(m->nstar, RT_MAX_STAR);
+    for (int k = 0; k < P.nstar; ++k) {
+        const double R = m->stars[5 * k], Ts = m->stars[5 * k + 1];
+        const double x = RT_H * nu / (RT_K * Ts);
+        lstar += 4.0 * PI_D * PI_D * R * R * (2.0 * RT_H * nu * nu * nu / (RT_C * RT_C) / std::expm1(x));
+        P.st.cum[k] = lstar;
+        for (int a = 0; a < 3; ++a) P.st.pos[k][a] = (float)(m->stars[5 * k + 2 + a] / RT_LEN_UNIT);
+    }
+    hipLaunchKernelGGL(k_scat_lum, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_rho, m->nspec, nc,
+                       P.kap, d_T, nu, m->d_vol, m->d_scum);
+    hipLaunchKernelGGL(k_scan_f64, dim3(1), dim3(1024), 0, sm, m->d_scum, nc, lstar);
+    double ltot = lstar;
+    HIPCHK(ctx, hipMemcpyAsync(&ltot, m->d_scum + (nc - 1), sizeof(double), hipMemcpyDeviceToHost, sm));
+    HIPCHK(ctx, hipStreamSynchronize(sm));
+    P.cum = m->d_scum;
+    P.ltot = ltot;
+    P.seed = seed;
+    P.nphot = nphot;
+    P.max_events = 1000000u;
+    P.invdr = m->d_invdr;
+    P.tally = m->d_jtally;
+    P.cnt = m->d_scnt;
+    hipLaunchKernelGGL(k_zero_u64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_jtally, (size_t)nc);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, sm, m->d_scnt, (size_t)8);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, sm);
+    const double eps = (ltot > 0 && nphot > 0) ? ltot / (double)nphot : 0.0;
+    if (eps > 0) {
+        const uint64_t blocks = std::min<uint64_t>((nphot + 255) / 256, (uint64_t)ctx->n_cu * 16);
+        hipLaunchKernelGGL(k_scat_mc, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0,
+                           sm, P);
+    }
+    (void)hipEventRecord(e1, sm);
+    hipLaunchKernelGGL(k_scat_J, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_jtally, m->d_invdr,
+                       m->d_vol, m->nr, nc, eps, d_J);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(sm));
+    float kms = 0.f;
+    (void)hipEventElapsedTime(&kms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) {
+        unsigned long long cnt[8];
+        HIPCHK(ctx, hipMemcpy(cnt, m->d_scnt, sizeof(cnt), hipMemcpyDeviceToHost));
+        st->n_escape = cnt[0];
+        st->n_abs = cnt[1];
+        st->n_scat = cnt[2];
+        st->n_cross = cnt[3];
+        st->n_killed = cnt[4];
+        st->lum_star = lstar;
+        st->lum_dust = ltot - lstar;
+        st->eps = eps;
+        st->kernel_ms = kms;
+    }
+    return RT_OK;
+}

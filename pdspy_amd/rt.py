@@ -204,11 +204,33 @@ class DeviceModel:
 
     @staticmethod
     def camera(npix, pix_cm, lam_um, incl=0.0, pa=0.0, phi=0.0, mode=0, incl_star=True,
-               nsub=1, tau_stop=40.0, ny=None, tile=(0, 0)):
-        return _lib.Camera(int(npix), int(ny if ny is not None else npix), int(nsub),
-                           float(pix_cm), float(incl), float(pa), float(phi), float(lam_um),
-                           int(mode), 1 if incl_star else 0, float(tau_stop), int(tile[0]),
-                           int(tile[1]))
+               nsub=1, tau_stop=40.0, ny=None, tile=(0, 0), jscat=None):
+        """Camera; ``jscat``: the mean intensity J[cells] (float32 device tensor)
+        of ``scattering_source`` at ``lam_um`` for images with the scattering
+        source (the camera keeps a reference to it)."""
+        if jscat is not None:
+            jscat = jscat.contiguous()
+        cam = _lib.Camera(int(npix), int(ny if ny is not None else npix), int(nsub),
+                          float(pix_cm), float(incl), float(pa), float(phi), float(lam_um),
+                          int(mode), 1 if incl_star else 0, float(tau_stop), int(tile[0]),
+                          int(tile[1]), None if jscat is None else jscat.data_ptr())
+        cam._jscat = jscat
+        return cam
+
+    def scattering_source(self, lam_um, nphot, T_dev, seed=0):
+        """Mean intensity J[cells] (float32 device tensor, erg/s/cm^2/Hz/sr) at
+        lam_um from the monochromatic scattering Monte Carlo with nphot packets
+        (stars and the dust's thermal emission at the imaging temperature
+        T_dev[cells]); returns (J, stats dict)."""
+        import torch
+        J = torch.empty(self.ncell, dtype=torch.float32, device="cuda")
+        T_dev = T_dev.contiguous()
+        st = _lib.ScatStats()
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_scattering_source(
+            self.h, float(lam_um), int(nphot), int(seed), C.c_void_p(T_dev.data_ptr()),
+            C.c_void_p(J.data_ptr()), C.byref(st)))
+        return J, {k: getattr(st, k) for k, _ in _lib.ScatStats._fields_}
 
     def image(self, cam, T_dev, rho_dev=None):
         """Intensity image (erg/s/cm^2/Hz/sr, or tau) as a float32 torch tensor [ny, nx]."""
@@ -417,17 +439,30 @@ def _camera_lams(model, lam, loadlambda):
     return np.atleast_1d(np.asarray([float(x) for x in np.atleast_1d(lam)], dtype=float))
 
 
-def _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device):
+def _scatters(model, keywords):
+    """RADMC-3D adds the scattering source when the opacities scatter, unless
+    scattering_mode_max = 0 (pdspy writes the option only when given)."""
+    if keywords.get("scattering_mode_max", None) == 0:
+        return False
+    return any(np.any(np.asarray(d.ksca) > 0) for d in model.grid.dust)
+
+
+def _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device,
+            nphot=0, keywords=None):
     import torch
+    keywords = keywords or {}
     dm = _device_model(model, device)
     if len(model.grid.temperature) == 0:
         raise RuntimeError("run_thermal (or grid.add_temperature) must come before imaging")
     T = torch.tensor(imaging_temperature(model.grid), dtype=torch.float32, device="cuda")
     pix_cm = pixelsize * dpc * AU
+    scat = int(nphot) > 0 and _scatters(model, keywords)
+    seed = int(keywords.get("iseed") or 0) & 0xFFFFFFFFFFFFFFFF
     chans = []
     for lam_um in lams:
+        J = dm.scattering_source(lam_um, int(nphot), T, seed=seed)[0] if scat else None
         cam = dm.camera(npix, pix_cm, lam_um, incl=incl, pa=pa, phi=phi, incl_star=not nostar,
-                        nsub=nsub)
+                        nsub=nsub, jscat=J)
         chans.append(dm.image(cam, T).double().cpu().numpy())
     img = np.stack(chans, axis=-1)  # [ny, nx, nlam], erg/s/cm^2/Hz/sr
     # Jy/pixel (Model.py:566-567)
@@ -443,7 +478,8 @@ def run_image(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="1300",
                      if k not in ("imolspec", "iline", "widthkms", "vkms", "linenlam",
                                   "doppcatch", "unstructured")})
     lams = _camera_lams(model, lam, loadlambda)
-    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device)
+    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device,
+                    nphot=nphot, keywords=keywords)
     x = (np.arange(npix) - npix // 2) * float(pixelsize)
     model.images[name] = Image(image, x=x.astype(float), y=x.astype(float).copy(),
                                wave=lams * 1.0e-4)
@@ -465,7 +501,8 @@ def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="
     import torch
     _check_keywords(keywords)
     lams = _camera_lams(model, lam, loadlambda)
-    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device)
+    image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device,
+                    nphot=nphot, keywords=keywords)
     u, v = fft_grid_uv(npix, pixelsize)
     real = np.empty((npix * npix, len(lams)))
     imag = np.empty((npix * npix, len(lams)))

@@ -283,3 +283,125 @@ def test_run_image_and_visibilities_units():
     scale = np.abs(V).max()
     assert np.abs(vis.real[:, 0] - V.real.ravel()).max() < 1e-5 * scale
     assert np.abs(vis.imag[:, 0] - V.imag.ravel()).max() < 1e-5 * scale
+
+
+# ---------------------------------------------------------------- scattering source
+def _kap_at(arr, key, lam):
+    return [rt._loglog_interp(np.array([lam]), arr["lam_um"], arr[key][s])[0]
+            for s in range(arr[key].shape[0])]
+
+
+def test_scattering_source_thin_shell_closed_form_gpu():
+    # star only (T = 0), optically thin: J = L_nu / (16 pi^2 r^2), shell averages
+    print("case: scattering source, thin shell around a star", flush=True)
+    m, _ = _uniform_sphere(rho=1e-30, nr=8, nt=3)
+    dm = _dev(m)
+    T0 = torch.zeros(24, dtype=torch.float32, device="cuda")
+    J, st = dm.scattering_source(1.0, 400000, T0, seed=4)
+    R, Ts = m["stars"][0, 0], m["stars"][0, 1]
+    Lnu = 4 * np.pi ** 2 * R ** 2 * np_planck(C_LIGHT / 1e-4, Ts)
+    assert st["lum_star"] == pytest.approx(Lnu, rel=1e-12) and st["lum_dust"] == 0.0
+    assert st["n_escape"] == 400000
+    rw = m["rw"]
+    want = Lnu / (16 * np.pi ** 2) * 3.0 * (rw[1:] - rw[:-1]) / (rw[1:] ** 3 - rw[:-1] ** 3)
+    got = J.double().cpu().numpy().reshape(3, 8)
+    for it in range(3):
+        assert np.allclose(got[it], want, rtol=0.03), (it, got[it] / want)
+
+
+def test_scattering_source_matches_oracle_and_is_reproducible():
+    # the same Philox streams as the FP64 oracle: per-cell J agrees up to the
+    # packets whose FP32 and FP64 paths part
+    print("case: scattering source vs oracle, disk+envelope 32x24 at 1.3 mm", flush=True)
+    arr, T = _disk_setup(nr=32, nt=24)
+    dm = _dev(arr)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    lam, n = 1300.0, 400000
+    J1, st = dm.scattering_source(lam, n, Td, seed=11)
+    J2, _ = dm.scattering_source(lam, n, Td, seed=11)
+    assert torch.equal(J1, J2)
+    ka, ks = _kap_at(arr, "kabs", lam), _kap_at(arr, "ksca", lam)
+    Jo, so = oracle.scat_source(arr, T, ka, ks, lam, n, seed=11)
+    Jg = J1.double().cpu().numpy()
+    assert st["lum_dust"] == pytest.approx(so["lum_dust"], rel=1e-5)
+    for k in ("n_escape", "n_abs", "n_scat"):
+        assert st[k] == pytest.approx(so[k], rel=2e-3), k
+    vol = np.asarray(oracle_vol(arr))
+    assert (Jg * vol).sum() == pytest.approx((Jo * vol).sum(), rel=2e-3)
+    sel = Jo > 1e-3 * Jo.max()
+    rel = np.abs(Jg[sel] - Jo[sel]) / Jo[sel]
+    print("J vs oracle: median %.2e, p90 %.2e" % (np.median(rel), np.percentile(rel, 90)))
+    assert np.median(rel) < 5e-3 and np.percentile(rel, 90) < 5e-2
+
+
+def oracle_vol(arr):
+    rw, tw, pw = arr["rw"], arr["tw"], arr["pw"]
+    vr = (rw[1:] ** 3 - rw[:-1] ** 3) / 3
+    vt = np.cos(tw[:-1]) - np.cos(tw[1:])
+    vp = pw[1:] - pw[:-1]
+    mirror = abs(tw[-1] - np.pi / 2) < 1e-9
+    return (vp[:, None, None] * vt[None, :, None] * vr[None, None, :]).ravel() * (2 if mirror else 1)
+
+
+def _scat_image_setup(seed=0):
+    arr, T = _disk_setup(nr=32, nt=16, seed=seed)
+    rng = np.random.default_rng(seed + 10)
+    B = np_planck(C_LIGHT / 1300e-4, T)
+    J = (B * rng.uniform(0.05, 1.5, T.size)).astype(np.float32).astype(np.float64)
+    return arr, T, J
+
+
+def test_image_with_scattering_source_matches_oracle():
+    print("case: image with scattering source vs oracle", flush=True)
+    arr, T, J = _scat_image_setup()
+    dm = _dev(arr)
+    lam, npix, pix = 1300.0, 64, 4.0 * AU
+    Jd = torch.tensor(J, dtype=torch.float32, device="cuda")
+    cam = dm.camera(npix, pix, lam, incl=60.0, incl_star=True, jscat=Jd)
+    img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+    ref = oracle.image(arr, T, _kap_at(arr, "kabs", lam), npix, pix, incl=60.0, lam_um=lam,
+                       incl_star=True, ksca=_kap_at(arr, "ksca", lam), J=J)
+    plain = oracle.image(arr, T, _kap_at(arr, "kabs", lam), npix, pix, incl=60.0, lam_um=lam,
+                         incl_star=True)
+    err = np.abs(img - ref).max() / np.abs(ref).max()
+    print("image (scattering) max rel err %.2e; scattering changes the image by %.2e" % (
+        err, np.abs(ref - plain).max() / np.abs(plain).max()))
+    assert err < 2e-3
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+    assert np.abs(ref - plain).max() > 0.05 * np.abs(plain).max()
+
+
+def test_image_adjoint_with_scattering_source_matches_oracle():
+    print("case: image adjoint with scattering source vs oracle", flush=True)
+    arr, T, J = _scat_image_setup(seed=2)
+    dm = _dev(arr)
+    lam, npix, pix = 1300.0, 48, 5.0 * AU
+    rng = np.random.default_rng(1)
+    dimg = rng.normal(size=(npix, npix))
+    Jd = torch.tensor(J, dtype=torch.float32, device="cuda")
+    cam = dm.camera(npix, pix, lam, incl=60.0, incl_star=True, jscat=Jd)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    g_lr, g_T = dm.image_adjoint(cam, Td, None, torch.tensor(dimg, dtype=torch.float32,
+                                                              device="cuda"))
+    o_lr, o_T = oracle.image_adjoint(arr, T, _kap_at(arr, "kabs", lam), dimg, npix, pix,
+                                     incl=60.0, lam_um=lam, incl_star=True,
+                                     ksca=_kap_at(arr, "ksca", lam), J=J)
+    e_lr = np.linalg.norm(g_lr.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
+    e_T = np.linalg.norm(g_T.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
+    print("relative gradient error (scattering): logrho %.2e  T %.2e" % (e_lr, e_T))
+    assert e_lr < 2e-3 and e_T < 2e-3
+
+
+def test_run_image_adds_scattering_source_unless_disabled():
+    print("case: run_image with and without the scattering source", flush=True)
+    m = configs.disk_c1(nr=24, nt=16)
+    rt.run_thermal(m, nphot=20000, seed=1, max_events=100000)
+    a = rt.run_image(m, name="s", nphot=100000, npix=32, pixelsize=0.5, lam="1300", incl=45,
+                     dpc=140).image
+    b = rt.run_image(m, name="n", nphot=100000, npix=32, pixelsize=0.5, lam="1300", incl=45,
+                     dpc=140, scattering_mode_max=0).image
+    c = rt.run_image(m, name="z", nphot=0, npix=32, pixelsize=0.5, lam="1300", incl=45,
+                     dpc=140).image
+    assert np.array_equal(b, c)
+    assert np.all(np.isfinite(a)) and a.sum() > 0
+    assert np.abs(a - b).max() > 1e-3 * np.abs(b).max()
