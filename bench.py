@@ -47,6 +47,8 @@ def parse():
                    help="target wall time of the bounded CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-image", action="store_true")
+    p.add_argument("--nphot-scat", type=float, default=1e5,
+                   help="scattering-source packets per image (pdspy's fits: 1e5)")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "thermal_traffic.json"))
     p.add_argument("--batch", action="store_true", help="batched sampler step instead")
     p.add_argument("--batch-models", type=int, default=64)
@@ -277,6 +279,29 @@ def main():
                         "ms_fwd": t_f * 1e3, "ms_fwd_adj": t_fa * 1e3,
                         "config": "256x256 px, 1.3 mm, incl 60, 2 arcsec field at 140 pc",
                         "ray_tile": list(tile)}
+        # with the scattering source, as RADMC-3D images when the dust scatters:
+        # J from the monochromatic Monte Carlo, then image + adjoint
+        nps = int(a.nphot_scat)
+        if nps > 0:
+            dm.scattering_source(1300.0, nps, T, seed=1)      # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                J, sst = dm.scattering_source(1300.0, nps, T, seed=0)
+            torch.cuda.synchronize()
+            t_s = (time.perf_counter() - t0) / 3
+            cam_s = dm.camera(256, pix_cm, 1300.0, incl=60.0, tile=tile, jscat=J)
+            t0 = time.perf_counter()
+            for _ in range(n_rep):
+                dm.image(cam_s, T)
+                dm.image_adjoint(cam_s, T, None, dimg)
+            torch.cuda.synchronize()
+            t_sa = (time.perf_counter() - t0) / n_rep
+            out["image"]["scattering"] = {
+                "nphot_scat": nps, "source_ms": t_s * 1e3, "source_packets_per_s": nps / t_s,
+                "source_kernel_ms": sst["kernel_ms"], "ms_fwd_adj": t_sa * 1e3,
+                "rays_per_s_fwd_adj_incl_source": 65536 / (t_s + t_sa),
+                "n_scat": sst["n_scat"], "n_abs": sst["n_abs"], "n_escape": sst["n_escape"]}
     if not a.no_cpu:
         cb = cpu_baseline(arr, a.max_events, a.cpu_seconds, a.mrw)
         out["cpu_baseline"] = cb

@@ -261,6 +261,11 @@ typedef struct {
     float *d_g_logrho;              /* out [nspec][cells] d chi^2 / d log rho, and  */
     float *d_g_T;                   /* out [cells] d chi^2 / d T_img; both NULL:
                                        no adjoint                                   */
+    uint64_t nphot_scat;            /* scattering-source packets before the image
+                                       (pdspy's fits: run_image nphot=1e5,
+                                       run_disk_model.py:301-315); 0: emission
+                                       only.  The step computes J itself; cam's
+                                       d_jscat is ignored.                         */
 } rt_step_item;
 
 int rt_batch_create(rt_ctx *ctx, rt_model *const *models, int nmodel, rt_batch **out);

@@ -97,7 +97,7 @@ class StepItem(C.Structure):
                 ("dra", C.c_double), ("ddec", C.c_double),
                 ("d_re", C.c_void_p), ("d_im", C.c_void_p), ("d_w", C.c_void_p),
                 ("d_img", C.c_void_p), ("d_chi2", C.c_void_p),
-                ("d_g_logrho", C.c_void_p), ("d_g_T", C.c_void_p)]
+                ("d_g_logrho", C.c_void_p), ("d_g_T", C.c_void_p), ("nphot_scat", C.c_uint64)]
 
 
 _lib = None

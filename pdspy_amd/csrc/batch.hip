@@ -37,7 +37,7 @@ struct rt_batch {
     rt_thermal_opts plan_opts{};
     bool planned = false;
     // step scratch per model
-    std::vector<float *> d_Timg, d_imgs, d_re, d_im, d_gre, d_gim, d_gimg;
+    std::vector<float *> d_Timg, d_imgs, d_re, d_im, d_gre, d_gim, d_gimg, d_J;
     std::vector<double *> d_ws;
     std::vector<int> K_cap, npix_cap;
     // captured step
@@ -133,6 +133,7 @@ int batch_create(rt_ctx *ctx, rt_model *const *models, int n, rt_batch **out) {
     b->d_gre.assign(n, nullptr);
     b->d_gim.assign(n, nullptr);
     b->d_gimg.assign(n, nullptr);
+    b->d_J.assign(n, nullptr);
     b->d_ws.assign(n, nullptr);
     b->K_cap.assign(n, 0);
     b->npix_cap.assign(n, 0);
@@ -157,6 +158,7 @@ int batch_destroy(rt_batch *b) {
         (void)hipFree(b->d_gre[i]);
         (void)hipFree(b->d_gim[i]);
         (void)hipFree(b->d_gimg[i]);
+        (void)hipFree(b->d_J[i]);
         (void)hipFree(b->d_ws[i]);
     }
     if (b->e0) (void)hipEventDestroy(b->e0);
@@ -277,6 +279,11 @@ static int batch_step_buffers(rt_batch *b, int i, const rt_step_item &it) {
     int rc = image_buffers(m);
     if (rc) return rc;
     if (!b->d_Timg[i]) HIPCHK(ctx, hipMalloc(&b->d_Timg[i], sizeof(float) * (size_t)m->ncell));
+    if (it.nphot_scat) {
+        int rc = scat_prepare(b->models[i]);
+        if (rc) return rc;
+        if (!b->d_J[i]) HIPCHK(ctx, hipMalloc(&b->d_J[i], sizeof(float) * b->models[i]->ncell));
+    }
     const int npix = it.cam.nx * it.cam.ny;
     if (b->npix_cap[i] < npix) {
         (void)hipFree(b->d_imgs[i]);
@@ -308,7 +315,14 @@ static int batch_enqueue_image_chain(rt_batch *b, int i, const rt_step_item &it,
     const int npix = it.cam.nx * it.cam.ny;
     hipLaunchKernelGGL(k_imaging_T, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, m->d_rho,
                        m->d_Tout, m->nspec, m->ncell, b->d_Timg[i]);
-    int rc = image_run(m, &it.cam, b->d_Timg[i], nullptr, it.d_img);
+    // scattering source at the image wavelength (Philox key uploaded per step)
+    rt_camera cam = it.cam;
+    cam.d_jscat = nullptr;
+    if (it.nphot_scat) {
+        scat_enqueue(m, cam.lam_um, it.nphot_scat, b->d_Timg[i], b->d_J[i], sm);
+        cam.d_jscat = b->d_J[i];
+    }
+    int rc = image_run(m, &cam, b->d_Timg[i], nullptr, it.d_img);
     if (rc) return rc;
     hipLaunchKernelGGL(k_scale_copy, dim3(nblk(npix, 256)), dim3(256), 0, sm, it.d_img,
                        (float)it.scale, (size_t)npix, b->d_imgs[i]);
@@ -324,7 +338,7 @@ static int batch_enqueue_image_chain(rt_batch *b, int i, const rt_step_item &it,
     if (rc) return rc;
     hipLaunchKernelGGL(k_scale_copy, dim3(nblk(npix, 256)), dim3(256), 0, sm, b->d_gimg[i],
                        (float)it.scale, (size_t)npix, b->d_gimg[i]);
-    return image_adjoint_run(m, &it.cam, b->d_Timg[i], nullptr, b->d_gimg[i], it.d_g_logrho,
+    return image_adjoint_run(m, &cam, b->d_Timg[i], nullptr, b->d_gimg[i], it.d_g_logrho,
                              it.d_g_T);
 }
 
@@ -335,7 +349,8 @@ static std::vector<const void *> model_buffers(const rt_batch *b) {
     for (const rt_model *m : b->models) {
         const void *p[] = {m->d_pipe, m->d_tally, m->d_tally2, m->d_xring, m->d_xrow, m->d_Tg,
                            m->d_Tout, m->d_rho, m->d_rhoL, m->d_alpha, m->d_src, m->d_galpha,
-                           m->d_gsrc, m->d_cnt, m->d_visits};
+                           m->d_gsrc, m->d_cnt, m->d_visits, m->d_vol, m->d_jtally, m->d_scum,
+                           m->d_scnt};
         v.insert(v.end(), p, p + sizeof(p) / sizeof(p[0]));
     }
     return v;
@@ -366,6 +381,13 @@ int batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
     for (int i = 0; i < n; ++i) {
         rc = batch_step_buffers(b, i, items[i]);
         if (rc) return rc;
+        if (items[i].nphot_scat) {
+            // the scattering Monte Carlo's key: the model's seed, decorrelated
+            // from its thermal streams (same key, other counters would do too)
+            const unsigned long long key = seeds[i] ^ 0x9E3779B97F4A7C15ull;
+            HIPCHK(ctx, hipMemcpy(scat_seed_slot(b->models[i]), &key, sizeof(key),
+                                  hipMemcpyHostToDevice));
+        }
     }
     // the step runs on the batch's stream, ordered after the caller's stream;
     // the library's enqueue functions use ctx->stream, so it is switched for

@@ -128,7 +128,8 @@ struct rt_model {
     double *d_vol = nullptr;                      // [cells] volumes (cm^3)
     unsigned long long *d_jtally = nullptr;       // [cells] path-length tallies
     double *d_scum = nullptr;                     // [cells] cumulative source luminosity
-    unsigned long long *d_scnt = nullptr;         // [8] counters
+    unsigned long long *d_scnt = nullptr;         // [8] counters; [7] the Philox key
+    double scat_lstar = 0;                        // star luminosity of the last call
     void *h_run = nullptr;        // host parameter block of the thermal kernel (ThermParams)
     void *d_run = nullptr;        // its device copy
     std::vector<unsigned long long> h_est;  // epoch starts of the last planned run
@@ -194,6 +195,10 @@ void density_enqueue(rt_model *m, const rt_density_comp *d_comps, int ncomp, hip
 int density_generate(rt_model *m, const rt_density_comp *h_comps, int ncomp);
 int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
                     const float *d_T, float *d_J, rt_scat_stats *st);
+int scat_prepare(rt_model *m);
+unsigned long long *scat_seed_slot(rt_model *m);
+void scat_enqueue(rt_model *m, double lam_um, uint64_t nphot, const float *d_T, float *d_J,
+                  hipStream_t sm);
 int thermal_run(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                 rt_thermal_stats *st);
 int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *d_rho,

@@ -47,9 +47,9 @@ struct ScatParams {
     int S, nstar;
     ScatKap kap;
     ScatStars st;
-    const double *cum;        // [cells] cumulative dust luminosity (after the stars)
-    double ltot;              // stars + dust
-    uint64_t seed, nphot;
+    const double *cum;        // [cells] cumulative luminosity, stars first (last = total)
+    const unsigned long long *seed;  // device word: Philox key (set per call or per step)
+    uint64_t nphot;
     uint32_t max_events;
     const float *invdr;       // [nr] 1/dr (au^-1)
     unsigned long long *tally;  // [cells]
@@ -124,12 +124,15 @@ __global__ void __launch_bounds__(256) k_scat_mc(ScatParams P) {
     const DevGrid &g = P.g;
     uint32_t c_esc = 0, c_abs = 0, c_sca = 0, c_cross = 0, c_kill = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < P.nphot;
+    const double ltot = P.cum[g.ncell - 1];
+    const uint64_t seed = *P.seed;
+    const uint64_t npk = ltot > 0.0 ? P.nphot : 0;
+    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npk;
          pid += stride) {
         float u[8];
-        rng8(P.seed, pid, 0u, u);
+        rng8(seed, pid, 0u, u);
         Pos p;
-        const double x = (double)u[0] * P.ltot;
+        const double x = (double)u[0] * ltot;
         int ks = -1;
         for (int k = 0; k < P.nstar; ++k)
             if (ks < 0 && x < P.st.cum[k]) ks = k;
@@ -180,7 +183,7 @@ __global__ void __launch_bounds__(256) k_scat_mc(ScatParams P) {
                 p.y = fmaf(sseg, p.dy, p.y);
                 p.z = fmaf(sseg, p.dz, p.z);
                 float v[8];
-                rng8(P.seed, pid, k++, v);
+                rng8(seed, pid, k++, v);
                 nev++;
                 if (v[0] * aext < asca) {
                     c_sca++;
@@ -225,12 +228,15 @@ __global__ void __launch_bounds__(256) k_scat_mc(ScatParams P) {
     }
 }
 
-// J_c = eps * l_c / (4 pi V_c), l in cm
+// J_c = eps * l_c / (4 pi V_c), l in cm, eps = L / nphot
 __global__ void k_scat_J(const unsigned long long *__restrict__ tally,
                          const float *__restrict__ invdr, const double *__restrict__ vol, int nr,
-                         int nc, double eps, float *__restrict__ J) {
+                         int nc, const double *__restrict__ cum, uint64_t nphot,
+                         float *__restrict__ J) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
+    const double ltot = cum[nc - 1];
+    const double eps = (ltot > 0.0 && nphot > 0) ? ltot / (double)nphot : 0.0;
     const double len_cm =
         (double)tally[c] / (double)TALLY_SCALE / (double)invdr[c % nr] * RT_LEN_UNIT;
     J[c] = (float)(eps * len_cm / (4.0 * PI_D * vol[c]));
@@ -248,11 +254,9 @@ inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 // opacity at the wavelength: the camera's log-log interpolation
 double kappa_at_lambda(const rt_model *m, const std::vector<double> &tab, int s, double lam);
 
-int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
-                    const float *d_T, float *d_J, rt_scat_stats *st) {
+// Allocates the model's scattering buffers (not capturable).
+int scat_prepare(rt_model *m) {
     rt_ctx *ctx = m->ctx;
-    if (!(lam_um > 0)) return set_err(ctx, RT_E_ARG, "wavelength must be > 0");
-    if (!d_T || !d_J) return set_err(ctx, RT_E_ARG, "NULL temperature or output pointer");
     if (m->vol.size() != (size_t)m->ncell) return set_err(ctx, RT_E_ARG, "model has no cell volumes");
     const int nc = m->ncell;
     if (!m->d_vol) {
@@ -262,7 +266,17 @@ int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
         HIPCHK(ctx, hipMalloc((void **)&m->d_scum, sizeof(double) * nc));
         HIPCHK(ctx, hipMalloc((void **)&m->d_scnt, sizeof(unsigned long long) * 8));
     }
-    hipStream_t sm = ctx->stream;
+    return RT_OK;
+}
+
+// The seed word the scattering kernels read (device memory).
+unsigned long long *scat_seed_slot(rt_model *m) { return m->d_scnt + 7; }
+
+// Source luminosities, prefix sum, Monte Carlo and J on stream sm, with the
+// Philox key in scat_seed_slot(m).  No host synchronisation: capturable.
+void scat_enqueue(rt_model *m, double lam_um, uint64_t nphot, const float *d_T, float *d_J,
+                  hipStream_t sm) {
+    const int nc = m->ncell;
     ScatParams P{};
     P.g = dev_grid(m);
     P.rhoL = m->d_rhoL;
@@ -281,53 +295,64 @@ int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
         P.st.cum[k] = lstar;
         for (int a = 0; a < 3; ++a) P.st.pos[k][a] = (float)(m->stars[5 * k + 2 + a] / RT_LEN_UNIT);
     }
+    m->scat_lstar = lstar;
     hipLaunchKernelGGL(k_scat_lum, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_rho, m->nspec, nc,
                        P.kap, d_T, nu, m->d_vol, m->d_scum);
     hipLaunchKernelGGL(k_scan_f64, dim3(1), dim3(1024), 0, sm, m->d_scum, nc, lstar);
-    double ltot = lstar;
-    HIPCHK(ctx, hipMemcpyAsync(&ltot, m->d_scum + (nc - 1), sizeof(double), hipMemcpyDeviceToHost, sm));
-    HIPCHK(ctx, hipStreamSynchronize(sm));
     P.cum = m->d_scum;
-    P.ltot = ltot;
-    P.seed = seed;
+    P.seed = scat_seed_slot(m);
     P.nphot = nphot;
     P.max_events = 1000000u;
     P.invdr = m->d_invdr;
     P.tally = m->d_jtally;
     P.cnt = m->d_scnt;
     hipLaunchKernelGGL(k_zero_u64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_jtally, (size_t)nc);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, sm, m->d_scnt, (size_t)8);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, sm, m->d_scnt, (size_t)5);
+    const uint64_t blocks = std::min<uint64_t>((nphot + 255) / 256, (uint64_t)m->ctx->n_cu * 16);
+    if (nphot > 0)
+        hipLaunchKernelGGL(k_scat_mc, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0,
+                           sm, P);
+    hipLaunchKernelGGL(k_scat_J, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_jtally, m->d_invdr,
+                       m->d_vol, m->nr, nc, (const double *)m->d_scum, nphot, d_J);
+}
+
+int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
+                    const float *d_T, float *d_J, rt_scat_stats *st) {
+    rt_ctx *ctx = m->ctx;
+    if (!(lam_um > 0)) return set_err(ctx, RT_E_ARG, "wavelength must be > 0");
+    if (!d_T || !d_J) return set_err(ctx, RT_E_ARG, "NULL temperature or output pointer");
+    int rc = scat_prepare(m);
+    if (rc) return rc;
+    hipStream_t sm = ctx->stream;
+    unsigned long long key = seed;
+    HIPCHK(ctx, hipMemcpyAsync(scat_seed_slot(m), &key, sizeof(key), hipMemcpyHostToDevice, sm));
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, sm);
-    const double eps = (ltot > 0 && nphot > 0) ? ltot / (double)nphot : 0.0;
-    if (eps > 0) {
-        const uint64_t blocks = std::min<uint64_t>((nphot + 255) / 256, (uint64_t)ctx->n_cu * 16);
-        hipLaunchKernelGGL(k_scat_mc, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0,
-                           sm, P);
-    }
+    scat_enqueue(m, lam_um, nphot, d_T, d_J, sm);
     (void)hipEventRecord(e1, sm);
-    hipLaunchKernelGGL(k_scat_J, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_jtally, m->d_invdr,
-                       m->d_vol, m->nr, nc, eps, d_J);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(sm));
-    float kms = 0.f;
-    (void)hipEventElapsedTime(&kms, e0, e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) {
         unsigned long long cnt[8];
+        double ltot = 0.0;
         HIPCHK(ctx, hipMemcpy(cnt, m->d_scnt, sizeof(cnt), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(&ltot, m->d_scum + (m->ncell - 1), sizeof(double),
+                              hipMemcpyDeviceToHost));
         st->n_escape = cnt[0];
         st->n_abs = cnt[1];
         st->n_scat = cnt[2];
         st->n_cross = cnt[3];
         st->n_killed = cnt[4];
-        st->lum_star = lstar;
-        st->lum_dust = ltot - lstar;
-        st->eps = eps;
-        st->kernel_ms = kms;
+        st->lum_star = m->scat_lstar;
+        st->lum_dust = ltot - m->scat_lstar;
+        st->eps = (ltot > 0 && nphot > 0) ? ltot / (double)nphot : 0.0;
+        st->kernel_ms = ms;
     }
     return RT_OK;
 }

@@ -310,13 +310,14 @@ class Batch:
 
     @staticmethod
     def item(cam, dxy, u, v, re, im, w, img, chi2, g_logrho=None, g_T=None, scale=1.0,
-             dra=0.0, ddec=0.0):
+             dra=0.0, ddec=0.0, nphot_scat=0):
         """A step item from device tensors (u, v float64; re, im, w, img,
-        g_logrho, g_T float32; chi2 float64 [1])."""
+        g_logrho, g_T float32; chi2 float64 [1]); nphot_scat > 0 adds the
+        scattering source (that many packets) before the image."""
         P = lambda t: None if t is None else t.data_ptr()
         return _lib.StepItem(cam, float(dxy), float(scale), P(u), P(v), int(u.numel()),
                              float(dra), float(ddec), P(re), P(im), P(w), P(img), P(chi2),
-                             P(g_logrho), P(g_T))
+                             P(g_logrho), P(g_T), int(nphot_scat))
 
     def step(self, items, nphot, seeds, use_graph=True, **opts):
         """One sampler step; returns (stats per model, device ms of the step)."""

@@ -6,7 +6,9 @@ gradient of chi^2 with respect to per-cell log density and temperature.
 
 ``SamplerStep`` builds the models of a parameter draw (``configs.sampler_batch``:
 uneven grids and photon counts), synthetic data for each (its own visibilities
-at a reference seed plus Gaussian noise of 1 % of the peak), and runs
+at a reference seed plus Gaussian noise of 1 % of the peak), and runs (images
+with the scattering source of ``nphot_scat`` packets, as pdspy's fits call
+run_image with nphot=1e5, run_disk_model.py:301-315)
 
 * ``step()``       -- the batched step (one thermal launch for all models,
   optionally replayed from a captured hipGraph), and
@@ -25,10 +27,13 @@ JY = 1e-23
 
 
 class SamplerStep:
+    SCAT_KEY = 0x9E3779B97F4A7C15  # scattering key = model seed ^ this (as the batch step)
+
     def __init__(self, params, npix=256, field_arcsec=2.0, K=5000, seed=0, max_events=1000000,
-                 mrw=1, device=0, data_seed=12345):
+                 mrw=1, device=0, data_seed=12345, nphot_scat=100000):
         import torch
         self.params = list(params)
+        self.nphot_scat = int(nphot_scat)
         self.n = len(self.params)
         self.npix = npix
         self.K = K
@@ -127,7 +132,8 @@ class SamplerStep:
         if self._items is None:
             self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u, self.v, self.re[i],
                                          self.im[i], self.w[i], self.img[i], self.chi2[i],
-                                         self.g_logrho[i], self.g_T[i], scale=self.scale)
+                                         self.g_logrho[i], self.g_T[i], scale=self.scale,
+                                         nphot_scat=self.nphot_scat)
                            for i in range(self.n)]
         return self._items
 
@@ -135,6 +141,33 @@ class SamplerStep:
         """The batched step; returns (stats per model, device ms)."""
         seeds = np.arange(self.n, dtype=np.uint64) + np.uint64(seed) * np.uint64(1000)
         return self.batch.step(self.items(), self.nphot, seeds, use_graph=use_graph, **self.opts)
+
+    def single_model(self, i, seed):
+        """Model i's share of the step through the single-model calls:
+        thermal run, imaging temperature, scattering source, image,
+        visibilities, chi^2 and the adjoint; returns (image, chi^2,
+        d chi^2 / d log rho, d chi^2 / d T)."""
+        import torch
+        m = self.models[i]
+        m.thermal(int(self.nphot[i]), int(seed), **self.opts)
+        T = m.temperature_device()
+        rho = m.rho_device()
+        dens = torch.where(rho == 0, torch.full_like(rho, 1e-30), rho)
+        Timg = ((dens * T).sum(0) / dens.sum(0)).contiguous()
+        cam = self.cams[i]
+        if self.nphot_scat:
+            J, _ = m.scattering_source(LAM_UM, self.nphot_scat, Timg,
+                                       seed=int(seed) ^ self.SCAT_KEY)
+            cam = rt.DeviceModel.camera(self.npix, self.pix_cm, LAM_UM,
+                                        incl=self.params[i]["incl"],
+                                        tile=(cam.tile_w, cam.tile_h), jscat=J)
+        img = m.image(cam, Timg)
+        re, im = rt.vis_dft(img * self.scale, self.dxy, self.u, self.v)
+        c2, gre, gim = rt.chisq(self.re[i], self.im[i], self.w[i], re, im, with_grad=True)
+        gimg = rt.vis_dft_adjoint(gre, gim, self.npix, self.npix, self.dxy, self.u,
+                                  self.v) * self.scale
+        glr, gT = m.image_adjoint(cam, Timg, None, gimg)
+        return img, c2, glr, gT
 
     def sequential(self, seed=0, progress=None):
         """The same work, model by model through the single-model calls; returns
@@ -145,18 +178,8 @@ class SamplerStep:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         chis = []
-        for i, m in enumerate(self.models):
-            m.thermal(int(self.nphot[i]), int(seeds[i]), **self.opts)
-            T = m.temperature_device()
-            rho = m.rho_device()
-            dens = torch.where(rho == 0, torch.full_like(rho, 1e-30), rho)
-            Timg = ((dens * T).sum(0) / dens.sum(0)).contiguous()
-            img = m.image(self.cams[i], Timg)
-            re, im = rt.vis_dft(img * self.scale, self.dxy, self.u, self.v)
-            c2, gre, gim = rt.chisq(self.re[i], self.im[i], self.w[i], re, im, with_grad=True)
-            gimg = rt.vis_dft_adjoint(gre, gim, self.npix, self.npix, self.dxy, self.u,
-                                      self.v) * self.scale
-            m.image_adjoint(self.cams[i], Timg, None, gimg)
+        for i in range(self.n):
+            _, c2, _, _ = self.single_model(i, int(seeds[i]))
             chis.append(c2)
             if progress is not None:
                 torch.cuda.synchronize()

@@ -83,26 +83,18 @@ def _close(a, b, rtol):
 def test_batch_step_matches_single_model_calls():
     import torch
     print("case: sampler step of 3 models, 32x32 images, K=300", flush=True)
-    s = SamplerStep(_small_params(), npix=32, K=300, max_events=100000)
+    s = SamplerStep(_small_params(), npix=32, K=300, max_events=100000, nphot_scat=20000)
     stats, ms = s.step(seed=5, use_graph=False)
     T_b, chi_b, img_b, gT_b, glr_b = _snapshot(s)
     assert all(st["n_killed"] <= 0.01 * st["n_escape"] for st in stats)
-    # the same work through the single-model calls
+    # the same work through the single-model calls (thermal, scattering
+    # source, image, visibilities, chi^2, adjoint)
     seeds = np.arange(s.n) + 5000
     for i, m in enumerate(s.models):
-        m.thermal(int(s.nphot[i]), int(seeds[i]), **s.opts)
+        img, c2, glr, gT = s.single_model(i, int(seeds[i]))
         assert np.array_equal(m.temperature(), T_b[i])
-        T = m.temperature_device()
-        rho = m.rho_device()
-        dens = torch.where(rho == 0, torch.full_like(rho, 1e-30), rho)
-        Timg = ((dens * T).sum(0) / dens.sum(0)).contiguous()
-        img = m.image(s.cams[i], Timg)
         assert _close(img.cpu().numpy(), img_b[i], 1e-6)
-        re, im = rt.vis_dft(img * s.scale, s.dxy, s.u, s.v)
-        c2, gre, gim = rt.chisq(s.re[i], s.im[i], s.w[i], re, im, with_grad=True)
         assert float(c2.item()) == pytest.approx(chi_b[i], rel=1e-5)
-        gimg = rt.vis_dft_adjoint(gre, gim, s.npix, s.npix, s.dxy, s.u, s.v) * s.scale
-        glr, gT = m.image_adjoint(s.cams[i], Timg, None, gimg)
         assert _close(gT.cpu().numpy(), gT_b[i], 1e-4)
         assert _close(glr.cpu().numpy(), glr_b[i], 1e-4)
         assert np.abs(gT_b[i]).max() > 0
@@ -111,7 +103,7 @@ def test_batch_step_matches_single_model_calls():
 def test_batch_step_graph_replays_match_eager():
     import torch
     print("case: graph capture + 3 replays vs eager steps", flush=True)
-    s = SamplerStep(_small_params(), npix=32, K=300, max_events=100000)
+    s = SamplerStep(_small_params(), npix=32, K=300, max_events=100000, nphot_scat=20000)
     ref = {}
     for seed in (7, 8):
         s.step(seed=seed, use_graph=False)

@@ -267,7 +267,8 @@ def test_chi2_of_visibilities_gradient_matches_oracle_chain():
 def test_run_image_and_visibilities_units():
     m = configs.disk_envelope(nr=32, nt=16)
     rt.run_thermal(m, nphot=20000, iseed=1, max_events=20000)
-    im = rt.run_image(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140)
+    im = rt.run_image(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140,
+                      scattering_mode_max=0)
     assert im.image.shape == (32, 32, 1, 1)
     assert im.x[16] == 0.0 and im.x[17] == pytest.approx(0.1)
     assert im.wave[0] == pytest.approx(0.13)
@@ -278,7 +279,15 @@ def test_run_image_and_visibilities_units():
     cgs = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60), T).double().cpu().numpy()
     assert np.allclose(im.image[:, :, 0, 0], cgs / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2,
                        rtol=1e-6)
-    vis = rt.run_visibilities(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140)
+    # by default the image carries the scattering source (nphot packets, iseed)
+    ims = rt.run_image(m, name="s", nphot=100000, npix=32, pixelsize=0.1, lam="1300", incl=60,
+                       dpc=140, iseed=3)
+    J, _ = dm.scattering_source(1300.0, 100000, T, seed=3)
+    cgs_s = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60, jscat=J), T).double().cpu().numpy()
+    assert np.allclose(ims.image[:, :, 0, 0],
+                       cgs_s / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2, rtol=1e-6)
+    vis = rt.run_visibilities(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140,
+                              scattering_mode_max=0)
     V = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(im.image[:, :, 0, 0])))
     scale = np.abs(V).max()
     assert np.abs(vis.real[:, 0] - V.real.ravel()).max() < 1e-5 * scale
