@@ -4,11 +4,15 @@ Workload (the production-size disk + envelope): spherical grid 256 x 128 x 1
 cells (0.1-5000 au, equatorial mirror), flared disk (1e-3 Msun, Sigma ~ r^-0.5)
 plus an Ulrich envelope (0.1 Msun, Rc = 50 au, 20 deg cavity), two dust species
 with kappa_abs = 1e3 (lam/um)^-1 (albedo 0.5, isotropic), a 4000 K 2 Rsun star,
-100 wavelengths, modified random walk on (pdspy's fitting setting); nphot
-packets per step (default 1e8).  One step = one thermal Monte Carlo run plus
-the copy of the per-species temperatures to the host (what run_thermal hands
-back); the image part renders the 256 x 256 1.3 mm image at 60 deg (2" field at
-140 pc) of the resulting temperatures and its adjoint.
+100 wavelengths, modified random walk on (pdspy's fitting setting), nphot = 1e8
+packets per run, seeds 0-3.  One step = the four runs (seeds 4k .. 4k+3) in
+one batched persistent launch plus the copy of each run's per-species
+temperatures to the host (what run_thermal hands back); each run's result is
+bit-identical to its plain single run, and the plain run's rate is reported
+beside (`single_run`; --seeds 1 times plain runs only).  The image part
+renders the 256 x 256 1.3 mm image at 60 deg (2" field at 140 pc) of the
+resulting temperatures and its adjoint, without and with the scattering
+source.
 
 Prints ONE JSON line.  --gpus above 1 is not supported (one GPU per box).
 
@@ -41,12 +45,17 @@ def parse():
     p.add_argument("--steps", type=int, default=3)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--nphot", type=float, default=1e8)
+    p.add_argument("--seeds", type=int, default=4,
+                   help="runs per step (seeds k*N .. k*N+N-1 of the same model) in one batched "
+                        "launch; the workload's seeds 0-3 are 4 runs; 1 = one plain run")
     p.add_argument("--max-events", type=float, default=1e6)
     p.add_argument("--mrw", type=int, default=1)
     p.add_argument("--cpu-seconds", type=float, default=15.0,
                    help="target wall time of the bounded CPU sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-image", action="store_true")
+    p.add_argument("--no-single", action="store_true",
+                   help="skip the reference plain run after the batched steps")
     p.add_argument("--nphot-scat", type=float, default=1e5,
                    help="scattering-source packets per image (pdspy's fits: 1e5)")
     p.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "thermal_traffic.json"))
@@ -184,47 +193,73 @@ def main():
     del src, dst
 
     # warmup (the first one also counts cell visits for the accuracy mask)
+    N = max(1, int(a.seeds))
+    batch = None
+    if N > 1:
+        models = [dm] + [rt.DeviceModel(arr) for _ in range(N - 1)]
+        batch = rt.Batch(models)
     vis = None
     for w in range(a.warmup):
         dm.thermal(nphot, 1000 + w, count_visits=1 if w == 0 else 0, **opts)
         if w == 0:
             _, vis = dm.temperature(with_visits=True)
+        if batch is not None:
+            batch.thermal([nphot] * N, [2000 + w * N + i for i in range(N)], **opts)
     torch.cuda.synchronize()
+    # one step: N runs (seeds k N .. k N + N - 1) in one launch, temperatures to the host
     stats, temps = [], []
     t0 = time.perf_counter()
     for k in range(a.steps):
-        stats.append(dm.thermal(nphot, k, **opts))
-        temps.append(dm.temperature())
+        if batch is None:
+            st = [dm.thermal(nphot, k, **opts)]
+            temps.append([dm.temperature()])
+        else:
+            st = batch.thermal([nphot] * N, [k * N + i for i in range(N)], **opts)
+            temps.append([m.temperature() for m in models])
+        stats.append(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    value = nphot * a.steps / wall
+    value = N * nphot * a.steps / wall
 
     # roofline of the MC kernel (one launch per step): algorithmic bytes
     # B_alg = N_cross * 4 S + N_abs * 16 over the launch's HIP-event duration
-    kern_ms = sum(s["kernel_ms"] for s in stats) / a.steps
-    b_alg = sum(s["n_cross"] * 4 * S + s["n_abs"] * 16 for s in stats) / a.steps
+    kern_ms = sum(st[0]["kernel_ms"] for st in stats) / a.steps
+    b_alg = sum(x["n_cross"] * 4 * S + x["n_abs"] * 16 for st in stats for x in st) / a.steps
     achieved = b_alg / (kern_ms * 1e-3) / 1e9
     traffic = None
     if os.path.exists(a.traffic_json):
         try:
             tj = json.load(open(a.traffic_json))
-            if int(tj.get("nphot", -1)) == nphot and int(tj.get("mrw", -1)) == a.mrw:
+            if (int(tj.get("nphot", -1)) == nphot and int(tj.get("mrw", -1)) == a.mrw and
+                    int(tj.get("runs", 1)) == N):
                 traffic = tj.get("bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
-    lane_steps = sum(s["n_cross"] + s["n_events"] for s in stats)
-    wave_iters = sum(s["wave_iters"] for s in stats)
+    lane_steps = sum(x["n_cross"] + x["n_events"] for st in stats for x in st)
+    wave_iters = sum(st[0]["wave_iters"] for st in stats)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "k_thermal", "b_alg_per_launch": b_alg, "kernel_ms_per_launch": kern_ms,
+            "kernel": "k_thermal" + (" (batched, %d runs per launch)" % N if N > 1 else ""),
+            "b_alg_per_launch": b_alg, "kernel_ms_per_launch": kern_ms,
             "launches_per_step": 1,
             "steps_per_s": lane_steps / (kern_ms * a.steps * 1e-3),
             "lane_utilisation": lane_steps / (64.0 * max(wave_iters, 1))}
 
     med = p99 = ncell = None
-    if vis is not None and len(temps) >= 2:
-        med, p99, ncell = dT_stats(temps[0], temps[1], vis)
-    h0 = hashlib.sha256(temps[0].astype(np.float32).tobytes()).hexdigest()[:16]
+    runs = [T for step in temps for T in step]   # seeds 0, 1, 2, ... in order
+    if vis is not None and len(runs) >= 2:
+        med, p99, ncell = dT_stats(runs[0], runs[1], vis)
+    h0 = hashlib.sha256(runs[0].astype(np.float32).tobytes()).hexdigest()[:16]
+    # one plain run for reference (seed 0: the same temperatures bit for bit)
+    single = None
+    if N > 1 and not a.no_single:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st1 = dm.thermal(nphot, 0, **opts)
+        T1 = dm.temperature()
+        t1 = time.perf_counter() - t0
+        single = {"packets_per_s": nphot / t1, "ms": t1 * 1e3, "kernel_ms": st1["kernel_ms"],
+                  "bitwise_equal_to_batched_seed0": bool(np.array_equal(T1, runs[0]))}
 
     out = {
         "metric": "photon packets/s (thermal MC) + image rays/s w/ adjoint, at <=1% median dT/T",
@@ -232,21 +267,23 @@ def main():
         "ms_per_step": wall / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "disk+envelope 256x128x1 cells, 2 species, 100 lambda, "
-                               "nphot=%g per step, MRW %s" % (nphot, "on" if a.mrw else "off"),
-                   "nphot": nphot, "cells": dm.ncell, "species": S, "max_events": int(a.max_events),
-                   "mrw": a.mrw},
+                               "nphot=%g per run, %d run(s) (seeds) per step in one launch, "
+                               "MRW %s" % (nphot, N, "on" if a.mrw else "off"),
+                   "nphot": nphot, "runs_per_step": N, "cells": dm.ncell, "species": S,
+                   "max_events": int(a.max_events), "mrw": a.mrw},
         "roofline": roof,
         "accuracy": {"median_dT_over_T": med, "p99_dT_over_T": p99, "cells_ge_100_visits": ncell,
-                     "compared": "timed steps seed 0 vs seed 1 at equal nphot",
+                     "compared": "timed runs seed 0 vs seed 1 at equal nphot",
                      "T_hash_seed0": h0},
-        "mc_stats": {k: stats[-1][k] for k in ("n_cross", "n_abs", "n_scat", "n_killed",
-                                               "n_events", "n_mrw", "n_epochs", "n_nudged",
-                                               "max_packet_steps")},
+        "mc_stats": {k: stats[-1][0][k] for k in ("n_cross", "n_abs", "n_scat", "n_killed",
+                                                  "n_events", "n_mrw", "n_epochs", "n_nudged",
+                                                  "max_packet_steps")},
+        "single_run": single,
     }
 
     if not a.no_image:
         # 1.3 mm, 256x256, 2" field at 140 pc, 60 deg; forward and adjoint
-        T = torch.tensor(rt.imaging_temperature(_with_T(model, temps[0], dm)),
+        T = torch.tensor(rt.imaging_temperature(_with_T(model, runs[0], dm)),
                          dtype=torch.float32, device="cuda")
         from pdspy_amd import tune
         pix_cm = 2.0 / 256 * 140 * inputs.AU

@@ -310,6 +310,9 @@ static double cone_hit(double x, double y, double z, double dx, double dy,
     double A = (dx * dx + dy * dy) * c2 - dz * dz * s2;
     double B = (x * dx + y * dy) * c2 - z * dz * s2;
     double C = (x * x + y * y) * c2 - z * z * s2;
+    /* a ray lying in the cone (direction and position on it, as a radial ray
+     * through the origin along a wall angle): it never leaves through it */
+    if (fabs(A) <= 1e-6 && fabs(C) <= 1e-6 * (x * x + y * y + z * z)) return INF_D;
     double D = B * B - A * C;
     if (D < 0) return INF_D;
     double sq = sqrt(D);

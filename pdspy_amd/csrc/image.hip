@@ -199,10 +199,18 @@ struct StarArg {
     float f[RT_MAX_STAR];
 };
 
+// Point stars: one block per star.  The star's pixel gets F e^-tau, tau along
+// the ray from the star to the observer.  A star at the origin (pdspy's
+// default) sees the grid along a radial ray: it stays in one (theta, phi) cell
+// column and crosses every shell over its full width, so tau is a sum over the
+// shells that the block's lanes share; any other star walks its ray cell by
+// cell on one lane.  For the adjoint (img NULL) each cell on the ray gets
+// -dimg F e^-tau l in g_alpha.
 __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux, float *img,
                         const float *dimg, double *g_alpha) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x;
     if (s >= nstar) return;
+    const int lane = threadIdx.x;
     float qx = spos[3 * s], qy = spos[3 * s + 1], qz = spos[3 * s + 2];
     float xp = qx * C.ex[0] + qy * C.ex[1] + qz * C.ex[2];
     float yp = qx * C.ey[0] + qy * C.ey[1] + qz * C.ey[2];
@@ -211,6 +219,35 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
     int k = (int)floorf(X / C.pix + 0.5f) + C.nx / 2;
     int j = (int)floorf(Y / C.pix + 0.5f) + C.ny / 2;
     if (k < 0 || k >= C.nx || j < 0 || j >= C.ny) return;
+    const DevGrid &g = C.g;
+    if (qx == 0.f && qy == 0.f && qz == 0.f) {
+        // radial ray: the (theta, phi) column of the direction
+        Pos p;
+        const float rm = 0.5f * (g.rw[0] + g.rw[1]);
+        p.x = rm * C.n[0];
+        p.y = rm * C.n[1];
+        p.z = rm * C.n[2];
+        p.dx = C.n[0];
+        p.dy = C.n[1];
+        p.dz = C.n[2];
+        locate(g, p);
+        const size_t col = (size_t)g.nr * (p.it + (size_t)g.nt * p.ip);
+        float part = 0.f;
+        for (int ir = lane; ir < g.nr; ir += blockDim.x)
+            part += C.alpha[col + ir] * (g.rw[ir + 1] - g.rw[ir]);
+        for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+        const float Is = sflux.f[s] * __expf(-part);
+        if (img) {
+            if (lane == 0) atomicAdd(&img[j * C.nx + k], Is);
+            return;
+        }
+        const float gI = dimg[j * C.nx + k];
+        if (gI == 0.f) return;
+        for (int ir = lane; ir < g.nr; ir += blockDim.x)
+            unsafeAtomicAdd(&g_alpha[col + ir], (double)(-gI * Is * (g.rw[ir + 1] - g.rw[ir])));
+        return;
+    }
+    if (lane != 0) return;
     Pos p;
     p.x = qx;
     p.y = qy;
@@ -218,17 +255,17 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
     p.dx = C.n[0];
     p.dy = C.n[1];
     p.dz = C.n[2];
-    locate(C.g, p);
+    locate(g, p);
     float tau = 0.f;
     Pos p0 = p;
     uint32_t nzero = 0;
-    for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
+    for (int guard = 0; guard < (1 << 24) && p.ir < g.nr; ++guard) {
         int kind;
-        float tw = next_wall(C.g, p, kind);
-        const float rcell = C.g.rw[p.ir + 1];
-        if (p.ir >= 0) tau += C.alpha[cell_index(C.g, p)] * tw;
-        if (cross_wall(C.g, p, tw, kind)) break;
-        unstick(C.g, p, tw, rcell, nzero);
+        float tw = next_wall(g, p, kind);
+        const float rcell = g.rw[p.ir + 1];
+        if (p.ir >= 0) tau += C.alpha[cell_index(g, p)] * tw;
+        if (cross_wall(g, p, tw, kind)) break;
+        unstick(g, p, tw, rcell, nzero);
     }
     float Is = sflux.f[s] * __expf(-tau);
     if (img) {
@@ -239,13 +276,13 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
     if (gI == 0.f) return;
     p = p0;
     nzero = 0;
-    for (int guard = 0; guard < (1 << 24) && p.ir < C.g.nr; ++guard) {
+    for (int guard = 0; guard < (1 << 24) && p.ir < g.nr; ++guard) {
         int kind;
-        float tw = next_wall(C.g, p, kind);
-        const float rcell = C.g.rw[p.ir + 1];
-        if (p.ir >= 0) unsafeAtomicAdd(&g_alpha[cell_index(C.g, p)], (double)(-gI * Is * tw));
-        if (cross_wall(C.g, p, tw, kind)) break;
-        unstick(C.g, p, tw, rcell, nzero);
+        float tw = next_wall(g, p, kind);
+        const float rcell = g.rw[p.ir + 1];
+        if (p.ir >= 0) unsafeAtomicAdd(&g_alpha[cell_index(g, p)], (double)(-gI * Is * tw));
+        if (cross_wall(g, p, tw, kind)) break;
+        unstick(g, p, tw, rcell, nzero);
     }
 }
 
@@ -440,7 +477,7 @@ int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *
     if (cam->incl_star && cam->mode == 0 && m->nstar > 0) {
         StarArg fl;
         star_fluxes(m, cam, fl);
-        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl, d_img,
+        hipLaunchKernelGGL(k_stars, dim3(m->nstar), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl, d_img,
                            (const float *)nullptr, (double *)nullptr);
     }
     HIPCHK(ctx, hipGetLastError());
@@ -473,7 +510,7 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
     if (cam->incl_star && m->nstar > 0) {
         StarArg fl;
         star_fluxes(m, cam, fl);
-        hipLaunchKernelGGL(k_stars, dim3(1), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl,
+        hipLaunchKernelGGL(k_stars, dim3(m->nstar), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl,
                            (float *)nullptr, d_dimg, m->d_galpha);
     }
     hipLaunchKernelGGL(k_cam_post, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, m->d_gsrc,

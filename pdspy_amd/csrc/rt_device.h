@@ -198,6 +198,9 @@ __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, f
     float A = (dx * dx + dy * dy) * c2 - dz * dz * s2;
     float B = (x * dx + y * dy) * c2 - z * dz * s2;
     float C = (x * x + y * y) * c2 - z * z * s2;
+    // a ray lying in the cone (direction and position on it, as a radial ray
+    // through the origin along a wall angle): it never leaves through it
+    if (fabsf(A) <= 1e-6f && fabsf(C) <= 1e-6f * (x * x + y * y + z * z)) return RT_INF;
     float D = B * B - A * C;
     int want_plus = (req * (cth > 0.f ? 1 : -1)) > 0;
     float t;

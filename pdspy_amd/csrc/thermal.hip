@@ -124,7 +124,9 @@ struct BatchArgs {
 };
 
 #define RT_PARK_WORDS 22   // words of a parked packet's state
+#ifndef RT_PARK_MAXLIVE
 #define RT_PARK_MAXLIVE 8  // a wave with at most this many live packets may park them
+#endif
 
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ unsigned ld_rlx32(const unsigned *p) {

@@ -343,6 +343,29 @@ def test_scattering_source_matches_oracle_and_is_reproducible():
     assert np.median(rel) < 5e-3 and np.percentile(rel, 90) < 5e-2
 
 
+def test_scattering_source_3d_hg_grid_matches_oracle():
+    # phi walls and Henyey-Greenstein dust (the image's scattering source is
+    # isotropic, as RADMC-3D's for pdspy's g-less opacity files)
+    print("case: scattering source vs oracle, 3D spiral 24x16x12 at 0.8 mm", flush=True)
+    m = configs.spiral_3d(nr=24, nt=16, nphi=12)
+    arr = oracle_arrays(m)
+    dm = _dev(arr)
+    rng = np.random.default_rng(5)
+    T = 15.0 + 150.0 * rng.uniform(size=dm.ncell)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    lam, n = 800.0, 300000
+    J, st = dm.scattering_source(lam, n, Td, seed=21)
+    Jo, so = oracle.scat_source(arr, T, _kap_at(arr, "kabs", lam), _kap_at(arr, "ksca", lam),
+                                lam, n, seed=21)
+    Jg = J.double().cpu().numpy()
+    vol = np.asarray(oracle_vol(arr))
+    assert (Jg * vol).sum() == pytest.approx((Jo * vol).sum(), rel=3e-3)
+    sel = Jo > 1e-3 * Jo.max()
+    rel = np.abs(Jg[sel] - Jo[sel]) / Jo[sel]
+    print("3D J vs oracle: median %.2e, p90 %.2e" % (np.median(rel), np.percentile(rel, 90)))
+    assert np.median(rel) < 5e-3 and np.percentile(rel, 90) < 5e-2
+
+
 def oracle_vol(arr):
     rw, tw, pw = arr["rw"], arr["tw"], arr["pw"]
     vr = (rw[1:] ** 3 - rw[:-1] ** 3) / 3

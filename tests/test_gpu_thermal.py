@@ -156,11 +156,13 @@ def test_device_density_generator_matches_host_builders():
     # cubic for mu0 has three real roots
     print("case: device density generator vs host builders", flush=True)
     cases = [dict(nr=256, nt=128, mdisk=1e-3, plrho=1.75, plh=1.25, menv=0.1),
+             dict(nr=32, nt=16, nphi=6, mdisk=2e-4, plrho=1.5, plh=1.15, menv=0.03),
              dict(nr=64, nt=64, mdisk=3e-5, plrho=1.2, plh=1.05, menv=2e-3, rcent=20.0),
              dict(nr=128, nt=64, mdisk=8e-3, plrho=2.6, plh=1.3, menv=0.25, rcent=120.0)]
     for c in cases:
         host = configs.disk_envelope(**c)
-        m = rt.DeviceModel(rt.model_arrays(configs.disk_envelope(nr=c["nr"], nt=c["nt"]).grid))
+        m = rt.DeviceModel(rt.model_arrays(configs.disk_envelope(
+            nr=c["nr"], nt=c["nt"], nphi=c.get("nphi", 1)).grid))
         comps = configs.disk_envelope_components(
             mdisk=c["mdisk"], plrho=c["plrho"], plh=c["plh"], menv=c["menv"],
             rcent=c.get("rcent", 50.0))

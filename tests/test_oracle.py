@@ -118,6 +118,19 @@ def test_trace_ray_matches_brute_force(nphi, mirror):
             t += l
 
 
+def test_trace_ray_through_origin_along_a_wall():
+    # the central pixel's ray is radial; on a theta wall every point of it lies
+    # in the cone: the traversal must neither cross it nor stall on it
+    m = _small_model(nphi=1, mirror=True)
+    R = m["rw"][-1]
+    for th in m["tw"][1:-1]:
+        d = np.array([0.0, -np.sin(th), -np.cos(th)])
+        o = -d * 3 * R
+        cell, ln, _ = oracle.trace_ray(m, o, d)
+        assert ln.sum() == pytest.approx(2 * R, rel=1e-9)
+        assert len(cell) <= 4 * len(m["rw"])
+
+
 # ------------------------------------------------------------------ thermal MC
 def _thin_shell_model(nr=4, nt=4, rho=1e-24, kappa=10.0):
     lam = np.logspace(-1, 4, 100)
