@@ -106,30 +106,34 @@ __device__ double disk_sigma(const rt_density_comp &C, double r) {
     return s;
 }
 
-// per disk component: mass / (2 pi trapz(r Sigma, r)) on 1000 log-spaced radii
-__global__ void k_disk_scale(const rt_density_comp *__restrict__ comps, int ncomp,
-                             double *__restrict__ scale) {
-    const int k = threadIdx.x;
+// per disk component (one wave each): mass / (2 pi trapz(r Sigma, r)) on 1000
+// log-spaced radii; the lanes share the points, the segment sums are reduced
+// in a fixed order
+#define RT_DISK_NPT 1000
+__global__ void __launch_bounds__(64) k_disk_scale(const rt_density_comp *__restrict__ comps,
+                                                   int ncomp, double *__restrict__ scale) {
+    __shared__ double xs[RT_DISK_NPT], ys[RT_DISK_NPT];
+    const int k = blockIdx.x, lane = threadIdx.x;
     if (k >= ncomp) return;
     const rt_density_comp C = comps[k];
     if (C.kind != RT_COMP_DISK) {
-        scale[k] = 1.0;
+        if (lane == 0) scale[k] = 1.0;
         return;
     }
+    const int n = RT_DISK_NPT;
     const double a = log10(C.rmin), b = log10(C.rmax);
-    const int n = 1000;
     const double step = (b - a) / (double)(n - 1);
-    double acc = 0.0, x0 = 0.0, y0 = 0.0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = lane; i < n; i += 64) {
         const double e = i == n - 1 ? b : (double)i * step + a;
         const double r = pow(10.0, e);
-        const double x = r * AU_CM;
-        const double y = r * AU_CM * disk_sigma(C, r);
-        if (i > 0) acc += (x - x0) * (y + y0) / 2.0;
-        x0 = x;
-        y0 = y;
+        xs[i] = r * AU_CM;
+        ys[i] = r * AU_CM * disk_sigma(C, r);
     }
-    scale[k] = C.mass * MSUN_G / (2.0 * PI_NP * acc);
+    __syncthreads();
+    double acc = 0.0;
+    for (int i = lane + 1; i < n; i += 64) acc += (xs[i] - xs[i - 1]) * (ys[i] + ys[i - 1]) / 2.0;
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) scale[k] = C.mass * MSUN_G / (2.0 * PI_NP * acc);
 }
 
 // component k's density at cell (ir, it) of the (r, theta) plane, envelopes
@@ -285,7 +289,7 @@ int density_prepare(rt_model *m, const rt_density_comp *h_comps, int ncomp) {
 void density_enqueue(rt_model *m, const rt_density_comp *d_comps, int ncomp, hipStream_t sm) {
     const size_t nrt = (size_t)m->nr * m->nt;
     double *scale = m->d_dens_fac, *fac = m->d_dens_fac + RT_MAX_COMP;
-    hipLaunchKernelGGL(k_disk_scale, dim3(1), dim3(64), 0, sm, d_comps, ncomp, scale);
+    hipLaunchKernelGGL(k_disk_scale, dim3(ncomp), dim3(64), 0, sm, d_comps, ncomp, scale);
     hipLaunchKernelGGL(k_comp_values, dim3(nblk(nrt * ncomp, 256)), dim3(256), 0, sm, d_comps,
                        ncomp, m->d_rc, m->d_tc, m->nr, m->nt, scale, m->d_dens_w);
     hipLaunchKernelGGL(k_env_rows, dim3(nblk((size_t)m->nr * ncomp, 64)), dim3(64), 0, sm,
