@@ -129,6 +129,7 @@ struct rt_model {
     unsigned long long *d_jtally = nullptr;       // [cells] path-length tallies
     double *d_scum = nullptr;                     // [cells] cumulative source luminosity
     unsigned long long *d_scnt = nullptr;         // [8] counters; [7] the Philox key
+    double *d_sblk = nullptr;                     // [1024] prefix-sum chunk totals
     double scat_lstar = 0;                        // star luminosity of the last call
     void *h_run = nullptr;        // host parameter block of the thermal kernel (ThermParams)
     void *d_run = nullptr;        // its device copy

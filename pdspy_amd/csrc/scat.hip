@@ -73,23 +73,64 @@ __global__ void k_scat_lum(const float *__restrict__ rho, int S, int nc, ScatKap
     lum[c] = 4.0 * PI_D * vol[c] * a * planck_d(nu, (double)T[c]);
 }
 
-// In-place inclusive prefix sum of n doubles, plus `base`, by one block:
-// each thread sums a contiguous chunk, the chunk totals are scanned in LDS,
-// then each thread rewrites its chunk.  Fixed order: deterministic.
-__global__ void __launch_bounds__(1024) k_scan_f64(double *__restrict__ x, int n, double base) {
-    __shared__ double part[1024];
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int chunk = (n + nt - 1) / nt;
-    const int i0 = min(n, t * chunk), i1 = min(n, i0 + chunk);
+// In-place inclusive prefix sum of n doubles plus `base`, in three kernels
+// with a fixed partition (deterministic): block b owns a contiguous chunk and
+// each of its threads a contiguous piece of that; the chunk totals are
+// scanned by one block; then every block rewrites its chunk from its offset.
+#define SCAN_BS 256
+#define SCAN_MAXB 1024
+__device__ __forceinline__ void scan_piece(int n, int nb, int b, int t, int &i0, int &i1) {
+    const int chunk = (n + nb - 1) / nb;
+    const int c0 = min(n, b * chunk), c1 = min(n, c0 + chunk);
+    const int piece = (c1 - c0 + SCAN_BS - 1) / SCAN_BS;
+    i0 = min(c1, c0 + t * piece);
+    i1 = min(c1, i0 + piece);
+}
+
+// per block: the sums of its threads' pieces (part_t) and of its chunk
+__global__ void __launch_bounds__(SCAN_BS) k_scan_sums(const double *__restrict__ x, int n,
+                                                       double *__restrict__ bsum) {
+    __shared__ double part[SCAN_BS];
+    const int b = blockIdx.x, t = threadIdx.x, nb = gridDim.x;
+    int i0, i1;
+    scan_piece(n, nb, b, t, i0, i1);
     double s = 0.0;
     for (int i = i0; i < i1; ++i) s += x[i];
     part[t] = s;
     __syncthreads();
     if (t == 0) {
-        double acc = base;
-        for (int k = 0; k < nt; ++k) {
+        double acc = 0.0;
+        for (int k = 0; k < SCAN_BS; ++k) acc += part[k];
+        bsum[b] = acc;
+    }
+}
+
+// exclusive prefix of the chunk totals, starting at base
+__global__ void k_scan_top(double *__restrict__ bsum, int nb, double base) {
+    if (threadIdx.x != 0) return;
+    double acc = base;
+    for (int b = 0; b < nb; ++b) {
+        const double v = bsum[b];
+        bsum[b] = acc;
+        acc += v;
+    }
+}
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_apply(double *__restrict__ x, int n,
+                                                        const double *__restrict__ boff) {
+    __shared__ double part[SCAN_BS];
+    const int b = blockIdx.x, t = threadIdx.x, nb = gridDim.x;
+    int i0, i1;
+    scan_piece(n, nb, b, t, i0, i1);
+    double s = 0.0;
+    for (int i = i0; i < i1; ++i) s += x[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        double acc = boff[b];
+        for (int k = 0; k < SCAN_BS; ++k) {
             const double v = part[k];
-            part[k] = acc;  // exclusive prefix of chunk k
+            part[k] = acc;
             acc += v;
         }
     }
@@ -265,6 +306,7 @@ int scat_prepare(rt_model *m) {
         HIPCHK(ctx, hipMalloc((void **)&m->d_jtally, sizeof(unsigned long long) * nc));
         HIPCHK(ctx, hipMalloc((void **)&m->d_scum, sizeof(double) * nc));
         HIPCHK(ctx, hipMalloc((void **)&m->d_scnt, sizeof(unsigned long long) * 8));
+        HIPCHK(ctx, hipMalloc((void **)&m->d_sblk, sizeof(double) * SCAN_MAXB));
     }
     return RT_OK;
 }
@@ -298,7 +340,12 @@ void scat_enqueue(rt_model *m, double lam_um, uint64_t nphot, const float *d_T, 
     m->scat_lstar = lstar;
     hipLaunchKernelGGL(k_scat_lum, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_rho, m->nspec, nc,
                        P.kap, d_T, nu, m->d_vol, m->d_scum);
-    hipLaunchKernelGGL(k_scan_f64, dim3(1), dim3(1024), 0, sm, m->d_scum, nc, lstar);
+    const int nb = (int)std::min<int64_t>(SCAN_MAXB, std::max<int64_t>(1, ((int64_t)nc + 4095) / 4096));
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(SCAN_BS), 0, sm, (const double *)m->d_scum, nc,
+                       m->d_sblk);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(64), 0, sm, m->d_sblk, nb, lstar);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_BS), 0, sm, m->d_scum, nc,
+                       (const double *)m->d_sblk);
     P.cum = m->d_scum;
     P.seed = scat_seed_slot(m);
     P.nphot = nphot;
