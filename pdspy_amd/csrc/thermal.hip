@@ -37,10 +37,21 @@
 // such fence appears here: counters, flags and tallies are only touched by
 // device-coherent atomics, each epoch's snapshot gets its own slot that is
 // written once before its flag is raised and never rewritten during the run
-// (so no cache can hold a stale copy), and workgroup-scope fences keep the
-// program order of the atomics.
+// (so no cache can hold a stale copy); a release waits for the wave's
+// outstanding memory operations (order_release), an acquire is a branch on
+// the acquired value.
 // Opacities and grid walls live in LDS; the current cell's densities and
 // temperature rows stay in registers.
+//
+// Batches (BATCH instance): one launch serves many runs, each with its own
+// parameter block, pipeline and tallies; a wavefront serves one run at a time
+// and moves to another when its run has nothing to hand out.  A wave left with
+// a few live packets while its run waits for a snapshot parks them in the
+// run's pool (their whole state: position, direction, cell, Philox counter,
+// frequency, remaining optical depth, event counts, slot) and moves on;
+// waves of that run adopt parked packets into dead lanes.  A packet's result
+// does not depend on which wave steps it, so every run's temperatures equal
+// those of its plain run bit for bit.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>

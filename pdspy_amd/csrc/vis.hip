@@ -8,9 +8,10 @@
 // chi^2 = sum w ((Re_d - Re_m)^2 + (Im_d - Im_m)^2) (libinterferometry.pyx:619-633).
 //
 // Forward: one (u,v) point per lane; the image is staged through LDS in
-// column strips of 32 and read as wave-wide broadcasts; the 32 column phasors
-// sit in registers and each row phasor is one sincospi, so the inner loop is
-// two FMAs per pixel.  Adjoint: one pixel per lane in 16x16 tiles; per chunk of
+// column strips and read as wave-wide broadcasts; the strip's column phasors
+// sit in registers and the row phasor is one sincospi every 16 rows and a
+// rotation by one row step in between, so the inner loop is two FMAs per
+// pixel.  Adjoint: one pixel per lane in 16x16 tiles; per chunk of
 // 64 points the tile's column phasors (times the point's complex cotangent) and
 // row phasors are built once in LDS and each lane does two FMAs per point.
 #include <cmath>
@@ -34,6 +35,8 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
         vd = (float)(v[q] * dxy);
     }
     float sr = 0.f, si = 0.f;
+    float ds, dc;  // one row step of the row phasor
+    sincospif(-2.f * vd, &ds, &dc);
     // process columns in strips of 8 so that a strip of up to 1024 rows fits LDS
     const int W = 8;
     for (int k0 = 0; k0 < nx; k0 += W) {
@@ -52,6 +55,9 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
             ar[kk] = c;
             ai[kk] = s;
         }
+        // row phasors: exact every 16 rows, rotated by one row step in between
+        // (FP32 drift below 2e-6 over 15 rotations)
+        float bs = 0.f, bc = 1.f;
         for (int j = 0; j < ny; ++j) {
             const float *row = strip + j * W;
             float rr = 0.f, ri = 0.f;
@@ -60,8 +66,13 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
                 rr = fmaf(row[kk], ar[kk], rr);
                 ri = fmaf(row[kk], ai[kk], ri);
             }
-            float bs, bc;
-            sincospif(-2.f * vd * (float)(j - ny / 2), &bs, &bc);
+            if ((j & 15) == 0) {
+                sincospif(-2.f * vd * (float)(j - ny / 2), &bs, &bc);
+            } else {
+                const float nc = bc * dc - bs * ds;
+                bs = fmaf(bc, ds, bs * dc);
+                bc = nc;
+            }
             sr += rr * bc - ri * bs;
             si += rr * bs + ri * bc;
         }
@@ -86,16 +97,21 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
                                                           float *gimg) {
     __shared__ float Ha_r[ADJ_CH][ADJ_T], Ha_i[ADJ_CH][ADJ_T];
     __shared__ float b_r[ADJ_CH][ADJ_T], b_i[ADJ_CH][ADJ_T];
+    __shared__ float H_r[ADJ_CH], H_i[ADJ_CH];
+    __shared__ float sa_c[ADJ_CH], sa_s[ADJ_CH], sb_c[ADJ_CH], sb_s[ADJ_CH];
     const int tx = threadIdx.x % ADJ_T, ty = threadIdx.x / ADJ_T;
     const int k0 = blockIdx.x * ADJ_T, j0 = blockIdx.y * ADJ_T;
     const int k = k0 + tx, j = j0 + ty;
     float acc = 0.f;
+    // loader: 4 threads per point, 4 consecutive columns / rows each
+    const int lq = threadIdx.x >> 2, lt = (threadIdx.x & 3) * 4;
     for (int q0 = 0; q0 < K; q0 += ADJ_CH) {
         __syncthreads();
-        for (int i = threadIdx.x; i < ADJ_CH * ADJ_T; i += ADJ_T * ADJ_T) {
-            int qq = i / ADJ_T, t = i - qq * ADJ_T;
-            int q = q0 + qq;
-            float hr = 0.f, hi = 0.f, ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+        // per point: the cotangent with the phase-centre shift and the
+        // one-pixel steps of its column and row phasors
+        if (threadIdx.x < ADJ_CH) {
+            const int q = q0 + threadIdx.x;
+            float hr = 0.f, hi = 0.f, s1 = 0.f, c1 = 1.f, s2 = 0.f, c2 = 1.f;
             if (q < K) {
                 double s, c;
                 sincospi(-2.0 * (u[q] * dra + v[q] * ddec), &s, &c);
@@ -103,18 +119,45 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
                 float gr = gre[q], gi = gim[q];
                 hr = (float)(gr * c + gi * s);
                 hi = (float)(gr * s - gi * c);
-                float sa, ca, sb, cb;
-                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + t - nx / 2), &sa, &ca);
-                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + t - ny / 2), &sb, &cb);
-                ar = ca;
-                ai = sa;
-                br = cb;
-                bi = sb;
+                sincospif(-2.f * (float)(u[q] * dxy), &s1, &c1);
+                sincospif(-2.f * (float)(v[q] * dxy), &s2, &c2);
             }
-            Ha_r[qq][t] = hr * ar - hi * ai;
-            Ha_i[qq][t] = hr * ai + hi * ar;
-            b_r[qq][t] = br;
-            b_i[qq][t] = bi;
+            H_r[threadIdx.x] = hr;
+            H_i[threadIdx.x] = hi;
+            sa_c[threadIdx.x] = c1;
+            sa_s[threadIdx.x] = s1;
+            sb_c[threadIdx.x] = c2;
+            sb_s[threadIdx.x] = s2;
+        }
+        __syncthreads();
+        {
+            const int q = q0 + lq;
+            float hr = 0.f, hi = 0.f, ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+            float dac = 1.f, das = 0.f, dbc = 1.f, dbs = 0.f;
+            if (q < K) {
+                hr = H_r[lq];
+                hi = H_i[lq];
+                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + lt - nx / 2), &ai, &ar);
+                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + lt - ny / 2), &bi, &br);
+                dac = sa_c[lq];
+                das = sa_s[lq];
+                dbc = sb_c[lq];
+                dbs = sb_s[lq];
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                Ha_r[lq][lt + e] = hr * ar - hi * ai;
+                Ha_i[lq][lt + e] = hr * ai + hi * ar;
+                b_r[lq][lt + e] = br;
+                b_i[lq][lt + e] = bi;
+                // next column / row: rotate by one pixel step
+                const float nar = ar * dac - ai * das;
+                ai = fmaf(ar, das, ai * dac);
+                ar = nar;
+                const float nbr = br * dbc - bi * dbs;
+                bi = fmaf(br, dbs, bi * dbc);
+                br = nbr;
+            }
         }
         __syncthreads();
 #pragma unroll 8
