@@ -192,8 +192,6 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
     }
 }
 
-// stars: one thread per star; adds F exp(-tau) to the pixel holding the star,
-// or (adjoint) dL/dalpha along the star's line of sight
 // point-source fluxes (erg/s/cm^2/Hz/sr over one pixel) passed by value
 struct StarArg {
     float f[RT_MAX_STAR];
