@@ -1,13 +1,16 @@
 // image.hip -- ray-traced images and their adjoint on gfx950: what RADMC-3D's
 // camera computes for pdspy's Model.run_image (pdspy/modeling/Model.py:515-577,
-// radmc3d/run.py:157-289) with scattering_mode_max=0, i.e. the formal solution
-//     I = sum_k S_k (1 - exp(-dtau_k)) exp(-tau_k),  S = B_nu(T),
-// along parallel rays, plus blackbody point stars attenuated by the model
-// (pdspy's nostar=False).  Pixel (j,k) is centred at ((k - nx/2) pix,
-// (j - ny/2) pix) in the sky plane, the centring pdspy's zoomau produces
-// (Model.py:522-527).
+// radmc3d/run.py:157-289), i.e. the formal solution
+//     I = sum_k S_k (1 - exp(-dtau_k)) exp(-tau_k)
+// along parallel rays, with S = B_nu(T) and absorption-only extinction
+// (scattering_mode_max=0), or, given the scattering source's mean intensity J
+// (scat.hip), extinction rho (kabs + ksca) and S = (rho kabs B + rho ksca J) /
+// (rho kext) (isotropic scattering); plus blackbody point stars attenuated by
+// the model (pdspy's nostar=False).  Pixel (j,k) is centred at
+// ((k - nx/2) pix, (j - ny/2) pix) in the sky plane, the centring pdspy's
+// zoomau produces (Model.py:522-527).
 //
-// Kernels: a per-cell prep pass (alpha = sum_s rho_s kappa_s, S = B_nu(T)) so
+// Kernels: a per-cell prep pass (alpha and S per cell) so
 // the ray loop reads 8 bytes per crossed cell; one ray per lane; the adjoint
 // re-integrates each ray twice (first for I, then for dI/d dtau_k = e^-tau_k
 // e^-dtau_k S_k - (I - I_<=k)) instead of storing tau, and accumulates
