@@ -1,0 +1,77 @@
+"""Edge cases on the GPU: packet counts below and around one wavefront, a grid
+without matter, a one-cell grid, several stars off the origin, empty
+scattering sources."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from pdspy_amd import rt
+from tests.helpers import dT_stats
+from tests.test_oracle import _small_model
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 1000])
+def test_tiny_packet_counts(n):
+    print("case: %d packets" % n, flush=True)
+    dm = rt.DeviceModel(_small_model())
+    st = dm.thermal(n, 2)
+    assert st["n_escape"] + st["n_killed"] == n
+    T1 = dm.temperature()
+    dm.thermal(n, 2)
+    assert np.array_equal(T1, dm.temperature())
+    _, _, so = oracle.thermal(_small_model(), n, seed=2)
+    assert so["n_escape"] + so["n_killed"] == n
+
+
+def test_grid_without_matter():
+    print("case: zero density everywhere", flush=True)
+    m = _small_model(nr=6, nt=5)
+    m["rho"] = np.zeros_like(m["rho"])
+    dm = rt.DeviceModel(m)
+    st = dm.thermal(5000, 1)
+    assert st["n_escape"] == 5000 and st["n_abs"] == 0 and st["n_scat"] == 0
+    assert np.all(dm.temperature() == 0.0)
+    J, sst = dm.scattering_source(1.0, 1000, torch.zeros(30, dtype=torch.float32,
+                                                        device="cuda"), seed=3)
+    # starlight still fills the empty grid: J follows the dilution law
+    assert sst["n_escape"] == 1000 and sst["n_abs"] == 0
+    assert torch.all(torch.isfinite(J)) and torch.all(J > 0)
+
+
+def test_one_cell_grid_matches_oracle():
+    print("case: one-cell grid", flush=True)
+    m = _small_model(nr=1, nt=1)
+    dm = rt.DeviceModel(m)
+    dm.thermal(20000, 1)
+    T = dm.temperature()
+    To, _, _ = oracle.thermal(m, 20000, seed=1)
+    assert T.shape == To.shape == (1, 1)
+    assert T[0, 0] == pytest.approx(To[0, 0], rel=2e-3)
+
+
+def test_two_stars_off_origin_3d_matches_oracle():
+    print("case: two stars, one off the origin, 6x5x4 grid", flush=True)
+    m = _small_model(nr=6, nt=5, nphi=4)
+    m["stars"] = np.array([[7e10, 4000.0, 0, 0, 0], [5e10, 6000.0, 3e13, 1e13, 2e13]])
+    dm = rt.DeviceModel(m)
+    dm.thermal(50000, 1, count_visits=1)
+    T, vis = dm.temperature(with_visits=True)
+    To, vo, _ = oracle.thermal(m, 50000, seed=1)
+    med, p99, n = dT_stats(To, T, vo)
+    print("two stars: median %.2e p99 %.2e over %d cells" % (med, p99, n))
+    assert n > 50 and med <= 0.01 and p99 <= 0.05
+
+
+def test_scattering_source_without_packets():
+    print("case: scattering source with 0 and 1 packets", flush=True)
+    m = _small_model(nr=6, nt=5)
+    dm = rt.DeviceModel(m)
+    Td = torch.full((30,), 50.0, dtype=torch.float32, device="cuda")
+    J0, s0 = dm.scattering_source(100.0, 0, Td, seed=1)
+    assert torch.all(J0 == 0) and s0["n_escape"] + s0["n_abs"] == 0
+    J1, s1 = dm.scattering_source(100.0, 1, Td, seed=1)
+    assert s1["n_escape"] + s1["n_abs"] + s1["n_killed"] == 1
+    assert torch.all(torch.isfinite(J1)) and torch.all(J1 >= 0)
