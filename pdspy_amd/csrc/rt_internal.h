@@ -197,6 +197,9 @@ int density_generate(rt_model *m, const rt_density_comp *h_comps, int ncomp);
 int scat_source_run(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
                     const float *d_T, float *d_J, rt_scat_stats *st);
 int scat_prepare(rt_model *m);
+// opacity table `tab` ([species][lambda]) of species s at wavelength lam (um):
+// the camera's log-log interpolation
+double kappa_at_lambda(const rt_model *m, const std::vector<double> &tab, int s, double lam);
 unsigned long long *scat_seed_slot(rt_model *m);
 void scat_enqueue(rt_model *m, double lam_um, uint64_t nphot, const float *d_T, float *d_J,
                   hipStream_t sm);

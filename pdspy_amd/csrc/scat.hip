@@ -292,9 +292,6 @@ inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
 
-// opacity at the wavelength: the camera's log-log interpolation
-double kappa_at_lambda(const rt_model *m, const std::vector<double> &tab, int s, double lam);
-
 // Allocates the model's scattering buffers (not capturable).
 int scat_prepare(rt_model *m) {
     rt_ctx *ctx = m->ctx;

@@ -135,6 +135,9 @@ struct BatchArgs {
 };
 
 #define RT_PARK_WORDS 22   // words of a parked packet's state
+#ifndef RT_SNAP_POLL
+#define RT_SNAP_POLL 64u  // a busy wave looks at its run's next snapshot every this many iterations
+#endif
 #ifndef RT_PARK_MAXLIVE
 #define RT_PARK_MAXLIVE 8  // a wave with at most this many live packets may park them
 #endif
@@ -641,7 +644,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
         }
 
         // ---- help with the run's next temperature snapshot
-        if ((iter & 63u) == 0u) {
+        if ((iter & (RT_SNAP_POLL - 1u)) == 0u) {
             snapshot_help(P, lane);
             // a wave holding a long random walk is issued first on its SIMD:
             // those walks are what the end of a run waits for
