@@ -22,6 +22,10 @@ on), each step = one thermal launch for all of them, then per model the 1.3 mm
 256x256 image, visibilities at 5e4 (u,v) points, chi^2 and the adjoint
 gradient, replayed from a hipGraph -- timed against the same work done model by
 model through the single-model calls.
+
+--vis: instead, the two visibility samplers (exact DFT; galario's method, the
+image's transform on its pixel grid + bilinear interpolation) on 256^2 and
+1024^2 images at 5e4 and 1e6 (u,v) points.
 """
 import argparse
 import hashlib
@@ -63,7 +67,55 @@ def parse():
     p.add_argument("--batch-models", type=int, default=64)
     p.add_argument("--batch-k", type=int, default=50000)
     p.add_argument("--no-sequential", action="store_true")
+    p.add_argument("--vis", action="store_true",
+                   help="visibility samplers only: exact DFT and galario's method")
     return p.parse_args()
+
+
+def vis_main(a):
+    """Visibilities of a resolved source (Gaussian of sigma 0.15 arcsec, offset
+    from the centre) at 256^2 and 1024^2 pixels, at the workload's 5e4 (u,v)
+    points and at 1e6: the exact DFT (rt_vis_dft) and galario's method
+    (rt_vis_grid, grid transform + bilinear), with the latter's deviation from
+    the former."""
+    import torch
+    from pdspy_amd import configs, inputs, rt
+    rows = []
+    for n in (256, 1024):
+        pix = 2.0 / n  # 2" field
+        x = (np.arange(n) - n // 2) * pix
+        X, Y = np.meshgrid(x - 0.05, x + 0.02)
+        img_h = np.exp(-(X ** 2 + Y ** 2) / (2 * 0.15 ** 2))
+        img = torch.tensor(img_h, dtype=torch.float32, device="cuda")
+        dxy = pix * inputs.arcsec
+        for K in (50000, 1000000):
+            u, v = configs.uv_points(K=K, lam_m=1.3e-3, seed=0)
+            u = torch.tensor(u, device="cuda")
+            v = torch.tensor(v, device="cuda")
+            row = {"npix": n, "K": K}
+            res = {}
+            for name, fn in (("dft", rt.vis_dft), ("grid", rt.vis_grid)):
+                fn(img, dxy, u, v)
+                torch.cuda.synchronize()
+                reps = 3 if (name == "dft" and n * n * K > 1e11) else 10
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    res[name] = fn(img, dxy, u, v)
+                torch.cuda.synchronize()
+                row[name + "_ms"] = (time.perf_counter() - t0) / reps * 1e3
+            d = torch.hypot(res["grid"][0] - res["dft"][0], res["grid"][1] - res["dft"][1])
+            row["grid_max_dev_over_V0"] = float(d.max()) / float(img_h.sum())
+            row["speedup"] = row["dft_ms"] / row["grid_ms"]
+            print("vis: %s" % row, file=sys.stderr, flush=True)
+            rows.append(row)
+    out = {"metric": "visibilities/s (galario's method, 256x256 image, 5e4 (u,v))",
+           "value": 50000 / (rows[0]["grid_ms"] * 1e-3), "unit": "visibilities/s", "n_gpus": 1,
+           "steps": 10, "warmup": 1, "ms_per_step": rows[0]["grid_ms"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": "Gaussian disk image, 2 arcsec field, (u,v) of 15-2000 m "
+                                  "baselines at 1.3 mm"},
+           "cases": rows}
+    print(json.dumps(out), flush=True)
 
 
 def batch_main(a):
@@ -174,6 +226,9 @@ def main():
         sys.exit(2)
     if a.batch:
         batch_main(a)
+        return
+    if a.vis:
+        vis_main(a)
         return
     import torch
     from pdspy_amd import configs, inputs, rt
@@ -339,6 +394,28 @@ def main():
                 "source_kernel_ms": sst["kernel_ms"], "ms_fwd_adj": t_sa * 1e3,
                 "rays_per_s_fwd_adj_incl_source": 65536 / (t_s + t_sa),
                 "n_scat": sst["n_scat"], "n_abs": sst["n_abs"], "n_escape": sst["n_escape"]}
+        # visibilities of that image at the workload's 5e4 (u,v) points and at
+        # 1e6 (a large interferometric data set): exact DFT and galario's method
+        img = dm.image(cam, T)
+        dxy = 2.0 / 256 * inputs.arcsec
+        vis_out = {}
+        for K in (50000, 1000000):
+            u, v = configs.uv_points(K=K, lam_m=1.3e-3, seed=0)
+            u = torch.tensor(u, device="cuda")
+            v = torch.tensor(v, device="cuda")
+            row = {}
+            for name, fn in (("dft", rt.vis_dft), ("grid", rt.vis_grid)):
+                fn(img, dxy, u, v)
+                torch.cuda.synchronize()
+                reps = 5
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    fn(img, dxy, u, v)
+                torch.cuda.synchronize()
+                row[name + "_ms"] = (time.perf_counter() - t0) / reps * 1e3
+            vis_out["K=%d" % K] = row
+        out["visibilities"] = dict(vis_out, config="256x256 image above, 1.3 mm, (u,v) of "
+                                   "15-2000 m baselines")
     if not a.no_cpu:
         cb = cpu_baseline(arr, a.max_events, a.cpu_seconds, a.mrw)
         out["cpu_baseline"] = cb

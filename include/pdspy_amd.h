@@ -233,6 +233,18 @@ int rt_vis_dft(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
 int rt_vis_dft_adjoint(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx,
                        int ny, double dxy, const double *d_u, const double *d_v, int K,
                        double dra, double ddec, float *d_g_img);
+/* Gridded sampler, galario's method (galario's sampleImage, called at
+ * interpolate_model.py:20-27; Tazzari et al. 2018, MNRAS 476, 4527): the exact
+ * DFT above on the image's pixel-conjugate grid (u_a = (a - nx/2)/(nx dxy),
+ * v_b = (b - ny/2)/(ny dxy)), bilinear interpolation at each (u,v), then the
+ * phase-centre shift.  Equal to rt_vis_dft at grid nodes; a point without
+ * four grid nodes around it gets NaN.  2 <= nx, ny <= 4096.  d_work: device
+ * scratch of rt_vis_grid_workspace(nx, ny) bytes, or NULL (allocated and freed
+ * stream-ordered). */
+int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                const double *d_u, const double *d_v, int K, double dra, double ddec,
+                float *d_re, float *d_im, void *d_work);
+size_t rt_vis_grid_workspace(int nx, int ny);
 /* chi^2 = sum w ((dre-mre)^2 + (dim-mim)^2) into d_out[0] (double), and
  * (optional, non-NULL) d_gre/d_gim = d chi^2 / d(mre, mim). */
 int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d_w,

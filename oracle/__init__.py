@@ -272,6 +272,33 @@ def vis_dft_adjoint(gre, gim, nx, ny, dxy, u, v, dra=0.0, ddec=0.0):
     return g
 
 
+def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0):
+    """galario's sampling method in FP64 numpy: the image's transform on its
+    pixel-conjugate grid (imtovis's fftshift(fft2(ifftshift(I))), imaging/
+    imtovis.py:13-24), bilinear interpolation at (u, v), phase-centre shift
+    (interferometry/interpolate_model.py:20-27 calls galario's sampleImage).
+    Pinned in the tests by the exact DFT at grid nodes and by scipy's
+    RegularGridInterpolator between them."""
+    img = _f64(img)
+    u = _f64(u)
+    v = _f64(v)
+    ny, nx = img.shape
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img)))
+    fa = u * nx * dxy + nx // 2
+    fb = v * ny * dxy + ny // 2
+    a0 = np.floor(fa)
+    b0 = np.floor(fb)
+    ok = (a0 >= 0) & (a0 <= nx - 2) & (b0 >= 0) & (b0 <= ny - 2)
+    a = np.where(ok, a0, 0).astype(np.int64)
+    b = np.where(ok, b0, 0).astype(np.int64)
+    ta = fa - a0
+    tb = fb - b0
+    V = ((1 - ta) * (1 - tb) * F[b, a] + ta * (1 - tb) * F[b, a + 1] +
+         (1 - ta) * tb * F[b + 1, a] + ta * tb * F[b + 1, a + 1])
+    V = V * np.exp(-2j * np.pi * (u * dra + v * ddec))
+    return np.where(ok, V.real, np.nan), np.where(ok, V.imag, np.nan)
+
+
 def chisq(dre, dim, w, mre, mim):
     a = [_f64(x) for x in (dre, dim, w, mre, mim)]
     return lib().or_chisq(*[x.ctypes.data for x in a], len(a[0]))

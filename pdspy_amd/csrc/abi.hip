@@ -426,6 +426,15 @@ int rt_vis_dft_adjoint(rt_ctx *ctx, const float *d_gre, const float *d_gim, int 
     return vis_adjoint_run(ctx, d_gre, d_gim, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_g_img);
 }
 
+int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
+                const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
+                void *d_work) {
+    if (!ctx) return RT_E_ARG;
+    return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, d_work);
+}
+
+size_t rt_vis_grid_workspace(int nx, int ny) { return vis_grid_workspace(nx, ny); }
+
 int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d_w,
              const float *d_mre, const float *d_mim, int K, double *d_out, float *d_gre,
              float *d_gim) {

@@ -218,3 +218,9 @@ int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx,
 int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, const float *mre,
               const float *mim, int K, double *out, float *gre, float *gim, double *ws);
 size_t chisq_workspace(int K);
+// gridded sampler; work: nullable, vis_grid_workspace(nx, ny) bytes (else
+// allocated stream-ordered)
+int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
+                 const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
+                 void *d_work);
+size_t vis_grid_workspace(int nx, int ny);

@@ -4,7 +4,8 @@
 // y_j = (j - ny/2) dxy, times the phase-centre shift exp(-2 pi i (u dra + v ddec)).
 // On the pixel-conjugate grid this is pdspy's imtovis (imaging/imtovis.py:13-24,
 // fftshift(fft2(ifftshift(I)))); at arbitrary (u,v) it is the exact sampler
-// behind interpolate_model (interferometry/interpolate_model.py:20-27).
+// behind interpolate_model (interferometry/interpolate_model.py:20-27), and the
+// gridded sampler further down samples it as galario does.
 // chi^2 = sum w ((Re_d - Re_m)^2 + (Im_d - Im_m)^2) (libinterferometry.pyx:619-633).
 //
 // Forward: one (u,v) point per lane; the image is staged through LDS in
@@ -165,6 +166,164 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
             acc = fmaf(Ha_r[qq][tx], b_r[qq][ty], fmaf(-Ha_i[qq][tx], b_i[qq][ty], acc));
     }
     if (k < nx && j < ny) gimg[(size_t)j * nx + k] = acc;
+}
+
+// ---- gridded sampler: galario's method (Tazzari, Beaujean & Testi 2018,
+// MNRAS 476, 4527: Fourier transform of the image on its pixel-conjugate grid,
+// bilinear interpolation at each (u,v), then the phase-centre shift), which is
+// what pdspy's interpolate_model calls (interpolate_model.py:20-27).
+// F[b][a] = sum_jk I[j][k] w_nx((a - nx/2)(k - nx/2)) w_ny((b - ny/2)(j - ny/2)),
+// w_n(r) = exp(-2 pi i r / n), is the exact DFT at u_a = (a - nx/2) / (nx dxy),
+// v_b = (b - ny/2) / (ny dxy); it is computed as two passes of 1D transforms
+// (rows, then columns) with integer phase indices into one twiddle table per
+// axis, so every twiddle is exact to FP32 rounding.  Work:
+// [A: ny x nx float2][F: ny x nx float2][twiddles: nx + ny float2].
+#define GRID_BS 256
+#define GRID_MAXN 4096
+
+__global__ void __launch_bounds__(GRID_BS) k_grid_twiddles(int nx, int ny, float2 *tw) {
+    const int i = blockIdx.x * GRID_BS + threadIdx.x;
+    if (i >= nx + ny) return;
+    const int n = i < nx ? nx : ny, r = i < nx ? i : i - nx;
+    double s, c;
+    sincospi(-2.0 * (double)r / (double)n, &s, &c);
+    tw[i] = make_float2((float)c, (float)s);
+}
+
+// n-periodic phase index of k * x0 and of the step k, both in [0, n)
+__device__ __forceinline__ void phase_index(int k, int x0, int n, int &r, int &dr) {
+    long long a = ((long long)k * (long long)x0) % n;
+    r = (int)(a < 0 ? a + n : a);
+    const int d = k % n;
+    dr = d < 0 ? d + n : d;
+}
+
+// rows: A[j][a] = sum_k I[j][k] w_nx((a - nx/2)(k - nx/2)); block = 256 a's of row j
+__global__ void __launch_bounds__(GRID_BS) k_grid_rows(const float *img, const float2 *twx,
+                                                       int nx, float2 *A) {
+    __shared__ float row[GRID_MAXN];
+    __shared__ float2 tw[GRID_MAXN];
+    const int j = blockIdx.y;
+    for (int i = threadIdx.x; i < nx; i += GRID_BS) {
+        row[i] = img[(size_t)j * nx + i];
+        tw[i] = twx[i];
+    }
+    __syncthreads();
+    const int a = blockIdx.x * GRID_BS + threadIdx.x;
+    if (a >= nx) return;
+    int r, dr;
+    phase_index(a - nx / 2, -(nx / 2), nx, r, dr);
+    float sr = 0.f, si = 0.f;
+    for (int k = 0; k < nx; ++k) {
+        const float p = row[k];
+        const float2 w = tw[r];
+        sr = fmaf(p, w.x, sr);
+        si = fmaf(p, w.y, si);
+        r += dr;
+        if (r >= nx) r -= nx;
+    }
+    A[(size_t)j * nx + a] = make_float2(sr, si);
+}
+
+// columns: F[b][a] = sum_j A[j][a] w_ny((b - ny/2)(j - ny/2)); block = 256 a's x
+// KB rows b, so each A element read serves KB outputs (KB = 1 for small grids,
+// where the launch would otherwise hold fewer blocks than the GPU has CUs)
+template <int GRID_KB>
+__global__ void __launch_bounds__(GRID_BS) k_grid_cols(const float2 *A, const float2 *twy, int nx,
+                                                       int ny, float2 *F) {
+    __shared__ float2 tw[GRID_MAXN];
+    for (int i = threadIdx.x; i < ny; i += GRID_BS) tw[i] = twy[i];
+    __syncthreads();
+    const int a = blockIdx.x * GRID_BS + threadIdx.x;
+    const int b0 = blockIdx.y * GRID_KB;
+    if (a >= nx) return;
+    int r[GRID_KB], dr[GRID_KB];
+    float fr[GRID_KB], fi[GRID_KB];
+#pragma unroll
+    for (int t = 0; t < GRID_KB; ++t) {
+        phase_index(b0 + t - ny / 2, -(ny / 2), ny, r[t], dr[t]);
+        fr[t] = 0.f;
+        fi[t] = 0.f;
+    }
+    for (int j = 0; j < ny; ++j) {
+        const float2 x = A[(size_t)j * nx + a];
+#pragma unroll
+        for (int t = 0; t < GRID_KB; ++t) {
+            const float2 w = tw[r[t]];
+            fr[t] = fmaf(x.x, w.x, fmaf(-x.y, w.y, fr[t]));
+            fi[t] = fmaf(x.x, w.y, fmaf(x.y, w.x, fi[t]));
+            r[t] += dr[t];
+            if (r[t] >= ny) r[t] -= ny;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < GRID_KB; ++t)
+        if (b0 + t < ny) F[(size_t)(b0 + t) * nx + a] = make_float2(fr[t], fi[t]);
+}
+
+// one (u,v) point per lane: bilinear interpolation between the four grid nodes
+// around it, then the phase-centre shift; a point whose four nodes are not all
+// on the grid gets NaN
+__global__ void __launch_bounds__(GRID_BS) k_grid_sample(const float2 *F, int nx, int ny,
+                                                         double dxy, const double *u,
+                                                         const double *v, int K, double dra,
+                                                         double ddec, float *re, float *im) {
+    const int q = blockIdx.x * GRID_BS + threadIdx.x;
+    if (q >= K) return;
+    const double uq = u[q], vq = v[q];
+    const double fa = uq * ((double)nx * dxy) + (double)(nx / 2);
+    const double fb = vq * ((double)ny * dxy) + (double)(ny / 2);
+    const double a0 = floor(fa), b0 = floor(fb);
+    if (!(a0 >= 0.0 && a0 <= (double)(nx - 2) && b0 >= 0.0 && b0 <= (double)(ny - 2))) {
+        re[q] = __builtin_nanf("");
+        im[q] = __builtin_nanf("");
+        return;
+    }
+    const size_t o = (size_t)b0 * nx + (size_t)a0;
+    const float2 f00 = F[o], f01 = F[o + 1], f10 = F[o + nx], f11 = F[o + nx + 1];
+    const double ta = fa - a0, tb = fb - b0;
+    const double w00 = (1.0 - ta) * (1.0 - tb), w01 = ta * (1.0 - tb), w10 = (1.0 - ta) * tb,
+                 w11 = ta * tb;
+    const double sr = w00 * f00.x + w01 * f01.x + w10 * f10.x + w11 * f11.x;
+    const double si = w00 * f00.y + w01 * f01.y + w10 * f10.y + w11 * f11.y;
+    double s, c;
+    sincospi(-2.0 * (uq * dra + vq * ddec), &s, &c);
+    re[q] = (float)(sr * c - si * s);
+    im[q] = (float)(sr * s + si * c);
+}
+
+size_t vis_grid_workspace(int nx, int ny) {
+    return sizeof(float2) * (2 * (size_t)nx * ny + (size_t)nx + ny);
+}
+
+int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
+                 const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
+                 void *d_work) {
+    if (nx < 2 || ny < 2 || nx > GRID_MAXN || ny > GRID_MAXN || K < 0 || !(dxy > 0))
+        return set_err(ctx, RT_E_ARG, "vis grid: invalid sizes (2 <= nx, ny <= %d)", GRID_MAXN);
+    if (K == 0) return RT_OK;
+    float2 *ws = (float2 *)d_work;
+    if (!ws) HIPCHK(ctx, hipMallocAsync((void **)&ws, vis_grid_workspace(nx, ny), ctx->stream));
+    float2 *A = ws, *F = ws + (size_t)nx * ny, *tw = ws + 2 * (size_t)nx * ny;
+    hipLaunchKernelGGL(k_grid_twiddles, dim3((nx + ny + GRID_BS - 1) / GRID_BS), dim3(GRID_BS), 0,
+                       ctx->stream, nx, ny, tw);
+    hipLaunchKernelGGL(k_grid_rows, dim3((nx + GRID_BS - 1) / GRID_BS, ny), dim3(GRID_BS), 0,
+                       ctx->stream, d_img, tw, nx, A);
+    const int bx = (nx + GRID_BS - 1) / GRID_BS;
+    if (bx * ((ny + 7) / 8) >= 512)
+        hipLaunchKernelGGL(k_grid_cols<8>, dim3(bx, (ny + 7) / 8), dim3(GRID_BS), 0, ctx->stream, A,
+                           tw + nx, nx, ny, F);
+    else if (bx * ((ny + 3) / 4) >= 512)
+        hipLaunchKernelGGL(k_grid_cols<4>, dim3(bx, (ny + 3) / 4), dim3(GRID_BS), 0, ctx->stream, A,
+                           tw + nx, nx, ny, F);
+    else
+        hipLaunchKernelGGL(k_grid_cols<1>, dim3(bx, ny), dim3(GRID_BS), 0, ctx->stream, A, tw + nx,
+                           nx, ny, F);
+    hipLaunchKernelGGL(k_grid_sample, dim3((K + GRID_BS - 1) / GRID_BS), dim3(GRID_BS), 0,
+                       ctx->stream, F, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im);
+    if (!d_work) HIPCHK(ctx, hipFreeAsync(ws, ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return RT_OK;
 }
 
 #define CHI_BS 256

@@ -351,6 +351,40 @@ def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0, device=0):
     return re, im
 
 
+def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
+    """V at (u, v) of a float32 torch image [ny, nx] by galario's method (the
+    image's DFT on its pixel-conjugate grid, bilinear interpolation, phase-centre
+    shift; galario sampleImage as called at interpolate_model.py:20-27).  A point
+    outside the grid's interpolation range raises ValueError.  ``work``: an
+    optional uint8 device tensor of ``rt_vis_grid_workspace(nx, ny)`` bytes."""
+    import torch
+    ctx = _lib.context(device)
+    img = img.contiguous().float()
+    ny, nx = img.shape
+    u = torch.as_tensor(u, dtype=torch.float64, device="cuda").contiguous()
+    v = torch.as_tensor(v, dtype=torch.float64, device="cuda").contiguous()
+    K = u.numel()
+    if K:
+        # the four nodes around each point must lie on the grid (galario's
+        # sampleImage checks the uv coverage against the image the same way)
+        fa = u * (nx * dxy) + nx // 2
+        fb = v * (ny * dxy) + ny // 2
+        ok = ((fa >= 0) & (fa < nx - 1) & (fb >= 0) & (fb < ny - 1)).all()
+        if not bool(ok):
+            raise ValueError("vis_grid: (u,v) beyond the image's uv grid (|u| < (nx/2 - 1) / "
+                             "(nx dxy)); use a finer pixel size or vis_dft")
+    re = torch.empty(K, dtype=torch.float32, device="cuda")
+    im = torch.empty(K, dtype=torch.float32, device="cuda")
+    need = int(ctx.L.rt_vis_grid_workspace(nx, ny))
+    if work is not None and work.numel() < need:
+        raise ValueError("vis_grid: work holds %d bytes, %d needed" % (work.numel(), need))
+    ctx.check(ctx.L.rt_vis_grid(ctx.h, C.c_void_p(img.data_ptr()), nx, ny, float(dxy),
+                                C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), K, float(dra),
+                                float(ddec), C.c_void_p(re.data_ptr()), C.c_void_p(im.data_ptr()),
+                                C.c_void_p(work.data_ptr() if work is not None else None)))
+    return re, im
+
+
 def vis_dft_adjoint(gre, gim, nx, ny, dxy, u, v, dra=0.0, ddec=0.0, device=0):
     import torch
     ctx = _lib.context(device)
@@ -517,15 +551,23 @@ def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="
     return model.visibilities[name]
 
 
-def interpolate_model(u, v, freq, model_image, dRA=0.0, dDec=0.0, device=0, **unused):
-    """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57),
-    by exact DFT; dRA/dDec in arcsec shift the phase centre."""
+def interpolate_model(u, v, freq, model_image, dRA=0.0, dDec=0.0, device=0, code="galario",
+                      **unused):
+    """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57);
+    dRA/dDec in arcsec shift the phase centre.  ``code="galario"`` (pdspy's
+    default) samples as galario does, by bilinear interpolation of the image's
+    transform on its pixel-conjugate grid (``vis_grid``); ``code="exact"`` takes
+    the exact DFT at each point (``vis_dft``).  pdspy's other codes
+    (galario-unstructured, trift) are not on this path."""
     import torch
+    if code not in ("galario", "exact"):
+        raise ValueError("interpolate_model: code %r is not supported (galario, exact)" % (code,))
+    sample = vis_grid if code == "galario" else vis_dft
     dxy = (model_image.x[1] - model_image.x[0]) * arcsec
     real, imag = [], []
     for i in range(model_image.image.shape[2]):
         img = torch.tensor(model_image.image[:, :, i, 0], dtype=torch.float32, device="cuda")
-        re, im = vis_dft(img, dxy, u, v, dRA * arcsec, dDec * arcsec, device=device)
+        re, im = sample(img, dxy, u, v, dRA * arcsec, dDec * arcsec, device=device)
         real.append(re.double().cpu().numpy().reshape(-1, 1))
         imag.append(im.double().cpu().numpy().reshape(-1, 1))
     real = np.concatenate(real, axis=1)

@@ -192,6 +192,65 @@ def test_vis_dft_matches_numpy_fft_and_oracle():
     assert err < 2e-5 * scale
 
 
+def test_vis_grid_matches_oracle_and_dft_at_nodes():
+    """galario's method on the GPU (rt_vis_grid) against the FP64 oracle at the
+    workload's (u,v) sampling, with a phase-centre shift, on a square and an odd
+    non-square image; at grid nodes it equals the exact DFT kernel."""
+    rng = np.random.default_rng(21)
+    uu, vv = configs.uv_points(K=20000, lam_m=1.3e-3, seed=3)
+    dxy = 0.0078125 * inputs.arcsec
+    for ny, nx in ((256, 256), (75, 64)):
+        print("case: vis_grid %dx%d" % (ny, nx), flush=True)
+        img = rng.uniform(size=(ny, nx))
+        t = torch.tensor(img, dtype=torch.float32, device="cuda")
+        d = dxy * (256 / nx)
+        for dra, ddec in ((0.0, 0.0), (0.1 * inputs.arcsec, -0.05 * inputs.arcsec)):
+            re, im = rt.vis_grid(t, d, uu, vv, dra, ddec)
+            ore, oim = oracle.vis_grid(img, d, uu, vv, dra, ddec)
+            scale = np.abs(ore + 1j * oim).max()
+            err = max(np.abs(re.double().cpu().numpy() - ore).max(),
+                      np.abs(im.double().cpu().numpy() - oim).max())
+            print("vis_grid max abs err / max|V| = %.2e" % (err / scale))
+            assert err < 2e-5 * scale
+        ug = (np.arange(nx - 1) - nx // 2) / (nx * d)
+        vg = (np.arange(ny - 1) - ny // 2) / (ny * d)
+        U, V = np.meshgrid(ug, vg)
+        g = rt.vis_grid(t, d, U.ravel(), V.ravel(), 3e-7, 1e-7)
+        e = rt.vis_dft(t, d, U.ravel(), V.ravel(), 3e-7, 1e-7)
+        scale = float(torch.hypot(e[0], e[1]).max())
+        assert float((g[0] - e[0]).abs().max()) < 2e-5 * scale
+        assert float((g[1] - e[1]).abs().max()) < 2e-5 * scale
+    # a caller's workspace gives the same result as the stream-ordered one
+    need = int(rt._lib.context(0).L.rt_vis_grid_workspace(nx, ny))
+    work = torch.empty(need, dtype=torch.uint8, device="cuda")
+    g2 = rt.vis_grid(t, d, U.ravel(), V.ravel(), 3e-7, 1e-7, work=work)
+    assert torch.equal(g2[0], g[0]) and torch.equal(g2[1], g[1])
+    with pytest.raises(ValueError):
+        rt.vis_grid(t, d, np.array([0.5 / d]), np.array([0.0]))
+
+
+def test_interpolate_model_codes():
+    """interpolate_model: code="galario" samples by galario's method, "exact" by
+    the DFT; on a Gaussian of width s the two agree to the bilinear bound
+    h^2/8 |F''| summed over both axes, |F''| <= (2 pi s)^2 F(0), h = 1/(n dxy)."""
+    from types import SimpleNamespace
+    n, pix = 256, 0.01
+    x = (np.arange(n) - n // 2) * pix
+    X, Y = np.meshgrid(x, x)
+    img = np.exp(-(X ** 2 + Y ** 2) / (2 * 0.15 ** 2))
+    im = SimpleNamespace(x=x, y=x, image=img[:, :, None, None])
+    u, v = configs.uv_points(K=5000, lam_m=1.3e-3, seed=4)
+    a = rt.interpolate_model(u, v, np.array([2.3e11]), im, dRA=0.05, dDec=-0.02)
+    b = rt.interpolate_model(u, v, np.array([2.3e11]), im, dRA=0.05, dDec=-0.02, code="exact")
+    d = np.abs((a.real - b.real) + 1j * (a.imag - b.imag)).max() / img.sum()
+    print("galario-method vs exact DFT: max |dV| / V(0) = %.2e" % d)
+    h = 1.0 / (n * pix * inputs.arcsec)
+    bound = 2 * h * h / 8 * (2 * np.pi * 0.15 * inputs.arcsec) ** 2
+    assert 0.1 * bound < d <= bound
+    with pytest.raises(ValueError):
+        rt.interpolate_model(u, v, np.array([2.3e11]), im, code="trift")
+
+
 def test_vis_adjoint_matches_oracle():
     rng = np.random.default_rng(6)
     ny = nx = 48

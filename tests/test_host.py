@@ -127,7 +127,7 @@ def test_loglog_opacity_interpolation():
 
 def _header_symbols():
     txt = open(os.path.join(ROOT, "include", "pdspy_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(rt_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|size_t|const char \*)\s*(rt_\w+)\(", txt, re.M)))
 
 
 def test_header_and_binding_agree():
