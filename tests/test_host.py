@@ -150,3 +150,11 @@ def test_product_never_imports_oracle():
                 src = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in src and "from oracle" not in src, f
                 assert "liboracle" not in src, f
+
+
+def test_interpolate_model_rejects_other_codes_before_any_device_work():
+    from types import SimpleNamespace
+    im = SimpleNamespace(x=np.arange(4.0), y=np.arange(4.0), image=np.zeros((4, 4, 1, 1)))
+    for code in ("trift", "galario-unstructured"):
+        with pytest.raises(ValueError, match="not supported"):
+            rt.interpolate_model(np.zeros(3), np.zeros(3), np.array([1e11]), im, code=code)
