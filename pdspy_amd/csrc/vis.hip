@@ -302,6 +302,8 @@ int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, co
     if (nx < 2 || ny < 2 || nx > GRID_MAXN || ny > GRID_MAXN || K < 0 || !(dxy > 0))
         return set_err(ctx, RT_E_ARG, "vis grid: invalid sizes (2 <= nx, ny <= %d)", GRID_MAXN);
     if (K == 0) return RT_OK;
+    if (!d_img || !d_u || !d_v || !d_re || !d_im)
+        return set_err(ctx, RT_E_ARG, "vis grid: NULL image, (u,v) or output");
     float2 *ws = (float2 *)d_work;
     if (!ws) HIPCHK(ctx, hipMallocAsync((void **)&ws, vis_grid_workspace(nx, ny), ctx->stream));
     float2 *A = ws, *F = ws + (size_t)nx * ny, *tw = ws + 2 * (size_t)nx * ny;
@@ -370,6 +372,8 @@ int vis_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const d
     if (nx < 1 || ny < 1 || ny > VIS_MAXROWS || K < 0 || !(dxy > 0))
         return set_err(ctx, RT_E_ARG, "vis: invalid sizes (ny <= %d)", VIS_MAXROWS);
     if (K == 0) return RT_OK;
+    if (!d_img || !d_u || !d_v || !d_re || !d_im)
+        return set_err(ctx, RT_E_ARG, "vis: NULL image, (u,v) or output");
     hipLaunchKernelGGL(k_vis, dim3((K + VIS_BS - 1) / VIS_BS), dim3(VIS_BS), 0, ctx->stream, d_img, nx,
                        ny, d_u, d_v, K, dxy, dra, ddec, d_re, d_im);
     HIPCHK(ctx, hipGetLastError());

@@ -356,7 +356,8 @@ def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
     image's DFT on its pixel-conjugate grid, bilinear interpolation, phase-centre
     shift; galario sampleImage as called at interpolate_model.py:20-27).  A point
     outside the grid's interpolation range raises ValueError.  ``work``: an
-    optional uint8 device tensor of ``rt_vis_grid_workspace(nx, ny)`` bytes."""
+    optional contiguous device tensor of at least ``rt_vis_grid_workspace(nx, ny)``
+    bytes, else the library allocates one stream-ordered per call."""
     import torch
     ctx = _lib.context(device)
     img = img.contiguous().float()
@@ -376,8 +377,11 @@ def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
     re = torch.empty(K, dtype=torch.float32, device="cuda")
     im = torch.empty(K, dtype=torch.float32, device="cuda")
     need = int(ctx.L.rt_vis_grid_workspace(nx, ny))
-    if work is not None and work.numel() < need:
-        raise ValueError("vis_grid: work holds %d bytes, %d needed" % (work.numel(), need))
+    if work is not None:
+        have = work.numel() * work.element_size()
+        if not (work.is_cuda and work.is_contiguous()) or have < need:
+            raise ValueError("vis_grid: work must be a contiguous device tensor of >= %d bytes "
+                             "(got %d)" % (need, have))
     ctx.check(ctx.L.rt_vis_grid(ctx.h, C.c_void_p(img.data_ptr()), nx, ny, float(dxy),
                                 C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), K, float(dra),
                                 float(ddec), C.c_void_p(re.data_ptr()), C.c_void_p(im.data_ptr()),
