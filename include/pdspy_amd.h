@@ -167,6 +167,16 @@ int rt_model_destroy(rt_model *m);
 int rt_model_ncells(const rt_model *m);
 /* Replace the model's densities (host, same layout as rt_model_desc.h_rho). */
 int rt_model_set_density(rt_model *m, const double *h_rho);
+/* Replace the dust opacities (pdspy: a new Dust per model draw,
+ * DustGenerator.__call__, dust/DustGenerator.py:71-97, as run_disk_model.py:79-85
+ * does for every walker): h_kabs, h_ksca [nspec][nlam] in cm^2/g on the model's
+ * wavelength grid, h_g likewise exactly when the model was created with g.  The
+ * emission, Planck- and Rosseland-mean tables are rebuilt; the species grouping
+ * (species with identical kabs share tables) must stay as it was, else RT_E_ARG
+ * and the model keeps its old opacities.  A batch's captured step graph is
+ * recaptured on its next step. */
+int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
+                         const double *h_g);
 
 /* Density components generated on the device (pdspy's model builders, which
  * run on the host there):

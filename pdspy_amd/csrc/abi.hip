@@ -93,6 +93,42 @@ static int upload_density(rt_model *m) {
     return 0;
 }
 
+// device copies of the opacities and of what derives from them besides the
+// emission tables: FP32 kabs/ksca/g [S][nlam], the group representatives'
+// kabs [G][nlam], 2^shift / kmax [G], and the per-cell tally-to-energy factors
+// [cells][G] (these need the cell volumes)
+struct OpacArrays {
+    std::vector<float> ka, ks, gg, gka, ginvk;
+    std::vector<double> econv;
+};
+
+static void opacity_arrays(const rt_model *m, OpacArrays &o) {
+    const size_t nk = (size_t)m->nspec * m->nlam;
+    const int G = m->tab.ngroup, nc = m->ncell;
+    o.ka.assign(nk, 0.f);
+    o.ks.assign(nk, 0.f);
+    o.gg.assign(nk, 0.f);
+    for (size_t i = 0; i < nk; ++i) {
+        o.ka[i] = (float)m->kabs[i];
+        o.ks[i] = (float)m->ksca[i];
+        if (m->has_g) o.gg[i] = (float)m->gsca[i];
+    }
+    o.gka.assign((size_t)G * m->nlam, 0.f);
+    for (int q = 0; q < G; ++q)
+        for (int i = 0; i < m->nlam; ++i)
+            o.gka[(size_t)q * m->nlam + i] = o.ka[(size_t)m->tab.grep[q] * m->nlam + i];
+    o.ginvk.assign(G, 0.f);
+    for (int q = 0; q < G; ++q) o.ginvk[q] = (float)(std::ldexp(1.0, RT_TALLY_SHIFT) / m->tab.kmax[q]);
+    o.econv.assign((size_t)nc * G, 0.0);
+    for (int c = 0; c < nc; ++c) {
+        int i = c % m->nr;
+        double dr_cm = m->rw[i + 1] - m->rw[i];
+        for (int q = 0; q < G; ++q)
+            o.econv[(size_t)c * G + q] =
+                m->tab.kmax[q] * dr_cm / (std::ldexp(1.0, RT_TALLY_SHIFT) * m->vol[c]);
+    }
+}
+
 static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
     if (!d) return set_err(ctx, RT_E_ARG, "NULL model description");
     if (d->nr < 1 || d->ntheta < 1 || d->nphi < 1 || d->nspec < 1 || d->nlam < 1)
@@ -211,34 +247,21 @@ static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
                 double vt = ctd[j] - ctd[j + 1];
                 m->vol[i + m->nr * (j + m->nt * k)] = vr * vt * (m->pw[k + 1] - m->pw[k]) * (m->mirror ? 2.0 : 1.0);
             }
-    std::vector<double> econv((size_t)nc * G);
-    std::vector<float> invdr(m->nr), ginvk(G);
+    std::vector<float> invdr(m->nr);
     for (int i = 0; i < m->nr; ++i) invdr[i] = (float)(1.0 / (rwd[i + 1] - rwd[i]));
-    for (int q = 0; q < G; ++q) ginvk[q] = (float)(std::ldexp(1.0, RT_TALLY_SHIFT) / m->tab.kmax[q]);
-    for (int c = 0; c < nc; ++c) {
-        int i = c % m->nr;
-        double dr_cm = m->rw[i + 1] - m->rw[i];
-        for (int q = 0; q < G; ++q)
-            econv[(size_t)c * G + q] = m->tab.kmax[q] * dr_cm / (std::ldexp(1.0, RT_TALLY_SHIFT) * m->vol[c]);
-    }
     // opacities and tables
-    std::vector<float> ka(nk), ks(nk), gg(nk, 0.f), gka((size_t)G * m->nlam);
-    for (size_t i = 0; i < nk; ++i) {
-        ka[i] = (float)m->kabs[i];
-        ks[i] = (float)m->ksca[i];
-        if (m->has_g) gg[i] = (float)m->gsca[i];
-    }
-    for (int q = 0; q < G; ++q)
-        for (int i = 0; i < m->nlam; ++i) gka[(size_t)q * m->nlam + i] = ka[(size_t)m->tab.grep[q] * m->nlam + i];
+    OpacArrays o;
+    opacity_arrays(m, o);
     std::vector<float> spos(3 * std::max(1, m->nstar), 0.f);
     for (int s = 0; s < m->nstar; ++s)
         for (int a = 0; a < 3; ++a) spos[3 * s + a] = (float)(m->stars[5 * s + 2 + a] / L);
     const size_t ncg = (size_t)nc * G;
     if (upload<float>(ctx, &m->d_rhoL, nullptr, nrho) || upload<float>(ctx, &m->d_rho, nullptr, nrho) ||
         upload<unsigned char>(ctx, &m->d_rhopos, nullptr, nrho) ||
-        upload(ctx, &m->d_ka, ka.data(), nk) || upload(ctx, &m->d_ks, ks.data(), nk) ||
-        upload(ctx, &m->d_g, gg.data(), nk) || upload(ctx, &m->d_grp, m->tab.grp.data(), m->nspec) ||
-        upload(ctx, &m->d_gka, gka.data(), gka.size()) || upload(ctx, &m->d_ginvk, ginvk.data(), ginvk.size()) ||
+        upload(ctx, &m->d_ka, o.ka.data(), nk) || upload(ctx, &m->d_ks, o.ks.data(), nk) ||
+        upload(ctx, &m->d_g, o.gg.data(), nk) || upload(ctx, &m->d_grp, m->tab.grp.data(), m->nspec) ||
+        upload(ctx, &m->d_gka, o.gka.data(), o.gka.size()) ||
+        upload(ctx, &m->d_ginvk, o.ginvk.data(), o.ginvk.size()) ||
         upload(ctx, &m->d_invdr, invdr.data(), invdr.size()) ||
         upload(ctx, &m->d_cdf, m->tab.cdf.data(), m->tab.cdf.size()) ||
         upload(ctx, &m->d_guide, m->tab.guide.data(), m->tab.guide.size()) ||
@@ -250,7 +273,7 @@ static int create_model(rt_ctx *ctx, const rt_model_desc *d, rt_model *m) {
         upload(ctx, &m->d_kross, m->tab.kross.data(), m->tab.kross.size()) ||
         upload(ctx, &m->d_kplanck, m->tab.kplanck.data(), m->tab.kplanck.size()) ||
         upload(ctx, &m->d_mrwy, m->tab.mrw_y.data(), m->tab.mrw_y.size()) ||
-        upload(ctx, &m->d_econv, econv.data(), econv.size()) ||
+        upload(ctx, &m->d_econv, o.econv.data(), o.econv.size()) ||
         upload<unsigned long long>(ctx, &m->d_tally, nullptr, ncg) ||
         upload<float>(ctx, &m->d_xrow, nullptr, ncg) || upload<float>(ctx, &m->d_Tg, nullptr, ncg) ||
         upload<unsigned int>(ctx, &m->d_visits, nullptr, nc) ||
@@ -341,6 +364,68 @@ int rt_model_set_density(rt_model *m, const double *h_rho) {
     m->have_T = false;
     m->rho_host_stale = false;
     return upload_density(m);
+}
+
+int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
+                         const double *h_g) {
+    if (!m || !h_kabs || !h_ksca) return RT_E_ARG;
+    rt_ctx *ctx = m->ctx;
+    if ((h_g != nullptr) != m->has_g)
+        return set_err(ctx, RT_E_ARG, "g must be given exactly when the model was created with g");
+    const size_t nk = (size_t)m->nspec * m->nlam;
+    if (!finite_all(h_kabs, nk) || !finite_all(h_ksca, nk))
+        return set_err(ctx, RT_E_ARG, "opacity contains NaN/Inf");
+    for (size_t i = 0; i < nk; ++i)
+        if (h_kabs[i] < 0 || h_ksca[i] < 0) return set_err(ctx, RT_E_ARG, "negative opacity");
+    if (h_g)
+        for (size_t i = 0; i < nk; ++i)
+            if (!(h_g[i] > -1.0 && h_g[i] < 1.0)) return set_err(ctx, RT_E_ARG, "HG g must be in (-1, 1)");
+    // the new tables must keep the device layout: same species grouping; the
+    // old host tables are kept for a rollback
+    std::vector<double> old_kabs, old_ksca, old_g;
+    old_kabs.swap(m->kabs);
+    old_ksca.swap(m->ksca);
+    old_g.swap(m->gsca);
+    const HostTables old_tab = m->tab;
+    m->kabs.assign(h_kabs, h_kabs + nk);
+    m->ksca.assign(h_ksca, h_ksca + nk);
+    if (h_g) m->gsca.assign(h_g, h_g + nk);
+    std::string err;
+    int rc = build_tables(m, err);
+    if (!rc && m->tab.grp != old_tab.grp) {
+        rc = RT_E_ARG;
+        err = "the new opacities group the species differently (species with identical kabs "
+              "share tables); create a new model instead";
+    }
+    if (!rc && (m->tab.cdf.size() != old_tab.cdf.size() || m->tab.kross.size() != old_tab.kross.size())) {
+        rc = RT_E_ARG;
+        err = "table sizes changed";
+    }
+    if (rc) {
+        m->kabs.swap(old_kabs);
+        m->ksca.swap(old_ksca);
+        m->gsca.swap(old_g);
+        m->tab = old_tab;
+        return set_err(ctx, rc, "%s", err.c_str());
+    }
+    OpacArrays o;
+    opacity_arrays(m, o);
+    const HostTables &t = m->tab;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpy(m->d_ka, o.ka.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_ks, o.ks.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_g, o.gg.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_gka, o.gka.data(), o.gka.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_ginvk, o.ginvk.data(), o.ginvk.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_econv, o.econv.data(), o.econv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_cdf, t.cdf.data(), t.cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_guide, t.guide.data(), t.guide.size(), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_lnq, t.lnq.data(), t.lnq.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_kross, t.kross.data(), t.kross.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(m->d_kplanck, t.kplanck.data(), t.kplanck.size() * sizeof(float), hipMemcpyHostToDevice));
+    m->have_T = false;
+    m->opac_version++;
+    return RT_OK;
 }
 
 int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp) {

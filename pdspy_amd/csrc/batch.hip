@@ -352,6 +352,8 @@ static std::vector<const void *> model_buffers(const rt_batch *b) {
                            m->d_gsrc, m->d_cnt, m->d_visits, m->d_vol, m->d_jtally, m->d_scum,
                            m->d_scnt, m->d_sblk};
         v.insert(v.end(), p, p + sizeof(p) / sizeof(p[0]));
+        // the image chains capture opacities at the camera wavelength by value
+        v.push_back((const void *)(uintptr_t)m->opac_version);
     }
     return v;
 }

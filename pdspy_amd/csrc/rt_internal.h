@@ -137,6 +137,7 @@ struct rt_model {
     double eps_last = 0;          // packet energy of the last thermal run
     std::vector<double> timeline; // last run: ms from kernel start to each snapshot, then the end
     bool have_T = false;
+    unsigned opac_version = 0;  // bumped by rt_model_set_opacity (captured graphs go stale)
 };
 
 int set_err(rt_ctx *ctx, int code, const char *fmt, ...);

@@ -134,6 +134,37 @@ class DeviceModel:
         except Exception:
             pass
 
+    def set_opacity(self, kabs, ksca, g=None):
+        """New dust opacities [nspec][nlam] (cm^2/g) on the model's wavelengths
+        (rt_model_set_opacity); the species grouping must stay as it was."""
+        nl = len(self._keep["lam_um"])
+        kabs = np.ascontiguousarray(kabs, dtype=float).reshape(self.nspec, nl)
+        ksca = np.ascontiguousarray(ksca, dtype=float).reshape(self.nspec, nl)
+        gg = None if g is None else np.ascontiguousarray(g, dtype=float).reshape(self.nspec, nl)
+        self.ctx.sync_stream()
+        self.ctx.check(self.ctx.L.rt_model_set_opacity(
+            self.h, kabs.ctypes.data, ksca.ctypes.data, None if gg is None else gg.ctypes.data))
+        self._keep.update(kabs=kabs, ksca=ksca)
+        if gg is not None:
+            self._keep["gsca"] = gg
+
+    def set_dust(self, dust):
+        """New opacities from pdspy Dust objects, one per species (lam in cm,
+        kabs, ksca and optionally g), resampled onto the model's wavelengths as
+        the model was built (model_arrays)."""
+        if len(dust) != self.nspec:
+            raise ValueError("set_dust: %d Dust objects for %d species" % (len(dust), self.nspec))
+        lam = self._keep["lam_um"]
+        kabs, ksca, gg = [], [], []
+        for d in dust:
+            lam_d = np.asarray(d.lam, dtype=float) * 1e4
+            kabs.append(_loglog_interp(lam, lam_d, d.kabs))
+            ksca.append(_loglog_interp(lam, lam_d, d.ksca))
+            g = getattr(d, "g", None)
+            gg.append(np.interp(np.log(lam), np.log(lam_d), g) if g is not None else np.zeros_like(lam))
+        self.set_opacity(np.array(kabs), np.array(ksca),
+                         np.array(gg) if self._keep.get("gsca") is not None else None)
+
     def set_density(self, rho):
         rho = np.ascontiguousarray(rho, dtype=float).reshape(self.nspec, self.ncell)
         self.ctx.sync_stream()
