@@ -62,6 +62,45 @@ void make_cdf(const std::vector<double> &lw, float *cdf, unsigned char *guide, i
     }
 }
 
+// modified random walk: y(xi) inverting the survival probability of a packet
+// diffusing from the centre of a sphere, P(y) = 2 sum_n (-1)^(n+1)
+// exp(-n^2 pi^2 y) (y = D t / R0^2).  The same for every model: built once per
+// process (about 1e7 exponentials).
+const std::vector<float> &mrw_escape_table() {
+    static const std::vector<float> table = [] {
+        auto surv = [](double y) {
+            if (y < 0.1) {  // Jacobi dual form, fast for small y
+                double s = 0;
+                for (int mm = -40; mm <= 40; ++mm) s += std::exp(-(mm + 0.5) * (mm + 0.5) / y);
+                return 1.0 - s / std::sqrt(kPi * y);
+            }
+            double s = 0;
+            for (int n = 1; n <= 60; ++n)
+                s += ((n & 1) ? 2.0 : -2.0) * std::exp(-(double)n * n * kPi * kPi * y);
+            return s;
+        };
+        // tabulated on a uniform grid of logit(xi) = ln(xi / (1 - xi)), in which y
+        // is close to linear at both ends (y ~ (ln 2 - ln xi)/pi^2 as xi -> 0 and
+        // y ~ 1 / (4 ln(1/(1-xi))) as xi -> 1)
+        std::vector<float> y(RT_MRW_NY, 0.f);
+        for (int k = 0; k < RT_MRW_NY; ++k) {
+            double L = -RT_MRW_LMAX + 2.0 * RT_MRW_LMAX * k / (RT_MRW_NY - 1);
+            double xi = 1.0 / (1.0 + std::exp(-L));
+            double lo = std::log(1e-9), hi = std::log(60.0);
+            for (int it = 0; it < 90; ++it) {
+                double mid = 0.5 * (lo + hi);
+                if (surv(std::exp(mid)) > xi)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            y[k] = (float)std::exp(0.5 * (lo + hi));
+        }
+        return y;
+    }();
+    return table;
+}
+
 }  // namespace
 
 int build_tables(rt_model *m, std::string &err) {
@@ -107,37 +146,62 @@ int build_tables(rt_model *m, std::string &err) {
     t.guide.assign((size_t)G * nT * RT_NGUIDE, 0);
     t.kross.assign((size_t)ns * nT, 0.f);
     t.kplanck.assign((size_t)G * nT, 0.f);
-    std::vector<double> lw(nl), lwb(nl);
+    // per-temperature quantities shared by every group and species, computed
+    // once: ln B_nu(T), ln dB_nu/dT, and the Planck and Rosseland weights
+    // exp(ln dnu + ln B - max) and exp(ln dnu + ln dB/dT - max) with their sums
+    std::vector<double> ldnu(nl);
+    for (int i = 0; i < nl; ++i) ldnu[i] = std::log(t.dnu[i]);
+    const size_t nTl = (size_t)nT * nl;
+    std::vector<double> LB(nTl), LD(nTl), Bw(nTl), Dw(nTl), wbT(nT), wdT(nT);
+    for (int it = 0; it < nT; ++it) {
+        const double T = std::exp(t.lnt0 + it * t.dlnt);
+        double *lb = &LB[(size_t)it * nl], *ld = &LD[(size_t)it * nl];
+        double *bw = &Bw[(size_t)it * nl], *dw = &Dw[(size_t)it * nl];
+        for (int i = 0; i < nl; ++i) {
+            lb[i] = lnB(t.nu[i], T);
+            ld[i] = lndB(t.nu[i], T);
+        }
+        double mb = -INFINITY, md = -INFINITY;
+        for (int i = 0; i < nl; ++i) {
+            mb = std::max(mb, ldnu[i] + lb[i]);
+            md = std::max(md, ldnu[i] + ld[i]);
+        }
+        double wb = 0, wd = 0;
+        for (int i = 0; i < nl; ++i) {
+            bw[i] = std::exp(ldnu[i] + lb[i] - mb);
+            wb += bw[i];
+            dw[i] = std::exp(ldnu[i] + ld[i] - md);
+            wd += dw[i];
+        }
+        wbT[it] = wb;
+        wdT[it] = wd;
+    }
+    std::vector<double> lw(nl), lwb(nl), lkd(nl);
     for (int q = 0; q < G; ++q) {
         const int s = t.grep[q];
         const double *ka = &m->kabs[(size_t)s * nl];
-        const double *ks = &m->ksca[(size_t)s * nl];
         double km = 0;
         for (int i = 0; i < nl; ++i) km = std::max(km, ka[i]);
         t.kmax[q] = km > 0 ? km : 1.0;
+        for (int i = 0; i < nl; ++i) lkd[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) : -INFINITY;
         for (int it = 0; it < nT; ++it) {
-            double T = std::exp(t.lnt0 + it * t.dlnt);
+            const double *lb = &LB[(size_t)it * nl], *ld = &LD[(size_t)it * nl];
             double mx = -INFINITY;
             for (int i = 0; i < nl; ++i) {
-                lwb[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) + lnB(t.nu[i], T) : -INFINITY;
+                lwb[i] = ka[i] > 0 ? lkd[i] + lb[i] : -INFINITY;
                 mx = std::max(mx, lwb[i]);
             }
             double s1 = 0;
             for (int i = 0; i < nl; ++i) s1 += std::exp(lwb[i] - mx);
             t.lnq[(size_t)q * nT + it] = std::log(4.0 * kPi) + mx + std::log(s1);
-            for (int i = 0; i < nl; ++i)
-                lw[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) + lndB(t.nu[i], T) : -INFINITY;
+            for (int i = 0; i < nl; ++i) lw[i] = ka[i] > 0 ? lkd[i] + ld[i] : -INFINITY;
             make_cdf(lw, &t.cdf[((size_t)q * nT + it) * nl],
                      &t.guide[((size_t)q * nT + it) * RT_NGUIDE], RT_NGUIDE);
             // Planck-mean absorption at T
-            double wb = 0, wkb = 0, mb = -INFINITY;
-            for (int i = 0; i < nl; ++i) mb = std::max(mb, std::log(t.dnu[i]) + lnB(t.nu[i], T));
-            for (int i = 0; i < nl; ++i) {
-                double b = std::exp(std::log(t.dnu[i]) + lnB(t.nu[i], T) - mb);
-                wb += b;
-                wkb += b * ka[i];
-            }
-            t.kplanck[(size_t)q * nT + it] = (float)(wkb / wb);
+            const double *bw = &Bw[(size_t)it * nl];
+            double wkb = 0;
+            for (int i = 0; i < nl; ++i) wkb += bw[i] * ka[i];
+            t.kplanck[(size_t)q * nT + it] = (float)(wkb / wbT[it]);
         }
     }
     // Rosseland-mean extinction per species (kabs + ksca), weights dB/dT dnu
@@ -145,46 +209,16 @@ int build_tables(rt_model *m, std::string &err) {
         const double *ka = &m->kabs[(size_t)s * nl];
         const double *ks = &m->ksca[(size_t)s * nl];
         for (int it = 0; it < nT; ++it) {
-            double T = std::exp(t.lnt0 + it * t.dlnt);
-            double md = -INFINITY;
-            for (int i = 0; i < nl; ++i) md = std::max(md, std::log(t.dnu[i]) + lndB(t.nu[i], T));
-            double wd = 0, wdk = 0;
+            const double *dw = &Dw[(size_t)it * nl];
+            double wdk = 0;
             for (int i = 0; i < nl; ++i) {
-                double d = std::exp(std::log(t.dnu[i]) + lndB(t.nu[i], T) - md);
                 double ke = ka[i] + ks[i];
-                wd += d;
-                if (ke > 0) wdk += d / ke;
+                if (ke > 0) wdk += dw[i] / ke;
             }
-            t.kross[(size_t)s * nT + it] = (float)(wdk > 0 ? wd / wdk : 0.0);
+            t.kross[(size_t)s * nT + it] = (float)(wdk > 0 ? wdT[it] / wdk : 0.0);
         }
     }
-    // modified random walk: y(xi) inverting the survival probability of a
-    // packet diffusing from the centre of a sphere, P(y) = 2 sum_n (-1)^(n+1)
-    // exp(-n^2 pi^2 y) (y = D t / R0^2), on xi_k = k / (NY - 1)
-    auto surv = [](double y) {
-        if (y < 0.1) {  // Jacobi dual form, fast for small y
-            double s = 0;
-            for (int mm = -40; mm <= 40; ++mm) s += std::exp(-(mm + 0.5) * (mm + 0.5) / y);
-            return 1.0 - s / std::sqrt(kPi * y);
-        }
-        double s = 0;
-        for (int n = 1; n <= 60; ++n) s += ((n & 1) ? 2.0 : -2.0) * std::exp(-(double)n * n * kPi * kPi * y);
-        return s;
-    };
-    // tabulated on a uniform grid of logit(xi) = ln(xi / (1 - xi)), in which y is
-    // close to linear at both ends (y ~ (ln 2 - ln xi)/pi^2 as xi -> 0 and
-    // y ~ 1 / (4 ln(1/(1-xi))) as xi -> 1)
-    t.mrw_y.assign(RT_MRW_NY, 0.f);
-    for (int k = 0; k < RT_MRW_NY; ++k) {
-        double L = -RT_MRW_LMAX + 2.0 * RT_MRW_LMAX * k / (RT_MRW_NY - 1);
-        double xi = 1.0 / (1.0 + std::exp(-L));
-        double lo = std::log(1e-9), hi = std::log(60.0);
-        for (int it = 0; it < 90; ++it) {
-            double mid = 0.5 * (lo + hi);
-            if (surv(std::exp(mid)) > xi) lo = mid; else hi = mid;
-        }
-        t.mrw_y[k] = (float)std::exp(0.5 * (lo + hi));
-    }
+    t.mrw_y = mrw_escape_table();
     // stars
     t.scdf.assign((size_t)std::max(1, m->nstar) * nl, 0.f);
     t.sguide.assign((size_t)std::max(1, m->nstar) * RT_NGUIDE, 0);

@@ -78,6 +78,23 @@ def _loglog_interp(x_new, x, y):
     return np.interp(x_new, x, y)
 
 
+def dust_arrays(dust, lam_um):
+    """pdspy Dust objects (lam in cm, kabs, ksca, optional g) -> kabs, ksca, g
+    [nspec][nlam] on the wavelengths lam_um (micron), log-log interpolated as
+    write_radmc3d's dustkappa tables are read (Model.py:770-775), and whether
+    any species has g."""
+    lam = np.asarray(lam_um, dtype=float)
+    kabs, ksca, gg = [], [], []
+    has_g = any(getattr(d, "g", None) is not None for d in dust)
+    for d in dust:
+        lam_d = np.asarray(d.lam, dtype=float) * 1e4
+        kabs.append(_loglog_interp(lam, lam_d, d.kabs))
+        ksca.append(_loglog_interp(lam, lam_d, d.ksca))
+        g = getattr(d, "g", None)
+        gg.append(np.interp(np.log(lam), np.log(lam_d), g) if g is not None else np.zeros_like(lam))
+    return kabs, ksca, gg, has_g
+
+
 def model_arrays(grid):
     """pdspy Grid -> the flat FP64 arrays of rt_model_desc (units of
     write_radmc3d, Model.py:733-778: walls in cm/rad/rad, rho r-fastest,
@@ -87,14 +104,7 @@ def model_arrays(grid):
     lam = np.asarray(grid.lam, dtype=float)
     rho = np.ascontiguousarray(np.array([np.asarray(d, dtype=float).transpose(2, 1, 0)
                                          for d in grid.density]))
-    kabs, ksca, gg = [], [], []
-    has_g = any(getattr(d, "g", None) is not None for d in grid.dust)
-    for d in grid.dust:
-        lam_d = np.asarray(d.lam, dtype=float) * 1e4
-        kabs.append(_loglog_interp(lam, lam_d, d.kabs))
-        ksca.append(_loglog_interp(lam, lam_d, d.ksca))
-        g = getattr(d, "g", None)
-        gg.append(np.interp(np.log(lam), np.log(lam_d), g) if g is not None else np.zeros_like(lam))
+    kabs, ksca, gg, has_g = dust_arrays(grid.dust, lam)
     stars = np.array([[s.radius * R_sun, s.temperature, s.x * AU, s.y * AU, s.z * AU]
                       for s in grid.stars], dtype=float).reshape(-1, 5)
     return dict(rw=np.asarray(grid.w1, dtype=float) * AU, tw=np.asarray(grid.w2, dtype=float),
@@ -154,14 +164,7 @@ class DeviceModel:
         the model was built (model_arrays)."""
         if len(dust) != self.nspec:
             raise ValueError("set_dust: %d Dust objects for %d species" % (len(dust), self.nspec))
-        lam = self._keep["lam_um"]
-        kabs, ksca, gg = [], [], []
-        for d in dust:
-            lam_d = np.asarray(d.lam, dtype=float) * 1e4
-            kabs.append(_loglog_interp(lam, lam_d, d.kabs))
-            ksca.append(_loglog_interp(lam, lam_d, d.ksca))
-            g = getattr(d, "g", None)
-            gg.append(np.interp(np.log(lam), np.log(lam_d), g) if g is not None else np.zeros_like(lam))
+        kabs, ksca, gg, _ = dust_arrays(dust, self._keep["lam_um"])
         self.set_opacity(np.array(kabs), np.array(ksca),
                          np.array(gg) if self._keep.get("gsca") is not None else None)
 

@@ -30,17 +30,23 @@ class SamplerStep:
     SCAT_KEY = 0x9E3779B97F4A7C15  # scattering key = model seed ^ this (as the batch step)
 
     def __init__(self, params, npix=256, field_arcsec=2.0, K=5000, seed=0, max_events=1000000,
-                 mrw=1, device=0, data_seed=12345, nphot_scat=100000):
+                 mrw=1, device=0, data_seed=12345, nphot_scat=100000, dust_gen=None,
+                 env_dust=None):
+        """``dust_gen``: optional ``pdspy_amd.dust.DustGenerator``; then each
+        parameter set's ``a_max`` (micron) and ``p`` draw the disk's dust, as
+        run_disk_model.py:79-85 does per walker, and ``env_dust`` (default: the
+        generator at 1 micron, p = 3.5) is the envelope's."""
         import torch
         self.params = list(params)
+        self.dust_gen = dust_gen
+        self.env_dust = env_dust if env_dust is not None or dust_gen is None else dust_gen(1e-4, 3.5)
         self.nphot_scat = int(nphot_scat)
         self.n = len(self.params)
         self.npix = npix
         self.K = K
         self.opts = dict(max_events=int(max_events), mrw=int(mrw))
         self.nphot = np.array([int(p["nphot"]) for p in self.params], dtype=np.uint64)
-        self.models = [rt.DeviceModel(rt.model_arrays(configs.batch_model(p).grid), device=device)
-                       for p in self.params]
+        self.models = [rt.DeviceModel(self._arrays(p), device=device) for p in self.params]
         self.batch = rt.Batch(self.models, device=device)
         pixel = field_arcsec / npix                      # arcsec
         self.pix_cm = pixel * DPC * AU
@@ -103,11 +109,22 @@ class SamplerStep:
         self._items = None
         return dict(thermal=best, tile=tile)
 
+    def _dust(self, p):
+        return [self.dust_gen(p["a_max"] / 1e4, p["p"]), self.env_dust]
+
+    def _arrays(self, p):
+        a = rt.model_arrays(configs.batch_model(p).grid)
+        if self.dust_gen is not None:
+            kabs, ksca, _, _ = rt.dust_arrays(self._dust(p), a["lam_um"])
+            a["kabs"], a["ksca"] = np.array(kabs), np.array(ksca)
+        return a
+
     def set_params(self, params):
         """A new parameter draw for the same models (grids and photon counts
         stay): densities are rebuilt on the device (disk + Ulrich envelope
-        generator) and the cameras take the new inclinations.  Returns the
-        wall seconds of the density generation."""
+        generator), the disk's dust is drawn again when the step has a dust
+        generator, and the cameras take the new inclinations.  Returns the wall
+        seconds of the density (and opacity) update."""
         import torch
         params = list(params)
         if len(params) != self.n:
@@ -119,6 +136,8 @@ class SamplerStep:
         t0 = time.perf_counter()
         for m, p in zip(self.models, params):
             m.generate_density(configs.batch_components(p))
+            if self.dust_gen is not None:
+                m.set_dust(self._dust(p))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         self.params = params

@@ -94,3 +94,37 @@ def test_batch_graph_recaptured_after_set_opacity():
     assert np.array_equal(m0.temperature(), got[0][0])
     assert _close(img.cpu().numpy(), got[2][0], 1e-6)
     assert float(c2.item()) == pytest.approx(got[1][0], rel=1e-5)
+
+
+def test_sampler_step_draws_dust_per_model():
+    """A sampler step with a dust generator: each model's disk dust comes from
+    its (a_max, p) draw (run_disk_model.py:79-85); after set_params the graph
+    step equals the single-model calls with the new opacities."""
+    import time
+    print("case: sampler step with DustGenerator, set_params, graph replay", flush=True)
+    ps = _small_params()
+    lam_um = rt.model_arrays(configs.batch_model(ps[0]).grid)["lam_um"]
+    gen = _generator(lam_um)
+    rng = np.random.default_rng(3)
+    for p in ps:
+        p.update(a_max=float(10 ** rng.uniform(0, 3)), p=float(rng.uniform(2.6, 4.4)))
+    s = SamplerStep(ps, npix=32, K=300, max_events=100000, nphot_scat=20000, dust_gen=gen)
+    want = gen(ps[1]["a_max"] / 1e4, ps[1]["p"])
+    assert np.allclose(s.models[1]._keep["kabs"][0], rt.dust_arrays([want], lam_um)[0][0])
+    s.step(seed=2, use_graph=True)
+    new = [dict(p) for p in ps]
+    for p in new:
+        p.update(a_max=float(10 ** rng.uniform(0, 3)), p=float(rng.uniform(2.6, 4.4)),
+                 mdisk=p["mdisk"] * 1.5)
+    t0 = time.perf_counter()
+    s.set_params(new)
+    print("set_params with new dust for %d models: %.1f ms" % (s.n, (time.perf_counter() - t0) * 1e3))
+    s.step(seed=2, use_graph=True)
+    got = _snapshot(s)
+    for i, m in enumerate(s.models):
+        d = gen(new[i]["a_max"] / 1e4, new[i]["p"])
+        assert np.allclose(m._keep["kabs"][0], rt.dust_arrays([d], lam_um)[0][0])
+        img, c2, _, _ = s.single_model(i, 2 * 1000 + i)
+        assert np.array_equal(m.temperature(), got[0][i]), i
+        assert _close(img.cpu().numpy(), got[2][i], 1e-6), i
+        assert float(c2.item()) == pytest.approx(got[1][i], rel=1e-5), i
