@@ -165,7 +165,9 @@ int rt_device_info(const rt_ctx *ctx, int *n_cu, int *wallclock_khz);
 int rt_model_create(rt_ctx *ctx, const rt_model_desc *desc, rt_model **out);
 int rt_model_destroy(rt_model *m);
 int rt_model_ncells(const rt_model *m);
-/* Replace the model's densities (host, same layout as rt_model_desc.h_rho). */
+/* Replace the model's densities (host, same layout as rt_model_desc.h_rho).
+ * This and rt_model_set_opacity copy synchronously: no work using the model
+ * may be in flight on the context's stream. */
 int rt_model_set_density(rt_model *m, const double *h_rho);
 /* Replace the dust opacities (pdspy: a new Dust per model draw,
  * DustGenerator.__call__, dust/DustGenerator.py:71-97, as run_disk_model.py:79-85
