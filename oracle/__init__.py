@@ -8,6 +8,9 @@ The C source (``rt_oracle.c``) restates what RADMC-3D computes for pdspy's
 ``Model.run_thermal`` / ``run_image`` (pdspy/modeling/Model.py:221-256,
 :515-577) plus the visibility DFT and chi^2 (imaging/imtovis.py:6-27,
 interferometry/interpolate_model.py:11-57, interferometry/libinterferometry.pyx:610-633).
+``vis_grid`` (numpy, in this file) restates galario's sampling method that
+interpolate_model calls: the grid transform, bilinear interpolation and the
+phase-centre shift.
 
 Model inputs are plain numpy arrays (a dict or any object with the fields
 ``rw, tw, pw, rho, lam_um, kabs, ksca, gsca, stars``), never product objects.
