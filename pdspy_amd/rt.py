@@ -454,6 +454,29 @@ def chisq(dre, dim, w, mre, mim, with_grad=False, device=0):
     return (out, gre, gim) if with_grad else out
 
 
+def lnlike_vis(data, model, device=0):
+    """pdspy's log-likelihood of one visibility data set (utils/emcee.py:31-43):
+    -1/2 sum w dRe^2 - sum_{w>0} log(w / 2 pi) - 1/2 sum w dIm^2 - sum_{w>0}
+    log(w / 2 pi), the weighted sums on the device (rt_chisq), the weight term
+    -- which does not depend on the model -- in FP64 numpy as pdspy has it.
+    ``data``: Visibilities (real, imag, weights [K, 1]); ``model``: Visibilities
+    or a (re, im) pair of float32 device tensors."""
+    import torch
+    w = np.asarray(data.weights, dtype=float).ravel()
+    lw = np.sum(np.log(w[w > 0] / (2 * np.pi)))
+
+    def dev(x):
+        return torch.as_tensor(np.asarray(x, dtype=float).ravel(), dtype=torch.float32,
+                               device="cuda")
+
+    if isinstance(model, tuple):
+        mre, mim = model
+    else:
+        mre, mim = dev(model.real), dev(model.imag)
+    c2 = chisq(dev(data.real), dev(data.imag), dev(w), mre, mim, device=device)
+    return -0.5 * float(c2.item()) - lw - lw
+
+
 # ---------------------------------------------------------------- pdspy mirror
 def _device_model(model, device=0, ntemp=1000, temp0=0.01, temp1=1e5):
     dm = getattr(model, "_pdspy_amd_dev", None)

@@ -280,6 +280,29 @@ def test_chisq_and_gradient():
     assert out2.item() == out.item()
 
 
+def test_lnlike_vis_follows_pdspy():
+    """rt.lnlike_vis against pdspy's formula (utils/emcee.py:31-43) in FP64 on
+    the same FP32-rounded inputs, with zero-weight points; a model equal to the
+    data leaves only the weight term."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(8)
+    K = 20011
+    f32 = lambda x: x.astype(np.float32).astype(float)
+    d = SimpleNamespace(real=f32(rng.normal(size=(K, 1))), imag=f32(rng.normal(size=(K, 1))),
+                        weights=f32(rng.uniform(0.1, 5.0, (K, 1))))
+    d.weights[::97] = 0.0
+    m = SimpleNamespace(real=f32(rng.normal(size=(K, 1))), imag=f32(rng.normal(size=(K, 1))))
+    good = d.weights > 0
+    want = (-0.5 * np.sum((d.real - m.real) ** 2 * d.weights) -
+            np.sum(np.log(d.weights[good] / (2 * np.pi))) +
+            -0.5 * np.sum((d.imag - m.imag) ** 2 * d.weights) -
+            np.sum(np.log(d.weights[good] / (2 * np.pi))))
+    got = rt.lnlike_vis(d, m)
+    assert got == pytest.approx(want, rel=1e-9, abs=1e-6 * abs(want))
+    same = rt.lnlike_vis(d, d)
+    assert same == -2 * np.sum(np.log(d.weights[good] / (2 * np.pi)))
+
+
 def test_chi2_of_visibilities_gradient_matches_oracle_chain():
     """d chi^2 / d log rho through image -> DFT -> chi^2 (config-3 chain, small)."""
     arr, T = _disk_setup(nr=32, nt=16, seed=8)
