@@ -18,6 +18,7 @@ int set_err(rt_ctx *ctx, int code, const char *fmt, ...) {
         va_start(ap, fmt);
         vsnprintf(buf, sizeof(buf), fmt, ap);
         va_end(ap);
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
         ctx->err = buf;
     }
     return code;

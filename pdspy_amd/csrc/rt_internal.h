@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,7 @@ struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    std::mutex err_mu;  // models of one context may be updated from several host threads
     int n_cu = 0;
     int wallclock_khz = 0;
 };

@@ -151,7 +151,8 @@ class DeviceModel:
         kabs = np.ascontiguousarray(kabs, dtype=float).reshape(self.nspec, nl)
         ksca = np.ascontiguousarray(ksca, dtype=float).reshape(self.nspec, nl)
         gg = None if g is None else np.ascontiguousarray(g, dtype=float).reshape(self.nspec, nl)
-        self.ctx.sync_stream()
+        # synchronous copies, no stream work: safe to call from several threads
+        # for different models
         self.ctx.check(self.ctx.L.rt_model_set_opacity(
             self.h, kabs.ctypes.data, ksca.ctypes.data, None if gg is None else gg.ctypes.data))
         self._keep.update(kabs=kabs, ksca=ksca)

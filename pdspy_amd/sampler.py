@@ -136,8 +136,13 @@ class SamplerStep:
         t0 = time.perf_counter()
         for m, p in zip(self.models, params):
             m.generate_density(configs.batch_components(p))
-            if self.dust_gen is not None:
-                m.set_dust(self._dust(p))
+        if self.dust_gen is not None:
+            # the host table rebuilds run in parallel (the ctypes calls release
+            # the GIL; each model's update touches only that model)
+            from concurrent.futures import ThreadPoolExecutor
+            dust = [self._dust(p) for p in params]
+            with ThreadPoolExecutor(max_workers=min(16, self.n)) as ex:
+                list(ex.map(lambda md: md[0].set_dust(md[1]), zip(self.models, dust)))
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         self.params = params
