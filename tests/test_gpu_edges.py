@@ -75,3 +75,20 @@ def test_scattering_source_without_packets():
     J1, s1 = dm.scattering_source(100.0, 1, Td, seed=1)
     assert s1["n_escape"] + s1["n_abs"] + s1["n_killed"] == 1
     assert torch.all(torch.isfinite(J1)) and torch.all(J1 >= 0)
+
+
+def test_vis_grid_edges():
+    print("case: vis_grid with no points and on a 2x2 image", flush=True)
+    img = torch.tensor([[1.0, 2.0], [3.0, 4.0]], dtype=torch.float32, device="cuda")
+    d = 1e-7
+    re, im = rt.vis_grid(img, d, np.zeros(0), np.zeros(0))
+    assert re.numel() == 0 and im.numel() == 0
+    # on a 2x2 grid only the cell between the nodes (-1/(2d), 0) in u and v interpolates
+    u = np.array([-0.5 / d, -0.25 / d, -1e-3 / d])
+    v = np.array([-0.4 / d, -0.5 / d, -0.1 / d])
+    re, im = rt.vis_grid(img, d, u, v, 1e-7, -2e-7)
+    ore, oim = oracle.vis_grid(img.cpu().numpy(), d, u, v, 1e-7, -2e-7)
+    assert np.allclose(re.cpu().numpy(), ore, rtol=1e-6, atol=1e-6)
+    assert np.allclose(im.cpu().numpy(), oim, rtol=1e-6, atol=1e-6)
+    with pytest.raises(ValueError):
+        rt.vis_grid(img, d, np.array([0.0]), np.array([0.0]))
