@@ -67,6 +67,9 @@ def parse():
     p.add_argument("--batch-models", type=int, default=64)
     p.add_argument("--batch-k", type=int, default=50000)
     p.add_argument("--no-sequential", action="store_true")
+    p.add_argument("--dust", action="store_true",
+                   help="batch: each model's disk dust drawn from a DustGenerator (a_max, p), "
+                        "drawn again with the new parameters")
     p.add_argument("--vis", action="store_true",
                    help="visibility samplers only: exact DFT and galario's method")
     return p.parse_args()
@@ -123,9 +126,11 @@ def batch_main(a):
     from pdspy_amd import configs
     from pdspy_amd.sampler import SamplerStep
 
-    params = configs.sampler_batch(a.batch_models, seed=0)
+    params = configs.sampler_batch(a.batch_models, seed=0, dust=a.dust)
+    gen = configs.synthetic_dust_generator() if a.dust else None
     t0 = time.perf_counter()
-    s = SamplerStep(params, npix=256, K=a.batch_k, max_events=int(a.max_events), mrw=a.mrw)
+    s = SamplerStep(params, npix=256, K=a.batch_k, max_events=int(a.max_events), mrw=a.mrw,
+                    dust_gen=gen)
     t_build = time.perf_counter() - t0
     tuned = s.autotune()
     print("batch: built and tuned", json.dumps(tuned), file=sys.stderr, flush=True)
@@ -142,7 +147,7 @@ def batch_main(a):
     print("batch: %d steps, %.2f s each" % (a.steps, wall), file=sys.stderr, flush=True)
     # a new parameter draw for the same grids: densities rebuilt on the device,
     # then a graph step with them; the host builders timed on 8 of the models
-    new = configs.sampler_batch(a.batch_models, seed=1)
+    new = configs.sampler_batch(a.batch_models, seed=1, dust=a.dust)
     for p, q in zip(new, params):
         p.update(nr=q["nr"], nt=q["nt"], nphot=q["nphot"])
     t_gen = s.set_params(new)
@@ -166,7 +171,7 @@ def batch_main(a):
     npk = int(s.nphot.sum())
     out = {
         "metric": "batched sampler step (thermal MC + 1.3 mm image + visibilities + chi^2 "
-                  "gradient), %d models" % s.n,
+                  "gradient), %d models%s" % (s.n, ", per-model dust" if a.dust else ""),
         "value": s.n / wall, "unit": "models/s", "n_gpus": 1, "steps": a.steps,
         "warmup": max(1, a.warmup), "ms_per_step": wall * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
@@ -179,6 +184,7 @@ def batch_main(a):
         "sequential_s": seq, "speedup_vs_sequential": (seq / wall) if seq else None,
         "killed": int(sum(x["n_killed"] for x in stats)),
         "new_params": {"device_density_ms": t_gen * 1e3, "step_s": t_new,
+                       "includes_dust_tables": bool(a.dust),
                        "host_builder_s_per_model": t_host_gen,
                        "chi2_finite": bool(np.all(np.isfinite(chi_new))),
                        "killed": int(sum(x["n_killed"] for x in stats_new))},

@@ -71,8 +71,29 @@ def uv_points(K=50000, bmin_m=15.0, bmax_m=2000.0, lam_m=1.3e-3, seed=0):
     return b * np.cos(pa) / lam_m, b * np.sin(pa) / lam_m
 
 
-def sampler_batch(n=64, seed=0):
-    """Parameter draws of one batched sampler step (log-uniform masses)."""
+def synthetic_dust_generator(lam_um=None):
+    """Opacity tables on pdspy's DustGenerator axes (amax = logspace(-4, 1, 60) cm,
+    p = linspace(2.5, 4.5, 11); dust/DustGenerator.py:13-14) with a smooth
+    synthetic dependence standing in for pdspy's Mie tables (HDF5 files):
+    kappa_abs = 1e3 (lam/1um)^-beta, flat below 1 um, beta falling from 1.7 for
+    small grains to 0.3 for large grains with shallow size distributions
+    (small p), and an albedo rising from 0.1 to 0.9 with amax."""
+    from .dust import DustGenerator
+    lam_um = np.logspace(-1, 4, 100) if lam_um is None else np.asarray(lam_um, dtype=float)
+    amax = np.logspace(-4.0, 1.0, 60)
+    p = np.linspace(2.5, 4.5, 11)
+    P, A, L = np.meshgrid(p, amax, lam_um, indexing="ij")
+    x = np.clip(np.log10(A / 1e-4) / 3.0, 0.0, 1.0)
+    beta = 1.7 - 1.4 * x * (4.5 - P) / 2.0
+    kabs = np.where(L > 1.0, 1e3 * L ** -beta, 1e3)
+    albedo = 0.1 + 0.8 * x
+    return DustGenerator(lam_um * 1e-4, amax, p, kabs, albedo / (1.0 - albedo) * kabs)
+
+
+def sampler_batch(n=64, seed=0, dust=False):
+    """Parameter draws of one batched sampler step (log-uniform masses); with
+    ``dust`` also the disk dust's a_max (micron, log-uniform 1-1e4) and p
+    (2.5-4.5), from a separate stream so the other draws stay as they are."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n):
@@ -86,6 +107,10 @@ def sampler_batch(n=64, seed=0):
             nt=int(rng.choice([64, 128])),
             nphot=int(10 ** rng.uniform(6, 7)),
         ))
+    if dust:
+        rd = np.random.default_rng(seed + 7919)
+        for p in out:
+            p.update(a_max=float(10 ** rd.uniform(0, 4)), p=float(rd.uniform(2.5, 4.5)))
     return out
 
 
@@ -112,4 +137,5 @@ def batch_components(p):
 
 
 __all__ = ["powerlaw_dust", "disk_c1", "disk_envelope", "spiral_3d", "uv_points",
-           "sampler_batch", "batch_model", "disk_envelope_components", "batch_components", "AU"]
+           "sampler_batch", "synthetic_dust_generator", "batch_model",
+           "disk_envelope_components", "batch_components", "AU"]

@@ -436,7 +436,7 @@ __device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uin
 // (register caches).  BATCH: the parameter blocks are read from `runs` in
 // device memory; otherwise the single run's block P1 is a kernel argument.
 template <int MS, int MG, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 ? 4 : MS <= 4 ? 3 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 && MG <= 1 ? 4 : MS <= 4 ? 3 : 2)))
 k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     extern __shared__ float lds[];
     const int lane = __lane_id();
@@ -1410,6 +1410,8 @@ int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, 
     void (*kern)(BatchArgs, const ThermParams *, ThermParams);
     if (max_s <= 2 && max_g <= 1)
         kern = batch ? k_thermal<2, 1, true> : k_thermal<2, 1, false>;
+    else if (max_s <= 2 && max_g <= 2)  // pdspy's fits: disk and envelope dust differ
+        kern = batch ? k_thermal<2, 2, true> : k_thermal<2, 2, false>;
     else if (max_s <= 4 && max_g <= 2)
         kern = batch ? k_thermal<4, 2, true> : k_thermal<4, 2, false>;
     else
