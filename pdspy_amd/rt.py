@@ -613,23 +613,25 @@ def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="
     return model.visibilities[name]
 
 
-def interpolate_model(u, v, freq, model_image, dRA=0.0, dDec=0.0, device=0, code="galario",
-                      **unused):
-    """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57);
-    dRA/dDec in arcsec shift the phase centre.  ``code="galario"`` (pdspy's
-    default) samples as galario does, by bilinear interpolation of the image's
-    transform on its pixel-conjugate grid (``vis_grid``); ``code="exact"`` takes
-    the exact DFT at each point (``vis_dft``).  pdspy's other codes
-    (galario-unstructured, trift) are not on this path."""
+def interpolate_model(u, v, freq, model, nthreads=1, dRA=0.0, dDec=0.0, code="galario",
+                      nxy=1024, dxy=0.01, device=0):
+    """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57,
+    same arguments in the same order); dRA/dDec in arcsec shift the phase
+    centre.  ``code="galario"`` (pdspy's default) samples as galario does, by
+    bilinear interpolation of the image's transform on its pixel-conjugate grid
+    (``vis_grid``); ``code="exact"`` takes the exact DFT at each point
+    (``vis_dft``).  pdspy's other codes (galario-unstructured, trift) are not on
+    this path; ``nthreads`` (galario's host threads) and ``nxy``/``dxy`` (used
+    only by galario-unstructured) are accepted and unused."""
     import torch
     if code not in ("galario", "exact"):
         raise ValueError("interpolate_model: code %r is not supported (galario, exact)" % (code,))
     sample = vis_grid if code == "galario" else vis_dft
-    dxy = (model_image.x[1] - model_image.x[0]) * arcsec
+    pix = (model.x[1] - model.x[0]) * arcsec  # rad (interpolate_model.py:18)
     real, imag = [], []
-    for i in range(model_image.image.shape[2]):
-        img = torch.tensor(model_image.image[:, :, i, 0], dtype=torch.float32, device="cuda")
-        re, im = sample(img, dxy, u, v, dRA * arcsec, dDec * arcsec, device=device)
+    for i in range(model.image.shape[2]):
+        img = torch.tensor(model.image[:, :, i, 0], dtype=torch.float32, device="cuda")
+        re, im = sample(img, pix, u, v, dRA * arcsec, dDec * arcsec, device=device)
         real.append(re.double().cpu().numpy().reshape(-1, 1))
         imag.append(im.double().cpu().numpy().reshape(-1, 1))
     real = np.concatenate(real, axis=1)

@@ -158,3 +158,12 @@ def test_interpolate_model_rejects_other_codes_before_any_device_work():
     for code in ("trift", "galario-unstructured"):
         with pytest.raises(ValueError, match="not supported"):
             rt.interpolate_model(np.zeros(3), np.zeros(3), np.array([1e11]), im, code=code)
+
+
+def test_interpolate_model_signature_is_pdspys():
+    """Same parameters in the same order as interpolate_model.py:11-12, so
+    positional and keyword calls written for pdspy carry over."""
+    import inspect
+    names = list(inspect.signature(rt.interpolate_model).parameters)
+    assert names[:10] == ["u", "v", "freq", "model", "nthreads", "dRA", "dDec", "code", "nxy",
+                          "dxy"]
