@@ -48,6 +48,22 @@ _CONTROL_KEYS = {
 _RUN_KEYS = {"verbose", "timelimit", "nice"}
 
 
+# run_image_radmc3d's line-transfer and unstructured-camera arguments
+# (Model.py:515-519, 682-686) with their defaults: accepted at the defaults,
+# refused otherwise (continuum only on this path)
+_LINE_DEFAULTS = {"imolspec": None, "iline": None, "widthkms": None, "vkms": None,
+                  "linenlam": None, "doppcatch": False, "unstructured": False}
+
+
+def _pop_line_options(keywords):
+    for k, d in _LINE_DEFAULTS.items():
+        if k in keywords:
+            v = keywords.pop(k)
+            if v is not d and v != d:
+                raise NotImplementedError("%s=%r: line transfer and unstructured cameras are "
+                                          "not on this path (continuum images only)" % (k, v))
+
+
 def _check_keywords(keywords):
     bad = set(keywords) - _CONTROL_KEYS - _RUN_KEYS
     if bad:
@@ -571,9 +587,8 @@ def run_image(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="1300",
               loadlambda=False, incl=0, pa=0, phi=0, dpc=1, nostar=False, nsub=1, device=0,
               **keywords):
     """Model.run_image(code="mi355x") (Model.py:515-577): ``model.images[name]``."""
-    _check_keywords({k: v for k, v in keywords.items()
-                     if k not in ("imolspec", "iline", "widthkms", "vkms", "linenlam",
-                                  "doppcatch", "unstructured")})
+    _pop_line_options(keywords)
+    _check_keywords(keywords)
     lams = _camera_lams(model, lam, loadlambda)
     image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device,
                     nphot=nphot, keywords=keywords)
@@ -596,6 +611,7 @@ def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="
     """Model.run_visibilities(code="mi355x") (Model.py:682-721): the image's
     visibilities on its pixel-conjugate grid (imtovis.py:6-27)."""
     import torch
+    _pop_line_options(keywords)
     _check_keywords(keywords)
     lams = _camera_lams(model, lam, loadlambda)
     image = _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, device,

@@ -167,3 +167,15 @@ def test_interpolate_model_signature_is_pdspys():
     names = list(inspect.signature(rt.interpolate_model).parameters)
     assert names[:10] == ["u", "v", "freq", "model", "nthreads", "dRA", "dDec", "code", "nxy",
                           "dxy"]
+
+
+def test_line_options_accepted_at_defaults_only():
+    """run_image's line-transfer arguments (Model.py:515-519) pass at pdspy's
+    defaults and are refused when a line is asked for."""
+    kw = dict(imolspec=None, iline=None, widthkms=None, vkms=None, linenlam=None,
+              doppcatch=False, unstructured=False, verbose=True)
+    rt._pop_line_options(kw)
+    assert kw == dict(verbose=True)
+    for k, v in (("imolspec", 1), ("doppcatch", True), ("unstructured", True)):
+        with pytest.raises(NotImplementedError):
+            rt._pop_line_options({k: v})
