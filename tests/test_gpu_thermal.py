@@ -50,6 +50,26 @@ def test_thermal_matches_oracle_disk_envelope_two_species():
     assert np.array_equal(Tg[0][both], Tg[1][both])
 
 
+def test_thermal_matches_oracle_two_absorption_groups():
+    """Disk and envelope with different dust (as pdspy's fits have them): two
+    absorption groups, the (2 species, 2 groups) kernel variant."""
+    print("case: disk+envelope 48x24, distinct dust, 5e4 packets, seed 2", flush=True)
+    arr = oracle_arrays(configs.disk_envelope(nr=48, nt=24))
+    lam = arr["lam_um"]
+    arr["kabs"] = np.array(arr["kabs"], dtype=float)
+    arr["ksca"] = np.array(arr["ksca"], dtype=float)
+    arr["kabs"][0] = np.where(lam > 1.0, 1e3 * lam ** -0.6, 1e3)   # grown grains in the disk
+    arr["ksca"][0] = 2.0 * arr["kabs"][0]
+    n = 50000
+    Tg, vg, sg = _gpu_T(arr, n, 2, max_events=20000)
+    To, vo, so = oracle.thermal(arr, n, seed=2, max_events=20000, lagged=True)
+    med, p99, ncell = dT_stats(To, Tg, vo)
+    print("median %.2e p99 %.2e cells %d" % (med, p99, ncell))
+    assert med <= 0.01 and p99 <= 0.05
+    both = (arr["rho"][0].ravel() > 0) & (arr["rho"][1].ravel() > 0)
+    assert both.sum() > 100 and not np.array_equal(Tg[0][both], Tg[1][both])
+
+
 def test_thermal_hg_3d_matches_oracle():
     print("case: 3D spiral 16x8x8 HG g=0.6, 2e4 packets", flush=True)
     arr = oracle_arrays(configs.spiral_3d(nr=16, nt=8, nphi=8))
