@@ -16,19 +16,7 @@ with the same dataset names).  The old format without a ``p`` axis
 """
 import numpy as np
 
-from .inputs import c as C_LIGHT
-
-
-class Dust:
-    """The fields pdspy's Dust.set_properties fills (dust/Dust.py:181-188)."""
-
-    def set_properties(self, lam, kabs, ksca):
-        self.lam = lam
-        self.nu = C_LIGHT / self.lam
-        self.kabs = kabs
-        self.ksca = ksca
-        self.kext = self.kabs + self.ksca
-        self.albedo = self.ksca / self.kext
+from .inputs import Dust  # the fields pdspy's Dust.set_properties fills (dust/Dust.py:181-188)
 
 
 def _bracket(grid, x, name):
