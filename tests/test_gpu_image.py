@@ -66,6 +66,28 @@ def test_image_matches_oracle(incl, pa):
     assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
 
 
+def test_image_matches_oracle_distinct_dust():
+    """Disk and envelope with different opacities (each species its own kappa at
+    the camera wavelength), as in pdspy's fits."""
+    print("case: image, two species with different dust", flush=True)
+    arr, T = _disk_setup(seed=3)
+    lam = arr["lam_um"]
+    arr["kabs"] = np.array(arr["kabs"], dtype=float)
+    arr["kabs"][0] = np.where(lam > 1.0, 1e3 * lam ** -0.6, 1e3)
+    dm = _dev(arr)
+    kaps = [rt._loglog_interp(np.array([1300.0]), lam, arr["kabs"][s])[0] for s in (0, 1)]
+    assert kaps[0] > 3 * kaps[1]
+    npix, pix = 64, 4.0 * AU
+    cam = dm.camera(npix, pix, 1300.0, incl=55.0, pa=20.0, incl_star=True)
+    img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+    ref = oracle.image(arr, T, kaps, npix, pix, incl=55.0, posang=20.0, lam_um=1300.0,
+                       incl_star=True)
+    err = np.abs(img - ref).max() / np.abs(ref).max()
+    print("image (distinct dust) max rel err %.2e" % err)
+    assert err < 2e-3
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+
+
 def test_ray_tiles_do_not_change_results():
     # the launch geometry (pixels per block) must not change any ray
     arr, T = _disk_setup()
