@@ -179,3 +179,20 @@ def test_line_options_accepted_at_defaults_only():
     for k, v in (("imolspec", 1), ("doppcatch", True), ("unstructured", True)):
         with pytest.raises(NotImplementedError):
             rt._pop_line_options({k: v})
+
+
+def test_dust_arrays_match_model_arrays_resampling():
+    """DeviceModel.set_dust resamples a Dust as model_arrays does at creation."""
+    m = configs.disk_envelope(nr=8, nt=6)
+    a = rt.model_arrays(m.grid)
+    kabs, ksca, gg, has_g = rt.dust_arrays(m.grid.dust, a["lam_um"])
+    assert np.array_equal(np.array(kabs), a["kabs"]) and np.array_equal(np.array(ksca), a["ksca"])
+    assert not has_g and a["gsca"] is None
+    # a Dust on its own coarser grid is log-log interpolated onto the model's
+    gen = configs.synthetic_dust_generator(np.logspace(-1, 4, 40))
+    d = gen(1e-2, 3.2)
+    k = rt.dust_arrays([d], a["lam_um"])[0][0]
+    sel = np.isin(np.round(np.log10(a["lam_um"]), 9), np.round(np.log10(d.lam * 1e4), 9))
+    assert sel.sum() >= 2
+    assert np.allclose(k[sel], d.kabs[np.isin(np.round(np.log10(d.lam * 1e4), 9),
+                                              np.round(np.log10(a["lam_um"]), 9))], rtol=1e-10)
