@@ -433,7 +433,9 @@ __device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uin
 // walls; when it has no packet left and the run has none to hand out (all
 // claimed, or the next epoch waits for its snapshot) it moves to the next run
 // that has.  MS, MG: compile-time bounds on species and absorption groups
-// (register caches).  BATCH: the parameter blocks are read from `runs` in
+// (register caches); two species in one absorption group fit 128 VGPRs (four
+// waves per SIMD), up to four species get three waves' registers and more
+// species two.  BATCH: the parameter blocks are read from `runs` in
 // device memory; otherwise the single run's block P1 is a kernel argument.
 template <int MS, int MG, bool BATCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 && MG <= 1 ? 4 : MS <= 4 ? 3 : 2)))
