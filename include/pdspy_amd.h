@@ -257,6 +257,20 @@ int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
                 const double *d_u, const double *d_v, int K, double dra, double ddec,
                 float *d_re, float *d_im, void *d_work);
 size_t rt_vis_grid_workspace(int nx, int ny);
+/* pdspy's interpolate_model(code="galario") for one channel of a pdspy Image
+ * (interpolate_model.py:20-27, which calls galario's
+ * sampleImage(image[::-1], dxy, u, v, dRA, dDec) and negates the imaginary
+ * part): galario's sampling of the row-flipped image -- transform on the
+ * pixel-conjugate grid with the phase origin at pixel (nx/2, ny/2), bilinear
+ * interpolation, shift factor exp(+2 pi i (u dra + v ddec)) -- returned
+ * complex-conjugated.  Equivalently: the gridded sampler of the unflipped
+ * transform evaluated as G(-u, -v) exp(-2 pi i (u dra + v ddec)) with
+ * G = transform of image[::-1]; on grid nodes this is
+ * exp(-2 pi i (u dra + v (ddec + dxy))) times rt_vis_dft at (-u, v).  Same
+ * arguments, limits and workspace as rt_vis_grid; dra, ddec in rad. */
+int rt_interpolate_galario(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                           const double *d_u, const double *d_v, int K, double dra,
+                           double ddec, float *d_re, float *d_im, void *d_work);
 /* chi^2 = sum w ((dre-mre)^2 + (dim-mim)^2) into d_out[0] (double), and
  * (optional, non-NULL) d_gre/d_gim = d chi^2 / d(mre, mim). */
 int rt_chisq(rt_ctx *ctx, const float *d_dre, const float *d_dim, const float *d_w,

@@ -108,6 +108,15 @@ def _setup(L):
     L.or_emis_per_gram.argtypes = [P, P, C.c_int, C.c_double]
     L.or_emis_per_gram.restype = C.c_double
     L.or_freq_grid.argtypes = [P, C.c_int, P, P]
+    L.or_tables_create.argtypes = [C.POINTER(_Model), C.c_int, C.c_double, C.c_double]
+    L.or_tables_create.restype = C.c_void_p
+    L.or_tables_free.argtypes = [P]
+    L.or_tables_shape.argtypes = [P, P, P, P]
+    L.or_tables_get.argtypes = [P, P, P, P, P, P]
+    L.or_reemit_bin.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.or_reemit_bin.restype = C.c_int
+    L.or_reemit_hist.argtypes = [P, C.c_int, C.c_double, C.c_uint64, C.c_uint64, P]
+    L.or_set_mutation.argtypes = [C.c_int]
     L.or_mrw_survival.argtypes = [C.c_double]
     L.or_mrw_survival.restype = C.c_double
     L.or_mrw_invert.argtypes = [C.c_double]
@@ -300,6 +309,107 @@ def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0):
          (1 - ta) * tb * F[b + 1, a] + ta * tb * F[b + 1, a + 1])
     V = V * np.exp(-2j * np.pi * (u * dra + v * ddec))
     return np.where(ok, V.real, np.nan), np.where(ok, V.imag, np.nan)
+
+
+class Tables:
+    """The thermal Monte Carlo's tables for a model (or_tables_create): ``lnq``
+    [ngroup][ntemp] (ln of the emission per gram), ``kplanck`` [ngroup][ntemp],
+    ``kross`` [nspec][ntemp], ``cdf`` [ngroup][ntemp][nlam] (re-emission),
+    ``grp`` [nspec]; ``temps`` the temperature grid.  ``reemit(q, T, u_row,
+    u_freq)`` is the thermal MC's re-emission draw, ``hist(q, T, n, seed)``
+    the histogram of 2n such draws."""
+
+    def __init__(self, m, ntemp=1000, temp0=0.01, temp1=1e5):
+        pm = _Pinned(m)
+        h = lib().or_tables_create(C.byref(pm.s), int(ntemp), float(temp0), float(temp1))
+        if not h:
+            raise RuntimeError("or_tables_create failed")
+        self.h = h
+        ng, nt, nl = C.c_int(), C.c_int(), C.c_int()
+        lib().or_tables_shape(h, C.byref(ng), C.byref(nt), C.byref(nl))
+        self.ngroup, self.ntemp, self.nlam = ng.value, nt.value, nl.value
+        self.lnq = np.zeros((self.ngroup, self.ntemp))
+        self.kplanck = np.zeros((self.ngroup, self.ntemp))
+        self.kross = np.zeros((pm.nspec, self.ntemp))
+        self.cdf = np.zeros((self.ngroup, self.ntemp, self.nlam))
+        self.grp = np.zeros(pm.nspec, dtype=np.int32)
+        lib().or_tables_get(h, self.lnq.ctypes.data, self.kplanck.ctypes.data,
+                            self.kross.ctypes.data, self.cdf.ctypes.data, self.grp.ctypes.data)
+        self.temps = np.exp(np.linspace(np.log(temp0), np.log(temp1), ntemp))
+        self.lnt0 = np.log(temp0)
+        self.dlnt = (np.log(temp1) - np.log(temp0)) / (ntemp - 1)
+
+    def reemit(self, q, T, u_row, u_freq):
+        return lib().or_reemit_bin(self.h, int(q), float(T), float(u_row), float(u_freq))
+
+    def hist(self, q, T, n, seed=0):
+        out = np.zeros(self.nlam, dtype=np.uint64)
+        lib().or_reemit_hist(self.h, int(q), float(T), int(n), int(seed), out.ctypes.data)
+        return out
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().or_tables_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def set_mutation(code):
+    """Test only: 0 = none, 1 = re-emission spectrum kappa B instead of
+    kappa dB/dT, 2 = temperature row shifted by one (applies to tables built
+    and draws made afterwards)."""
+    lib().or_set_mutation(int(code))
+
+
+def galario_sample_image(image, dxy, u, v, dRA=0.0, dDec=0.0, origin="upper"):
+    """galario's sampleImage (Tazzari, Beaujean & Testi 2018, MNRAS 476, 4527),
+    restated in FP64 numpy from its published interface and method: ``image``
+    [N, N] (Jy/pixel), ``dxy`` the pixel size (rad), (u, v) in wavelengths,
+    dRA / dDec (rad) translate the image towards the East (left, decreasing
+    column index) / North (up), ``origin`` says whether row 0 is the top
+    ('upper', galario's default) or the bottom ('lower') of the image.
+    Method: real-to-complex transform of fftshift(image) (phase origin at pixel
+    (N/2, N/2)), fftshift along the row axis, bilinear interpolation at
+    (|u|/du, N/2 + s v_o v/du) with du = 1/(N dxy), s = sign(u), v_o = +1
+    ('upper') or -1 ('lower'), complex-conjugated where u < 0 (the stored half
+    plane), then the factor exp(+2 pi i (u dRA + v dDec)).  pdspy calls it at
+    interferometry/interpolate_model.py:23-24.  Returns complex V."""
+    img = _f64(image)
+    u = _f64(u).reshape(-1)
+    v = _f64(v).reshape(-1)
+    n = img.shape[0]
+    if img.shape != (n, n):
+        raise ValueError("galario_sample_image: square image expected")
+    vo = {"upper": 1.0, "lower": -1.0}[origin]
+    F = np.fft.fftshift(np.fft.rfft2(np.fft.fftshift(img)), axes=0)   # [n, n/2 + 1]
+    du = 1.0 / (n * dxy)
+    neg = u < 0
+    fu = np.abs(u) / du
+    fv = n // 2 + np.where(neg, -1.0, 1.0) * vo * v / du
+    a0 = np.floor(fu)
+    b0 = np.floor(fv)
+    ok = (a0 >= 0) & (a0 <= F.shape[1] - 2) & (b0 >= 0) & (b0 <= n - 2)
+    a = np.where(ok, a0, 0).astype(np.int64)
+    b = np.where(ok, b0, 0).astype(np.int64)
+    ta = fu - a0
+    tb = fv - b0
+    V = ((1 - ta) * (1 - tb) * F[b, a] + ta * (1 - tb) * F[b, a + 1] +
+         (1 - ta) * tb * F[b + 1, a] + ta * tb * F[b + 1, a + 1])
+    V = np.where(neg, np.conj(V), V)
+    V = V * np.exp(2j * np.pi * (u * dRA + v * dDec))
+    return np.where(ok, V, np.nan + 1j * np.nan)
+
+
+def pdspy_interpolate_galario(image, dxy, u, v, dRA=0.0, dDec=0.0):
+    """What pdspy's interpolate_model(code="galario") computes for one channel
+    (interferometry/interpolate_model.py:20-27): galario's sampleImage of the
+    row-flipped image (pdspy's row 0 is the bottom of the image, galario's
+    default origin is the top) and the imaginary part negated.  ``image``
+    [ny, nx] in pdspy's orientation; dRA, dDec in rad.  Returns (re, im)."""
+    V = galario_sample_image(_f64(image)[::-1], dxy, u, v, dRA, dDec)
+    return V.real, -V.imag
 
 
 def chisq(dre, dim, w, mre, mim):

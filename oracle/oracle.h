@@ -165,6 +165,25 @@ double or_emis_per_gram(const double *kabs, const double *lam_um, int nlam,
 /* Modified-random-walk escape law: survival probability of a diffusing packet
  * started at the centre of a sphere, P(y) = 2 sum_n (-1)^(n+1) exp(-n^2 pi^2 y)
  * with y = D t / R0^2, and its inverse. */
+/* The thermal Monte Carlo's tables for a model (temperature grid
+ * exp(linspace(ln temp0, ln temp1, ntemp))): ln q(T) per absorption group
+ * [ngroup][ntemp], Planck-mean absorption [ngroup][ntemp], Rosseland-mean
+ * extinction per species [nspec][ntemp], re-emission CDFs
+ * [ngroup][ntemp][nlam], group of each species.  or_reemit_bin is the
+ * re-emission draw the thermal MC makes (T, two uniforms -> frequency bin);
+ * or_reemit_hist histograms 2n such draws from Philox uniforms. */
+typedef struct or_tables or_tables;
+or_tables *or_tables_create(const or_model *m, int ntemp, double temp0, double temp1);
+void or_tables_free(or_tables *t);
+void or_tables_shape(const or_tables *t, int *ngroup, int *ntemp, int *nlam);
+void or_tables_get(const or_tables *t, double *lnq, double *kplanck, double *kross,
+                   double *cdf, int *grp);
+int or_reemit_bin(const or_tables *t, int q, double T, double u_row, double u_freq);
+void or_reemit_hist(const or_tables *t, int q, double T, uint64_t n, uint64_t seed,
+                    uint64_t *counts);
+/* Test only: plants a known mistake in the tables / draw (see rt_oracle.c). */
+void or_set_mutation(int code);
+
 double or_mrw_survival(double y);
 double or_mrw_invert(double xi);
 

@@ -605,6 +605,12 @@ static void cdf_from_log(const double *lw, int n, double *cdf) {
     cdf[n - 1] = 1.0;
 }
 
+/* Deliberate mistakes a test plants to show that the re-emission pins catch
+ * them (0 = none; 1 = re-emission spectrum kappa B instead of kappa dB/dT;
+ * 2 = temperature row shifted up by one).  Never set outside such a test. */
+static int g_mutation = 0;
+void or_set_mutation(int code) { g_mutation = code; }
+
 static int tab_init(tab_t *t, const or_model *m, const or_thermal_opts *o) {
     memset(t, 0, sizeof(*t));
     int nl = m->nlam, ns = m->nspec;
@@ -655,7 +661,9 @@ static int tab_init(tab_t *t, const or_model *m, const or_thermal_opts *o) {
                                     : -INFINITY;
             t->lnq[q * t->ntemp + it] = log(4.0 * PI_D) + logsumexp(lw, nl);
             for (int i = 0; i < nl; ++i)
-                lw[i] = (ka[i] > 0) ? log(ka[i] * t->dnu[i]) + ln_dplanck(t->nu[i], T)
+                lw[i] = (ka[i] > 0) ? log(ka[i] * t->dnu[i]) +
+                                          (g_mutation == 1 ? ln_planck(t->nu[i], T)
+                                                           : ln_dplanck(t->nu[i], T))
                                     : -INFINITY;
             cdf_from_log(lw, nl, t->cdf + ((size_t)q * t->ntemp + it) * nl);
         }
@@ -759,6 +767,22 @@ static int sample_cdf(const double *cdf, int n, double u) {
 }
 
 /* isotropic direction: cos(theta) = 2 u1 - 1, azimuth 2 pi u2 */
+/* Bjorkman & Wood re-emission frequency of absorption group q at temperature
+ * T: the table row below or above T's position on the log temperature grid,
+ * the upper one with probability equal to the fractional position (u_row),
+ * then the inverse CDF of that row's kappa_nu dB_nu/dT dnu spectrum (u_freq) */
+static int reemit_bin(const tab_t *t, int q, double T, double u_row, double u_freq) {
+    double x = (T > 0) ? (log(T) - t->lnt0) / t->dlnt : 0.0;
+    if (x < 0) x = 0;
+    if (x > t->ntemp - 1) x = t->ntemp - 1;
+    int lo = (int)x;
+    if (lo > t->ntemp - 2) lo = t->ntemp - 2;
+    double w = x - lo;
+    int row = lo + (u_row < w ? 1 : 0);
+    if (g_mutation == 2 && row < t->ntemp - 1) ++row;
+    return sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * t->nlam, t->nlam, u_freq);
+}
+
 static void iso_dir(double u1, double u2, double *d) {
     double mu = 2.0 * u1 - 1.0;
     double ph = TWO_PI_D * u2;
@@ -961,7 +985,7 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, uint64_t pid, uint32
     const or_model *m = mc->m;
     const geom_t *g = mc->g;
     const tab_t *t = mc->t;
-    const int ns = m->nspec, ng = t->ngroup, nl = t->nlam;
+    const int ns = m->nspec, ng = t->ngroup;
     double aR = 0;
     for (int s = 0; s < ns; ++s) {
         double rho = m->rho[(size_t)s * g->ncell + c];
@@ -1011,15 +1035,7 @@ static int mrw_step(mc_t *mc, uint64_t *tl, pk_t *p, int c, uint64_t pid, uint32
         }
     }
     int q = t->grp[sp];
-    double T = cur_T(mc, c, q);
-    double x = (T > 0) ? (log(T) - t->lnt0) / t->dlnt : 0.0;
-    if (x < 0) x = 0;
-    if (x > t->ntemp - 1) x = t->ntemp - 1;
-    int lo = (int)x;
-    if (lo > t->ntemp - 2) lo = t->ntemp - 2;
-    double w = x - lo;
-    int row = lo + (v[6] < w ? 1 : 0);
-    *ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, v[7]);
+    *ifr = reemit_bin(t, q, cur_T(mc, c, q), v[6], v[7]);
     return 1;
 }
 
@@ -1144,14 +1160,7 @@ static void run_packet(mc_t *mc, uint64_t pid, int epoch, cnt_t *cnt) {
                     tv = mc->tally[(size_t)c * ng + q];
                     T = temp_from_e(t, q, (double)tv * mc->econv[(size_t)c * ng + q]);
                 }
-                double x = (T > 0) ? (log(T) - t->lnt0) / t->dlnt : 0.0;
-                if (x < 0) x = 0;
-                if (x > t->ntemp - 1) x = t->ntemp - 1;
-                int lo = (int)x;
-                if (lo > t->ntemp - 2) lo = t->ntemp - 2;
-                double w = x - lo;
-                int row = lo + (u[2] < w ? 1 : 0);
-                ifr = sample_cdf(t->cdf + ((size_t)q * t->ntemp + row) * nl, nl, u[3]);
+                ifr = reemit_bin(t, q, T, u[2], u[3]);
                 iso_dir(u[4], u[5], d);
                 p.dx = d[0];
                 p.dy = d[1];
@@ -1407,6 +1416,64 @@ int or_thermal(const or_model *m, const or_thermal_opts *o, double *T_out,
  * uniform), u4 u5 direction, u6 first tau, u7 hemisphere (mirrored grids);
  * k >= 1 at the k-th interaction: u0 scatter or end, u1 u2 new direction,
  * u3 next tau.                                                           */
+/* ------------------------------------------------- tables, for the tests */
+struct or_tables {
+    tab_t t;
+};
+
+or_tables *or_tables_create(const or_model *m, int ntemp, double temp0, double temp1) {
+    or_thermal_opts o;
+    memset(&o, 0, sizeof(o));
+    o.ntemp = ntemp;
+    o.temp0 = temp0;
+    o.temp1 = temp1;
+    or_tables *h = calloc(1, sizeof(or_tables));
+    if (!h) return NULL;
+    if (tab_init(&h->t, m, &o) != 0) {
+        tab_free(&h->t);
+        free(h);
+        return NULL;
+    }
+    return h;
+}
+
+void or_tables_free(or_tables *h) {
+    if (!h) return;
+    tab_free(&h->t);
+    free(h);
+}
+
+void or_tables_shape(const or_tables *h, int *ngroup, int *ntemp, int *nlam) {
+    *ngroup = h->t.ngroup;
+    *ntemp = h->t.ntemp;
+    *nlam = h->t.nlam;
+}
+
+void or_tables_get(const or_tables *h, double *lnq, double *kplanck, double *kross, double *cdf,
+                   int *grp) {
+    const tab_t *t = &h->t;
+    if (lnq) memcpy(lnq, t->lnq, sizeof(double) * t->ngroup * t->ntemp);
+    if (kplanck) memcpy(kplanck, t->kplanck, sizeof(double) * t->ngroup * t->ntemp);
+    if (kross) memcpy(kross, t->kross, sizeof(double) * t->nspec * t->ntemp);
+    if (cdf) memcpy(cdf, t->cdf, sizeof(double) * (size_t)t->ngroup * t->ntemp * t->nlam);
+    if (grp) memcpy(grp, t->grp, sizeof(int) * t->nspec);
+}
+
+int or_reemit_bin(const or_tables *h, int q, double T, double u_row, double u_freq) {
+    return reemit_bin(&h->t, q, T, u_row, u_freq);
+}
+
+void or_reemit_hist(const or_tables *h, int q, double T, uint64_t n, uint64_t seed,
+                    uint64_t *counts) {
+    memset(counts, 0, sizeof(uint64_t) * h->t.nlam);
+    for (uint64_t k = 0; k < n; ++k) {
+        double u[8];
+        rng8(seed, k, 0, u);
+        counts[reemit_bin(&h->t, q, T, u[0], u[1])]++;
+        counts[reemit_bin(&h->t, q, T, u[2], u[3])]++;
+    }
+}
+
 static int sample_cum(const double *cum, int n, double x) {
     /* first i with cum[i] > x (cum ascending, cum[n-1] = total) */
     int lo = 0, hi = n - 1;

@@ -19,7 +19,7 @@ EXPORTS = [
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_vis_grid",
-    "rt_vis_grid_workspace", "rt_chisq",
+    "rt_vis_grid_workspace", "rt_interpolate_galario", "rt_chisq",
     "rt_batch_create", "rt_batch_destroy", "rt_batch_thermal", "rt_batch_step",
 ]
 
@@ -138,6 +138,7 @@ def load():
         "rt_vis_dft_adjoint": ([P, P, P, I, I, D, P, P, I, D, D, P], I),
         "rt_vis_grid": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_vis_grid_workspace": ([I, I], C.c_size_t),
+        "rt_interpolate_galario": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_chisq": ([P, P, P, P, P, P, I, P, P, P], I),
         "rt_batch_create": ([P, P, I, C.POINTER(P)], I),
         "rt_batch_destroy": ([P], I),

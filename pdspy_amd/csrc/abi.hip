@@ -516,7 +516,14 @@ int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, con
                 const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
                 void *d_work) {
     if (!ctx) return RT_E_ARG;
-    return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, d_work);
+    return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, d_work, 0);
+}
+
+int rt_interpolate_galario(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                           const double *d_u, const double *d_v, int K, double dra, double ddec,
+                           float *d_re, float *d_im, void *d_work) {
+    if (!ctx) return RT_E_ARG;
+    return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, d_work, 1);
 }
 
 size_t rt_vis_grid_workspace(int nx, int ny) { return vis_grid_workspace(nx, ny); }

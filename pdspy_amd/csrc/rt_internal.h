@@ -222,8 +222,9 @@ int chisq_run(rt_ctx *ctx, const float *dre, const float *dim, const float *w, c
               const float *mim, int K, double *out, float *gre, float *gim, double *ws);
 size_t chisq_workspace(int K);
 // gridded sampler; work: nullable, vis_grid_workspace(nx, ny) bytes (else
-// allocated stream-ordered)
+// allocated stream-ordered); conv 0 = imtovis convention, 1 = pdspy's galario
+// call (image[::-1], imaginary part negated)
 int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
                  const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
-                 void *d_work);
+                 void *d_work, int conv);
 size_t vis_grid_workspace(int nx, int ny);

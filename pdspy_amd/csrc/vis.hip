@@ -198,14 +198,17 @@ __device__ __forceinline__ void phase_index(int k, int x0, int n, int &r, int &d
     dr = d < 0 ? d + n : d;
 }
 
-// rows: A[j][a] = sum_k I[j][k] w_nx((a - nx/2)(k - nx/2)); block = 256 a's of row j
+// rows: A[j][a] = sum_k I[j'][k] w_nx((a - nx/2)(k - nx/2)); block = 256 a's of
+// row j, which reads image row j' = j, or j' = ny - 1 - j when flip is set (the
+// transform of image[::-1])
 __global__ void __launch_bounds__(GRID_BS) k_grid_rows(const float *img, const float2 *twx,
-                                                       int nx, float2 *A) {
+                                                       int nx, int ny, int flip, float2 *A) {
     __shared__ float row[GRID_MAXN];
     __shared__ float2 tw[GRID_MAXN];
     const int j = blockIdx.y;
+    const int js = flip ? ny - 1 - j : j;
     for (int i = threadIdx.x; i < nx; i += GRID_BS) {
-        row[i] = img[(size_t)j * nx + i];
+        row[i] = img[(size_t)js * nx + i];
         tw[i] = twx[i];
     }
     __syncthreads();
@@ -262,17 +265,18 @@ __global__ void __launch_bounds__(GRID_BS) k_grid_cols(const float2 *A, const fl
 }
 
 // one (u,v) point per lane: bilinear interpolation between the four grid nodes
-// around it, then the phase-centre shift; a point whose four nodes are not all
-// on the grid gets NaN
+// around (usgn u, usgn v), then the phase-centre shift at (u, v); a point whose
+// four nodes are not all on the grid gets NaN
 __global__ void __launch_bounds__(GRID_BS) k_grid_sample(const float2 *F, int nx, int ny,
                                                          double dxy, const double *u,
-                                                         const double *v, int K, double dra,
-                                                         double ddec, float *re, float *im) {
+                                                         const double *v, int K, double usgn,
+                                                         double dra, double ddec, float *re,
+                                                         float *im) {
     const int q = blockIdx.x * GRID_BS + threadIdx.x;
     if (q >= K) return;
     const double uq = u[q], vq = v[q];
-    const double fa = uq * ((double)nx * dxy) + (double)(nx / 2);
-    const double fb = vq * ((double)ny * dxy) + (double)(ny / 2);
+    const double fa = usgn * uq * ((double)nx * dxy) + (double)(nx / 2);
+    const double fb = usgn * vq * ((double)ny * dxy) + (double)(ny / 2);
     const double a0 = floor(fa), b0 = floor(fb);
     if (!(a0 >= 0.0 && a0 <= (double)(nx - 2) && b0 >= 0.0 && b0 <= (double)(ny - 2))) {
         re[q] = __builtin_nanf("");
@@ -296,9 +300,16 @@ size_t vis_grid_workspace(int nx, int ny) {
     return sizeof(float2) * (2 * (size_t)nx * ny + (size_t)nx + ny);
 }
 
+// conv 0: the imtovis convention above.  conv 1: pdspy's galario call
+// (interpolate_model.py:23-27): galario's sampleImage of image[::-1] with the
+// imaginary part negated.  galario samples G = transform of image[::-1] at
+// (u, v) and multiplies by exp(+2 pi i (u dra + v ddec)); for a real image
+// conj(G(u, v)) = G(-u, -v) holds node by node, hence also for the bilinear
+// interpolant, so the negated imaginary part is G interpolated at (-u, -v)
+// times exp(-2 pi i (u dra + v ddec)).
 int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const double *d_u,
                  const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
-                 void *d_work) {
+                 void *d_work, int conv) {
     if (nx < 2 || ny < 2 || nx > GRID_MAXN || ny > GRID_MAXN || K < 0 || !(dxy > 0))
         return set_err(ctx, RT_E_ARG, "vis grid: invalid sizes (2 <= nx, ny <= %d)", GRID_MAXN);
     if (K == 0) return RT_OK;
@@ -310,7 +321,7 @@ int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, co
     hipLaunchKernelGGL(k_grid_twiddles, dim3((nx + ny + GRID_BS - 1) / GRID_BS), dim3(GRID_BS), 0,
                        ctx->stream, nx, ny, tw);
     hipLaunchKernelGGL(k_grid_rows, dim3((nx + GRID_BS - 1) / GRID_BS, ny), dim3(GRID_BS), 0,
-                       ctx->stream, d_img, tw, nx, A);
+                       ctx->stream, d_img, tw, nx, ny, conv == 1 ? 1 : 0, A);
     const int bx = (nx + GRID_BS - 1) / GRID_BS;
     if (bx * ((ny + 7) / 8) >= 512)
         hipLaunchKernelGGL(k_grid_cols<8>, dim3(bx, (ny + 7) / 8), dim3(GRID_BS), 0, ctx->stream, A,
@@ -322,7 +333,8 @@ int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, co
         hipLaunchKernelGGL(k_grid_cols<1>, dim3(bx, ny), dim3(GRID_BS), 0, ctx->stream, A, tw + nx,
                            nx, ny, F);
     hipLaunchKernelGGL(k_grid_sample, dim3((K + GRID_BS - 1) / GRID_BS), dim3(GRID_BS), 0,
-                       ctx->stream, F, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im);
+                       ctx->stream, F, nx, ny, dxy, d_u, d_v, K, conv == 1 ? -1.0 : 1.0, dra, ddec,
+                       d_re, d_im);
     if (!d_work) HIPCHK(ctx, hipFreeAsync(ws, ctx->stream));
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;

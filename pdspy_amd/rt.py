@@ -402,14 +402,20 @@ def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0, device=0):
     return re, im
 
 
-def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
+def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None, conv="imtovis"):
     """V at (u, v) of a float32 torch image [ny, nx] by galario's method (the
     image's DFT on its pixel-conjugate grid, bilinear interpolation, phase-centre
-    shift; galario sampleImage as called at interpolate_model.py:20-27).  A point
-    outside the grid's interpolation range raises ValueError.  ``work``: an
-    optional contiguous device tensor of at least ``rt_vis_grid_workspace(nx, ny)``
-    bytes, else the library allocates one stream-ordered per call."""
+    shift; galario sampleImage as called at interpolate_model.py:20-27).
+    ``conv="imtovis"``: the imtovis sign convention (rt_vis_grid);
+    ``conv="galario"``: what pdspy's interpolate_model makes of galario's
+    output -- sampleImage(image[::-1]) with the imaginary part negated
+    (rt_interpolate_galario).  A point outside the grid's interpolation range
+    raises ValueError.  ``work``: an optional contiguous device tensor of at
+    least ``rt_vis_grid_workspace(nx, ny)`` bytes, else the library allocates
+    one stream-ordered per call."""
     import torch
+    if conv not in ("imtovis", "galario"):
+        raise ValueError("vis_grid: conv must be 'imtovis' or 'galario'")
     ctx = _lib.context(device)
     img = img.contiguous().float()
     ny, nx = img.shape
@@ -418,9 +424,11 @@ def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
     K = u.numel()
     if K:
         # the four nodes around each point must lie on the grid (galario's
-        # sampleImage checks the uv coverage against the image the same way)
-        fa = u * (nx * dxy) + nx // 2
-        fb = v * (ny * dxy) + ny // 2
+        # sampleImage checks the uv coverage against the image the same way);
+        # the galario convention interpolates at (-u, -v)
+        sg = -1.0 if conv == "galario" else 1.0
+        fa = sg * u * (nx * dxy) + nx // 2
+        fb = sg * v * (ny * dxy) + ny // 2
         ok = ((fa >= 0) & (fa < nx - 1) & (fb >= 0) & (fb < ny - 1)).all()
         if not bool(ok):
             raise ValueError("vis_grid: (u,v) beyond the image's uv grid (|u| < (nx/2 - 1) / "
@@ -433,10 +441,11 @@ def vis_grid(img, dxy, u, v, dra=0.0, ddec=0.0, device=0, work=None):
         if not (work.is_cuda and work.is_contiguous()) or have < need:
             raise ValueError("vis_grid: work must be a contiguous device tensor of >= %d bytes "
                              "(got %d)" % (need, have))
-    ctx.check(ctx.L.rt_vis_grid(ctx.h, C.c_void_p(img.data_ptr()), nx, ny, float(dxy),
-                                C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), K, float(dra),
-                                float(ddec), C.c_void_p(re.data_ptr()), C.c_void_p(im.data_ptr()),
-                                C.c_void_p(work.data_ptr() if work is not None else None)))
+    fn = ctx.L.rt_interpolate_galario if conv == "galario" else ctx.L.rt_vis_grid
+    ctx.check(fn(ctx.h, C.c_void_p(img.data_ptr()), nx, ny, float(dxy),
+                 C.c_void_p(u.data_ptr()), C.c_void_p(v.data_ptr()), K, float(dra),
+                 float(ddec), C.c_void_p(re.data_ptr()), C.c_void_p(im.data_ptr()),
+                 C.c_void_p(work.data_ptr() if work is not None else None)))
     return re, im
 
 
@@ -632,22 +641,35 @@ def run_visibilities(model, name=None, nphot=1e6, npix=256, pixelsize=1.0, lam="
 def interpolate_model(u, v, freq, model, nthreads=1, dRA=0.0, dDec=0.0, code="galario",
                       nxy=1024, dxy=0.01, device=0):
     """Model visibilities of an Image at arbitrary (u, v) (interpolate_model.py:11-57,
-    same arguments in the same order); dRA/dDec in arcsec shift the phase
-    centre.  ``code="galario"`` (pdspy's default) samples as galario does, by
-    bilinear interpolation of the image's transform on its pixel-conjugate grid
-    (``vis_grid``); ``code="exact"`` takes the exact DFT at each point
-    (``vis_dft``).  pdspy's other codes (galario-unstructured, trift) are not on
-    this path; ``nthreads`` (galario's host threads) and ``nxy``/``dxy`` (used
-    only by galario-unstructured) are accepted and unused."""
+    same arguments in the same order); dRA/dDec in arcsec.  ``code="galario"``
+    (pdspy's default) gives what pdspy computes there: galario's
+    sampleImage(image[::-1], dxy, u, v, dRA, dDec) with the imaginary part
+    negated (interpolate_model.py:23-27), i.e. bilinear interpolation of the
+    flipped image's transform on its pixel-conjugate grid
+    (``vis_grid(conv="galario")``).  ``code="exact"`` evaluates the same
+    visibility function exactly at each point instead of interpolating: on the
+    grid nodes pdspy's galario call equals
+    exp(-2 pi i (u dRA + v (dDec + dxy))) DFT(-u, v) with DFT the imtovis
+    transform of the unflipped image (the flip moves the phase origin by one
+    pixel), so ``vis_dft`` is called at (-u, v) with the shift (-dRA, dDec + dxy).
+    pdspy's other codes (galario-unstructured, trift) are not on this path;
+    ``nthreads`` (galario's host threads) and ``nxy``/``dxy`` (used only by
+    galario-unstructured) are accepted and unused."""
     import torch
     if code not in ("galario", "exact"):
         raise ValueError("interpolate_model: code %r is not supported (galario, exact)" % (code,))
-    sample = vis_grid if code == "galario" else vis_dft
-    pix = (model.x[1] - model.x[0]) * arcsec  # rad (interpolate_model.py:18)
+    pix = (model.x[1] - model.x[0]) * arcsec  # rad (interpolate_model.py:20)
+    u_h = np.asarray(u, dtype=float).reshape(-1)
+    v_h = np.asarray(v, dtype=float).reshape(-1)
     real, imag = [], []
     for i in range(model.image.shape[2]):
         img = torch.tensor(model.image[:, :, i, 0], dtype=torch.float32, device="cuda")
-        re, im = sample(img, pix, u, v, dRA * arcsec, dDec * arcsec, device=device)
+        if code == "galario":
+            re, im = vis_grid(img, pix, u_h, v_h, dRA * arcsec, dDec * arcsec, device=device,
+                              conv="galario")
+        else:
+            re, im = vis_dft(img, pix, -u_h, v_h, -dRA * arcsec, dDec * arcsec + pix,
+                             device=device)
         real.append(re.double().cpu().numpy().reshape(-1, 1))
         imag.append(im.double().cpu().numpy().reshape(-1, 1))
     real = np.concatenate(real, axis=1)

@@ -273,6 +273,41 @@ def test_interpolate_model_codes():
         rt.interpolate_model(u, v, np.array([2.3e11]), im, code="trift")
 
 
+def test_interpolate_model_follows_pdspys_galario_call():
+    """interpolate_model(code="galario") against the oracle's restatement of
+    what pdspy computes (interpolate_model.py:23-27: galario's sampleImage of
+    image[::-1], imaginary part negated) on an asymmetric, off-centre image
+    with nonzero dRA and dDec; code="exact" against the exact form of the same
+    visibility function (exp(-2 pi i (u dRA + v (dDec + dxy))) DFT(-u, v))."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(31)
+    n, pix = 256, 0.0078125
+    x = (np.arange(n) - n // 2) * pix
+    X, Y = np.meshgrid(x, x)
+    img = (np.exp(-((X - 0.21) ** 2 + (Y + 0.08) ** 2) / (2 * 0.05 ** 2)) +
+           0.4 * np.exp(-((X + 0.3) ** 2 / (2 * 0.12 ** 2) + (Y - 0.25) ** 2 / (2 * 0.03 ** 2))) +
+           0.05 * rng.uniform(size=(n, n)))
+    im = SimpleNamespace(x=x, y=x, image=img[:, :, None, None])
+    u, v = configs.uv_points(K=20000, lam_m=1.3e-3, seed=8)
+    dra, ddec = 0.037, -0.052                      # arcsec
+    dxy = pix * inputs.arcsec
+    a = rt.interpolate_model(u, v, np.array([2.3e11]), im, dRA=dra, dDec=ddec)
+    ore, oim = oracle.pdspy_interpolate_galario(img, dxy, u, v, dra * inputs.arcsec,
+                                                ddec * inputs.arcsec)
+    scale = np.abs(ore + 1j * oim).max()
+    err = max(np.abs(a.real[:, 0] - ore).max(), np.abs(a.imag[:, 0] - oim).max()) / scale
+    print("galario convention: max |dV| / max|V| = %.2e" % err)
+    assert err < 2e-5
+    b = rt.interpolate_model(u, v, np.array([2.3e11]), im, dRA=dra, dDec=ddec, code="exact")
+    ere, eim = oracle.vis_dft(img, dxy, -u, v, -dra * inputs.arcsec, ddec * inputs.arcsec + dxy)
+    err = max(np.abs(b.real[:, 0] - ere).max(), np.abs(b.imag[:, 0] - eim).max()) / scale
+    print("exact form: max |dV| / max|V| = %.2e" % err)
+    assert err < 2e-5
+    # the imtovis convention is a different function for this image
+    c, d = oracle.vis_grid(img, dxy, u, v, dra * inputs.arcsec, ddec * inputs.arcsec)
+    assert np.abs((c - ore) + 1j * (d - oim)).max() > 0.05 * scale
+
+
 def test_vis_adjoint_matches_oracle():
     rng = np.random.default_rng(6)
     ny = nx = 48

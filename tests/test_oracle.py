@@ -560,3 +560,244 @@ def test_image_adjoint_with_scattering_matches_finite_differences():
         Tm[c] -= dT
         fdT = (L(m, Tp) - L(m, Tm)) / (2 * dT)
         assert g_T[c] == pytest.approx(fdT, rel=1e-4, abs=1e-40)
+
+
+# ---- galario's sampleImage and pdspy's call of it (interpolate_model.py:20-27)
+
+def _grid_nodes(n, dxy, rng, k):
+    """k random nodes of the pixel-conjugate grid, away from its edges"""
+    du = 1.0 / (n * dxy)
+    a = rng.integers(-n // 2 + 2, n // 2 - 2, size=k)
+    b = rng.integers(-n // 2 + 2, n // 2 - 2, size=k)
+    return a * du, b * du
+
+
+def _bordered_image(n, rng, border=6):
+    img = np.zeros((n, n))
+    img[border:-border, border:-border] = rng.random((n - 2 * border, n - 2 * border))
+    return img
+
+
+def test_galario_restatement_documented_shifts():
+    """galario's documented meaning of dRA / dDec: dRA > 0 translates the image
+    to the left (East), dDec > 0 translates it up (North); 'upper' puts row 0
+    at the top, 'lower' at the bottom.  Translating the array by k pixels must
+    equal the shift factor with dRA (dDec) = k dxy, at every grid node; a sign
+    error in the shift factor or in the transform's kernel fails this."""
+    rng = np.random.default_rng(3)
+    n, dxy = 64, 1e-7
+    img = _bordered_image(n, rng)
+    u, v = _grid_nodes(n, dxy, rng, 200)
+    for k in (1, 3):
+        left = np.roll(img, -k, axis=1)             # k columns towards column 0
+        up = np.roll(img, -k, axis=0)               # k rows towards row 0
+        down = np.roll(img, k, axis=0)
+        V0 = oracle.galario_sample_image(img, dxy, u, v, dRA=k * dxy)
+        np.testing.assert_allclose(V0, oracle.galario_sample_image(left, dxy, u, v),
+                                   rtol=1e-10, atol=1e-10 * np.abs(V0).max())
+        V1 = oracle.galario_sample_image(img, dxy, u, v, dDec=k * dxy)
+        np.testing.assert_allclose(V1, oracle.galario_sample_image(up, dxy, u, v),
+                                   rtol=1e-10, atol=1e-10 * np.abs(V1).max())
+        V2 = oracle.galario_sample_image(img, dxy, u, v, dDec=k * dxy, origin="lower")
+        np.testing.assert_allclose(V2, oracle.galario_sample_image(down, dxy, u, v,
+                                                                   origin="lower"),
+                                   rtol=1e-10, atol=1e-10 * np.abs(V2).max())
+        # the wrong direction does not match
+        assert np.abs(V0 - oracle.galario_sample_image(np.roll(img, k, axis=1), dxy, u,
+                                                       v)).max() > 1e-3 * np.abs(V0).max()
+
+
+def test_galario_restatement_point_source_and_half_plane():
+    """A point at galario's phase origin (pixel (N/2, N/2)) has V = its flux at
+    every (u, v), on and off the grid; for a real image the stored half plane
+    and its conjugate give V(-u, -v) = conj V(u, v); at grid nodes the
+    restatement equals the full complex FFT of fftshift(image)."""
+    rng = np.random.default_rng(4)
+    n, dxy = 64, 1e-7
+    du = 1.0 / (n * dxy)
+    img = np.zeros((n, n))
+    img[n // 2, n // 2] = 2.5
+    u = rng.uniform(-0.45, 0.45, 500) * n * du
+    v = rng.uniform(-0.45, 0.45, 500) * n * du
+    V = oracle.galario_sample_image(img, dxy, u, v)
+    np.testing.assert_allclose(V, 2.5 + 0j, atol=1e-12)
+    img = _bordered_image(n, rng)
+    Vp = oracle.galario_sample_image(img, dxy, u, v)
+    Vm = oracle.galario_sample_image(img, dxy, -u, -v)
+    np.testing.assert_allclose(Vm, np.conj(Vp), rtol=1e-10, atol=1e-10 * np.abs(Vp).max())
+    a, b = _grid_nodes(n, dxy, rng, 100)
+    F = np.fft.fftshift(np.fft.fft2(np.fft.fftshift(img)))
+    ia = np.rint(a / du).astype(int) + n // 2
+    ib = np.rint(b / du).astype(int) + n // 2
+    np.testing.assert_allclose(oracle.galario_sample_image(img, dxy, a, b), F[ib, ia],
+                               rtol=1e-10, atol=1e-10 * np.abs(F).max())
+
+
+def test_pdspy_galario_call_equals_shifted_mirrored_dft():
+    """pdspy's interpolate_model negates galario's imaginary part and flips the
+    image rows.  On grid nodes this is exactly
+    exp(-2 pi i (u dRA + v (dDec + dxy))) * DFT(-u, v) of the unflipped image
+    in the imtovis convention (the flip moves the phase origin by one pixel);
+    between nodes it is the imtovis-convention bilinear sampler of the flipped
+    image at (-u, -v) with the shift factor exp(-2 pi i (u dRA + v dDec)).
+    These are the identities the GPU's interpolate_model uses; dropping the
+    flip, the negation or the one-pixel shift fails them."""
+    rng = np.random.default_rng(5)
+    n, dxy = 64, 4e-8
+    img = _bordered_image(n, rng)
+    img[20:24, 40:47] += 5.0                        # asymmetric, off centre
+    dra, ddec = 0.37 * dxy, -1.9 * dxy
+    u, v = _grid_nodes(n, dxy, rng, 300)
+    re, im = oracle.pdspy_interpolate_galario(img, dxy, u, v, dra, ddec)
+    er, ei = oracle.vis_dft(img, dxy, -u, v, -dra, ddec + dxy)
+    scale = np.abs(er + 1j * ei).max()
+    np.testing.assert_allclose(re, er, atol=1e-10 * scale)
+    np.testing.assert_allclose(im, ei, atol=1e-10 * scale)
+    # without the one-pixel shift, or without the mirror, the identity fails
+    xr, xi = oracle.vis_dft(img, dxy, -u, v, -dra, ddec)
+    assert np.abs((re - xr) + 1j * (im - xi)).max() > 1e-3 * scale
+    xr, xi = oracle.vis_dft(img, dxy, u, v, dra, ddec + dxy)
+    assert np.abs((re - xr) + 1j * (im - xi)).max() > 1e-3 * scale
+    # off the grid: the imtovis-convention sampler of image[::-1] at (-u, -v)
+    du = 1.0 / (n * dxy)
+    u = rng.uniform(-0.4, 0.4, 400) * n * du
+    v = rng.uniform(-0.4, 0.4, 400) * n * du
+    re, im = oracle.pdspy_interpolate_galario(img, dxy, u, v, dra, ddec)
+    gr, gi = oracle.vis_grid(img[::-1], dxy, -u, -v)
+    G = (gr + 1j * gi) * np.exp(-2j * np.pi * (u * dra + v * ddec))
+    np.testing.assert_allclose(re, G.real, atol=1e-10 * scale)
+    np.testing.assert_allclose(im, G.imag, atol=1e-10 * scale)
+
+
+# ---- Bjorkman & Wood re-emission spectrum and the mean-opacity tables
+#      (temperature grid: RADMC-3D's ntemp / temp0 / temp1, radmc3d/write.py:47-52)
+
+def _np_dB_nu(nu, T):
+    """pdspy's dB_nu (misc/dB_nu.py:4-7); it carries a minus sign, the
+    spectrum uses its magnitude"""
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        x = H * nu / (K_B * T)
+        d = (-2 * H ** 2 * nu ** 4 / (C_LIGHT ** 2 * K_B * T ** 2)) / np.expm1(x) / (-np.expm1(-x))
+    return np.nan_to_num(np.abs(d))
+
+
+def _np_B_nu(nu, T):
+    with np.errstate(over="ignore"):
+        return (2 * H * nu ** 3 / C_LIGHT ** 2) / np.expm1(H * nu / (K_B * T))
+
+
+def _np_dnu(lam_um):
+    """trapezoid weights of the frequency grid nu = c / lam"""
+    nu = C_LIGHT / (lam_um * 1e-4)
+    d = np.zeros_like(nu)
+    d[1:-1] = 0.5 * np.abs(nu[:-2] - nu[2:])
+    d[0] = 0.5 * abs(nu[0] - nu[1])
+    d[-1] = 0.5 * abs(nu[-2] - nu[-1])
+    return nu, d
+
+
+def _two_dust_model():
+    lam = np.logspace(-1, 4, 100)
+    m = _small_model()
+    k1 = np.where(lam > 1.0, 1e3 / lam, 1e3)
+    k2 = np.where(lam > 1.0, 1e3 * lam ** -0.6, 1e3)
+    m.update(lam_um=lam, rho=np.ones((2, 1, 4, 5)) * 1e-20, kabs=np.array([k1, k2]),
+             ksca=np.array([k1, 2 * k2]))
+    return m
+
+
+def _reemit_expected(m, q, T, tabs):
+    """the expected distribution of the draw at T: kappa_nu |dB_nu/dT| dnu on
+    the two temperature rows around T, mixed linearly in the row coordinate"""
+    nu, dnu = _np_dnu(m["lam_um"])
+    ka = m["kabs"][q]
+    x = (np.log(T) - tabs.lnt0) / tabs.dlnt
+    lo = int(np.floor(x))
+    w = x - lo
+    out = np.zeros(len(nu))
+    for row, wt in ((lo, 1 - w), (lo + 1, w)):
+        if wt > 0:
+            p = ka * _np_dB_nu(nu, np.exp(tabs.lnt0 + row * tabs.dlnt)) * dnu
+            out += wt * p / p.sum()
+    return out
+
+
+def _chi2(counts, expected_p):
+    n = counts.sum()
+    e = expected_p * n
+    keep = e >= 20
+    c = counts[keep].astype(float)
+    # everything in the sparse tail as one bin
+    c = np.append(c, counts[~keep].sum())
+    e = np.append(e[keep], e[~keep].sum())
+    sel = e > 0
+    return float(((c[sel] - e[sel]) ** 2 / e[sel]).sum()), int(sel.sum() - 1)
+
+
+# temperatures on a table row (row 400 = 6.5 K... row 860 = 2.6e3 K) and between rows
+_REEMIT_CASES = [(0, 400, 0.0), (0, 612, 0.37), (1, 700, 0.0), (1, 860, 0.81), (0, 520, 0.5)]
+
+
+def _reemit_pin(m, seed=11, n=1000000):
+    """largest chi^2 excess over the cases, in standard deviations"""
+    tabs = oracle.Tables(m)
+    worst = -np.inf
+    for q, row, frac in _REEMIT_CASES:
+        T = np.exp(tabs.lnt0 + (row + frac) * tabs.dlnt)
+        counts = tabs.hist(q, T, n, seed=seed + row)
+        c2, dof = _chi2(counts, _reemit_expected(m, q, T, tabs))
+        worst = max(worst, (c2 - dof) / np.sqrt(2 * dof))
+    return worst
+
+
+def test_reemission_draw_follows_kappa_dBdT():
+    """2e6 draws of the thermal MC's re-emission frequency (the same function
+    run_packet and the MRW step call) at temperatures on and between table
+    rows, against kappa_nu dB_nu/dT dnu from pdspy's dB_nu formula in numpy,
+    mixed linearly between the two rows around T: chi^2 within 5 sigma of its
+    degrees of freedom."""
+    assert _reemit_pin(_two_dust_model()) < 5.0
+
+
+def test_reemission_pin_catches_planted_mistakes():
+    """The same pin fails when the oracle's re-emission spectrum is kappa B
+    instead of kappa dB/dT, and when the temperature row is off by one."""
+    m = _two_dust_model()
+    try:
+        for code in (1, 2):
+            oracle.set_mutation(code)
+            assert _reemit_pin(m) > 50.0, code
+    finally:
+        oracle.set_mutation(0)
+    assert _reemit_pin(m) < 5.0
+
+
+def test_mean_opacity_tables_match_numpy_quadrature():
+    """Planck-mean absorption, Rosseland-mean extinction and the emission per
+    gram q(T) = 4 pi sum kappa B dnu on the temperature grid, against numpy
+    quadrature of pdspy's B_nu / dB_nu on the same frequency weights; species
+    with different kabs are different absorption groups."""
+    m = _two_dust_model()
+    tabs = oracle.Tables(m)
+    assert list(tabs.grp) == [0, 1]
+    nu, dnu = _np_dnu(m["lam_um"])
+    for row in (150, 400, 620, 800, 990):
+        T = np.exp(tabs.lnt0 + row * tabs.dlnt)
+        B = _np_B_nu(nu, T)
+        dB = _np_dB_nu(nu, T)
+        for q in (0, 1):
+            ka = m["kabs"][q]
+            if B.sum() > 0:
+                assert tabs.kplanck[q, row] == pytest.approx((ka * B * dnu).sum() / (B * dnu).sum(),
+                                                             rel=1e-10)
+                assert tabs.lnq[q, row] == pytest.approx(np.log(4 * np.pi * (ka * B * dnu).sum()),
+                                                         rel=1e-12, abs=1e-9)
+        for s in (0, 1):
+            ke = m["kabs"][s] + m["ksca"][s]
+            if dB.sum() > 0:
+                want = (dB * dnu).sum() / (dB * dnu / ke).sum()
+                assert tabs.kross[s, row] == pytest.approx(want, rel=1e-10)
+    # identical absorption opacity: one group
+    m2 = _two_dust_model()
+    m2["kabs"] = np.array([m2["kabs"][0], m2["kabs"][0]])
+    assert list(oracle.Tables(m2).grp) == [0, 0]
