@@ -22,7 +22,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PDSPY_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of tests/sanitize)
+_LIB_PATH = os.environ.get("PDSPY_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 
 
 def build():
@@ -89,6 +91,7 @@ def _setup(L):
     L.or_image.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P]
     L.or_image_adjoint.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P, P, P]
     L.or_trace_ray.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P]
+    L.or_trace_ray_n.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P, P]
     L.or_scat_source.argtypes = [C.POINTER(_Model), P, P, P, C.c_double, C.c_uint64,
                                  C.c_uint64, C.c_uint64, C.c_int, P, C.POINTER(_ScatStats)]
     L.or_image_scat.argtypes = [C.POINTER(_Model), P, P, P, P, C.POINTER(_Camera), P]
@@ -246,18 +249,25 @@ def image_adjoint(m, T, kap, dimg, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0,
     return g_lr, g_T
 
 
-def trace_ray(m, o, d, maxseg=100000):
+def trace_ray(m, o, d, maxseg=100000, with_nudges=False):
+    """cells, segment lengths and hemispheres along a ray from o along d
+    (and, with_nudges, how often the ping-pong guard moved it and the number
+    of segments, which the walk caps at 1e7)"""
     pm = _Pinned(m)
     o = _f64(o)
     d = _f64(d)
     cell = np.zeros(maxseg, dtype=np.int32)
     ln = np.zeros(maxseg)
     hemi = np.zeros(maxseg, dtype=np.int32)
-    n = lib().or_trace_ray(C.byref(pm.s), o.ctypes.data, d.ctypes.data, maxseg,
-                           cell.ctypes.data, ln.ctypes.data, hemi.ctypes.data)
+    nud = C.c_int(0)
+    n = lib().or_trace_ray_n(C.byref(pm.s), o.ctypes.data, d.ctypes.data, maxseg,
+                             cell.ctypes.data, ln.ctypes.data, hemi.ctypes.data, C.byref(nud))
     if n < 0:
         raise RuntimeError("or_trace_ray failed: %d" % n)
-    return cell[:n], ln[:n], hemi[:n]
+    m_ = min(n, maxseg)
+    if with_nudges:
+        return cell[:m_], ln[:m_], hemi[:m_], nud.value, n
+    return cell[:m_], ln[:m_], hemi[:m_]
 
 
 def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0):
@@ -359,7 +369,7 @@ class Tables:
 def set_mutation(code):
     """Test only: 0 = none, 1 = re-emission spectrum kappa B instead of
     kappa dB/dT, 2 = temperature row shifted by one (applies to tables built
-    and draws made afterwards)."""
+    and draws made afterwards), 3 = ping-pong guard off."""
     lib().or_set_mutation(int(code))
 
 

@@ -134,6 +134,9 @@ int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
  * hole.  Returns the number of segments or a negative code.                 */
 int or_trace_ray(const or_model *m, const double *o, const double *d,
                  int maxseg, int *cell, double *len, int *hemi);
+/* the same, counting the ping-pong guard's moves in *nudged (nullable) */
+int or_trace_ray_n(const or_model *m, const double *o, const double *d, int maxseg,
+                   int *cell, double *len, int *hemi, int *nudged);
 
 /* Direct DFT: V(u,v) = sum_jk I[j][k] exp(-2 pi i (u x_k + v y_j)) with
  * x_k=(k-nx/2)dxy, y_j=(j-ny/2)dxy (rad), times exp(-2 pi i (u dra + v ddec)). */

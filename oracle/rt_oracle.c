@@ -42,6 +42,13 @@
 
 #define TALLY_SHIFT 28 /* fixed-point tally resolution: 2^-28 of kmax*dr */
 
+/* Deliberate mistakes a test plants to show that a pin catches them (0 =
+ * none; 1 = re-emission spectrum kappa B instead of kappa dB/dT; 2 =
+ * re-emission temperature row shifted up by one; 3 = ping-pong guard off).
+ * Never set outside such a test. */
+static int g_mutation = 0;
+void or_set_mutation(int code) { g_mutation = code; }
+
 /* ------------------------------------------------------------------ Philox */
 #define PH_M0 0xD2511F53u
 #define PH_M1 0xCD9E8D57u
@@ -481,6 +488,7 @@ static inline int cell_of(const geom_t *g, const pk_t *p) {
  * in) the packet moves 2^-36 r along its ray and is located again.  Returns 1
  * when it moved the packet. */
 static int unstick(const geom_t *g, pk_t *p, double tw, double rcell, int *nzero) {
+    if (g_mutation == 3) return 0;
     if (tw > rcell * 0x1p-44) {
         *nzero = 0;
         return 0;
@@ -495,8 +503,8 @@ static int unstick(const geom_t *g, pk_t *p, double tw, double rcell, int *nzero
     return 1;
 }
 
-int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg,
-                 int *cell, double *len, int *hemi) {
+int or_trace_ray_n(const or_model *m, const double *o, const double *d, int maxseg,
+                   int *cell, double *len, int *hemi, int *nudged) {
     geom_t g;
     int rc = geom_init(&g, m);
     if (rc) return rc;
@@ -540,11 +548,16 @@ int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg
         }
         n++;
         if (cross_wall(&g, &p, t, kind)) break;
-        unstick(&g, &p, t, rcell, &nzero);
+        if (unstick(&g, &p, t, rcell, &nzero) && nudged) ++*nudged;
         if (n > 10000000) break;
     }
     geom_free(&g);
     return n;
+}
+
+int or_trace_ray(const or_model *m, const double *o, const double *d, int maxseg,
+                 int *cell, double *len, int *hemi) {
+    return or_trace_ray_n(m, o, d, maxseg, cell, len, hemi, NULL);
 }
 
 /* ------------------------------------------------------------- tables */
@@ -604,12 +617,6 @@ static void cdf_from_log(const double *lw, int n, double *cdf) {
     for (int i = 0; i < n; ++i) cdf[i] /= s;
     cdf[n - 1] = 1.0;
 }
-
-/* Deliberate mistakes a test plants to show that the re-emission pins catch
- * them (0 = none; 1 = re-emission spectrum kappa B instead of kappa dB/dT;
- * 2 = temperature row shifted up by one).  Never set outside such a test. */
-static int g_mutation = 0;
-void or_set_mutation(int code) { g_mutation = code; }
 
 static int tab_init(tab_t *t, const or_model *m, const or_thermal_opts *o) {
     memset(t, 0, sizeof(*t));

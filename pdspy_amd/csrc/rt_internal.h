@@ -8,17 +8,14 @@
 
 #include "../../include/pdspy_amd.h"
 #include "rt_device.h"
+#include "tables.h"
 
 #define RT_MAX_SPEC 8    // species per model
-#define RT_MAX_GROUP 4   // distinct absorption opacities per model
 #define RT_MAX_LAM 255   // wavelength bins (guide tables are uint8)
 #define RT_MAX_STAR 16   // point stars per model
 #define RT_MAX_COMP 8    // density components of the device generator
-#define RT_NGUIDE 128    // guide-table entries per CDF row
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
-#define RT_MRW_NY 2048   // entries of the diffusion escape-law inverse table
-#define RT_MRW_LMAX 18.0f  // its logit range: ln(xi/(1-xi)) in [-LMAX, LMAX]
 
 struct rt_ctx {
     int device = 0;
@@ -50,24 +47,7 @@ enum {
     CNT_N = CNT_TK0 + 8
 };
 
-struct HostTables {
-    int ngroup = 0;
-    std::vector<int> grp;    // species -> group
-    std::vector<int> grep;   // group -> representative species
-    std::vector<double> nu, dnu;
-    std::vector<double> kmax;        // per group
-    std::vector<double> lnq;         // [G][ntemp]
-    std::vector<float> cdf;          // [G][ntemp][nlam]
-    std::vector<unsigned char> guide;  // [G][ntemp][NGUIDE]
-    std::vector<float> scdf;         // [nstar][nlam]
-    std::vector<unsigned char> sguide; // [nstar][NGUIDE]
-    std::vector<float> spick;        // [nstar]
-    std::vector<float> kross;        // [S][ntemp] Rosseland-mean extinction (MRW)
-    std::vector<float> mrw_y;        // [RT_MRW_NY] inverse of the escape law
-    std::vector<float> kplanck;      // [G][ntemp] Planck-mean absorption (MRW)
-    double lum = 0;
-    double lnt0 = 0, dlnt = 0;
-};
+
 
 struct rt_model {
     rt_ctx *ctx = nullptr;
@@ -159,7 +139,12 @@ struct ThermPlan {
     int lds_wave = 0;     // LDS floats per wavefront
 };
 
-int build_tables(rt_model *m, std::string &err);
+// the model's thermal-MC tables from its host opacities and stars
+inline int build_tables(rt_model *m, std::string &err) {
+    TableSpec sp{m->nlam, m->nspec, m->ntemp, m->nstar, m->temp0, m->temp1,
+                 m->lam.data(), m->kabs.data(), m->ksca.data(), m->stars.data()};
+    return build_host_tables(sp, m->tab, err);
+}
 int thermal_plan(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_opts *o,
                  ThermPlan &rp);
 // upload_est: also copy the epoch starts (a captured step has them uploaded once)

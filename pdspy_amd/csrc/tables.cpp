@@ -11,7 +11,10 @@
 #include <cmath>
 #include <cstring>
 
-#include "rt_internal.h"
+#include <algorithm>
+
+#include "../../include/pdspy_amd.h"
+#include "tables.h"
 
 namespace {
 
@@ -103,8 +106,8 @@ const std::vector<float> &mrw_escape_table() {
 
 }  // namespace
 
-int build_tables(rt_model *m, std::string &err) {
-    HostTables &t = m->tab;
+int build_host_tables(const TableSpec &sp, HostTables &t, std::string &err) {
+    const TableSpec *m = &sp;
     const int nl = m->nlam, ns = m->nspec, nT = m->ntemp;
     t.nu.resize(nl);
     t.dnu.resize(nl);

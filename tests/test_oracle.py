@@ -801,3 +801,39 @@ def test_mean_opacity_tables_match_numpy_quadrature():
     m2 = _two_dust_model()
     m2["kabs"] = np.array([m2["kabs"][0], m2["kabs"][0]])
     assert list(oracle.Tables(m2).grp) == [0, 0]
+
+
+# ---- ping-pong guard: rays that graze a sphere at FP64 resolution
+#      (found by a random search over tangent rays on the 64x32 disk+envelope
+#      grid; each makes the traversal bounce between two cells without moving)
+_GRAZING_RAYS = [
+    ([-325621223417906.8, 49386887234287.49, 58561204497278.32],
+     [-0.1493119313775693, -0.9887901431286116, -3.648638127115638e-15]),
+    ([142879425523101.4, 254365375019529.78, 163650054660660.66],
+     [-0.8718814457963383, 0.48971700447920624, 7.225896500954678e-15]),
+    ([-4725679007135499.0, -1204404995739213.8, 1112849858861335.4],
+     [0.24716839945556102, -0.9689725395028366, 3.8854976619789003e-16]),
+]
+
+
+def test_ping_pong_guard_moves_grazing_rays_on():
+    """Each grazing ray leaves the grid after a few dozen segments with the
+    guard's move counted; with the guard planted off the same ray bounces
+    until the walk's 1e7-segment cap.  The thermal MC and the camera use the
+    same traversal and guard."""
+    arr = oracle_arrays(configs.disk_envelope(nr=64, nt=32))
+    for o, d in _GRAZING_RAYS:
+        c, ln, h, nud, n = oracle.trace_ray(arr, o, d, maxseg=1000, with_nudges=True)
+        assert nud >= 1 and n < 1000
+        # it walks from its start to the outer sphere (it passes outside the
+        # inner hole): the chord's closed form
+        o_, d_ = np.array(o), np.array(d) / np.linalg.norm(d)
+        b = o_ @ d_
+        t_exit = -b + np.sqrt(b * b - (o_ @ o_ - arr["rw"][-1] ** 2))
+        assert ln.sum() == pytest.approx(t_exit, rel=1e-9)
+    try:
+        oracle.set_mutation(3)
+        c, ln, h, nud, n = oracle.trace_ray(arr, *_GRAZING_RAYS[0], maxseg=10, with_nudges=True)
+        assert nud == 0 and n > 9000000
+    finally:
+        oracle.set_mutation(0)
