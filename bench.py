@@ -7,8 +7,13 @@ with kappa_abs = 1e3 (lam/um)^-1 (albedo 0.5, isotropic), a 4000 K 2 Rsun star,
 100 wavelengths, modified random walk on (pdspy's fitting setting), nphot = 1e8
 packets per run, seeds 0-3.  One step = one thermal run (seed = step mod 4, so
 consecutive steps cover the four seeds) plus the copy of its per-species
-temperatures to the host (what run_thermal hands back).  A warm-up step is a
-plain run of another seed.  No timed step is started once it would end past
+temperatures to the host (what run_thermal hands back).  Steps run on
+--streams (default 2) host threads, each with its own context, HIP stream and
+copy of the model, step k on thread k mod 2: a run's waves leave its launch as
+soon as its packets are all handed out, so the next run's launch takes over
+the GPU while the last long random walks of the previous one finish (each
+run's temperatures are bit-identical to a run on its own).  A warm-up step is
+a run of another seed.  No timed step is started once it would end past
 --step-budget-s seconds after the process started; `steps` reports the steps
 actually timed.
 
@@ -65,6 +70,9 @@ def parse():
                         "seconds after the process started")
     p.add_argument("--side-budget-s", type=float, default=540.0,
                    help="the batched four-seed launch runs only if it ends before this")
+    p.add_argument("--streams", type=int, default=2,
+                   help="steps run on this many host threads / HIP streams, overlapping "
+                        "one run's last random walks with the next run (1 = back to back)")
     p.add_argument("--acc-nphot", type=float, default=2e5,
                    help="packets of the GPU-vs-oracle accuracy comparison")
     p.add_argument("--max-events", type=float, default=1e6)
@@ -321,41 +329,90 @@ def main():
     torch.cuda.synchronize()
     del src, dst
 
-    # warm-up: plain runs (the first also counts cell visits for the accuracy mask)
+    # steps run on `streams` host threads, each with its own context, stream and
+    # copy of the model: step k goes to lane k mod streams.  A single-run
+    # launch's waves leave once its packets are all handed out, so the next
+    # run (queued on the other stream) fills the GPU while the last random
+    # walks of the previous one finish.
+    from pdspy_amd import _lib
+    nl = max(1, int(a.streams))
+    lanes = [(dm, torch.cuda.current_stream())]
+    for _ in range(nl - 1):
+        cx = _lib.Context(0)
+        lanes.append((rt.DeviceModel(arr, ctx=cx), torch.cuda.Stream()))
+
+    def run_steps(nsteps, seed_of, budget, tag, first_visits=False):
+        """nsteps runs over the lanes; returns per step (stats, T, seconds),
+        None for a step not started because of the budget"""
+        import threading
+        out = [None] * nsteps
+        err = []
+
+        def worker(li):
+            dml, stream = lanes[li]
+            last = None
+            try:
+                with torch.cuda.stream(stream):
+                    for k in range(li, nsteps, nl):
+                        if budget and last is not None and elapsed() + 1.2 * last > budget:
+                            log("%s: lane %d stops before step %d (budget)" % (tag, li, k + 1))
+                            return
+                        ts = time.perf_counter()
+                        cv = 1 if (first_visits and k == 0) else 0
+                        st = dml.thermal(nphot, seed_of(k), count_visits=cv, **opts)
+                        res = dml.temperature(with_visits=bool(cv))
+                        last = time.perf_counter() - ts
+                        out[k] = (st, res, last)
+                        log("%s %d/%d (lane %d): seed %d, %.0f ms"
+                            % (tag, k + 1, nsteps, li, seed_of(k), last * 1e3))
+            except Exception as e:  # re-raised in the main thread
+                err.append(e)
+
+        th = [threading.Thread(target=worker, args=(li,)) for li in range(nl)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if err:
+            raise err[0]
+        return out
+
+    # warm-up: runs of other seeds (the first also counts cell visits for the
+    # accuracy mask)
     vis = None
-    for w in range(a.warmup):
-        st = dm.thermal(nphot, 1000 + w, count_visits=1 if w == 0 else 0, **opts)
-        if w == 0:
-            _, vis = dm.temperature(with_visits=True)
-        log("warm-up run %d/%d: %.0f ms" % (w + 1, a.warmup, st["kernel_ms"]))
+    wres = run_steps(a.warmup, lambda k: 1000 + k, None, "warm-up", first_visits=True)
+    if a.warmup > 0:
+        vis = wres[0][1][1]
     torch.cuda.synchronize()
 
     # timed steps: step k = one run of seed k mod 4 (the workload's seeds 0-3)
     # plus the copy of its per-species temperatures to the host; no step is
-    # started once the next one would end past the step budget
-    stats, temps = [], []
+    # started once it would end past the step budget
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    last = None
-    for k in range(a.steps):
-        if last is not None and elapsed() + 1.2 * last > a.step_budget_s:
-            log("step budget reached after %d of %d steps" % (k, a.steps))
-            break
-        ts = time.perf_counter()
-        st = dm.thermal(nphot, k % 4, **opts)
-        temps.append(dm.temperature())
-        stats.append(st)
-        last = time.perf_counter() - ts
-        log("step %d/%d: seed %d, %.0f ms" % (k + 1, a.steps, k % 4, last * 1e3))
+    res = run_steps(a.steps, lambda k: k % 4, a.step_budget_s, "step")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    steps = len(stats)
+    done = [r for r in res if r is not None]
+    stats = [r[0] for r in done]
+    temps = [r[1] for r in done]
+    steps = len(done)
+    seeds_done = [k % 4 for k, r in enumerate(res) if r is not None]
     value = nphot * steps / wall
+    last = max(r[2] for r in done)
 
-    # roofline of the MC kernel (one launch per step): algorithmic bytes
-    # B_alg = N_cross * 4 S + N_abs * 16 over the launch's HIP-event duration
+    # roofline of the MC kernel: algorithmic bytes B_alg = N_cross * 4 S +
+    # N_abs * 16 per launch.  With one lane the launches run back to back and
+    # the rate is B_alg over the launch's HIP-event duration; with several the
+    # launches overlap (a HIP-event duration then includes the time the launch
+    # shares the GPU with its neighbours), so the rate is the launches'
+    # algorithmic bytes over the timed region's wall time
     kern_ms = sum(st["kernel_ms"] for st in stats) / steps
     b_alg = sum(x["n_cross"] * 4 * S + x["n_abs"] * 16 for x in stats) / steps
-    achieved = b_alg / (kern_ms * 1e-3) / 1e9
+    if nl == 1:
+        achieved = b_alg / (kern_ms * 1e-3) / 1e9
+    else:
+        achieved = b_alg * steps / wall / 1e9
     traffic, traffic_src = None, None
     if os.path.exists(a.traffic_json):
         try:
@@ -372,7 +429,12 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
             "kernel": "k_thermal", "b_alg_per_launch": b_alg, "kernel_ms_per_launch": kern_ms,
-            "launches_per_step": 1, "timing": "HIP events on the launch's stream",
+            "launches_per_step": 1, "lanes": nl,
+            "timing": ("HIP events on the launch's stream" if nl == 1 else
+                       "launches overlap on %d streams: algorithmic bytes of the timed "
+                       "launches over the timed wall time; kernel_ms_per_launch is the "
+                       "HIP-event duration of a launch (it includes the time shared with "
+                       "the neighbouring launch)" % nl),
             "steps_per_s": lane_steps / (kern_ms * steps * 1e-3),
             "lane_utilisation": lane_steps / (64.0 * max(wave_iters, 1))}
 
@@ -391,8 +453,9 @@ def main():
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "disk+envelope 256x128x1 cells, 2 species, 100 lambda, "
                                "nphot=%g per run, one run per step (seed = step mod 4), "
-                               "MRW %s" % (nphot, "on" if a.mrw else "off"),
-                   "nphot": nphot, "seeds": sorted(set(k % 4 for k in range(steps))),
+                               "MRW %s, steps on %d streams" % (nphot, "on" if a.mrw else "off",
+                                                                nl),
+                   "nphot": nphot, "seeds": sorted(set(seeds_done)), "streams": nl,
                    "cells": dm.ncell, "species": S, "max_events": int(a.max_events),
                    "mrw": a.mrw, "steps_requested": a.steps},
         "roofline": roof,

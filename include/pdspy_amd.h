@@ -152,10 +152,18 @@ typedef struct {
 int rt_scattering_source(rt_model *m, double lam_um, uint64_t nphot, uint64_t seed,
                          const float *d_T, float *d_J, rt_scat_stats *stats);
 
+/* A context is one device and one stream; the models created on it run their
+ * work on that stream.  Contexts of one device are independent: two contexts
+ * on two streams, each with its own models, used from two host threads, run
+ * their thermal launches concurrently (a single-run launch's waves leave as
+ * soon as its packets are all handed out, so a launch queued on the other
+ * stream takes over their CUs). */
 int rt_create(int device, rt_ctx **out);
 int rt_destroy(rt_ctx *ctx);
 /* HIP stream (hipStream_t) for all later work; NULL = the null stream. */
 int rt_set_stream(rt_ctx *ctx, void *hip_stream);
+/* Message of the context's last error; a copy owned by the calling thread,
+ * valid until that thread's next call. */
 const char *rt_last_error(const rt_ctx *ctx);
 /* Number of CUs of the context's device and its wall-clock rate (kHz). */
 int rt_device_info(const rt_ctx *ctx, int *n_cu, int *wallclock_khz);

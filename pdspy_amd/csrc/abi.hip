@@ -322,7 +322,17 @@ int rt_set_stream(rt_ctx *ctx, void *s) {
     return RT_OK;
 }
 
-const char *rt_last_error(const rt_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+// the message is copied under the context's lock into a buffer of the calling
+// thread, so a message written meanwhile by another thread (opacity updates of
+// several models run on host threads) never tears the one returned; the
+// pointer stays valid until this thread's next call
+const char *rt_last_error(const rt_ctx *ctx) {
+    if (!ctx) return "no context";
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(const_cast<rt_ctx *>(ctx)->err_mu);
+    copy = ctx->err;
+    return copy.c_str();
+}
 
 int rt_device_info(const rt_ctx *ctx, int *n_cu, int *khz) {
     if (!ctx) return RT_E_ARG;

@@ -627,6 +627,15 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                                        A.parked, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0u))
                         break;
                     const bool helped = snapshot_help(run_at<BATCH>(runs, P1, scan), lane);
+                    // one run whose packets are all handed out: a wave without
+                    // packets and without snapshot work leaves the launch, so a
+                    // launch queued behind it (another run on another stream)
+                    // takes over its CU while the last walks finish here.  The
+                    // waves holding those walks stay and publish the remaining
+                    // snapshots, which no packet of this run reads any more.
+                    if (!BATCH && !helped &&
+                        uniform32((unsigned)ld_rlx64(P1.ep_cur)) >= (unsigned)P1.nepoch)
+                        break;
                     scan = scan + 1 == A.nrun ? 0 : scan + 1;
                     parked_from = -1;  // no other run has work: its own pool is fair game
                     n_idle++;

@@ -111,10 +111,14 @@ def dust_arrays(dust, lam_um):
     return kabs, ksca, gg, has_g
 
 
-def model_arrays(grid):
+def model_arrays(grid, hg=False):
     """pdspy Grid -> the flat FP64 arrays of rt_model_desc (units of
     write_radmc3d, Model.py:733-778: walls in cm/rad/rad, rho r-fastest,
-    opacities on the grid's wavelengths in micron)."""
+    opacities on the grid's wavelengths in micron).  pdspy writes the dust
+    opacity files with kabs and ksca only (Model.py:771-775), so RADMC-3D
+    scatters isotropically even for Dust objects that carry an asymmetry
+    parameter ``g``; the same holds here unless ``hg=True`` asks for
+    Henyey-Greenstein scattering with the Dust's g."""
     if getattr(grid, "coordsystem", "spherical") != "spherical":
         raise ValueError("only spherical grids are supported")
     lam = np.asarray(grid.lam, dtype=float)
@@ -126,14 +130,16 @@ def model_arrays(grid):
     return dict(rw=np.asarray(grid.w1, dtype=float) * AU, tw=np.asarray(grid.w2, dtype=float),
                 pw=np.asarray(grid.w3, dtype=float), rho=rho, lam_um=lam,
                 kabs=np.ascontiguousarray(kabs), ksca=np.ascontiguousarray(ksca),
-                gsca=np.ascontiguousarray(gg) if has_g else None, stars=stars)
+                gsca=np.ascontiguousarray(gg) if (has_g and hg) else None, stars=stars)
 
 
 class DeviceModel:
     """A model resident on the GPU (``rt_model``)."""
 
-    def __init__(self, arrays, device=0, ntemp=1000, temp0=0.01, temp1=1e5):
-        self.ctx = _lib.context(device)
+    def __init__(self, arrays, device=0, ntemp=1000, temp0=0.01, temp1=1e5, ctx=None):
+        # ctx: a Context of its own (its own stream, e.g. for a host thread that
+        # runs models concurrently with others); default: the device's shared one
+        self.ctx = ctx if ctx is not None else _lib.context(device)
         a = {k: (None if v is None else np.ascontiguousarray(v, dtype=float))
              for k, v in arrays.items()}
         self._keep = a
@@ -504,38 +510,61 @@ def lnlike_vis(data, model, device=0):
 
 
 # ---------------------------------------------------------------- pdspy mirror
-def _device_model(model, device=0, ntemp=1000, temp0=0.01, temp1=1e5):
+def _arrays_digest(arrays):
+    """digest of a model's input arrays (walls, densities, opacities, stars)"""
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for k in sorted(arrays):
+        v = arrays[k]
+        h.update(k.encode())
+        if v is None:
+            h.update(b"-")
+        else:
+            v = np.ascontiguousarray(v, dtype=float)
+            h.update(str(v.shape).encode())
+            h.update(v.tobytes())
+    return h.hexdigest()
+
+
+def _device_model(model, device=0, ntemp=1000, temp0=0.01, temp1=1e5, hg=False):
+    """The model's resident device copy.  pdspy rewrites every input file on
+    each run, so any change of walls, densities (also in place), dust or stars
+    is honoured: the inputs are gathered again and compared by digest, and the
+    device model is rebuilt when they differ."""
+    arrays = model_arrays(model.grid, hg=hg)
+    key = (_arrays_digest(arrays), ntemp, temp0, temp1, device)
     dm = getattr(model, "_pdspy_amd_dev", None)
-    key = (id(model.grid), tuple(id(d) for d in model.grid.density), ntemp, temp0, temp1, device)
     if dm is None or dm[0] != key:
-        dev = DeviceModel(model_arrays(model.grid), device=device, ntemp=ntemp, temp0=temp0,
-                          temp1=temp1)
+        dev = DeviceModel(arrays, device=device, ntemp=ntemp, temp0=temp0, temp1=temp1)
         model._pdspy_amd_dev = (key, dev)
         return dev
     return dm[1]
 
 
 def run_thermal(model, nphot=1e6, seed=None, device=0, max_events=None, epoch_first=None,
-                epoch_growth=None, count_visits=False, return_stats=False, **keywords):
+                epoch_growth=None, count_visits=False, return_stats=False, timelimit=7200,
+                hg=False, **keywords):
     """Model.run_thermal(code="mi355x"): fills ``model.grid.temperature``.
 
     Accepts the RADMC-3D keyword set pdspy forwards (``iseed`` is the seed,
-    ``modified_random_walk``/``mrw_*`` select MRW, ``timelimit`` bounds the
-    kernel time, ``ntemp``/``temp0``/``temp1`` set the emission tables);
-    thread, niceness and verbosity keywords are accepted and ignored.
+    ``modified_random_walk``/``mrw_*`` select MRW, ``ntemp``/``temp0``/``temp1``
+    set the emission tables); ``timelimit`` (seconds, pdspy's default 7200,
+    Model.py:221) bounds the kernel time; thread, niceness and verbosity
+    keywords are accepted and ignored.  ``hg=True`` (no pdspy counterpart)
+    scatters with the Dust objects' Henyey-Greenstein g.
     """
     _check_keywords(keywords)
     if seed is None:
         seed = keywords.get("iseed", 0) or 0
     dm = _device_model(model, device, keywords.get("ntemp") or 1000,
-                       keywords.get("temp0") or 0.01, keywords.get("temp1") or 1e5)
+                       keywords.get("temp0") or 0.01, keywords.get("temp1") or 1e5, hg=hg)
     opts = dict(max_events=max_events, epoch_first=epoch_first, epoch_growth=epoch_growth,
                 count_visits=1 if count_visits else 0,
                 mrw=1 if keywords.get("modified_random_walk") else 0,
                 mrw_tauthres=keywords.get("mrw_tauthres"),
                 mrw_count_trigger=keywords.get("mrw_count_trigger"),
                 mrw_gamma=keywords.get("mrw_gamma"),
-                max_seconds=keywords.get("timelimit"))
+                max_seconds=float(timelimit) if timelimit else None)
     stats = dm.thermal(int(nphot), int(seed), **opts)
     T, vis = dm.temperature(with_visits=True)
     nr, nt, np_ = dm.nr, dm.nt, dm.np

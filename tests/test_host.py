@@ -196,3 +196,35 @@ def test_dust_arrays_match_model_arrays_resampling():
     assert sel.sum() >= 2
     assert np.allclose(k[sel], d.kabs[np.isin(np.round(np.log10(d.lam * 1e4), 9),
                                               np.round(np.log10(a["lam_um"]), 9))], rtol=1e-10)
+
+
+def test_dust_with_g_scatters_isotropically_unless_asked():
+    """pdspy writes dustkappa files with kabs and ksca only (Model.py:771-775),
+    so a Dust carrying g (pdspy's PAH class sets one) scatters isotropically in
+    RADMC-3D; model_arrays does the same unless hg=True."""
+    m = configs.spiral_3d(nr=6, nt=4, nphi=4)
+    assert getattr(m.grid.dust[0], "g", None) is not None
+    assert rt.model_arrays(m.grid)["gsca"] is None
+    g = rt.model_arrays(m.grid, hg=True)["gsca"]
+    assert g is not None and np.allclose(g, 0.6)
+
+
+def test_device_model_cache_key_follows_every_input():
+    """The resident device model is reused only while walls, densities (also
+    edited in place), dust and stars are unchanged."""
+    m = configs.disk_envelope(nr=8, nt=6)
+    d0 = rt._arrays_digest(rt.model_arrays(m.grid))
+    assert rt._arrays_digest(rt.model_arrays(m.grid)) == d0
+    m.grid.density[0][3, 2, 0] *= 1.5                 # in place
+    d1 = rt._arrays_digest(rt.model_arrays(m.grid))
+    assert d1 != d0
+    m.grid.dust[1].kabs = m.grid.dust[1].kabs * 1.01  # another dust
+    d2 = rt._arrays_digest(rt.model_arrays(m.grid))
+    assert d2 != d1
+    m.grid.stars[0].temperature = 4100.0
+    assert rt._arrays_digest(rt.model_arrays(m.grid)) != d2
+
+
+def test_run_thermal_time_limit_default_is_pdspys():
+    import inspect
+    assert inspect.signature(rt.run_thermal).parameters["timelimit"].default == 7200
