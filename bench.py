@@ -53,6 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STAGGER_S = 0.1  # least time between two thermal launches of the overlapped steps
 CALIB_BYTES = 256 << 20  # device copy whose FETCH_SIZE / WRITE_SIZE scale the counters
 
 
@@ -341,10 +342,24 @@ def main():
         cx = _lib.Context(0)
         lanes.append((rt.DeviceModel(arr, ctx=cx), torch.cuda.Stream()))
 
+    import threading
+    launch_mu = threading.Lock()
+    last_launch = [-1.0]
+
+    def stagger():
+        """A run is launched at least STAGGER_S after the previous one, so its
+        blocks queue behind that launch's resident ones instead of being
+        dispatched alongside them: it then starts on the CUs the previous
+        run's waves leave once its packets are handed out."""
+        with launch_mu:
+            wait = last_launch[0] + STAGGER_S - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            last_launch[0] = time.perf_counter()
+
     def run_steps(nsteps, seed_of, budget, tag, first_visits=False):
         """nsteps runs over the lanes; returns per step (stats, T, seconds),
         None for a step not started because of the budget"""
-        import threading
         out = [None] * nsteps
         err = []
 
@@ -357,6 +372,7 @@ def main():
                         if budget and last is not None and elapsed() + 1.2 * last > budget:
                             log("%s: lane %d stops before step %d (budget)" % (tag, li, k + 1))
                             return
+                        stagger()
                         ts = time.perf_counter()
                         cv = 1 if (first_visits and k == 0) else 0
                         st = dml.thermal(nphot, seed_of(k), count_visits=cv, **opts)
@@ -371,6 +387,7 @@ def main():
         th = [threading.Thread(target=worker, args=(li,)) for li in range(nl)]
         for t in th:
             t.start()
+            time.sleep(0.01)  # lane 0 launches first
         for t in th:
             t.join()
         if err:

@@ -135,6 +135,17 @@ typedef struct {
                              (isotropic scattering).  NULL: emission only, as
                              RADMC-3D with scattering_mode_max = 0.  The
                              adjoint holds J fixed.                           */
+    int nrefine;          /* recursive sub-pixeling, RADMC-3D's camera_nrrefine
+                             (write.py:57-58): at most this many levels, capped
+                             at 24; 0 = off (one ray per pixel / sub-pixel)   */
+    double refine_crit;   /* camera_refine_criterion (write.py:59-61; 0 = 1.0):
+                             a (sub)pixel of side d is split into 2 x 2 when its
+                             central ray crosses a cell of size l with
+                             d > refine_crit * l, l = min(dr, r_c dtheta,
+                             r_c sin(theta_c) dphi) at the cell centre; the
+                             pixel is the mean of its sub-pixels.  Sub-pixels
+                             live in a node pool of the context, grown as
+                             needed by calls outside a graph capture          */
 } rt_camera;
 
 /* Scattering source (RADMC-3D's monochromatic Monte Carlo before an image,
@@ -167,6 +178,15 @@ int rt_set_stream(rt_ctx *ctx, void *hip_stream);
 const char *rt_last_error(const rt_ctx *ctx);
 /* Number of CUs of the context's device and its wall-clock rate (kHz). */
 int rt_device_info(const rt_ctx *ctx, int *n_cu, int *wallclock_khz);
+
+/* Rays traced by the context's last sub-pixelled image (all levels) and
+ * sub-pixels left unsplit for want of pool space since the last call of this
+ * function (a captured graph cannot grow the pool; eager calls grow it and
+ * retrace, so for them this stays 0).  Synchronises the context's stream. */
+int rt_refine_stats(rt_ctx *ctx, uint64_t *nrays, uint64_t *unsplit);
+/* Grows the context's sub-pixel pool to at least n nodes (n > 0); for graph
+ * captures that refine.  Synchronous. */
+int rt_refine_reserve(rt_ctx *ctx, uint64_t n);
 
 /* Uploads grid, densities and opacities, builds the emission tables.
  * Replaces write_radmc3d's input files (Model.py:723-778). */

@@ -81,7 +81,8 @@ class _Camera(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsub", C.c_int),
                 ("pix_cm", C.c_double), ("incl", C.c_double), ("posang", C.c_double),
                 ("phi", C.c_double), ("lam_um", C.c_double), ("mode", C.c_int),
-                ("incl_star", C.c_int), ("tau_stop", C.c_double)]
+                ("incl_star", C.c_int), ("tau_stop", C.c_double), ("nrefine", C.c_int),
+                ("refine_crit", C.c_double)]
 
 
 def _setup(L):
@@ -89,6 +90,8 @@ def _setup(L):
     L.or_thermal.argtypes = [C.POINTER(_Model), C.POINTER(_ThermOpts), P, P,
                              C.POINTER(_ThermStats)]
     L.or_image.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P]
+    L.or_image_last_nrays.argtypes = []
+    L.or_image_last_nrays.restype = C.c_long
     L.or_image_adjoint.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P, P, P]
     L.or_trace_ray.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P]
     L.or_trace_ray_n.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P, P]
@@ -184,10 +187,10 @@ def thermal(m, nphot, seed=0, ntemp=1000, temp0=0.01, temp1=1e5, max_events=0,
 
 
 def _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop,
-            ny=None):
+            ny=None, nrefine=0, refine_crit=1.0):
     return _Camera(int(npix), int(ny if ny is not None else npix), int(nsub), float(pix_cm),
                    float(incl), float(posang), float(phi), float(lam_um), int(mode),
-                   int(incl_star), float(tau_stop))
+                   int(incl_star), float(tau_stop), int(nrefine), float(refine_crit))
 
 
 def scat_source(m, T, kap_abs, kap_sca, lam_um, nphot, seed=0, max_events=1000000,
@@ -209,13 +212,17 @@ def scat_source(m, T, kap_abs, kap_sca, lam_um, nphot, seed=0, max_events=100000
 
 
 def image(m, T, kap, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0, lam_um=1300.0,
-          mode=0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None):
+          mode=0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None, nrefine=0,
+          refine_crit=1.0, with_nrays=False):
     """Returns img[ny][nx] (erg/s/cm^2/Hz/sr, or tau for mode=1).  With ksca
-    and J (from scat_source): isotropic scattering source."""
+    and J (from scat_source): isotropic scattering source.  nrefine > 0:
+    recursive sub-pixeling (RADMC-3D camera_nrrefine / camera_refine_criterion);
+    with_nrays: also the number of rays traced."""
     pm = _Pinned(m)
     T = _f64(T).reshape(-1)
     kap = _f64(kap).reshape(-1)
-    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop)
+    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, mode, incl_star, nsub, tau_stop,
+                  nrefine=nrefine, refine_crit=refine_crit)
     img = np.zeros((cam.ny, cam.nx))
     ks = None if ksca is None else _f64(ksca).reshape(-1)
     Jv = None if J is None else _f64(J).reshape(-1)
@@ -225,16 +232,20 @@ def image(m, T, kap, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0, lam_um=1300.0,
                              img.ctypes.data)
     if rc != 0:
         raise RuntimeError("or_image failed: %d" % rc)
+    if with_nrays:
+        return img, int(lib().or_image_last_nrays())
     return img
 
 
 def image_adjoint(m, T, kap, dimg, npix, pix_cm, incl=0.0, posang=0.0, phi=0.0,
-                  lam_um=1300.0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None):
+                  lam_um=1300.0, incl_star=False, nsub=1, tau_stop=40.0, ksca=None, J=None,
+                  nrefine=0, refine_crit=1.0):
     pm = _Pinned(m)
     T = _f64(T).reshape(-1)
     kap = _f64(kap).reshape(-1)
     dimg = _f64(dimg)
-    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, 0, incl_star, nsub, tau_stop)
+    cam = _camera(npix, pix_cm, incl, posang, phi, lam_um, 0, incl_star, nsub, tau_stop,
+                  nrefine=nrefine, refine_crit=refine_crit)
     g_lr = np.zeros((pm.nspec, pm.ncell))
     g_T = np.zeros(pm.ncell)
     ks = None if ksca is None else _f64(ksca).reshape(-1)

@@ -109,10 +109,21 @@ typedef struct {
     int mode;
     int incl_star;
     double tau_stop;
+    /* recursive sub-pixeling (RADMC-3D camera_nrrefine /
+     * camera_refine_criterion): a (sub)pixel of side d whose central ray
+     * crosses a cell of size l with d > crit l is replaced by the mean of its
+     * 2x2 sub-pixels, at most nrefine levels deep (0 = off); a smaller crit
+     * refines further.  Cell size: the
+     * least of dr, r_c dtheta and (3D) r_c sin(theta_c) dphi, r_c and theta_c
+     * the cell's centre.                                                     */
+    int nrefine;
+    double refine_crit;
 } or_camera;
 
 int or_image(const or_model *m, const double *T, const double *kap,
              const or_camera *cam, double *img);
+/* rays traced by the last or_image / or_image_scat call (sub-pixels included) */
+long or_image_last_nrays(void);
 
 /* Reverse-mode gradient of L = sum(dimg * img) w.r.t. log(rho[s][c]) and
  * T[c], FP64.  The star term is included when cam->incl_star.              */

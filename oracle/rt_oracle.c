@@ -1691,10 +1691,38 @@ typedef struct {
 
 /* Integrate one ray entering from the observer side.  Collects segments when
  * segs != NULL.  Returns intensity (mode 0) or total tau (mode 1).         */
+/* size of cell (ir, it, ip) for the camera's sub-pixeling: the least of its
+ * radial width, r_c dtheta and (with phi cells) r_c sin(theta_c) dphi */
+static double cell_size(const geom_t *g, int ir, int it, int ip) {
+    double rc = 0.5 * (g->rw[ir] + g->rw[ir + 1]);
+    double l = g->rw[ir + 1] - g->rw[ir];
+    double lt = rc * (g->tw[it + 1] - g->tw[it]);
+    if (lt < l) l = lt;
+    if (g->np > 1) {
+        double lp = rc * sin(0.5 * (g->tw[it] + g->tw[it + 1])) * (g->pw[ip + 1] - g->pw[ip]);
+        if (lp < l) l = lp;
+    }
+    return l;
+}
+
+static double trace_cam_ray_l(const or_model *m, const geom_t *g, const double *T,
+                              const double *kap, const double *ksca, const double *Jsc,
+                              const or_camera *cam, const cam_frame_t *f, double X, double Y,
+                              seg_t *segs, int maxseg, int *nseg, double *lmin);
+
 static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
                             const double *kap, const double *ksca, const double *Jsc,
                             const or_camera *cam, const cam_frame_t *f, double X, double Y,
                             seg_t *segs, int maxseg, int *nseg) {
+    return trace_cam_ray_l(m, g, T, kap, ksca, Jsc, cam, f, X, Y, segs, maxseg, nseg, NULL);
+}
+
+/* the same, also returning the smallest cell size along the traced part of
+ * the ray in *lmin (INF_D when it crosses no cell) */
+static double trace_cam_ray_l(const or_model *m, const geom_t *g, const double *T,
+                              const double *kap, const double *ksca, const double *Jsc,
+                              const or_camera *cam, const cam_frame_t *f, double X, double Y,
+                              seg_t *segs, int maxseg, int *nseg, double *lmin) {
     /* sky (X,Y) -> model image plane (x', y') rotated by -posang */
     double xp = X * f->cpa + Y * f->spa;
     double yp = -X * f->spa + Y * f->cpa;
@@ -1705,6 +1733,7 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
         d[a] = -f->n[a];
     }
     if (nseg) *nseg = 0;
+    if (lmin) *lmin = INF_D;
     double rr = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
     double b = o[0] * d[0] + o[1] * d[1] + o[2] * d[2];
     double D = b * b - (rr - R * R);
@@ -1731,6 +1760,10 @@ static double trace_cam_ray(const or_model *m, const geom_t *g, const double *T,
         double rcell = g->rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_of(g, &p);
+            if (lmin) {
+                double l = cell_size(g, p.ir, p.it, p.ip);
+                if (l < *lmin) *lmin = l;
+            }
             /* extinction and source function; with a scattering source
              * (isotropic): S = (rho kabs B + rho ksca J) / (rho (kabs + ksca)) */
             double aa = 0, as = 0;
@@ -1815,6 +1848,31 @@ static int star_pixel(const or_camera *cam, const cam_frame_t *f, const double *
     return 1;
 }
 
+/* Value of the (sub)pixel of side d centred at sky (X, Y), level lev of the
+ * recursive sub-pixeling: the central ray, or, when that ray crosses a cell of
+ * size l with d > crit l and lev < nrefine, the mean of the 2x2 sub-pixels
+ * taken in the order (-,-), (+,-), (-,+), (+,+) in (X, Y).  *nrays counts
+ * the rays traced.                                                          */
+static double subpixel_value(const or_model *m, const geom_t *g, const double *T,
+                             const double *kap, const double *ksca, const double *Jsc,
+                             const or_camera *cam, const cam_frame_t *f, double X, double Y,
+                             double d, int lev, long *nrays) {
+    double lmin;
+    double I = trace_cam_ray_l(m, g, T, kap, ksca, Jsc, cam, f, X, Y, NULL, 0, NULL, &lmin);
+    if (nrays) ++*nrays;
+    if (lev >= cam->nrefine || !(d > cam->refine_crit * lmin)) return I;
+    double h = 0.25 * d, acc = 0;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            acc += subpixel_value(m, g, T, kap, ksca, Jsc, cam, f, X + (b ? h : -h),
+                                  Y + (a ? h : -h), 0.5 * d, lev + 1, nrays);
+    return 0.25 * acc;
+}
+
+/* rays traced by the last or_image / or_image_scat call */
+static long g_last_nrays = 0;
+long or_image_last_nrays(void) { return g_last_nrays; }
+
 int or_image(const or_model *m, const double *T, const double *kap,
              const or_camera *cam, double *img) {
     return or_image_scat(m, T, kap, NULL, NULL, cam, img);
@@ -1828,7 +1886,9 @@ int or_image_scat(const or_model *m, const double *T, const double *kap,
     cam_frame_t f;
     cam_frame(cam, &f);
     int nsub = cam->nsub > 0 ? cam->nsub : 1;
-#pragma omp parallel for schedule(dynamic, 4)
+    const double dsub = cam->pix_cm / nsub;
+    long nrays = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nrays)
     for (int j = 0; j < cam->ny; ++j)
         for (int k = 0; k < cam->nx; ++k) {
             double acc = 0;
@@ -1836,11 +1896,12 @@ int or_image_scat(const or_model *m, const double *T, const double *kap,
                 for (int b = 0; b < nsub; ++b) {
                     double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
                     double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
-                    acc += trace_cam_ray(m, &g, T, kap, Jsc ? ksca : NULL, Jsc, cam, &f, X, Y,
-                                         NULL, 0, NULL);
+                    acc += subpixel_value(m, &g, T, kap, Jsc ? ksca : NULL, Jsc, cam, &f, X, Y,
+                                          dsub, 0, &nrays);
                 }
             img[(size_t)j * cam->nx + k] = acc / (nsub * nsub);
         }
+    g_last_nrays = nrays;
     if (cam->incl_star && cam->mode == 0) {
         double nu = C_C / (cam->lam_um * 1e-4);
         for (int s = 0; s < m->nstar; ++s) {
@@ -1869,6 +1930,72 @@ int or_image_adjoint(const or_model *m, const double *T, const double *kap,
  *               + w (rho_s kabs_s B + rho_s ksca_s J - S rho_s (kabs_s + ksca_s)) / a
  * and dI/dT = w (sum rho kabs / a) dB/dT, w = e^-tau0 (1 - e^-dtau).  Without
  * J this reduces to the emission-only gradient.                            */
+/* adds gI * dI/dlog rho and gI * dI/dT of the ray through sky (X, Y) */
+static int adjoint_ray(const or_model *m, const geom_t *g, const double *T, const double *kap,
+                       const double *ksca, const double *Jsc, const or_camera *cam,
+                       const cam_frame_t *f, double X, double Y, double gI, seg_t *segs,
+                       int maxseg, double *g_logrho, double *g_T) {
+    const int nc = g->ncell, ns = m->nspec;
+    const double nu = C_C / (cam->lam_um * 1e-4);
+    int n;
+    double I = trace_cam_ray(m, g, T, kap, Jsc ? ksca : NULL, Jsc, cam, f, X, Y, segs, maxseg, &n);
+    if (n > maxseg) return -30;
+    double cum = 0;
+    for (int q = 0; q < n; ++q) {
+        seg_t *sg = &segs[q];
+        double att = exp(-sg->tau0);
+        double contrib = att * (-expm1(-sg->dtau)) * sg->S;
+        cum += contrib;
+        double Rk = I - cum;
+        double dI_ddtau = att * exp(-sg->dtau) * sg->S - Rk;
+        double w = att * (-expm1(-sg->dtau));
+        double a = 0, aa = 0;
+        for (int s = 0; s < ns; ++s) {
+            double rho = m->rho[(size_t)s * nc + sg->cell];
+            aa += rho * kap[s];
+            a += rho * (kap[s] + (Jsc ? ksca[s] : 0.0));
+        }
+        double B = or_planck(nu, T[sg->cell]);
+        for (int s = 0; s < ns; ++s) {
+            double rho = m->rho[(size_t)s * nc + sg->cell];
+            double ks = Jsc ? ksca[s] : 0.0;
+            double gS = Jsc && a > 0 ? (rho * kap[s] * B + rho * ks * Jsc[sg->cell] -
+                                        sg->S * rho * (kap[s] + ks)) / a
+                                     : 0.0;
+            g_logrho[(size_t)s * nc + sg->cell] +=
+                gI * (dI_ddtau * sg->s * rho * (kap[s] + ks) + w * gS);
+        }
+        double fa = Jsc ? (a > 0 ? aa / a : 0.0) : 1.0;
+        g_T[sg->cell] += gI * w * fa * or_dplanck_dT(nu, T[sg->cell]);
+    }
+    return 0;
+}
+
+/* the adjoint of subpixel_value: the same refinement decisions, each leaf
+ * ray weighted by gI / 4^(its depth) */
+static int subpixel_adjoint(const or_model *m, const geom_t *g, const double *T,
+                            const double *kap, const double *ksca, const double *Jsc,
+                            const or_camera *cam, const cam_frame_t *f, double X, double Y,
+                            double d, int lev, double gI, seg_t *segs, int maxseg,
+                            double *g_logrho, double *g_T) {
+    if (cam->nrefine > lev) {
+        double lmin;
+        trace_cam_ray_l(m, g, T, kap, Jsc ? ksca : NULL, Jsc, cam, f, X, Y, NULL, 0, NULL, &lmin);
+        if (d > cam->refine_crit * lmin) {
+            double h = 0.25 * d;
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    int rc = subpixel_adjoint(m, g, T, kap, ksca, Jsc, cam, f, X + (b ? h : -h),
+                                              Y + (a ? h : -h), 0.5 * d, lev + 1, 0.25 * gI,
+                                              segs, maxseg, g_logrho, g_T);
+                    if (rc) return rc;
+                }
+            return 0;
+        }
+    }
+    return adjoint_ray(m, g, T, kap, ksca, Jsc, cam, f, X, Y, gI, segs, maxseg, g_logrho, g_T);
+}
+
 int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
                           const double *ksca, const double *Jsc, const or_camera *cam,
                           const double *dimg, double *g_logrho, double *g_T) {
@@ -1881,6 +2008,7 @@ int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
     memset(g_logrho, 0, sizeof(double) * ns * nc);
     memset(g_T, 0, sizeof(double) * nc);
     int nsub = cam->nsub > 0 ? cam->nsub : 1;
+    const double dsub = cam->pix_cm / nsub;
     int maxseg = 4 * (g.nr + g.nt + g.np) + 64;
     seg_t *segs = malloc(sizeof(seg_t) * maxseg);
     double nu = C_C / (cam->lam_um * 1e-4);
@@ -1892,42 +2020,12 @@ int or_image_adjoint_scat(const or_model *m, const double *T, const double *kap,
                 for (int b = 0; b < nsub; ++b) {
                     double X = (k - cam->nx / 2 + ((b + 0.5) / nsub - 0.5)) * cam->pix_cm;
                     double Y = (j - cam->ny / 2 + ((a + 0.5) / nsub - 0.5)) * cam->pix_cm;
-                    int n;
-                    double I = trace_cam_ray(m, &g, T, kap, Jsc ? ksca : NULL, Jsc, cam, &f, X,
-                                             Y, segs, maxseg, &n);
-                    if (n > maxseg) {
+                    rc = subpixel_adjoint(m, &g, T, kap, ksca, Jsc, cam, &f, X, Y, dsub, 0, gI,
+                                          segs, maxseg, g_logrho, g_T);
+                    if (rc) {
                         free(segs);
                         geom_free(&g);
-                        return -30;
-                    }
-                    double cum = 0;
-                    for (int q = 0; q < n; ++q) {
-                        seg_t *sg = &segs[q];
-                        double att = exp(-sg->tau0);
-                        double contrib = att * (-expm1(-sg->dtau)) * sg->S;
-                        cum += contrib;
-                        double Rk = I - cum;
-                        double dI_ddtau = att * exp(-sg->dtau) * sg->S - Rk;
-                        double w = att * (-expm1(-sg->dtau));
-                        double a = 0, aa = 0;
-                        for (int s = 0; s < ns; ++s) {
-                            double rho = m->rho[(size_t)s * nc + sg->cell];
-                            aa += rho * kap[s];
-                            a += rho * (kap[s] + (Jsc ? ksca[s] : 0.0));
-                        }
-                        double B = or_planck(nu, T[sg->cell]);
-                        for (int s = 0; s < ns; ++s) {
-                            double rho = m->rho[(size_t)s * nc + sg->cell];
-                            double ks = Jsc ? ksca[s] : 0.0;
-                            double gS = Jsc && a > 0
-                                            ? (rho * kap[s] * B + rho * ks * Jsc[sg->cell] -
-                                               sg->S * rho * (kap[s] + ks)) / a
-                                            : 0.0;
-                            g_logrho[(size_t)s * nc + sg->cell] +=
-                                gI * (dI_ddtau * sg->s * rho * (kap[s] + ks) + w * gS);
-                        }
-                        double fa = Jsc ? (a > 0 ? aa / a : 0.0) : 1.0;
-                        g_T[sg->cell] += gI * w * fa * or_dplanck_dT(nu, T[sg->cell]);
+                        return rc;
                     }
                 }
         }

@@ -21,6 +21,7 @@ EXPORTS = [
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_vis_grid",
     "rt_vis_grid_workspace", "rt_interpolate_galario", "rt_chisq",
     "rt_batch_create", "rt_batch_destroy", "rt_batch_thermal", "rt_batch_step",
+    "rt_refine_stats", "rt_refine_reserve",
 ]
 
 RT_ERRORS = {-1: "RT_E_ARG", -2: "RT_E_HIP", -3: "RT_E_NOMEM", -4: "RT_E_WATCHDOG",
@@ -84,7 +85,7 @@ class Camera(C.Structure):
                 ("incl_deg", C.c_double), ("posang_deg", C.c_double), ("phi_deg", C.c_double),
                 ("lam_um", C.c_double), ("mode", C.c_int), ("incl_star", C.c_int),
                 ("tau_stop", C.c_double), ("tile_w", C.c_int), ("tile_h", C.c_int),
-                ("d_jscat", C.c_void_p)]
+                ("d_jscat", C.c_void_p), ("nrefine", C.c_int), ("refine_crit", C.c_double)]
 
 
 class ScatStats(C.Structure):
@@ -138,6 +139,8 @@ def load():
         "rt_vis_dft_adjoint": ([P, P, P, I, I, D, P, P, I, D, D, P], I),
         "rt_vis_grid": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_vis_grid_workspace": ([I, I], C.c_size_t),
+        "rt_refine_stats": ([P, C.POINTER(U64), C.POINTER(U64)], I),
+        "rt_refine_reserve": ([P, U64], I),
         "rt_interpolate_galario": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_chisq": ([P, P, P, P, P, P, I, P, P, P], I),
         "rt_batch_create": ([P, P, I, C.POINTER(P)], I),

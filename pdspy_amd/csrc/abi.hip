@@ -66,6 +66,7 @@ static void free_model(rt_model *m) {
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
+                    m->d_csize,
                     m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
                     m->d_dens_fac, m->d_comps, m->d_vol, m->d_jtally, m->d_scum, m->d_scnt, m->d_sblk};
     for (void *p : ptrs)
@@ -311,9 +312,34 @@ int rt_create(int device, rt_ctx **out) {
     return RT_OK;
 }
 
+rt_ctx::~rt_ctx() {
+    void *p[] = {ref.xy, ref.val, ref.child, ref.gw, ref.ctl};
+    for (void *q : p)
+        if (q) (void)hipFree(q);
+}
+
 int rt_destroy(rt_ctx *ctx) {
     delete ctx;
     return RT_OK;
+}
+
+int rt_refine_stats(rt_ctx *ctx, uint64_t *nrays, uint64_t *unsplit) {
+    if (!ctx) return RT_E_ARG;
+    unsigned c[RT_REF_NCTL] = {0};
+    if (ctx->ref.ctl) {
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipMemcpy(c, ctx->ref.ctl, sizeof(c), hipMemcpyDeviceToHost));
+        unsigned z = 0;
+        HIPCHK(ctx, hipMemcpy(ctx->ref.ctl + RT_REF_UNSPLIT, &z, sizeof(z), hipMemcpyHostToDevice));
+    }
+    if (nrays) *nrays = c[RT_REF_RAYS];
+    if (unsplit) *unsplit = c[RT_REF_UNSPLIT];
+    return RT_OK;
+}
+
+int rt_refine_reserve(rt_ctx *ctx, uint64_t n) {
+    if (!ctx || n == 0) return RT_E_ARG;
+    return refine_reserve(ctx, (size_t)n);
 }
 
 int rt_set_stream(rt_ctx *ctx, void *s) {

@@ -285,6 +285,13 @@ static int batch_step_buffers(rt_batch *b, int i, const rt_step_item &it) {
         if (!b->d_J[i]) HIPCHK(ctx, hipMalloc(&b->d_J[i], sizeof(float) * b->models[i]->ncell));
     }
     const int npix = it.cam.nx * it.cam.ny;
+    if (it.cam.nrefine > 0) {
+        // sub-pixel pool for the captured image chain: room for the pixels'
+        // rays and 15 times as many sub-pixels
+        rc = refine_reserve(ctx, std::max(ctx->ref.cap, (size_t)16 * (size_t)(npix + 256) *
+                                                            (size_t)std::max(1, it.cam.nsub * it.cam.nsub)));
+        if (rc) return rc;
+    }
     if (b->npix_cap[i] < npix) {
         (void)hipFree(b->d_imgs[i]);
         (void)hipFree(b->d_gimg[i]);
@@ -354,7 +361,11 @@ static std::vector<const void *> model_buffers(const rt_batch *b) {
         v.insert(v.end(), p, p + sizeof(p) / sizeof(p[0]));
         // the image chains capture opacities at the camera wavelength by value
         v.push_back((const void *)(uintptr_t)m->opac_version);
+        v.push_back(m->d_csize);
     }
+    const RefinePool &P = b->ctx->ref;
+    const void *q[] = {P.xy, P.val, P.child, P.gw, P.ctl};
+    v.insert(v.end(), q, q + 5);
     return v;
 }
 

@@ -16,6 +16,7 @@
 // e^-dtau_k S_k - (I - I_<=k)) instead of storing tau, and accumulates
 // dL/dalpha, dL/dS per cell with FP64 atomics; a post pass applies the chain
 // rule to log(rho_s) and T.
+#include <algorithm>
 #include <cmath>
 
 #include "rt_internal.h"
@@ -33,6 +34,7 @@ struct CamParams {
     float tau_stop;
     const float *alpha;  // [cell] per au
     const float *src;    // [cell]
+    const float *csize;  // [cell] cell size for the sub-pixeling (au)
 };
 
 __device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, Pos &p) {
@@ -54,7 +56,10 @@ __device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, 
     return true;
 }
 
-__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
+// formal solution along the ray from p; with LMIN also the smallest cell size
+// on the part of the ray traced (RT_INF when it crosses no cell)
+template <bool LMIN>
+__device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, float &lmin) {
     float I = 0.f, tau = 0.f;
     uint32_t nzero = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {
@@ -63,6 +68,7 @@ __device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
         const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) {
             int c = cell_index(C.g, p);
+            if (LMIN) lmin = fminf(lmin, C.csize[c]);
             float dtau = C.alpha[c] * tw;
             if (C.mode == 0) {
                 I = fmaf(__expf(-tau) * (-expm1f(-dtau)), C.src[c], I);
@@ -76,6 +82,11 @@ __device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
         if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
     }
     return C.mode == 0 ? I : tau;
+}
+
+__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
+    float unused = RT_INF;
+    return integrate_ray_l<false>(C, p, unused);
 }
 
 // pixel of this lane: blocks cover tw x th tiles of the image, so the rays of
@@ -133,6 +144,41 @@ __device__ __forceinline__ void adj_add(int c, double va, double vs, int *key, d
     unsafeAtomicAdd(&g_src[c], vs);
 }
 
+// gI times the gradient of the ray through sky (X, Y) w.r.t. the per-cell
+// alpha and S: the ray is integrated for I, then swept again front to back
+// recomputing tau, with R_k = I - I_<=k (nothing is stored per segment)
+__device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y, float gI,
+                                            int *key, double *ga, double *gs, double *g_alpha,
+                                            double *g_src) {
+    Pos p0;
+    if (!ray_start(C, X, Y, p0)) return;
+    float I = integrate_ray(C, p0);
+    Pos p = p0;
+    float tau = 0.f, cum = 0.f;
+    uint32_t nzero = 0;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+        int kind;
+        float tw = next_wall(C.g, p, kind);
+        const float rcell = C.g.rw[p.ir + 1];
+        if (p.ir >= 0) {
+            int c = cell_index(C.g, p);
+            float dtau = C.alpha[c] * tw;
+            float att = __expf(-tau);
+            float om = -expm1f(-dtau);
+            float S = C.src[c];
+            cum = fmaf(att * om, S, cum);
+            float Rk = I - cum;
+            float dI_ddtau = att * (1.f - om) * S - Rk;
+            adj_add(c, (double)(gI * dI_ddtau * tw), (double)(gI * att * om), key, ga, gs,
+                    C.hbits, g_alpha, g_src);
+            tau += dtau;
+            if (tau > C.tau_stop) break;
+        }
+        if (cross_wall(C.g, p, tw, kind)) break;
+        if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
                                                    double *g_src) {
     extern __shared__ double adj_lds[];
@@ -156,34 +202,7 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
         for (int b = 0; b < C.nsub; ++b) {
             float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
             float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
-            Pos p0;
-            if (!ray_start(C, X, Y, p0)) continue;
-            float I = integrate_ray(C, p0);
-            // reverse sweep: recompute tau front to back, R_k = I - I_<=k
-            Pos p = p0;
-            float tau = 0.f, cum = 0.f;
-            uint32_t nzero = 0;
-            for (int guard = 0; guard < (1 << 24); ++guard) {
-                int kind;
-                float tw = next_wall(C.g, p, kind);
-                const float rcell = C.g.rw[p.ir + 1];
-                if (p.ir >= 0) {
-                    int c = cell_index(C.g, p);
-                    float dtau = C.alpha[c] * tw;
-                    float att = __expf(-tau);
-                    float om = -expm1f(-dtau);
-                    float S = C.src[c];
-                    cum = fmaf(att * om, S, cum);
-                    float Rk = I - cum;
-                    float dI_ddtau = att * (1.f - om) * S - Rk;
-                    adj_add(c, (double)(gI * dI_ddtau * tw), (double)(gI * att * om), key, ga, gs,
-                            C.hbits, g_alpha, g_src);
-                    tau += dtau;
-                    if (tau > C.tau_stop) break;
-                }
-                if (cross_wall(C.g, p, tw, kind)) break;
-                if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
-            }
+            adjoint_ray(C, X, Y, gI, key, ga, gs, g_alpha, g_src);
         }
     __syncthreads();
     for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
@@ -192,6 +211,220 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
             unsafeAtomicAdd(&g_alpha[c], ga[i]);
             unsafeAtomicAdd(&g_src[c], gs[i]);
         }
+    }
+}
+
+// ---------------------------------------------------------------- sub-pixels
+// Recursive sub-pixeling (RADMC-3D's camera_nrrefine / camera_refine_criterion,
+// as pdspy leaves them at RADMC-3D's defaults): a (sub)pixel of side d whose
+// central ray crosses a cell of size l with d > crit l is replaced by the mean
+// of its 2 x 2 sub-pixels.  Done breadth first over a node pool: every node of
+// level L is traced by one lane, which hands its four children (at most
+// RT_MAX_REFINE levels deep) their slots in level L + 1 with one atomic per
+// wavefront; the values are then averaged bottom-up, level by level.  Siblings
+// are summed in a fixed order, so the image does not depend on the order in
+// which slots were handed out.  The adjoint hands each leaf ray the pixel's
+// cotangent divided by 4 per level and runs the per-ray reverse sweep.
+#define RT_REF_DEAD 1.0e30f  // sky position of a node with no ray (tile padding, no room)
+
+struct RefArgs {
+    float2 *xy;       // sky position of the node's central ray (au)
+    float *val;       // its value (intensity or tau)
+    int *child;       // first of its four children, -1: a leaf
+    float *gw;        // adjoint: its cotangent
+    unsigned *ctl;    // control words (RT_REF_*)
+    unsigned cap;     // pool size (nodes)
+    float crit;       // refine criterion
+    int nref;         // levels that may split
+    float d0;         // side of a level-0 node (au): pixel / nsub
+    unsigned n0;      // level-0 nodes: every tile's pixels x nsub^2
+    int ns2;          // nsub^2
+};
+
+// level-0 node i: tiles in order, the tile's pixels, then the pixel's nsub^2
+// sub-rays, so the rays of a wavefront are neighbours on the sky
+__device__ __forceinline__ bool node0_pixel(const CamParams &C, int ns2, unsigned i, int &j, int &k,
+                                            int &a, int &b) {
+    const unsigned tp = (unsigned)(C.tw * C.th);
+    const unsigned sub = i % (unsigned)ns2, q = i / (unsigned)ns2;
+    const unsigned t = q / tp, r = q - t * tp;
+    const int ty = (int)t / C.ntx, tx = (int)t - ty * C.ntx;
+    const int dy = (int)r / C.tw, dx = (int)r - dy * C.tw;
+    k = tx * C.tw + dx;
+    j = ty * C.th + dy;
+    a = (int)sub / C.nsub;
+    b = (int)sub - a * C.nsub;
+    return k < C.nx && j < C.ny;
+}
+
+__device__ __forceinline__ unsigned pixel_node0(const CamParams &C, int ns2, int j, int k, int sub) {
+    const int ty = j / C.th, tx = k / C.tw;
+    const unsigned t = (unsigned)(ty * C.ntx + tx);
+    const unsigned r = (unsigned)((j - ty * C.th) * C.tw + (k - tx * C.tw));
+    return (t * (unsigned)(C.tw * C.th) + r) * (unsigned)ns2 + (unsigned)sub;
+}
+
+__global__ void k_ref_init(CamParams C, RefArgs R) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        R.ctl[0] = R.n0;
+        R.ctl[1] = 0u;
+        for (int L = 1; L <= RT_MAX_REFINE + 1; ++L) R.ctl[1 + L] = R.n0;
+        R.ctl[RT_REF_RAYS] = 0u;
+    }
+    if (i >= R.n0) return;
+    int j, k, a, b;
+    float2 xy = make_float2(RT_REF_DEAD, RT_REF_DEAD);  // tile padding: no ray
+    if (node0_pixel(C, R.ns2, i, j, k, a, b)) {
+        xy.x = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
+        xy.y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
+    }
+    R.xy[i] = xy;
+}
+
+// traces level L: value of every node, children for those that split
+__global__ void __launch_bounds__(256) k_ref_trace(CamParams C, RefArgs R, int L) {
+    const unsigned start = R.ctl[1 + L], end = R.ctl[2 + L];
+    const float d = ldexpf(R.d0, -L);
+    const bool may_split = L < R.nref;
+    const int lane = __lane_id();
+    unsigned nrays = 0;
+    for (unsigned base = start + blockIdx.x * blockDim.x; base < end;
+         base += gridDim.x * blockDim.x) {
+        const unsigned i = base + threadIdx.x;
+        const bool act = i < end;
+        float I = 0.f, lmin = RT_INF;
+        float2 xy = make_float2(RT_REF_DEAD, RT_REF_DEAD);
+        if (act) {
+            xy = R.xy[i];
+            Pos p;
+            if (xy.x < 0.5f * RT_REF_DEAD && ray_start(C, xy.x, xy.y, p)) {
+                I = integrate_ray_l<true>(C, p, lmin);
+                ++nrays;
+            }
+            R.val[i] = I;
+        }
+        const bool want = act && may_split && d > R.crit * lmin;
+        const uint64_t mk = __ballot(want);
+        int ch = -1;
+        if (mk) {
+            const int cnt = __popcll(mk);
+            const int leader = __ffsll((unsigned long long)mk) - 1;
+            // slots are only handed out while the pool has room (the count
+            // stays within a wavefront's 256 slots of the pool size)
+            unsigned b0 = R.cap;
+            if (lane == leader && __hip_atomic_load(&R.ctl[0], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT) < R.cap)
+                b0 = atomicAdd(&R.ctl[0], 4u * (unsigned)cnt);
+            b0 = __shfl(b0, leader);
+            if (want) {
+                const unsigned c0 = b0 + 4u * (unsigned)__popcll(mk & ((1ull << lane) - 1ull));
+                if (b0 < R.cap && c0 + 4u <= R.cap) {
+                    ch = (int)c0;
+                    const float h = 0.25f * d;
+                    R.xy[c0] = make_float2(xy.x - h, xy.y - h);
+                    R.xy[c0 + 1] = make_float2(xy.x + h, xy.y - h);
+                    R.xy[c0 + 2] = make_float2(xy.x - h, xy.y + h);
+                    R.xy[c0 + 3] = make_float2(xy.x + h, xy.y + h);
+                } else {
+                    // no room: this node stays a leaf; slots of its that lie in
+                    // the pool become rayless orphans of the next level
+                    if (b0 < R.cap)
+                        for (unsigned q = 0; q < 4u; ++q)
+                            if (c0 + q < R.cap) {
+                                R.xy[c0 + q] = make_float2(RT_REF_DEAD, RT_REF_DEAD);
+                                R.gw[c0 + q] = 0.f;
+                            }
+                    atomicAdd(&R.ctl[RT_REF_UNSPLIT], 1u);
+                }
+            }
+        }
+        if (act) R.child[i] = ch;
+    }
+    for (int off = 32; off > 0; off >>= 1) nrays += __shfl_down(nrays, off);
+    if (lane == 0 && nrays) atomicAdd(&R.ctl[RT_REF_RAYS], nrays);
+}
+
+// level L + 1 ends where the slots handed out so far end
+__global__ void k_ref_close(RefArgs R, int L) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned c = R.ctl[0];
+        R.ctl[3 + L] = c < R.cap ? c : R.cap;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ref_combine(RefArgs R, int L) {
+    const unsigned start = R.ctl[1 + L], end = R.ctl[2 + L];
+    for (unsigned i = start + blockIdx.x * blockDim.x + threadIdx.x; i < end;
+         i += gridDim.x * blockDim.x) {
+        const int c = R.child[i];
+        if (c >= 0) R.val[i] = 0.25f * (((R.val[c] + R.val[c + 1]) + R.val[c + 2]) + R.val[c + 3]);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ref_out(CamParams C, RefArgs R, float *img) {
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= C.nx * C.ny) return;
+    const int j = pix / C.nx, k = pix - j * C.nx;
+    float acc = 0.f;
+    for (int sub = 0; sub < R.ns2; ++sub) acc += R.val[pixel_node0(C, R.ns2, j, k, sub)];
+    img[pix] = acc / (float)R.ns2;
+}
+
+__global__ void __launch_bounds__(256) k_ref_gw0(CamParams C, RefArgs R, const float *dimg) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R.n0) return;
+    int j, k, a, b;
+    R.gw[i] = node0_pixel(C, R.ns2, i, j, k, a, b) ? dimg[j * C.nx + k] / (float)R.ns2 : 0.f;
+}
+
+__global__ void __launch_bounds__(256) k_ref_down(RefArgs R, int L) {
+    const unsigned start = R.ctl[1 + L], end = R.ctl[2 + L];
+    for (unsigned i = start + blockIdx.x * blockDim.x + threadIdx.x; i < end;
+         i += gridDim.x * blockDim.x) {
+        const int c = R.child[i];
+        if (c >= 0) {
+            const float g = 0.25f * R.gw[i];
+            R.gw[c] = g;
+            R.gw[c + 1] = g;
+            R.gw[c + 2] = g;
+            R.gw[c + 3] = g;
+        }
+    }
+}
+
+// the leaves' reverse sweeps; each block gathers a chunk of 256 leaves' cell
+// gradients in its LDS table before the global atomics, as k_image_adj does
+__global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double *g_alpha,
+                                                 double *g_src) {
+    extern __shared__ double adj_lds[];
+    const unsigned H = 1u << C.hbits;
+    double *ga = adj_lds, *gs = adj_lds + H;
+    int *key = (int *)(adj_lds + 2 * H);
+    const unsigned n = R.ctl[0] < R.cap ? R.ctl[0] : R.cap;
+    for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
+            key[t] = -1;
+            ga[t] = 0.0;
+            gs[t] = 0.0;
+        }
+        __syncthreads();
+        const unsigned i = base + threadIdx.x;
+        if (i < n && R.child[i] < 0) {
+            const float g = R.gw[i];
+            const float2 xy = R.xy[i];
+            if (g != 0.f && xy.x < 0.5f * RT_REF_DEAD)
+                adjoint_ray(C, xy.x, xy.y, g, key, ga, gs, g_alpha, g_src);
+        }
+        __syncthreads();
+        for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
+            const int c = key[t];
+            if (c >= 0) {
+                unsafeAtomicAdd(&g_alpha[c], ga[t]);
+                unsafeAtomicAdd(&g_src[c], gs[t]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -385,7 +618,9 @@ static double kappa_at(const rt_model *m, const std::vector<double> &tab, int s,
     return std::exp(std::log(k[i]) + w * (std::log(k[i + 1]) - std::log(k[i])));
 }
 
-// per-cell scratch of the camera kernels (allocated once per model)
+// per-cell scratch of the camera kernels (allocated once per model) and the
+// cells' sizes for the sub-pixeling: the least of dr, r_c dtheta and (with phi
+// cells) r_c sin(theta_c) dphi at the cell centre, in au
 int image_buffers(rt_model *m) {
     rt_ctx *ctx = m->ctx;
     size_t nc = (size_t)m->ncell;
@@ -395,6 +630,50 @@ int image_buffers(rt_model *m) {
         HIPCHK(ctx, hipMalloc(&m->d_galpha, nc * sizeof(double)));
         HIPCHK(ctx, hipMalloc(&m->d_gsrc, nc * sizeof(double)));
     }
+    if (!m->d_csize) {
+        std::vector<float> cs(nc);
+        for (int ip = 0; ip < m->np; ++ip)
+            for (int it = 0; it < m->nt; ++it)
+                for (int ir = 0; ir < m->nr; ++ir) {
+                    const double rc = 0.5 * (m->rw[ir] + m->rw[ir + 1]);
+                    double l = m->rw[ir + 1] - m->rw[ir];
+                    l = std::min(l, rc * (m->tw[it + 1] - m->tw[it]));
+                    if (m->np > 1)
+                        l = std::min(l, rc * std::sin(0.5 * (m->tw[it] + m->tw[it + 1])) *
+                                            (m->pw[ip + 1] - m->pw[ip]));
+                    cs[(size_t)ir + (size_t)m->nr * (it + (size_t)m->nt * ip)] =
+                        (float)(l / RT_LEN_UNIT);
+                }
+        HIPCHK(ctx, hipMalloc(&m->d_csize, nc * sizeof(float)));
+        HIPCHK(ctx, hipMemcpy(m->d_csize, cs.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return RT_OK;
+}
+
+int refine_reserve(rt_ctx *ctx, size_t n) {
+    RefinePool &P = ctx->ref;
+    if (n > ((size_t)1 << 30)) return set_err(ctx, RT_E_LIMIT, "sub-pixel pool of %zu nodes > 2^30", n);
+    if (P.cap >= n && P.ctl) return RT_OK;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    void *old[] = {P.xy, P.val, P.child, P.gw};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    P.xy = nullptr;
+    P.val = nullptr;
+    P.child = nullptr;
+    P.gw = nullptr;
+    P.cap = 0;
+    if (hipMalloc(&P.xy, n * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&P.val, n * sizeof(float)) != hipSuccess ||
+        hipMalloc(&P.child, n * sizeof(int)) != hipSuccess ||
+        hipMalloc(&P.gw, n * sizeof(float)) != hipSuccess)
+        return set_err(ctx, RT_E_NOMEM, "sub-pixel pool of %zu nodes", n);
+    if (!P.ctl) {
+        HIPCHK(ctx, hipMalloc(&P.ctl, RT_REF_NCTL * sizeof(unsigned)));
+        std::vector<unsigned> z(RT_REF_NCTL, 0u);
+        HIPCHK(ctx, hipMemcpy(P.ctl, z.data(), RT_REF_NCTL * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    P.cap = n;
     return RT_OK;
 }
 
@@ -445,7 +724,82 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
     if (rc) return rc;
     C.alpha = m->d_alpha;
     C.src = m->d_src;
+    C.csize = m->d_csize;
     return 0;
+}
+
+static bool stream_capturing(hipStream_t sm) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(sm, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+// The sub-pixelled image (d_img) or, with d_dimg, the gradients w.r.t. the
+// per-cell alpha and S accumulated into m->d_galpha / m->d_gsrc.  Outside a
+// graph capture a pool too small for the refinement is grown and the image
+// traced again; inside one the pool must have been reserved (rt_refine_reserve)
+// and a sub-pixel without room stays a leaf (counted for rt_refine_stats).
+static int refine_run(rt_model *m, const rt_camera *cam, CamParams C, float *d_img,
+                      const float *d_dimg) {
+    rt_ctx *ctx = m->ctx;
+    hipStream_t sm = ctx->stream;
+    const int nref = std::min(cam->nrefine, RT_MAX_REFINE);
+    const int ns2 = C.nsub * C.nsub;
+    const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
+    const size_t n0 = (size_t)ntiles * (size_t)(C.tw * C.th) * (size_t)ns2;
+    const bool capturing = stream_capturing(sm);
+    // a fresh pool starts at 2^21 nodes (40 MB); a reserved one is used as
+    // it is while it holds the pixels' rays twice
+    size_t want = ctx->ref.cap ? std::max(ctx->ref.cap, 2 * n0)
+                               : std::max((size_t)1 << 21, 8 * n0);
+    if (capturing) {
+        if (ctx->ref.cap < 2 * n0)
+            return set_err(ctx, RT_E_ARG, "sub-pixel pool (%zu nodes) too small to capture an image "
+                                          "of %zu rays: reserve it first", ctx->ref.cap, n0);
+        want = ctx->ref.cap;
+    }
+    const unsigned gblk = (unsigned)(ctx->n_cu > 0 ? 8 * ctx->n_cu : 2048);
+    for (;;) {
+        int rc = refine_reserve(ctx, want);
+        if (rc) return rc;
+        RefinePool &P = ctx->ref;
+        RefArgs R{P.xy, P.val, P.child, P.gw, P.ctl, (unsigned)P.cap,
+                  (float)(cam->refine_crit > 0 ? cam->refine_crit : 1.0), nref, C.pix / C.nsub,
+                  (unsigned)n0, ns2};
+        hipLaunchKernelGGL(k_ref_init, dim3(nblk(n0, 256)), dim3(256), 0, sm, C, R);
+        for (int L = 0; L <= nref; ++L) {
+            hipLaunchKernelGGL(k_ref_trace, dim3(gblk), dim3(256), 0, sm, C, R, L);
+            if (L < nref) hipLaunchKernelGGL(k_ref_close, dim3(1), dim3(64), 0, sm, R, L);
+        }
+        if (!capturing) {
+            unsigned uns = 0;
+            HIPCHK(ctx, hipMemcpyAsync(&uns, P.ctl + RT_REF_UNSPLIT, sizeof(uns),
+                                       hipMemcpyDeviceToHost, sm));
+            HIPCHK(ctx, hipStreamSynchronize(sm));
+            if (uns && P.cap < ((size_t)1 << 30)) {
+                unsigned z = 0;
+                HIPCHK(ctx, hipMemcpy(P.ctl + RT_REF_UNSPLIT, &z, sizeof(z), hipMemcpyHostToDevice));
+                want = std::min((size_t)1 << 30, 4 * P.cap);
+                continue;  // trace again in a larger pool
+            }
+        }
+        if (!d_dimg) {
+            for (int L = nref - 1; L >= 0; --L)
+                hipLaunchKernelGGL(k_ref_combine, dim3(gblk), dim3(256), 0, sm, R, L);
+            hipLaunchKernelGGL(k_ref_out, dim3(nblk((size_t)C.nx * C.ny, 256)), dim3(256), 0, sm, C,
+                               R, d_img);
+        } else {
+            hipLaunchKernelGGL(k_ref_gw0, dim3(nblk(n0, 256)), dim3(256), 0, sm, C, R, d_dimg);
+            for (int L = 0; L < nref; ++L)
+                hipLaunchKernelGGL(k_ref_down, dim3(gblk), dim3(256), 0, sm, R, L);
+            CamParams Ca = C;
+            Ca.hbits = 10;  // >= 4 entries per ray of a 256-ray chunk
+            const size_t lds = (size_t)(1u << Ca.hbits) * (2 * sizeof(double) + sizeof(int));
+            hipLaunchKernelGGL(k_ref_adj, dim3(gblk), dim3(256), lds, sm, Ca, R, m->d_galpha,
+                               m->d_gsrc);
+        }
+        HIPCHK(ctx, hipGetLastError());
+        return RT_OK;
+    }
 }
 
 static void star_fluxes(const rt_model *m, const rt_camera *cam, StarArg &fl) {
@@ -472,9 +826,13 @@ int image_run(rt_model *m, const rt_camera *cam, const float *d_T, const float *
     hipLaunchKernelGGL(k_cam_prep, dim3(nblk(m->ncell, 256)), dim3(256), 0, sm, rho, m->nspec,
                        m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->mode,
                        cam->d_jscat, m->d_alpha, m->d_src);
-    int npx = cam->nx * cam->ny;
     const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
-    hipLaunchKernelGGL(k_image, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_img);
+    if (cam->nrefine > 0) {
+        rc = refine_run(m, cam, C, d_img, nullptr);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(k_image, dim3(ntiles), dim3(C.tw * C.th), 0, sm, C, d_img);
+    }
     if (cam->incl_star && cam->mode == 0 && m->nstar > 0) {
         StarArg fl;
         star_fluxes(m, cam, fl);
@@ -503,11 +861,15 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
                        m->d_src);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, nc);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
-    int npx = cam->nx * cam->ny;
     const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
-    const size_t adj_lds = (size_t)(1u << C.hbits) * (2 * sizeof(double) + sizeof(int));
-    hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), adj_lds, sm, C, d_dimg, m->d_galpha,
-                       m->d_gsrc);
+    if (cam->nrefine > 0) {
+        rc = refine_run(m, cam, C, nullptr, d_dimg);
+        if (rc) return rc;
+    } else {
+        const size_t adj_lds = (size_t)(1u << C.hbits) * (2 * sizeof(double) + sizeof(int));
+        hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), adj_lds, sm, C, d_dimg,
+                           m->d_galpha, m->d_gsrc);
+    }
     if (cam->incl_star && m->nstar > 0) {
         StarArg fl;
         star_fluxes(m, cam, fl);

@@ -17,6 +17,27 @@
 #define RT_TALLY_SHIFT 28  // fixed-point tally unit: 2^-28 of kmax*dr
 #define RT_LEN_UNIT 1.495978707e13  // device length unit: au in cm (pdspy AU)
 
+#define RT_MAX_REFINE 24  // sub-pixel levels of the camera's recursive refinement
+// control words of the sub-pixel pool: [0] nodes handed out, [1 + L] first node
+// of level L (L = 0 .. RT_MAX_REFINE + 1: the last is the end of the deepest
+// level), then the unsplit count (cumulative, cleared by rt_refine_stats) and
+// the rays of the last image
+#define RT_REF_UNSPLIT (RT_MAX_REFINE + 3)
+#define RT_REF_RAYS (RT_MAX_REFINE + 4)
+#define RT_REF_NCTL (RT_MAX_REFINE + 5)
+
+// Node pool of the camera's sub-pixeling, shared by the context's models (their
+// work is ordered on the context's stream): sky position, value, first child
+// (-1: a leaf), cotangent
+struct RefinePool {
+    size_t cap = 0;
+    float2 *xy = nullptr;
+    float *val = nullptr;
+    int *child = nullptr;
+    float *gw = nullptr;
+    unsigned *ctl = nullptr;
+};
+
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -24,6 +45,8 @@ struct rt_ctx {
     std::mutex err_mu;  // models of one context may be updated from several host threads
     int n_cu = 0;
     int wallclock_khz = 0;
+    RefinePool ref;
+    ~rt_ctx();
 };
 
 // counters[] slots used by the thermal kernels
@@ -95,6 +118,7 @@ struct rt_model {
     float *d_Tout = nullptr;      // [S][cells]
     // image scratch
     float *d_alpha = nullptr, *d_src = nullptr;
+    float *d_csize = nullptr;     // [cell] cell size for the camera's sub-pixeling (au)
     double *d_galpha = nullptr, *d_gsrc = nullptr;
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
@@ -170,6 +194,9 @@ struct PipeUpload {
 };
 PipeUpload thermal_epoch_starts(rt_model *m);
 int image_buffers(rt_model *m);
+// the context's sub-pixel pool holds at least n nodes (synchronous; never
+// inside a capture)
+int refine_reserve(rt_ctx *ctx, size_t n);
 // batches (batch.hip)
 int batch_create(rt_ctx *ctx, rt_model *const *models, int n, rt_batch **out);
 int batch_destroy(rt_batch *b);

@@ -44,6 +44,11 @@ _CONTROL_KEYS = {
     "setthreads", "camera_scatsrc_allfreq", "mc_scat_maxtauabs", "writeimage_unformatted",
     "camera_circ_nrphiinf", "camera_circ_dbdr",
 }
+# RADMC-3D's defaults for camera_nrrefine / camera_refine_criterion, which
+# pdspy leaves unset unless the caller passes them
+RADMC3D_NRREFINE = 100
+RADMC3D_REFINE_CRITERION = 1.0
+
 # keys of the run_* wrappers themselves (Model.py:221-222, 515-519)
 _RUN_KEYS = {"verbose", "timelimit", "nice"}
 
@@ -261,18 +266,29 @@ class DeviceModel:
 
     @staticmethod
     def camera(npix, pix_cm, lam_um, incl=0.0, pa=0.0, phi=0.0, mode=0, incl_star=True,
-               nsub=1, tau_stop=40.0, ny=None, tile=(0, 0), jscat=None):
+               nsub=1, tau_stop=40.0, ny=None, tile=(0, 0), jscat=None, nrefine=0,
+               refine_crit=1.0):
         """Camera; ``jscat``: the mean intensity J[cells] (float32 device tensor)
         of ``scattering_source`` at ``lam_um`` for images with the scattering
-        source (the camera keeps a reference to it)."""
+        source (the camera keeps a reference to it).  ``nrefine`` /
+        ``refine_crit``: RADMC-3D's recursive sub-pixeling (camera_nrrefine /
+        camera_refine_criterion), off at nrefine = 0."""
         if jscat is not None:
             jscat = jscat.contiguous()
         cam = _lib.Camera(int(npix), int(ny if ny is not None else npix), int(nsub),
                           float(pix_cm), float(incl), float(pa), float(phi), float(lam_um),
                           int(mode), 1 if incl_star else 0, float(tau_stop), int(tile[0]),
-                          int(tile[1]), None if jscat is None else jscat.data_ptr())
+                          int(tile[1]), None if jscat is None else jscat.data_ptr(),
+                          int(nrefine), float(refine_crit))
         cam._jscat = jscat
         return cam
+
+    def refine_stats(self):
+        """(rays traced by the last sub-pixelled image of this model's context,
+        sub-pixels left unsplit for want of pool space since the last call)"""
+        n, u = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.rt_refine_stats(self.ctx.h, C.byref(n), C.byref(u)))
+        return n.value, u.value
 
     def scattering_source(self, lam_um, nphot, T_dev, seed=0):
         """Mean intensity J[cells] (float32 device tensor, erg/s/cm^2/Hz/sr) at
@@ -608,12 +624,18 @@ def _render(model, npix, pixelsize, lams, incl, pa, phi, dpc, nostar, nsub, devi
     T = torch.tensor(imaging_temperature(model.grid), dtype=torch.float32, device="cuda")
     pix_cm = pixelsize * dpc * AU
     scat = int(nphot) > 0 and _scatters(model, keywords)
+    # RADMC-3D's recursive sub-pixeling at its defaults unless pdspy passes the
+    # options (write.py:57-61 writes them only when given)
+    nrefine = keywords.get("camera_nrrefine")
+    nrefine = RADMC3D_NRREFINE if nrefine is None else int(nrefine)
+    crit = keywords.get("camera_refine_criterion")
+    crit = RADMC3D_REFINE_CRITERION if crit is None else float(crit)
     seed = int(keywords.get("iseed") or 0) & 0xFFFFFFFFFFFFFFFF
     chans = []
     for lam_um in lams:
         J = dm.scattering_source(lam_um, int(nphot), T, seed=seed)[0] if scat else None
         cam = dm.camera(npix, pix_cm, lam_um, incl=incl, pa=pa, phi=phi, incl_star=not nostar,
-                        nsub=nsub, jscat=J)
+                        nsub=nsub, jscat=J, nrefine=nrefine, refine_crit=crit)
         chans.append(dm.image(cam, T).double().cpu().numpy())
     img = np.stack(chans, axis=-1)  # [ny, nx, nlam], erg/s/cm^2/Hz/sr
     # Jy/pixel (Model.py:566-567)

@@ -415,14 +415,22 @@ def test_run_image_and_visibilities_units():
     dm = rt._device_model(m)
     T = torch.tensor(rt.imaging_temperature(m.grid), dtype=torch.float32, device="cuda")
     pix_cm = 0.1 * 140 * AU
-    cgs = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60), T).double().cpu().numpy()
+    # RADMC-3D's sub-pixeling at its defaults (camera_nrrefine 100, criterion 1)
+    cgs = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60, nrefine=100), T).double().cpu().numpy()
     assert np.allclose(im.image[:, :, 0, 0], cgs / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2,
                        rtol=1e-6)
+    im0 = rt.run_image(m, name="p", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140,
+                       scattering_mode_max=0, camera_nrrefine=0)
+    plain = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60), T).double().cpu().numpy()
+    assert np.allclose(im0.image[:, :, 0, 0],
+                       plain / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2, rtol=1e-6)
+    assert not np.allclose(im0.image, im.image, rtol=1e-3)
     # by default the image carries the scattering source (nphot packets, iseed)
     ims = rt.run_image(m, name="s", nphot=100000, npix=32, pixelsize=0.1, lam="1300", incl=60,
                        dpc=140, iseed=3)
     J, _ = dm.scattering_source(1300.0, 100000, T, seed=3)
-    cgs_s = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60, jscat=J), T).double().cpu().numpy()
+    cgs_s = dm.image(dm.camera(32, pix_cm, 1300.0, incl=60, jscat=J, nrefine=100),
+                     T).double().cpu().numpy()
     assert np.allclose(ims.image[:, :, 0, 0],
                        cgs_s / inputs.Jy * (pix_cm / (140 * inputs.pc)) ** 2, rtol=1e-6)
     vis = rt.run_visibilities(m, name="1300", npix=32, pixelsize=0.1, lam="1300", incl=60, dpc=140,
