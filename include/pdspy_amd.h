@@ -207,6 +207,21 @@ int rt_model_set_density(rt_model *m, const double *h_rho);
  * recaptured on its next step. */
 int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
                          const double *h_g);
+/* The same, with the emission, re-emission and mean-opacity tables built by a
+ * kernel on the context's stream instead of on the host (FP64, the host
+ * builder's operations and order: the FP32 tables agree with it to
+ * rounding); the groups and the other per-species arrays are set on the host.
+ * For the per-walker dust of a sampler step. */
+int rt_model_set_opacity_device(rt_model *m, const double *h_kabs, const double *h_ksca,
+                                const double *h_g);
+/* Copies the model's device tables to the host (synchronises the stream):
+ * ln q(T) [G][ntemp] (double), re-emission CDFs [G][ntemp][nlam], their guide
+ * tables [G][ntemp][128] (uint8), Planck-mean absorption [G][ntemp],
+ * Rosseland-mean extinction [nspec][ntemp]; any pointer may be NULL.  G =
+ * rt_model_ngroups. */
+int rt_model_get_tables(rt_model *m, double *h_lnq, float *h_cdf, unsigned char *h_guide,
+                        float *h_kplanck, float *h_kross);
+int rt_model_ngroups(const rt_model *m);
 
 /* Density components generated on the device (pdspy's model builders, which
  * run on the host there):

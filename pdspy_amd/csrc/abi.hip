@@ -66,7 +66,7 @@ static void free_model(rt_model *m) {
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
-                    m->d_csize,
+                    m->d_csize, m->d_tabin,
                     m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
                     m->d_dens_fac, m->d_comps, m->d_vol, m->d_jtally, m->d_scum, m->d_scnt, m->d_sblk};
     for (void *p : ptrs)
@@ -403,8 +403,8 @@ int rt_model_set_density(rt_model *m, const double *h_rho) {
     return upload_density(m);
 }
 
-int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
-                         const double *h_g) {
+static int set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
+                       const double *h_g, bool on_device) {
     if (!m || !h_kabs || !h_ksca) return RT_E_ARG;
     rt_ctx *ctx = m->ctx;
     if ((h_g != nullptr) != m->has_g)
@@ -428,13 +428,22 @@ int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca
     m->ksca.assign(h_ksca, h_ksca + nk);
     if (h_g) m->gsca.assign(h_g, h_g + nk);
     std::string err;
-    int rc = build_tables(m, err);
+    int rc;
+    if (on_device) {
+        // groups, frequency grid and kmax on the host; the tables by a kernel
+        TableSpec sp{m->nlam, m->nspec, m->ntemp, m->nstar, m->temp0, m->temp1,
+                     m->lam.data(), m->kabs.data(), m->ksca.data(), m->stars.data()};
+        rc = host_table_groups(sp, m->tab, err);
+    } else {
+        rc = build_tables(m, err);
+    }
     if (!rc && m->tab.grp != old_tab.grp) {
         rc = RT_E_ARG;
         err = "the new opacities group the species differently (species with identical kabs "
               "share tables); create a new model instead";
     }
-    if (!rc && (m->tab.cdf.size() != old_tab.cdf.size() || m->tab.kross.size() != old_tab.kross.size())) {
+    if (!rc && !on_device &&
+        (m->tab.cdf.size() != old_tab.cdf.size() || m->tab.kross.size() != old_tab.kross.size())) {
         rc = RT_E_ARG;
         err = "table sizes changed";
     }
@@ -455,15 +464,47 @@ int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca
     HIPCHK(ctx, hipMemcpy(m->d_gka, o.gka.data(), o.gka.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(m->d_ginvk, o.ginvk.data(), o.ginvk.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(m->d_econv, o.econv.data(), o.econv.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_cdf, t.cdf.data(), t.cdf.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_guide, t.guide.data(), t.guide.size(), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_lnq, t.lnq.data(), t.lnq.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_kross, t.kross.data(), t.kross.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_kplanck, t.kplanck.data(), t.kplanck.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (on_device) {
+        rc = tables_device_build(m);
+        if (rc) return rc;
+    } else {
+        HIPCHK(ctx, hipMemcpy(m->d_cdf, t.cdf.data(), t.cdf.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_guide, t.guide.data(), t.guide.size(), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_lnq, t.lnq.data(), t.lnq.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_kross, t.kross.data(), t.kross.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_kplanck, t.kplanck.data(), t.kplanck.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     m->have_T = false;
     m->opac_version++;
     return RT_OK;
 }
+
+int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
+                         const double *h_g) {
+    return set_opacity(m, h_kabs, h_ksca, h_g, false);
+}
+
+int rt_model_set_opacity_device(rt_model *m, const double *h_kabs, const double *h_ksca,
+                                const double *h_g) {
+    return set_opacity(m, h_kabs, h_ksca, h_g, true);
+}
+
+int rt_model_get_tables(rt_model *m, double *h_lnq, float *h_cdf, unsigned char *h_guide,
+                        float *h_kplanck, float *h_kross) {
+    if (!m) return RT_E_ARG;
+    rt_ctx *ctx = m->ctx;
+    const size_t G = (size_t)m->tab.ngroup, nT = (size_t)m->ntemp, nl = (size_t)m->nlam;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_lnq) HIPCHK(ctx, hipMemcpy(h_lnq, m->d_lnq, G * nT * sizeof(double), hipMemcpyDeviceToHost));
+    if (h_cdf) HIPCHK(ctx, hipMemcpy(h_cdf, m->d_cdf, G * nT * nl * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_guide) HIPCHK(ctx, hipMemcpy(h_guide, m->d_guide, G * nT * RT_NGUIDE, hipMemcpyDeviceToHost));
+    if (h_kplanck) HIPCHK(ctx, hipMemcpy(h_kplanck, m->d_kplanck, G * nT * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_kross)
+        HIPCHK(ctx, hipMemcpy(h_kross, m->d_kross, (size_t)m->nspec * nT * sizeof(float), hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+int rt_model_ngroups(const rt_model *m) { return m ? m->tab.ngroup : RT_E_ARG; }
 
 int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp) {
     if (!m) return RT_E_ARG;

@@ -119,6 +119,8 @@ struct rt_model {
     // image scratch
     float *d_alpha = nullptr, *d_src = nullptr;
     float *d_csize = nullptr;     // [cell] cell size for the camera's sub-pixeling (au)
+    double *d_tabin = nullptr;    // inputs of the device table build
+    size_t tabin_cap = 0;
     double *d_galpha = nullptr, *d_gsrc = nullptr;
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
@@ -162,6 +164,10 @@ struct ThermPlan {
     bool count_visits = false;
     int lds_wave = 0;     // LDS floats per wavefront
 };
+
+// the opacity-dependent thermal-MC tables built by a kernel on the context's
+// stream from m->kabs / ksca, m->tab holding the groups (tables_dev.hip)
+int tables_device_build(rt_model *m);
 
 // the model's thermal-MC tables from its host opacities and stars
 inline int build_tables(rt_model *m, std::string &err) {

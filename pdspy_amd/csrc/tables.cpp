@@ -106,7 +106,7 @@ const std::vector<float> &mrw_escape_table() {
 
 }  // namespace
 
-int build_host_tables(const TableSpec &sp, HostTables &t, std::string &err) {
+int host_table_groups(const TableSpec &sp, HostTables &t, std::string &err) {
     const TableSpec *m = &sp;
     const int nl = m->nlam, ns = m->nspec, nT = m->ntemp;
     t.nu.resize(nl);
@@ -144,6 +144,21 @@ int build_host_tables(const TableSpec &sp, HostTables &t, std::string &err) {
     t.lnt0 = std::log(m->temp0);
     t.dlnt = (std::log(m->temp1) - t.lnt0) / (nT - 1);
     t.kmax.assign(G, 1.0);
+    for (int q = 0; q < G; ++q) {
+        const double *ka = &m->kabs[(size_t)t.grep[q] * nl];
+        double km = 0;
+        for (int i = 0; i < nl; ++i) km = std::max(km, ka[i]);
+        t.kmax[q] = km > 0 ? km : 1.0;
+    }
+    return 0;
+}
+
+int build_host_tables(const TableSpec &sp, HostTables &t, std::string &err) {
+    const TableSpec *m = &sp;
+    const int nl = m->nlam, ns = m->nspec, nT = m->ntemp;
+    int rc = host_table_groups(sp, t, err);
+    if (rc) return rc;
+    const int G = t.ngroup;
     t.lnq.assign((size_t)G * nT, 0.0);
     t.cdf.assign((size_t)G * nT * nl, 0.f);
     t.guide.assign((size_t)G * nT * RT_NGUIDE, 0);
@@ -183,9 +198,6 @@ int build_host_tables(const TableSpec &sp, HostTables &t, std::string &err) {
     for (int q = 0; q < G; ++q) {
         const int s = t.grep[q];
         const double *ka = &m->kabs[(size_t)s * nl];
-        double km = 0;
-        for (int i = 0; i < nl; ++i) km = std::max(km, ka[i]);
-        t.kmax[q] = km > 0 ? km : 1.0;
         for (int i = 0; i < nl; ++i) lkd[i] = ka[i] > 0 ? std::log(ka[i] * t.dnu[i]) : -INFINITY;
         for (int it = 0; it < nT; ++it) {
             const double *lb = &LB[(size_t)it * nl], *ld = &LD[(size_t)it * nl];

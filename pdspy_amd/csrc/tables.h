@@ -40,3 +40,6 @@ struct TableSpec {
 };
 
 int build_host_tables(const TableSpec &s, HostTables &t, std::string &err);
+// only the parts the device builder needs from the host: frequency grid,
+// absorption groups, their largest kabs and the temperature grid
+int host_table_groups(const TableSpec &s, HostTables &t, std::string &err);

@@ -118,6 +118,7 @@ struct ThermParams {
     unsigned long long *tslot;           // [E+1][slot] tallies by deposit slot (last: all later)
     unsigned long long *cum;             // [cell][G] tallies of all folded epochs
     float *xsnap;                        // [E][slot] temperature-row snapshots, one per epoch
+    double *slot_sum;                    // [E+1] sum of all adds into each deposit slot
     size_t slot;                         // floats per snapshot slot (>= cells * G, aligned)
     const double *econv;                 // [cell][G]
     const double *lnq;                   // [G][ntemp]
@@ -220,23 +221,32 @@ __device__ __forceinline__ float xrow_of(const double *lq, int ntemp, double e) 
     return (float)(m + (le - lq[m]) / (lq[m + 1] - lq[m]));
 }
 
-// Adds the packet's pending deposits to the tallies of cell c.  Overflow is
-// checked without waiting for the atomics: every add and the value each add
-// found (kept in old[], OR-ed into ovf at the next flush or at the end) must
-// stay below 2^62, so no tally can wrap unnoticed.
+// Adds the packet's pending deposits to the tallies of cell c with atomics
+// whose result nobody waits for.  Overflow is ruled out by bounds instead of
+// by the values the atomics find: every add stays below 2^62 (ovf), and the
+// packet's adds into its current slot are summed (ssum, FP64) and added to the
+// slot's total when the packet leaves the slot; a slot whose total of adds
+// stays below 2^61 has no tally near 2^64 (checked after the launch).
 template <int MG>
 __device__ __forceinline__ void flush_tally(const ThermParams &P, unsigned long long *tb, int c,
                                             unsigned long long (&acc)[MG],
-                                            unsigned long long (&old)[MG],
-                                            unsigned long long &ovf) {
+                                            unsigned long long &ovf, double &ssum) {
 #pragma unroll
     for (int q = 0; q < MG; ++q) {
         if (q < P.ngroup && acc[q]) {
-            ovf |= old[q] | acc[q];
-            old[q] = atomicAdd(&tb[(size_t)c * P.ngroup + q], acc[q]);
+            ovf |= acc[q];
+            ssum += (double)acc[q];
+            __hip_atomic_fetch_add(&tb[(size_t)c * P.ngroup + q], acc[q], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
             acc[q] = 0;
         }
     }
+}
+
+// the packet leaves slot `slot`: its adds there join the slot's total
+__device__ __forceinline__ void flush_ssum(const ThermParams &P, int slot, double &ssum) {
+    if (ssum > 0.0) unsafeAtomicAdd(&P.slot_sum[slot], ssum);
+    ssum = 0.0;
 }
 
 // One chunk of the next temperature snapshot, if its epoch is complete.
@@ -498,13 +508,11 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     for (int s = 0; s < MS; ++s) rcache[s] = 0.f;
 #pragma unroll
     for (int q = 0; q < MG; ++q) xcache[q] = -1.f;
-    unsigned long long acc[MG], aold[MG];
-    unsigned long long ovf = 0;  // OR of tally values seen and of adds (overflow check)
+    unsigned long long acc[MG];
+    unsigned long long ovf = 0;  // OR of the adds (overflow check)
+    double ssum = 0.0;           // the packet's adds into its current slot
 #pragma unroll
-    for (int q = 0; q < MG; ++q) {
-        acc[q] = 0;
-        aold[q] = 0;
-    }
+    for (int q = 0; q < MG; ++q) acc[q] = 0;
     // per-run wave counters: crossings, absorptions, scatterings, escapes,
     // kills, interactions, MRW steps, nudges
     uint32_t cnt8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -565,7 +573,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             if (stop) {
                 if (lane == 0) atomicOr(A.abort, 1ull);
                 if (alive) {
-                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, aold, ovf);
+                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, ovf, ssum);
+                    flush_ssum(P, slot, ssum);
                     max_pkt = max(max_pkt, nev + nstep);
                     n_kill++;
                     alive = false;
@@ -591,14 +600,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 if (r >= 0) {
                     order_acquire();
                     if (BATCH && r != m) {
-                        if (m >= 0) {
-#pragma unroll
-                            for (int q = 0; q < MG; ++q) {
-                                ovf |= aold[q];
-                                aold[q] = 0;
-                            }
-                            flush_counts(P, lane, cnt8, max_pkt, ovf);
-                        }
+                        if (m >= 0) flush_counts(P, lane, cnt8, max_pkt, ovf);
                         m = r;
                         Pp = BATCH ? runs + m : &P1;
                         stage_run(P, wl, lane, g, s_ka, s_ks, s_gka, s_g, s_invdr);
@@ -708,7 +710,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 ok = lane64(ok, leader);
                 base = lane64(base, leader);
                 if (ok && alive) {
-                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, aold, ovf);
+                    if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, ovf, ssum);
+                    flush_ssum(P, slot, ssum);
                     const unsigned long long idx = base + __popcll(live & ((1ull << lane) - 1ull));
                     const size_t s0 = (size_t)(idx % P.park_cap);
                     unsigned *w = P.park + s0 * RT_PARK_WORDS;
@@ -946,7 +949,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             nzero = 0;
             if (--slice_left == 0u && slot < E) {
                 // next slot: earlier deposits are final for snapshots <= slot
-                flush_tally(P, tb, c, acc, aold, ovf);
+                flush_tally(P, tb, c, acc, ovf, ssum);
+                flush_ssum(P, slot, ssum);
                 if (slot + 1 < E) count_up(&P.slot_in[slot + 1]);
                 count_up(&P.slot_out[slot]);
                 slot++;
@@ -1003,7 +1007,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             tau = -__logf(u[6]);
             RT_PHASE(6);
             if (P.max_events && nev >= P.max_events) {
-                flush_tally(P, tb, c, acc, aold, ovf);
+                flush_tally(P, tb, c, acc, ovf, ssum);
+                flush_ssum(P, slot, ssum);
                 max_pkt = max(max_pkt, nev + nstep);
                 n_kill++;
                 alive = false;
@@ -1083,7 +1088,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 for (int q = 0; q < MG; ++q)
                     if (q < G)
                         acc[q] += (unsigned long long)rintf(s_gka[q * NL + ifr] * wf);
-                flush_tally(P, tb, c, acc, aold, ovf);
+                flush_tally(P, tb, c, acc, ovf, ssum);
             }
             tau -= aext * tw;
             n_cross++;
@@ -1105,6 +1110,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             }
             RT_PHASE(5);
             if (end) {
+                flush_ssum(P, slot, ssum);
                 max_pkt = max(max_pkt, nev + nstep);
                 alive = false;
                 if (slot < E) count_up(&P.slot_out[slot]);
@@ -1114,11 +1120,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
         }
     }
     // ---- statistics of the last run served, batch counters
-    if (m >= 0) {
-#pragma unroll
-        for (int q = 0; q < MG; ++q) ovf |= aold[q];
-        flush_counts(P, lane, cnt8, max_pkt, ovf);
-    }
+    if (m >= 0) flush_counts(P, lane, cnt8, max_pkt, ovf);
     if (lane == 0) {
         for (int r = 0; r < A.nrun; ++r)
             atomicMax(&run_at<BATCH>(runs, P1, r).snap_time[RT_MAX_EPOCH + 1],
@@ -1194,11 +1196,13 @@ static std::vector<unsigned long long> epoch_starts(uint64_t nphot, uint64_t fir
 // park_tail] then u32 [snap_claim | snap_done | snap_ready | next_snap, runs_done]
 #define PIPE_U64 (6 * RT_MAX_EPOCH + 7)
 #define PIPE_U32 (3 * RT_MAX_EPOCH + 4)
+#define PIPE_F64 (RT_MAX_EPOCH + 1)  // after the u32 words (8-byte aligned): slot totals
 
 struct PipeView {
     unsigned long long *ep_start, *ep_claim, *slot_start, *slot_in, *slot_out, *snap_time, *ep_cur,
         *abort, *park_head, *park_tail;
     unsigned int *u32base, *snap_claim, *snap_done, *snap_ready, *next_snap, *runs_done;
+    double *slot_sum;
 };
 
 static PipeView pipe_view(void *d_pipe) {
@@ -1219,6 +1223,7 @@ static PipeView pipe_view(void *d_pipe) {
     v.snap_ready = v.snap_done + RT_MAX_EPOCH;
     v.next_snap = v.snap_ready + RT_MAX_EPOCH;
     v.runs_done = v.next_snap + 1;
+    v.slot_sum = (double *)(v.u32base + PIPE_U32);
     return v;
 }
 
@@ -1226,7 +1231,8 @@ static int ensure_pipe_buffers(rt_model *m) {
     rt_ctx *ctx = m->ctx;
     if (!m->d_pipe)
         HIPCHK(ctx, hipMalloc(&m->d_pipe, sizeof(unsigned long long) * PIPE_U64 +
-                                              sizeof(unsigned int) * PIPE_U32));
+                                              sizeof(unsigned int) * PIPE_U32 +
+                                              sizeof(double) * PIPE_F64));
     if (!m->d_run) HIPCHK(ctx, hipMalloc(&m->d_run, sizeof(ThermParams)));
     if (!m->h_run) m->h_run = std::calloc(1, sizeof(ThermParams));
     if (!m->h_run) return set_err(ctx, RT_E_NOMEM, "host allocation failed");
@@ -1347,6 +1353,7 @@ int thermal_plan(rt_model *m, uint64_t nphot, uint64_t seed, const rt_thermal_op
     P.tslot = m->d_tally2;
     P.cum = m->d_tally;
     P.xsnap = m->d_xring;
+    P.slot_sum = pv.slot_sum;
     P.slot = snap_slot(ncg);
     P.econv = m->d_econv;
     P.lnq = m->d_lnq;
@@ -1377,6 +1384,8 @@ int thermal_enqueue_reset(rt_model *m, const ThermPlan &rp, hipStream_t sm, bool
     hipLaunchKernelGGL(k_fill_u64, dim3(nblk(n64, 256)), dim3(256), 0, sm, pv.ep_claim, n64, 0ull);
     hipLaunchKernelGGL(k_fill_u32, dim3(nblk(PIPE_U32, 256)), dim3(256), 0, sm, pv.u32base,
                        (size_t)PIPE_U32, 0u);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(PIPE_F64, 256)), dim3(256), 0, sm,
+                       (unsigned long long *)pv.slot_sum, (size_t)PIPE_F64, 0ull);
     if (upload_est)
         HIPCHK(ctx, hipMemcpyAsync(pv.ep_start, m->h_est.data(),
                                    sizeof(unsigned long long) * m->h_est.size(),
@@ -1500,7 +1509,13 @@ int thermal_collect(rt_model *m, const ThermPlan &rp, bool aborted, float kms, f
     }
     if (aborted)
         return set_err(ctx, RT_E_WATCHDOG, "thermal kernel exceeded its time budget; packets were killed");
-    if (cnt[CNT_OVERFLOW]) return set_err(ctx, RT_E_OVERFLOW, "fixed-point tally overflow");
+    // every slot's adds sum below 2^61: no tally of it reached 2^64 (wrapped)
+    std::vector<double> ssum(E + 1);
+    HIPCHK(ctx, hipMemcpy(ssum.data(), pv.slot_sum, sizeof(double) * (E + 1), hipMemcpyDeviceToHost));
+    bool slot_ovf = false;
+    for (double v : ssum) slot_ovf |= !(v < 0x1p61);
+    if (cnt[CNT_OVERFLOW] || slot_ovf)
+        return set_err(ctx, RT_E_OVERFLOW, "fixed-point tally overflow");
     if (nsnap != (unsigned)E)
         return set_err(ctx, RT_E_HIP, "thermal pipeline ended after %u of %d epochs", nsnap, E);
     return RT_OK;

@@ -160,6 +160,7 @@ class DeviceModel:
         h = C.c_void_p()
         self.ctx.check(self.ctx.L.rt_model_create(self.ctx.h, C.byref(d), C.byref(h)))
         self.h = h
+        self._ntemp = int(ntemp) if ntemp else 1000
         self.rho_host = rho.reshape(self.nspec, self.ncell)
         self._rho_dev = None
 
@@ -171,30 +172,52 @@ class DeviceModel:
         except Exception:
             pass
 
-    def set_opacity(self, kabs, ksca, g=None):
+    def set_opacity(self, kabs, ksca, g=None, on_device=False):
         """New dust opacities [nspec][nlam] (cm^2/g) on the model's wavelengths
-        (rt_model_set_opacity); the species grouping must stay as it was."""
+        (rt_model_set_opacity); the species grouping must stay as it was.
+        ``on_device``: the thermal tables are built by a kernel on the stream
+        (rt_model_set_opacity_device) instead of on the host."""
         nl = len(self._keep["lam_um"])
         kabs = np.ascontiguousarray(kabs, dtype=float).reshape(self.nspec, nl)
         ksca = np.ascontiguousarray(ksca, dtype=float).reshape(self.nspec, nl)
         gg = None if g is None else np.ascontiguousarray(g, dtype=float).reshape(self.nspec, nl)
         # synchronous copies, no stream work: safe to call from several threads
         # for different models
-        self.ctx.check(self.ctx.L.rt_model_set_opacity(
-            self.h, kabs.ctypes.data, ksca.ctypes.data, None if gg is None else gg.ctypes.data))
+        fn = self.ctx.L.rt_model_set_opacity_device if on_device else self.ctx.L.rt_model_set_opacity
+        if on_device:
+            self.ctx.sync_stream()
+        self.ctx.check(fn(self.h, kabs.ctypes.data, ksca.ctypes.data,
+                          None if gg is None else gg.ctypes.data))
         self._keep.update(kabs=kabs, ksca=ksca)
         if gg is not None:
             self._keep["gsca"] = gg
 
-    def set_dust(self, dust):
+    def set_dust(self, dust, on_device=False):
         """New opacities from pdspy Dust objects, one per species (lam in cm,
         kabs, ksca and optionally g), resampled onto the model's wavelengths as
-        the model was built (model_arrays)."""
+        the model was built (model_arrays); ``on_device`` as in set_opacity."""
         if len(dust) != self.nspec:
             raise ValueError("set_dust: %d Dust objects for %d species" % (len(dust), self.nspec))
         kabs, ksca, gg, _ = dust_arrays(dust, self._keep["lam_um"])
         self.set_opacity(np.array(kabs), np.array(ksca),
-                         np.array(gg) if self._keep.get("gsca") is not None else None)
+                         np.array(gg) if self._keep.get("gsca") is not None else None,
+                         on_device=on_device)
+
+    def tables(self):
+        """The device's thermal tables: lnq [G][ntemp], cdf [G][ntemp][nlam],
+        guide [G][ntemp][128], kplanck [G][ntemp], kross [nspec][ntemp]."""
+        self.ctx.sync_stream()
+        G = int(self.ctx.L.rt_model_ngroups(self.h))
+        nT = int(self._ntemp)
+        nl = len(self._keep["lam_um"])
+        out = dict(lnq=np.zeros((G, nT)), cdf=np.zeros((G, nT, nl), dtype=np.float32),
+                   guide=np.zeros((G, nT, 128), dtype=np.uint8),
+                   kplanck=np.zeros((G, nT), dtype=np.float32),
+                   kross=np.zeros((self.nspec, nT), dtype=np.float32))
+        self.ctx.check(self.ctx.L.rt_model_get_tables(
+            self.h, out["lnq"].ctypes.data, out["cdf"].ctypes.data, out["guide"].ctypes.data,
+            out["kplanck"].ctypes.data, out["kross"].ctypes.data))
+        return out
 
     def set_density(self, rho):
         rho = np.ascontiguousarray(rho, dtype=float).reshape(self.nspec, self.ncell)

@@ -57,6 +57,12 @@ class SamplerStep:
         dev = "cuda"
         self.u = torch.tensor(u, dtype=torch.float64, device=dev)
         self.v = torch.tensor(v, dtype=torch.float64, device=dev)
+        # model visibilities as pdspy's fits compute them (interpolate_model
+        # with galario, interpolate_model.py:23-27), evaluated exactly: on the
+        # grid nodes that is exp(-2 pi i v dxy) DFT(-u, v) of the image, so the
+        # DFT runs at (-u, v) with a phase-centre shift of one pixel in Dec
+        self.u_vis = (-self.u).contiguous()
+        self.ddec = self.dxy
         self.cams = [rt.DeviceModel.camera(npix, self.pix_cm, LAM_UM, incl=p["incl"])
                      for p in self.params]
         f32 = dict(dtype=torch.float32, device=dev)
@@ -72,7 +78,8 @@ class SamplerStep:
         self.step(seed=data_seed, use_graph=False)
         rng = np.random.default_rng(data_seed)
         for i in range(self.n):
-            re, im = rt.vis_dft(self.img[i] * self.scale, self.dxy, self.u, self.v)
+            re, im = rt.vis_dft(self.img[i] * self.scale, self.dxy, self.u_vis, self.v, 0.0,
+                                self.ddec)
             sig = 0.01 * float(torch.sqrt(re * re + im * im).max())
             sig = sig if sig > 0 else 1.0
             self.re[i] = (re + torch.tensor(rng.normal(0, sig, K), **f32)).contiguous()
@@ -137,12 +144,10 @@ class SamplerStep:
         for m, p in zip(self.models, params):
             m.generate_density(configs.batch_components(p))
         if self.dust_gen is not None:
-            # the host table rebuilds run in parallel (the ctypes calls release
-            # the GIL; each model's update touches only that model)
-            from concurrent.futures import ThreadPoolExecutor
-            dust = [self._dust(p) for p in params]
-            with ThreadPoolExecutor(max_workers=min(16, self.n)) as ex:
-                list(ex.map(lambda md: md[0].set_dust(md[1]), zip(self.models, dust)))
+            # each model's new dust: interpolated on the host (2 x nlam values),
+            # the thermal tables rebuilt by a kernel on the stream
+            for m, p in zip(self.models, params):
+                m.set_dust(self._dust(p), on_device=True)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         self.params = params
@@ -154,10 +159,10 @@ class SamplerStep:
 
     def items(self):
         if self._items is None:
-            self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u, self.v, self.re[i],
+            self._items = [rt.Batch.item(self.cams[i], self.dxy, self.u_vis, self.v, self.re[i],
                                          self.im[i], self.w[i], self.img[i], self.chi2[i],
                                          self.g_logrho[i], self.g_T[i], scale=self.scale,
-                                         nphot_scat=self.nphot_scat)
+                                         ddec=self.ddec, nphot_scat=self.nphot_scat)
                            for i in range(self.n)]
         return self._items
 
@@ -186,10 +191,10 @@ class SamplerStep:
                                         incl=self.params[i]["incl"],
                                         tile=(cam.tile_w, cam.tile_h), jscat=J)
         img = m.image(cam, Timg)
-        re, im = rt.vis_dft(img * self.scale, self.dxy, self.u, self.v)
+        re, im = rt.vis_dft(img * self.scale, self.dxy, self.u_vis, self.v, 0.0, self.ddec)
         c2, gre, gim = rt.chisq(self.re[i], self.im[i], self.w[i], re, im, with_grad=True)
-        gimg = rt.vis_dft_adjoint(gre, gim, self.npix, self.npix, self.dxy, self.u,
-                                  self.v) * self.scale
+        gimg = rt.vis_dft_adjoint(gre, gim, self.npix, self.npix, self.dxy, self.u_vis,
+                                  self.v, 0.0, self.ddec) * self.scale
         glr, gT = m.image_adjoint(cam, Timg, None, gimg)
         return img, c2, glr, gT
 

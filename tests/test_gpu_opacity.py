@@ -128,3 +128,37 @@ def test_sampler_step_draws_dust_per_model():
         assert np.array_equal(m.temperature(), got[0][i]), i
         assert _close(img.cpu().numpy(), got[2][i], 1e-6), i
         assert float(c2.item()) == pytest.approx(got[1][i], rel=1e-5), i
+
+
+def test_device_built_tables_match_host_build():
+    """rt_model_set_opacity_device builds the emission, re-emission and mean
+    tables with a kernel; they equal the host build of a model created with
+    the same opacities to FP32 rounding, and the thermal MC run on them is
+    that model's to within the Monte Carlo tolerance."""
+    print("case: device-built opacity tables vs host build", flush=True)
+    base = _two_species(nr=24, nt=16)
+    gen = _generator(base["lam_um"])
+    new = rt.dust_arrays([gen(3e-3, 3.1), gen(1e-4, 3.5)], base["lam_um"])
+    fresh = dict(base, kabs=np.array(new[0]), ksca=np.array(new[1]))
+    host = rt.DeviceModel(fresh)
+    dev = rt.DeviceModel(base)
+    dev.set_opacity(np.array(new[0]), np.array(new[1]), on_device=True)
+    a, b = host.tables(), dev.tables()
+    assert np.allclose(a["lnq"], b["lnq"], rtol=1e-12, atol=0)
+    d = np.abs(a["cdf"].astype(float) - b["cdf"].astype(float))
+    print("cdf max |diff| %.2e, entries differing %d of %d" % (d.max(), (d > 0).sum(), d.size))
+    assert d.max() <= 2.0 ** -22
+    for k in ("kplanck", "kross"):
+        assert np.allclose(a[k], b[k], rtol=3e-7, atol=0), k
+    # a guide entry can only move where a CDF value sits at its threshold
+    assert (a["guide"] != b["guide"]).mean() < 1e-4
+    arr = fresh
+    sh = host.thermal(50000, 3, max_events=100000, mrw=1, count_visits=1)
+    Th, vis = host.temperature(with_visits=True)
+    dev.thermal(50000, 3, max_events=100000, mrw=1)
+    Td = dev.temperature()
+    sel = (vis >= 100)[None, :] & (Th > 0)
+    rel = np.abs(Td[sel] - Th[sel]) / Th[sel]
+    print("thermal on device tables vs host tables: median %.2e p99 %.2e"
+          % (np.median(rel), np.percentile(rel, 99)))
+    assert np.median(rel) <= 0.01 and np.percentile(rel, 99) <= 0.05
