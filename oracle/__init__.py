@@ -95,6 +95,7 @@ def _setup(L):
     L.or_image_adjoint.argtypes = [C.POINTER(_Model), P, P, C.POINTER(_Camera), P, P, P]
     L.or_trace_ray.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P]
     L.or_trace_ray_n.argtypes = [C.POINTER(_Model), P, P, C.c_int, P, P, P, P]
+    L.or_trace_ray_from.argtypes = [C.POINTER(_Model), P, P, P, C.c_int, P, P, P]
     L.or_scat_source.argtypes = [C.POINTER(_Model), P, P, P, C.c_double, C.c_uint64,
                                  C.c_uint64, C.c_uint64, C.c_int, P, C.POINTER(_ScatStats)]
     L.or_image_scat.argtypes = [C.POINTER(_Model), P, P, P, P, C.POINTER(_Camera), P]
@@ -281,6 +282,25 @@ def trace_ray(m, o, d, maxseg=100000, with_nudges=False):
     return cell[:m_], ln[:m_], hemi[:m_]
 
 
+def trace_ray_from(m, o, d, idx, maxseg=100000):
+    """cells, segment lengths and hemispheres along a ray from o (inside the
+    grid) along d, starting in the cell idx = (ir, it, ip, hemi) as given
+    rather than the one o lies in"""
+    pm = _Pinned(m)
+    o = _f64(o)
+    d = _f64(d)
+    ix = np.ascontiguousarray(idx, dtype=np.int32)
+    cell = np.zeros(maxseg, dtype=np.int32)
+    ln = np.zeros(maxseg)
+    hemi = np.zeros(maxseg, dtype=np.int32)
+    n = lib().or_trace_ray_from(C.byref(pm.s), o.ctypes.data, d.ctypes.data, ix.ctypes.data,
+                                maxseg, cell.ctypes.data, ln.ctypes.data, hemi.ctypes.data)
+    if n < 0:
+        raise RuntimeError("or_trace_ray_from failed: %d" % n)
+    m_ = min(n, maxseg)
+    return cell[:m_], ln[:m_], hemi[:m_]
+
+
 def vis_dft(img, dxy, u, v, dra=0.0, ddec=0.0):
     img = _f64(img)
     u = _f64(u)
@@ -380,7 +400,8 @@ class Tables:
 def set_mutation(code):
     """Test only: 0 = none, 1 = re-emission spectrum kappa B instead of
     kappa dB/dT, 2 = temperature row shifted by one (applies to tables built
-    and draws made afterwards), 3 = ping-pong guard off."""
+    and draws made afterwards), 3 = ping-pong guard off, 4 = a wall crossing
+    found just behind the position is dropped instead of taken at once."""
     lib().or_set_mutation(int(code))
 
 

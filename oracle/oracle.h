@@ -148,6 +148,12 @@ int or_trace_ray(const or_model *m, const double *o, const double *d,
 /* the same, counting the ping-pong guard's moves in *nudged (nullable) */
 int or_trace_ray_n(const or_model *m, const double *o, const double *d, int maxseg,
                    int *cell, double *len, int *hemi, int *nudged);
+/* the same from a position inside the grid whose cell indices are given,
+ * idx = (ir, it, ip, hemi), not located from the position (a test sets up a
+ * position a hair past a wall with the index one cell behind it); -1 for
+ * indices outside the grid */
+int or_trace_ray_from(const or_model *m, const double *o, const double *d, const int *idx,
+                      int maxseg, int *cell, double *len, int *hemi);
 
 /* Direct DFT: V(u,v) = sum_jk I[j][k] exp(-2 pi i (u x_k + v y_j)) with
  * x_k=(k-nx/2)dxy, y_j=(j-ny/2)dxy (rad), times exp(-2 pi i (u dra + v ddec)). */

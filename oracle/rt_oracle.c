@@ -44,7 +44,9 @@
 
 /* Deliberate mistakes a test plants to show that a pin catches them (0 =
  * none; 1 = re-emission spectrum kappa B instead of kappa dB/dT; 2 =
- * re-emission temperature row shifted up by one; 3 = ping-pong guard off).
+ * re-emission temperature row shifted up by one; 3 = ping-pong guard off;
+ * 4 = a wall crossing found just behind the position is dropped instead of
+ * taken at once).
  * Never set outside such a test. */
 static int g_mutation = 0;
 void or_set_mutation(int code) { g_mutation = code; }
@@ -321,16 +323,30 @@ static double cone_hit(double x, double y, double z, double dx, double dy,
      * through the origin along a wall angle): it never leaves through it */
     if (fabs(A) <= 1e-6 && fabs(C) <= 1e-6 * (x * x + y * y + z * z)) return INF_D;
     double D = B * B - A * C;
-    if (D < 0) return INF_D;
-    double sq = sqrt(D);
-    int sgn_c = cth > 0 ? 1 : -1;
+    /* sreq > 0: leaving through this wall makes f grow */
+    double sreq = cth > 0 ? req : -req;
     double t;
-    if (req * sgn_c > 0) {
-        t = (B < 0) ? (-B + sq) / A : -C / (B + sq);
+    if (D < 0) {
+        /* no crossing: f keeps the sign of C along the line; the far side's
+         * sign means the position is already past the wall */
+        if (g_mutation == 4 || !(C * sreq > 0)) return INF_D;
+        t = 0;
     } else {
-        t = (B > 0) ? -(B + sq) / A : C / (-B + sq);
+        double sq = sqrt(D);
+        if (sreq > 0) {
+            t = (B < 0) ? (-B + sq) / A : -C / (B + sq);
+        } else {
+            t = (B > 0) ? -(B + sq) / A : C / (-B + sq);
+        }
     }
-    if (!(t >= 0) || !(t < INF_D)) return INF_D;
+    /* exit root behind the position while the ray moves through the wall:
+     * the position is a hair past the wall with the index one cell behind it
+     * (two crossings closer than the arithmetic resolves): cross at once */
+    if (!(t >= 0)) {
+        if (g_mutation == 4 || !(B * sreq > 0)) return INF_D;
+        t = 0;
+    }
+    if (!(t < INF_D)) return INF_D;
     double zh = z + t * dz;
     if (zh * cth < 0) return INF_D;
     return t;
@@ -342,7 +358,11 @@ static double phi_hit(double x, double y, double dx, double dy, double cph,
     double g1 = -dx * sph + dy * cph;
     if (req < 0 ? !(g1 < 0) : !(g1 > 0)) return INF_D;
     double t = -g0 / g1;
-    if (!(t >= 0)) return INF_D;
+    /* already past the wall, moving through it: cross at once */
+    if (!(t >= 0)) {
+        if (g_mutation == 4) return INF_D;
+        t = 0;
+    }
     double xh = x + t * dx, yh = y + t * dy;
     if (xh * cph + yh * sph < 0) return INF_D;
     return t;
@@ -370,7 +390,7 @@ static double next_wall(const geom_t *g, const pk_t *p, int *kind) {
             /* equatorial plane seen from the lower hemisphere: g = -z, need g'<0 */
             if (p->dz > 0) {
                 t = -p->z / p->dz;
-                if (t < 0) t = INF_D;
+                if (t < 0) t = g_mutation == 4 ? INF_D : 0;
                 if (t < tbest) {
                     tbest = t;
                     *kind = W_TLO;
@@ -397,7 +417,7 @@ static double next_wall(const geom_t *g, const pk_t *p, int *kind) {
         } else if (!g->mirror && fabs(g->ct[j + 1]) < 1e-15) {
             if (p->dz < 0) {
                 t = -p->z / p->dz;
-                if (t < 0) t = INF_D;
+                if (t < 0) t = g_mutation == 4 ? INF_D : 0;
                 if (t < tbest) {
                     tbest = t;
                     *kind = W_THI;
@@ -503,6 +523,28 @@ static int unstick(const geom_t *g, pk_t *p, double tw, double rcell, int *nzero
     return 1;
 }
 
+/* walk from p (cell indices as given) until it leaves the grid, recording
+ * up to maxseg segments; returns the number of segments */
+static int walk_segments(const geom_t *g, pk_t *p, int maxseg, int *cell, double *len,
+                         int *hemi, int *nudged) {
+    int n = 0, nzero = 0;
+    while (p->ir >= -1 && p->ir < g->nr) {
+        int kind;
+        double t = next_wall(g, p, &kind);
+        double rcell = g->rw[p->ir + 1];
+        if (n < maxseg) {
+            cell[n] = (p->ir < 0) ? -1 : cell_of(g, p);
+            len[n] = t;
+            if (hemi) hemi[n] = p->hemi;
+        }
+        n++;
+        if (cross_wall(g, p, t, kind)) break;
+        if (unstick(g, p, t, rcell, &nzero) && nudged) ++*nudged;
+        if (n > 10000000) break;
+    }
+    return n;
+}
+
 int or_trace_ray_n(const or_model *m, const double *o, const double *d, int maxseg,
                    int *cell, double *len, int *hemi, int *nudged) {
     geom_t g;
@@ -536,21 +578,34 @@ int or_trace_ray_n(const or_model *m, const double *o, const double *d, int maxs
     } else {
         locate(&g, &p);
     }
-    int nzero = 0;
-    while (p.ir >= -1 && p.ir < g.nr) {
-        int kind;
-        double t = next_wall(&g, &p, &kind);
-        double rcell = g.rw[p.ir + 1];
-        if (n < maxseg) {
-            cell[n] = (p.ir < 0) ? -1 : cell_of(&g, &p);
-            len[n] = t;
-            if (hemi) hemi[n] = p.hemi;
-        }
-        n++;
-        if (cross_wall(&g, &p, t, kind)) break;
-        if (unstick(&g, &p, t, rcell, &nzero) && nudged) ++*nudged;
-        if (n > 10000000) break;
+    n = walk_segments(&g, &p, maxseg, cell, len, hemi, nudged);
+    geom_free(&g);
+    return n;
+}
+
+int or_trace_ray_from(const or_model *m, const double *o, const double *d, const int *idx,
+                      int maxseg, int *cell, double *len, int *hemi) {
+    geom_t g;
+    int rc = geom_init(&g, m);
+    if (rc) return rc;
+    if (idx[0] < -1 || idx[0] >= g.nr || idx[1] < 0 || idx[1] >= g.nt || idx[2] < 0 ||
+        idx[2] >= g.np || (idx[3] != 1 && idx[3] != -1) || (!g.mirror && idx[3] != 1)) {
+        geom_free(&g);
+        return -1;
     }
+    pk_t p;
+    memset(&p, 0, sizeof(p));
+    p.x = o[0];
+    p.y = o[1];
+    p.z = o[2];
+    p.dx = d[0];
+    p.dy = d[1];
+    p.dz = d[2];
+    p.ir = idx[0];
+    p.it = idx[1];
+    p.ip = idx[2];
+    p.hemi = idx[3];
+    int n = walk_segments(&g, &p, maxseg, cell, len, hemi, NULL);
     geom_free(&g);
     return n;
 }

@@ -37,29 +37,65 @@ struct CamParams {
     const float *csize;  // [cell] cell size for the sub-pixeling (au)
 };
 
-__device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, Pos &p) {
+// A camera ray is the line p0 + s d: p0 the point closest to the origin (in
+// the sky plane through it), d = -n.  Marching the FP32 position from the
+// outer sphere accumulates a transverse error of a few ulp of the outer radius
+// (~1e-3 au at 5000 au); on a nearly radial ray (the central pixels) the theta
+// crossings then move along the ray by that error times r/b, a sizeable part
+// of a cell.  At every radial crossing the position is therefore set back on
+// the exact line at s = -+sqrt(r_wall^2 - b^2) (FP64), which leaves only the
+// error gathered since the last radial crossing, a few ulp of the local radius.
+struct CamRay {
+    float x0, y0, z0;  // p0 (au)
+    double b2;         // |p0|^2
+};
+
+__device__ __forceinline__ bool ray_start(const CamParams &C, float X, float Y, Pos &p,
+                                          CamRay &cr) {
     float xp = X * C.cpa + Y * C.spa;
     float yp = -X * C.spa + Y * C.cpa;
-    float R = C.g.rw[C.g.nr];
-    float q2 = xp * xp + yp * yp;
-    if (q2 >= C.g.rw2[C.g.nr]) return false;
-    float h = sqrtf(C.g.rw2[C.g.nr] - q2);
-    p.x = xp * C.ex[0] + yp * C.ey[0] + h * C.n[0];
-    p.y = xp * C.ex[1] + yp * C.ey[1] + h * C.n[1];
-    p.z = xp * C.ex[2] + yp * C.ey[2] + h * C.n[2];
+    cr.x0 = xp * C.ex[0] + yp * C.ey[0];
+    cr.y0 = xp * C.ex[1] + yp * C.ey[1];
+    cr.z0 = xp * C.ex[2] + yp * C.ey[2];
+    cr.b2 = (double)cr.x0 * cr.x0 + (double)cr.y0 * cr.y0 + (double)cr.z0 * cr.z0;
+    const double R = C.g.rwd[C.g.nr];
+    if (cr.b2 >= R * R) return false;
     p.dx = -C.n[0];
     p.dy = -C.n[1];
     p.dz = -C.n[2];
-    snap_radius(p, R);
+    const float s = (float)-sqrt(R * R - cr.b2);
+    p.x = fmaf(s, p.dx, cr.x0);
+    p.y = fmaf(s, p.dy, cr.y0);
+    p.z = fmaf(s, p.dz, cr.z0);
     p.ir = C.g.nr - 1;
     locate_angles(C.g, p);
     return true;
 }
 
+// cross_wall for a camera ray: radial crossings put the position on the exact
+// line (see CamRay); theta, phi and mirror crossings as cross_wall
+__device__ __forceinline__ bool cross_wall_cam(const DevGrid &g, Pos &p, float t, int kind,
+                                               const CamRay &cr) {
+    if (kind != W_RIN && kind != W_ROUT && kind != W_HOLE) return cross_wall(g, p, t, kind);
+    const bool out = kind != W_RIN;
+    const int ir = kind == W_HOLE ? 0 : p.ir + (out ? 1 : -1);
+    p.ir = ir;
+    if (ir >= g.nr) return true;
+    const double rw = g.rwd[out ? ir : ir + 1];
+    float s = (float)sqrt(fmax(rw * rw - cr.b2, 0.0));
+    if (!out) s = -s;
+    p.x = fmaf(s, p.dx, cr.x0);
+    p.y = fmaf(s, p.dy, cr.y0);
+    p.z = fmaf(s, p.dz, cr.z0);
+    if (kind == W_HOLE) locate_angles(g, p);
+    return false;
+}
+
 // formal solution along the ray from p; with LMIN also the smallest cell size
 // on the part of the ray traced (RT_INF when it crosses no cell)
 template <bool LMIN>
-__device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, float &lmin) {
+__device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, const CamRay &cr,
+                                                 float &lmin) {
     float I = 0.f, tau = 0.f;
     uint32_t nzero = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {
@@ -78,15 +114,15 @@ __device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, floa
                 tau += dtau;
             }
         }
-        if (cross_wall(C.g, p, tw, kind)) break;
+        if (cross_wall_cam(C.g, p, tw, kind, cr)) break;
         if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
     }
     return C.mode == 0 ? I : tau;
 }
 
-__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p) {
+__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p, const CamRay &cr) {
     float unused = RT_INF;
-    return integrate_ray_l<false>(C, p, unused);
+    return integrate_ray_l<false>(C, p, cr, unused);
 }
 
 // pixel of this lane: blocks cover tw x th tiles of the image, so the rays of
@@ -109,7 +145,8 @@ __global__ void __launch_bounds__(256) k_image(CamParams C, float *img) {
             float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
             float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
             Pos p;
-            if (ray_start(C, X, Y, p)) acc += integrate_ray(C, p);
+            CamRay cr;
+            if (ray_start(C, X, Y, p, cr)) acc += integrate_ray(C, p, cr);
         }
     img[pix] = acc / (float)(C.nsub * C.nsub);
 }
@@ -151,8 +188,9 @@ __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y
                                             int *key, double *ga, double *gs, double *g_alpha,
                                             double *g_src) {
     Pos p0;
-    if (!ray_start(C, X, Y, p0)) return;
-    float I = integrate_ray(C, p0);
+    CamRay cr;
+    if (!ray_start(C, X, Y, p0, cr)) return;
+    float I = integrate_ray(C, p0, cr);
     Pos p = p0;
     float tau = 0.f, cum = 0.f;
     uint32_t nzero = 0;
@@ -174,7 +212,7 @@ __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y
             tau += dtau;
             if (tau > C.tau_stop) break;
         }
-        if (cross_wall(C.g, p, tw, kind)) break;
+        if (cross_wall_cam(C.g, p, tw, kind, cr)) break;
         if (unstick(C.g, p, tw, rcell, nzero) && p.ir >= C.g.nr) break;
     }
 }
@@ -298,8 +336,9 @@ __global__ void __launch_bounds__(256) k_ref_trace(CamParams C, RefArgs R, int L
         if (act) {
             xy = R.xy[i];
             Pos p;
-            if (xy.x < 0.5f * RT_REF_DEAD && ray_start(C, xy.x, xy.y, p)) {
-                I = integrate_ray_l<true>(C, p, lmin);
+            CamRay cr;
+            if (xy.x < 0.5f * RT_REF_DEAD && ray_start(C, xy.x, xy.y, p, cr)) {
+                I = integrate_ray_l<true>(C, p, cr, lmin);
                 ++nrays;
             }
             R.val[i] = I;

@@ -180,6 +180,14 @@ __device__ __forceinline__ float sph_in(const Pos &p, float b, float rr, float R
 // z^2 sin^2; root with f' > 0 is (-B+sqrt(D))/A, with f' < 0 is (-B-sqrt(D))/A;
 // on the nappe sign(z) = sign(cos th) the sign of dtheta/dt is
 // sign(f') * sign(cos th).  req = -1 leaves towards smaller theta, +1 larger.
+// A ray whose cell index is consistent with its position always meets its
+// exit root ahead of it.  When a radial and a theta crossing lie within FP32
+// resolution of each other, the ray can take the radial one first and stand a
+// hair past the cone while the index still names the cell before it: the exit
+// root is then just behind the position (the ray moving through the wall in
+// the req direction), or, near tangency, missing while f already has the far
+// side's sign.  The wall is then crossed at once (t = 0); rejecting it would
+// leave the theta index behind for the rest of the ray.
 __device__ __forceinline__ float cone_root(double A, double B, double C, double D,
                                            int want_plus) {
     double sq = sqrt(D);
@@ -202,7 +210,8 @@ __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, f
     // through the origin along a wall angle): it never leaves through it
     if (fabsf(A) <= 1e-6f && fabsf(C) <= 1e-6f * (x * x + y * y + z * z)) return RT_INF;
     float D = B * B - A * C;
-    int want_plus = (req * (cth > 0.f ? 1 : -1)) > 0;
+    const float sreq = (float)(cth > 0.f ? req : -req);
+    int want_plus = sreq > 0.f;
     float t;
     if (D < 1e-4f * B * B) {
         double c2d = cthd * cthd, s2d = sthd * sthd;
@@ -210,8 +219,13 @@ __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, f
         double Bd = ((double)x * dx + (double)y * dy) * c2d - (double)z * dz * s2d;
         double Cd = ((double)x * x + (double)y * y) * c2d - (double)z * z * s2d;
         double Dd = Bd * Bd - Ad * Cd;
-        if (Dd < 0.0) return RT_INF;
-        t = cone_root(Ad, Bd, Cd, Dd, want_plus);
+        if (Dd < 0.0) {
+            // no crossing: f keeps the sign of C along the whole line
+            if (!(Cd * sreq > 0.0)) return RT_INF;
+            t = 0.f;
+        } else {
+            t = cone_root(Ad, Bd, Cd, Dd, want_plus);
+        }
     } else {
         float sq = sqrtf(D);
         if (want_plus)
@@ -219,19 +233,25 @@ __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, f
         else
             t = (B > 0.f) ? -(B + sq) / A : C / (-B + sq);
     }
-    if (!(t >= 0.f) || !(t < RT_INF)) return RT_INF;
+    if (!(t >= 0.f)) {
+        if (!(B * sreq > 0.f)) return RT_INF;
+        t = 0.f;
+    }
+    if (!(t < RT_INF)) return RT_INF;
     float zh = fmaf(t, dz, z);
     if (zh * cth < 0.f) return RT_INF;
     return t;
 }
 
+// Exit through the phi half-plane (cos ph, sin ph); a crossing just behind
+// the position (the ray already past the wall, moving through it) is taken at
+// once, as for the cones.
 __device__ __forceinline__ float phi_hit(float x, float y, float dx, float dy, float cph,
                                          float sph, int req) {
     float g0 = -x * sph + y * cph;
     float g1 = -dx * sph + dy * cph;
     if (req < 0 ? !(g1 < 0.f) : !(g1 > 0.f)) return RT_INF;
-    float t = -g0 / g1;
-    if (!(t >= 0.f)) return RT_INF;
+    float t = fmaxf(-g0 / g1, 0.f);
     float xh = fmaf(t, dx, x), yh = fmaf(t, dy, y);
     if (xh * cph + yh * sph < 0.f) return RT_INF;
     return t;
@@ -256,9 +276,10 @@ __device__ __forceinline__ float next_wall(const DevGrid &g, const Pos &p, int &
     int j = p.it;
     if (j > 0) {
         if (!g.mirror && g.ct[j] == 0.f) {
+            // the equatorial plane, left upwards (a crossing behind the
+            // position is taken at once)
             if (p.dz > 0.f) {
-                t = -p.z / p.dz;
-                if (t < 0.f) t = RT_INF;
+                t = fmaxf(-p.z / p.dz, 0.f);
                 if (t < tb) {
                     tb = t;
                     kind = W_TLO;
@@ -283,8 +304,7 @@ __device__ __forceinline__ float next_wall(const DevGrid &g, const Pos &p, int &
     } else if (j + 1 < g.nt) {
         if (!g.mirror && g.ct[j + 1] == 0.f) {
             if (p.dz < 0.f) {
-                t = -p.z / p.dz;
-                if (t < 0.f) t = RT_INF;
+                t = fmaxf(-p.z / p.dz, 0.f);
                 if (t < tb) {
                     tb = t;
                     kind = W_THI;

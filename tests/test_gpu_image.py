@@ -61,9 +61,9 @@ def test_image_matches_oracle(incl, pa):
                        incl_star=True)
     err = np.abs(img - ref).max() / np.abs(ref).max()
     print("image max rel err %.2e" % err)
-    assert err < 2e-3
-    # FP32 vs FP64 traversal: total flux to 1e-4
-    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+    assert err < 2e-5
+    # FP32 vs FP64 traversal: total flux to 1e-6
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-6)
 
 
 def test_image_matches_oracle_distinct_dust():
@@ -84,8 +84,8 @@ def test_image_matches_oracle_distinct_dust():
                        incl_star=True)
     err = np.abs(img - ref).max() / np.abs(ref).max()
     print("image (distinct dust) max rel err %.2e" % err)
-    assert err < 2e-3
-    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+    assert err < 5e-5
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-6)
 
 
 def test_ray_tiles_do_not_change_results():
@@ -116,14 +116,45 @@ def test_image_3d_and_tau_mode_match_oracle():
     dm = _dev(arr)
     kap = rt._loglog_interp(np.array([870.0]), arr["lam_um"], arr["kabs"][0])[0]
     npix, pix = 40, 6.0 * AU
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
     for mode in (0, 1):
-        cam = dm.camera(npix, pix, 870.0, incl=45.0, pa=10.0, phi=30.0, mode=mode, incl_star=False)
-        img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
-        ref = oracle.image(arr, T, [kap, kap], npix, pix, incl=45.0, posang=10.0, phi=30.0,
+        cam = dm.camera(npix, pix, 870.0, incl=44.0, pa=10.0, phi=31.0, mode=mode, incl_star=False)
+        img = dm.image(cam, Td).double().cpu().numpy()
+        ref = oracle.image(arr, T, [kap, kap], npix, pix, incl=44.0, posang=10.0, phi=31.0,
                            lam_um=870.0, mode=mode)
         err = np.abs(img - ref).max() / np.abs(ref).max()
         print("mode %d max rel err %.2e" % (mode, err))
-        assert err < 2e-3
+        assert err < 2e-5
+
+
+def test_image_3d_central_ray_on_a_theta_and_a_phi_wall():
+    """At incl 45 deg and phi 30 deg the central pixel's radial ray lies in a
+    theta cone (walls every 5.625 deg) and in a phi half-plane (walls every
+    30 deg).  Which side of each wall it is charged to is a matter of rounding,
+    FP32 and FP64 alike, so that pixel is held to the range the oracle spans
+    with the camera turned 1e-5 deg to either side of both walls; every other
+    pixel to the oracle itself."""
+    arr, T = _disk_setup(nr=24, nt=16, nphi=12)
+    dm = _dev(arr)
+    kap = rt._loglog_interp(np.array([870.0]), arr["lam_um"], arr["kabs"][0])[0]
+    npix, pix = 40, 6.0 * AU
+    cam = dm.camera(npix, pix, 870.0, incl=45.0, pa=10.0, phi=30.0, incl_star=False)
+    img = dm.image(cam, torch.tensor(T, dtype=torch.float32, device="cuda")).double().cpu().numpy()
+
+    def ref(di=0.0, dp=0.0):
+        return oracle.image(arr, T, [kap, kap], npix, pix, incl=45.0 + di, posang=10.0,
+                            phi=30.0 + dp, lam_um=870.0)
+
+    r0 = ref()
+    c = npix // 2
+    d = np.abs(img - r0) / r0.max()
+    d[c, c] = 0.0
+    print("off-centre max rel err %.2e" % d.max())
+    assert d.max() < 2e-5
+    side = [ref(di, dp)[c, c] for di in (-1e-5, 1e-5) for dp in (-1e-5, 1e-5)]
+    lo, hi = min(side), max(side)
+    print("centre %.5e, oracle beside the walls [%.5e, %.5e]" % (img[c, c], lo, hi))
+    assert lo - 0.02 * hi <= img[c, c] <= 1.02 * hi
 
 
 def test_image_adjoint_matches_oracle_fp64_gradient():
@@ -141,7 +172,7 @@ def test_image_adjoint_matches_oracle_fp64_gradient():
     e_lr = np.linalg.norm(g_lr.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
     e_T = np.linalg.norm(g_T.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
     print("relative gradient error: logrho %.2e  T %.2e" % (e_lr, e_T))
-    assert e_lr < 2e-3 and e_T < 2e-3
+    assert e_lr < 5e-5 and e_T < 5e-5
 
 
 def test_image_adjoint_central_differences_gpu():
@@ -393,13 +424,29 @@ def test_chi2_of_visibilities_gradient_matches_oracle_chain():
     oimg = oracle.image(arr, T, [kap, kap], npix, pix_cm, incl=60.0, lam_um=1300.0, incl_star=True)
     ore, oim = oracle.vis_dft(oimg, dxy, u, v)
     d_re, d_im, ww = dre.double().cpu().numpy(), dim.double().cpu().numpy(), w.double().cpu().numpy()
-    gre, gim = -2 * ww * (d_re - ore), -2 * ww * (d_im - oim)
-    gimg = oracle.vis_dft_adjoint(gre, gim, npix, npix, dxy, u, v)
-    o_lr, o_T = oracle.image_adjoint(arr, T, [kap, kap], gimg, npix, pix_cm, incl=60.0,
-                                     lam_um=1300.0, incl_star=True)
-    e_lr = np.linalg.norm(lr.grad.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
-    e_T = np.linalg.norm(Tt.grad.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
-    print("chi2 chain relative gradient error: logrho %.2e T %.2e" % (e_lr, e_T))
+    g_lr = lr.grad.double().cpu().numpy()
+    g_T = Tt.grad.double().cpu().numpy()
+
+    def oracle_chain(mre_, mim_):
+        gre, gim = -2 * ww * (d_re - mre_), -2 * ww * (d_im - mim_)
+        gimg = oracle.vis_dft_adjoint(gre, gim, npix, npix, dxy, u, v)
+        o_lr, o_T = oracle.image_adjoint(arr, T, [kap, kap], gimg, npix, pix_cm, incl=60.0,
+                                         lam_um=1300.0, incl_star=True)
+        return (np.linalg.norm(g_lr - o_lr) / np.linalg.norm(o_lr),
+                np.linalg.norm(g_T - o_T) / np.linalg.norm(o_T))
+
+    # the chain itself: the oracle's adjoints applied to the GPU model's
+    # residual (data - model)
+    e_lr, e_T = oracle_chain(re.double().cpu().numpy(), im.double().cpu().numpy())
+    print("chi2 chain relative gradient error, same residual: logrho %.2e T %.2e" % (e_lr, e_T))
+    assert e_lr < 5e-5 and e_T < 5e-5
+    # end to end, the oracle with its own FP64 model: the data here are the
+    # model plus noise, so the residual is pure noise and the ~1e-6 difference
+    # of the FP32 model (exp of a float32 log-density included) enters it
+    # coherently; the DFT adjoint sums that smooth difference over the
+    # (u,v) points in phase and the noise out of phase, which raises it ~40x
+    e_lr, e_T = oracle_chain(ore, oim)
+    print("chi2 chain relative gradient error, FP64 model: logrho %.2e T %.2e" % (e_lr, e_T))
     assert e_lr < 1e-2 and e_T < 1e-2
 
 
@@ -545,8 +592,8 @@ def test_image_with_scattering_source_matches_oracle():
     err = np.abs(img - ref).max() / np.abs(ref).max()
     print("image (scattering) max rel err %.2e; scattering changes the image by %.2e" % (
         err, np.abs(ref - plain).max() / np.abs(plain).max()))
-    assert err < 2e-3
-    assert img.sum() == pytest.approx(ref.sum(), rel=1e-4)
+    assert err < 2e-5
+    assert img.sum() == pytest.approx(ref.sum(), rel=1e-6)
     assert np.abs(ref - plain).max() > 0.05 * np.abs(plain).max()
 
 
@@ -568,7 +615,7 @@ def test_image_adjoint_with_scattering_source_matches_oracle():
     e_lr = np.linalg.norm(g_lr.double().cpu().numpy() - o_lr) / np.linalg.norm(o_lr)
     e_T = np.linalg.norm(g_T.double().cpu().numpy() - o_T) / np.linalg.norm(o_T)
     print("relative gradient error (scattering): logrho %.2e  T %.2e" % (e_lr, e_T))
-    assert e_lr < 2e-3 and e_T < 2e-3
+    assert e_lr < 5e-5 and e_T < 5e-5
 
 
 def test_run_image_adds_scattering_source_unless_disabled():

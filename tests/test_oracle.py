@@ -131,6 +131,52 @@ def test_trace_ray_through_origin_along_a_wall():
         assert len(cell) <= 4 * len(m["rw"])
 
 
+def _past_wall_cases():
+    """(model, position a hair past a wall, direction through it, cell index
+    one cell behind): a theta cone (mirror grid), a phi half-plane (3D grid)
+    and the equatorial plane (grid over [0, pi])"""
+    eps = 1e-11
+    m = _small_model(nt=8)
+    r = np.sqrt(m["rw"][2] * m["rw"][3])
+    th = m["tw"][4] + eps
+    e_r = np.array([np.sin(th), 0.0, np.cos(th)])
+    e_th = np.array([np.cos(th), 0.0, -np.sin(th)])
+    cone = (m, r * e_r, 0.8 * e_th + 0.6 * e_r, (2, 3, 0, 1))
+    m = _small_model(nphi=6)
+    th, ph = 0.3, m["pw"][2] + eps
+    e_r = np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    e_ph = np.array([-np.sin(ph), np.cos(ph), 0.0])
+    phi = (m, r * e_r, 0.8 * e_ph + 0.6 * e_r, (2, 0, 1, 1))
+    m = _small_model(mirror=False)
+    plane = (m, np.array([r, 0.0, -eps * r]), np.array([0.6, 0.0, -0.8]), (2, 1, 0, 1))
+    return [cone, phi, plane]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2], ids=["cone", "phi", "plane"])
+def test_wall_just_behind_the_position_is_crossed_at_once(case):
+    """Two crossings closer than the arithmetic resolves can leave a ray a
+    hair past a wall while its cell index still names the cell before it.
+    The walk then crosses that wall at once (a zero-length segment) and
+    continues exactly as a walk started from the cell the position lies in;
+    with the crossing planted to be dropped instead (the rule before), the
+    index stays behind and the cells walked differ."""
+    m, o, d, idx = _past_wall_cases()[case]
+    d = d / np.linalg.norm(d)
+    nr, nt = len(m["rw"]) - 1, len(m["tw"]) - 1
+    c_ref, l_ref, h_ref = oracle.trace_ray(m, o, d)
+    assert c_ref[0] == _locate(m, o) != idx[0] + nr * (idx[1] + nt * idx[2])
+    c, ln, h = oracle.trace_ray_from(m, o, d, idx)
+    assert c[0] == idx[0] + nr * (idx[1] + nt * idx[2]) and ln[0] == 0.0
+    assert np.array_equal(c[1:], c_ref) and np.array_equal(h[1:], h_ref)
+    assert np.allclose(ln[1:], l_ref, rtol=1e-12, atol=1e-9 * m["rw"][-1])
+    try:
+        oracle.set_mutation(4)
+        c4, _, _ = oracle.trace_ray_from(m, o, d, idx)
+    finally:
+        oracle.set_mutation(0)
+    assert len(c4) != len(c_ref) + 1 or not np.array_equal(c4[1:], c_ref)
+
+
 # ------------------------------------------------------------------ thermal MC
 def _thin_shell_model(nr=4, nt=4, rho=1e-24, kappa=10.0):
     lam = np.logspace(-1, 4, 100)
