@@ -300,6 +300,17 @@ int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
                 const double *d_u, const double *d_v, int K, double dra, double ddec,
                 float *d_re, float *d_im, void *d_work);
 size_t rt_vis_grid_workspace(int nx, int ny);
+/* rt_vis_grid without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under stream capture,
+ * tight sampler loops, harnesses that forbid hipMalloc.  Its workspace is a
+ * static device array of the library (16.8 MB on each GPU it runs on, held
+ * until the process ends), so nx, ny <= 1024 (RT_E_LIMIT beyond).  Same
+ * arguments and results as rt_vis_grid with d_work.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (two streams, two host
+ * threads) race and must be serialised by the caller. */
+int rt_vis_grid_noalloc(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                        const double *d_u, const double *d_v, int K, double dra, double ddec,
+                        float *d_re, float *d_im);
 /* pdspy's interpolate_model(code="galario") for one channel of a pdspy Image
  * (interpolate_model.py:20-27, which calls galario's
  * sampleImage(image[::-1], dxy, u, v, dRA, dDec) and negates the imaginary

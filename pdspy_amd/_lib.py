@@ -20,7 +20,7 @@ EXPORTS = [
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_vis_grid",
-    "rt_vis_grid_workspace", "rt_interpolate_galario", "rt_chisq",
+    "rt_vis_grid_workspace", "rt_vis_grid_noalloc", "rt_interpolate_galario", "rt_chisq",
     "rt_batch_create", "rt_batch_destroy", "rt_batch_thermal", "rt_batch_step",
     "rt_refine_stats", "rt_refine_reserve",
 ]
@@ -143,6 +143,7 @@ def load():
         "rt_vis_dft_adjoint": ([P, P, P, I, I, D, P, P, I, D, D, P], I),
         "rt_vis_grid": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_vis_grid_workspace": ([I, I], C.c_size_t),
+        "rt_vis_grid_noalloc": ([P, P, I, I, D, P, P, I, D, D, P, P], I),
         "rt_refine_stats": ([P, C.POINTER(U64), C.POINTER(U64)], I),
         "rt_refine_reserve": ([P, U64], I),
         "rt_interpolate_galario": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),

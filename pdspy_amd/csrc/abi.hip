@@ -308,6 +308,11 @@ int rt_create(int device, rt_ctx **out) {
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
         khz = 100000;
     c->wallclock_khz = khz;
+    // the allocation-free sampler's scratch address on this device
+    if (vis_scratch_init(c, device) != RT_OK) {
+        delete c;
+        return RT_E_HIP;
+    }
     *out = c;
     return RT_OK;
 }
@@ -594,6 +599,13 @@ int rt_vis_grid(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, con
                 void *d_work) {
     if (!ctx) return RT_E_ARG;
     return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, d_work, 0);
+}
+
+int rt_vis_grid_noalloc(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                        const double *d_u, const double *d_v, int K, double dra, double ddec,
+                        float *d_re, float *d_im) {
+    if (!ctx) return RT_E_ARG;
+    return vis_grid_noalloc_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im);
 }
 
 int rt_interpolate_galario(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,

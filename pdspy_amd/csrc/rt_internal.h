@@ -246,3 +246,9 @@ int vis_grid_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, co
                  const double *d_v, int K, double dra, double ddec, float *d_re, float *d_im,
                  void *d_work, int conv);
 size_t vis_grid_workspace(int nx, int ny);
+// the allocation-free gridded sampler: its scratch's device address for the
+// current device (at context creation), and the run
+int vis_scratch_init(rt_ctx *ctx, int device);
+int vis_grid_noalloc_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                         const double *d_u, const double *d_v, int K, double dra, double ddec,
+                         float *d_re, float *d_im);

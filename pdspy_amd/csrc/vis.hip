@@ -16,6 +16,7 @@
 // 64 points the tile's column phasors (times the point's complex cotangent) and
 // row phasors are built once in LDS and each lane does two FMAs per point.
 #include <cmath>
+#include <mutex>
 
 #include "rt_internal.h"
 
@@ -298,6 +299,52 @@ __global__ void __launch_bounds__(GRID_BS) k_grid_sample(const float2 *F, int nx
 
 size_t vis_grid_workspace(int nx, int ny) {
     return sizeof(float2) * (2 * (size_t)nx * ny + (size_t)nx + ny);
+}
+
+// Scratch of the allocation-free gridded sampler (rt_vis_grid_noalloc): the
+// workspace of an image of up to VIS_NOALLOC_MAXN x VIS_NOALLOC_MAXN pixels
+// (16.8 MB), one copy per GPU for the life of the process.  Host code never
+// passes the array's name to a kernel: it is a host-side stand-in, and the
+// device address of each GPU's copy is looked up once per device.
+#define VIS_NOALLOC_MAXN 1024
+static __device__ __align__(256) float2 scratch[2 * VIS_NOALLOC_MAXN * VIS_NOALLOC_MAXN +
+                                                 2 * VIS_NOALLOC_MAXN];
+#define VIS_MAX_DEVICES 64
+static void *g_scratch_ptr[VIS_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// with `device` current: the device address of this GPU's scratch array
+int vis_scratch_init(rt_ctx *ctx, int device) {
+    if (device < 0 || device >= VIS_MAX_DEVICES)
+        return set_err(ctx, RT_E_LIMIT, "device %d beyond the %d the library supports", device,
+                       VIS_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return RT_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_err(ctx, RT_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return RT_OK;
+}
+
+int vis_grid_noalloc_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy,
+                         const double *d_u, const double *d_v, int K, double dra, double ddec,
+                         float *d_re, float *d_im) {
+    if (nx > VIS_NOALLOC_MAXN || ny > VIS_NOALLOC_MAXN)
+        return set_err(ctx, RT_E_LIMIT, "rt_vis_grid_noalloc: %d x %d image exceeds its scratch "
+                                        "(at most %d x %d pixels); use rt_vis_grid", nx, ny,
+                       VIS_NOALLOC_MAXN, VIS_NOALLOC_MAXN);
+    void *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < VIS_MAX_DEVICES) ptr = g_scratch_ptr[ctx->device];
+    }
+    if (!ptr) return set_err(ctx, RT_E_HIP, "rt_vis_grid_noalloc: scratch of device %d not set up",
+                             ctx->device);
+    // every scratch word the kernels read is written first in the same call
+    // (twiddles, row pass, column pass), so nothing needs clearing
+    return vis_grid_run(ctx, d_img, nx, ny, dxy, d_u, d_v, K, dra, ddec, d_re, d_im, ptr, 0);
 }
 
 // conv 0: the imtovis convention above.  conv 1: pdspy's galario call
