@@ -53,6 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_TFLOPS = 78.6  # 256 CU x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz, no packed FP32
 STAGGER_S = 0.1  # least time between two thermal launches of the overlapped steps
 CALIB_BYTES = 256 << 20  # device copy whose FETCH_SIZE / WRITE_SIZE scale the counters
 
@@ -419,17 +420,15 @@ def main():
     last = max(r[2] for r in done)
 
     # roofline of the MC kernel: algorithmic bytes B_alg = N_cross * 4 S +
-    # N_abs * 16 per launch.  With one lane the launches run back to back and
-    # the rate is B_alg over the launch's HIP-event duration; with several the
-    # launches overlap (a HIP-event duration then includes the time the launch
-    # shares the GPU with its neighbours), so the rate is the launches'
-    # algorithmic bytes over the timed region's wall time
+    # N_abs * 16 per launch over the launch's average duration (HIP events on
+    # the stream it runs on, the figure rocprofv3's kernel trace reports).
+    # With several lanes the launches overlap and a duration includes the time
+    # a launch shares the GPU with its neighbour; the timed launches'
+    # algorithmic bytes over the timed wall time are reported beside it
     kern_ms = sum(st["kernel_ms"] for st in stats) / steps
     b_alg = sum(x["n_cross"] * 4 * S + x["n_abs"] * 16 for x in stats) / steps
-    if nl == 1:
-        achieved = b_alg / (kern_ms * 1e-3) / 1e9
-    else:
-        achieved = b_alg * steps / wall / 1e9
+    achieved = b_alg / (kern_ms * 1e-3) / 1e9
+    achieved_wall = b_alg * steps / wall / 1e9
     traffic, traffic_src = None, None
     if os.path.exists(a.traffic_json):
         try:
@@ -448,10 +447,11 @@ def main():
             "kernel": "k_thermal", "b_alg_per_launch": b_alg, "kernel_ms_per_launch": kern_ms,
             "launches_per_step": 1, "lanes": nl,
             "timing": ("HIP events on the launch's stream" if nl == 1 else
-                       "launches overlap on %d streams: algorithmic bytes of the timed "
-                       "launches over the timed wall time; kernel_ms_per_launch is the "
-                       "HIP-event duration of a launch (it includes the time shared with "
-                       "the neighbouring launch)" % nl),
+                       "HIP events on each launch's stream; the launches overlap on %d "
+                       "streams, so a duration includes the time shared with the "
+                       "neighbouring launch (achieved_over_wall: the timed launches' "
+                       "algorithmic bytes over the timed wall time)" % nl),
+            "achieved_over_wall": achieved_wall,
             "steps_per_s": lane_steps / (kern_ms * steps * 1e-3),
             "lane_utilisation": lane_steps / (64.0 * max(wave_iters, 1))}
 
@@ -519,10 +519,38 @@ def main():
             dm.image_adjoint(cam, T, None, dimg)
         torch.cuda.synchronize()
         t_fa = (time.perf_counter() - t0) / n_rep
+        # cells crossed by one image's rays and by one adjoint's reverse sweeps
+        dm.image_crossings(reset=True)
+        dm.image(cam, T)
+        n_img, _ = dm.image_crossings(reset=True)
+        dm.image_adjoint(cam, T, None, dimg)
+        _, n_adj = dm.image_crossings(reset=True)
+        # algorithmic bytes: k_image reads alpha and S (8 B) per crossed cell
+        # and writes 4 B per pixel; the adjoint reads them twice (forward and
+        # reverse sweep), reads the cotangent image and writes two FP64
+        # gradients per cell
+        npx, ncell_img = 65536, dm.ncell
+        b_fwd = 8.0 * n_img + 4.0 * npx
+        b_adj = 16.0 * n_adj + 4.0 * npx + 16.0 * ncell_img
+        t_adj = max(t_fa - t_f, 1e-9)
         out["image"] = {"rays_per_s_fwd": 65536 / t_f, "rays_per_s_fwd_adj": 65536 / t_fa,
                         "ms_fwd": t_f * 1e3, "ms_fwd_adj": t_fa * 1e3,
                         "config": "256x256 px, 1.3 mm, incl 60, 2 arcsec field at 140 pc",
-                        "ray_tile": list(tile)}
+                        "ray_tile": list(tile),
+                        "crossings_per_ray": n_img / npx,
+                        "roofline_fwd": {"kernel": "k_image (+ k_cam_prep, k_stars in the "
+                                         "timing)", "bound": "hbm",
+                                         "achieved": b_fwd / t_f / 1e9, "peak": HBM_PEAK_GBS,
+                                         "unit": "GB/s", "frac": b_fwd / t_f / 1e9 / HBM_PEAK_GBS,
+                                         "b_alg": b_fwd,
+                                         "note": "per-cell data (8 B x cells) stay in L2; "
+                                                 "the kernel is bound by VALU issue per "
+                                                 "crossing, not by HBM"},
+                        "roofline_adj": {"kernel": "k_image_adj (+ k_cam_prep, k_cam_post)",
+                                         "bound": "hbm", "achieved": b_adj / t_adj / 1e9,
+                                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": b_adj / t_adj / 1e9 / HBM_PEAK_GBS,
+                                         "b_alg": b_adj}}
         # with the scattering source, as RADMC-3D images when the dust scatters:
         # J from the monochromatic Monte Carlo, then image + adjoint
         nps = int(a.nphot_scat)
@@ -565,9 +593,18 @@ def main():
                     fn(img, dxy, u, v)
                 torch.cuda.synchronize()
                 row[name + "_ms"] = (time.perf_counter() - t0) / reps * 1e3
+            # exact DFT of a real image: a real pixel times a complex phasor
+            # per pixel and point, 2 FMAs = 4 FP32 flops
+            fl = 4.0 * 65536 * K
+            row["dft_tflops"] = fl / (row["dft_ms"] * 1e-3) / 1e12
+            row["dft_frac_fp32_vector"] = row["dft_tflops"] / FP32_VECTOR_TFLOPS
             vis_out["K=%d" % K] = row
         out["visibilities"] = dict(vis_out, config="256x256 image above, 1.3 mm, (u,v) of "
-                                   "15-2000 m baselines")
+                                   "15-2000 m baselines",
+                                   peak_note="FP32 vector peak without packed FP32 "
+                                             "(256 CU x 4 SIMD x 16 lanes x 2 flop x 2.4 GHz "
+                                             "= 78.6 TFLOP/s; the packed / MFMA figure is "
+                                             "157.3); packed FP32 is off in this build")
         log("image legs done")
     if not a.no_cpu:
         out["accuracy"] = accuracy_vs_oracle(dm, arr, int(a.acc_nphot), 0, a.max_events, a.mrw,

@@ -187,6 +187,12 @@ int rt_refine_stats(rt_ctx *ctx, uint64_t *nrays, uint64_t *unsplit);
 /* Grows the context's sub-pixel pool to at least n nodes (n > 0); for graph
  * captures that refine.  Synchronous. */
 int rt_refine_reserve(rt_ctx *ctx, uint64_t n);
+/* Cells crossed by the context's camera rays since the last reset: *image by
+ * image rays (sub-pixels included), *adjoint by the adjoints' reverse sweeps
+ * (either pointer may be NULL); reset != 0 zeroes the counts.  Instrumentation
+ * with no counterpart in the reference (the images' bytes per ray for their
+ * roofline).  Synchronises the context's stream. */
+int rt_image_crossings(rt_ctx *ctx, uint64_t *image, uint64_t *adjoint, int reset);
 
 /* Uploads grid, densities and opacities, builds the emission tables.
  * Replaces write_radmc3d's input files (Model.py:723-778). */

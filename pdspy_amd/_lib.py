@@ -22,7 +22,7 @@ EXPORTS = [
     "rt_image_adjoint", "rt_vis_dft", "rt_vis_dft_adjoint", "rt_vis_grid",
     "rt_vis_grid_workspace", "rt_vis_grid_noalloc", "rt_interpolate_galario", "rt_chisq",
     "rt_batch_create", "rt_batch_destroy", "rt_batch_thermal", "rt_batch_step",
-    "rt_refine_stats", "rt_refine_reserve",
+    "rt_refine_stats", "rt_refine_reserve", "rt_image_crossings",
 ]
 
 RT_ERRORS = {-1: "RT_E_ARG", -2: "RT_E_HIP", -3: "RT_E_NOMEM", -4: "RT_E_WATCHDOG",
@@ -145,6 +145,7 @@ def load():
         "rt_vis_grid_workspace": ([I, I], C.c_size_t),
         "rt_vis_grid_noalloc": ([P, P, I, I, D, P, P, I, D, D, P, P], I),
         "rt_refine_stats": ([P, C.POINTER(U64), C.POINTER(U64)], I),
+        "rt_image_crossings": ([P, C.POINTER(U64), C.POINTER(U64), I], I),
         "rt_refine_reserve": ([P, U64], I),
         "rt_interpolate_galario": ([P, P, I, I, D, P, P, I, D, D, P, P, P], I),
         "rt_chisq": ([P, P, P, P, P, P, I, P, P, P], I),

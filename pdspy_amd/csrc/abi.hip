@@ -313,18 +313,34 @@ int rt_create(int device, rt_ctx **out) {
         delete c;
         return RT_E_HIP;
     }
+    if (hipMalloc(&c->d_icount, 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(c->d_icount, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return RT_E_HIP;
+    }
     *out = c;
     return RT_OK;
 }
 
 rt_ctx::~rt_ctx() {
-    void *p[] = {ref.xy, ref.val, ref.child, ref.gw, ref.ctl};
+    void *p[] = {ref.xy, ref.val, ref.child, ref.gw, ref.ctl, d_icount};
     for (void *q : p)
         if (q) (void)hipFree(q);
 }
 
 int rt_destroy(rt_ctx *ctx) {
     delete ctx;
+    return RT_OK;
+}
+
+int rt_image_crossings(rt_ctx *ctx, uint64_t *image, uint64_t *adjoint, int reset) {
+    if (!ctx || !ctx->d_icount) return RT_E_ARG;
+    unsigned long long c[2] = {0, 0};
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(c, ctx->d_icount, sizeof(c), hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(ctx, hipMemset(ctx->d_icount, 0, sizeof(c)));
+    if (image) *image = c[0];
+    if (adjoint) *adjoint = c[1];
     return RT_OK;
 }
 

@@ -35,7 +35,19 @@ struct CamParams {
     const float *alpha;  // [cell] per au
     const float *src;    // [cell]
     const float *csize;  // [cell] cell size for the sub-pixeling (au)
+    unsigned long long *icount;  // cells crossed: [0] image rays, [1] adjoint sweeps
 };
+
+// adds the block's crossings n (summed over its threads) to *dst; every thread
+// of the block calls it
+__device__ __forceinline__ void count_crossings(unsigned long long *dst, uint32_t n) {
+    __shared__ unsigned s_n;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    if (n) atomicAdd(&s_n, n);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n && dst) atomicAdd(dst, (unsigned long long)s_n);
+}
 
 // A camera ray is the line p0 + s d: p0 the point closest to the origin (in
 // the sky plane through it), d = -n.  Marching the FP32 position from the
@@ -95,7 +107,7 @@ __device__ __forceinline__ bool cross_wall_cam(const DevGrid &g, Pos &p, float t
 // on the part of the ray traced (RT_INF when it crosses no cell)
 template <bool LMIN>
 __device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, const CamRay &cr,
-                                                 float &lmin) {
+                                                 float &lmin, uint32_t &nseg) {
     float I = 0.f, tau = 0.f;
     uint32_t nzero = 0;
     for (int guard = 0; guard < (1 << 24); ++guard) {
@@ -103,6 +115,7 @@ __device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, cons
         float tw = next_wall(C.g, p, kind);
         const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) {
+            ++nseg;
             int c = cell_index(C.g, p);
             if (LMIN) lmin = fminf(lmin, C.csize[c]);
             float dtau = C.alpha[c] * tw;
@@ -120,9 +133,10 @@ __device__ __forceinline__ float integrate_ray_l(const CamParams &C, Pos p, cons
     return C.mode == 0 ? I : tau;
 }
 
-__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p, const CamRay &cr) {
+__device__ __forceinline__ float integrate_ray(const CamParams &C, Pos p, const CamRay &cr,
+                                              uint32_t &nseg) {
     float unused = RT_INF;
-    return integrate_ray_l<false>(C, p, cr, unused);
+    return integrate_ray_l<false>(C, p, cr, unused, nseg);
 }
 
 // pixel of this lane: blocks cover tw x th tiles of the image, so the rays of
@@ -137,18 +151,21 @@ __device__ __forceinline__ bool tile_pixel(const CamParams &C, int &j, int &k) {
 
 __global__ void __launch_bounds__(256) k_image(CamParams C, float *img) {
     int j, k;
-    if (!tile_pixel(C, j, k)) return;
-    const int pix = j * C.nx + k;
-    float acc = 0.f;
-    for (int a = 0; a < C.nsub; ++a)
-        for (int b = 0; b < C.nsub; ++b) {
-            float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
-            float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
-            Pos p;
-            CamRay cr;
-            if (ray_start(C, X, Y, p, cr)) acc += integrate_ray(C, p, cr);
-        }
-    img[pix] = acc / (float)(C.nsub * C.nsub);
+    uint32_t nseg = 0;
+    if (tile_pixel(C, j, k)) {
+        const int pix = j * C.nx + k;
+        float acc = 0.f;
+        for (int a = 0; a < C.nsub; ++a)
+            for (int b = 0; b < C.nsub; ++b) {
+                float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
+                float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
+                Pos p;
+                CamRay cr;
+                if (ray_start(C, X, Y, p, cr)) acc += integrate_ray(C, p, cr, nseg);
+            }
+        img[pix] = acc / (float)(C.nsub * C.nsub);
+    }
+    count_crossings(C.icount, nseg);
 }
 
 // The adjoint's per-cell sums are first gathered in a small open-addressing
@@ -186,11 +203,12 @@ __device__ __forceinline__ void adj_add(int c, double va, double vs, int *key, d
 // recomputing tau, with R_k = I - I_<=k (nothing is stored per segment)
 __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y, float gI,
                                             int *key, double *ga, double *gs, double *g_alpha,
-                                            double *g_src) {
+                                            double *g_src, uint32_t &nseg) {
     Pos p0;
     CamRay cr;
     if (!ray_start(C, X, Y, p0, cr)) return;
-    float I = integrate_ray(C, p0, cr);
+    uint32_t nfwd = 0;
+    float I = integrate_ray(C, p0, cr, nfwd);
     Pos p = p0;
     float tau = 0.f, cum = 0.f;
     uint32_t nzero = 0;
@@ -199,6 +217,7 @@ __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y
         float tw = next_wall(C.g, p, kind);
         const float rcell = C.g.rw[p.ir + 1];
         if (p.ir >= 0) {
+            ++nseg;
             int c = cell_index(C.g, p);
             float dtau = C.alpha[c] * tw;
             float att = __expf(-tau);
@@ -236,11 +255,12 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
         gI = dimg[j * C.nx + k] / (float)(C.nsub * C.nsub);
         act = gI != 0.f;
     }
+    uint32_t nseg = 0;
     for (int a = 0; act && a < C.nsub; ++a)
         for (int b = 0; b < C.nsub; ++b) {
             float X = ((float)(k - C.nx / 2) + ((b + 0.5f) / C.nsub - 0.5f)) * C.pix;
             float Y = ((float)(j - C.ny / 2) + ((a + 0.5f) / C.nsub - 0.5f)) * C.pix;
-            adjoint_ray(C, X, Y, gI, key, ga, gs, g_alpha, g_src);
+            adjoint_ray(C, X, Y, gI, key, ga, gs, g_alpha, g_src, nseg);
         }
     __syncthreads();
     for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
@@ -250,6 +270,7 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
             unsafeAtomicAdd(&g_src[c], gs[i]);
         }
     }
+    count_crossings(C.icount ? C.icount + 1 : nullptr, nseg);
 }
 
 // ---------------------------------------------------------------- sub-pixels
@@ -327,6 +348,7 @@ __global__ void __launch_bounds__(256) k_ref_trace(CamParams C, RefArgs R, int L
     const bool may_split = L < R.nref;
     const int lane = __lane_id();
     unsigned nrays = 0;
+    uint32_t nseg = 0;
     for (unsigned base = start + blockIdx.x * blockDim.x; base < end;
          base += gridDim.x * blockDim.x) {
         const unsigned i = base + threadIdx.x;
@@ -338,7 +360,7 @@ __global__ void __launch_bounds__(256) k_ref_trace(CamParams C, RefArgs R, int L
             Pos p;
             CamRay cr;
             if (xy.x < 0.5f * RT_REF_DEAD && ray_start(C, xy.x, xy.y, p, cr)) {
-                I = integrate_ray_l<true>(C, p, cr, lmin);
+                I = integrate_ray_l<true>(C, p, cr, lmin, nseg);
                 ++nrays;
             }
             R.val[i] = I;
@@ -382,6 +404,7 @@ __global__ void __launch_bounds__(256) k_ref_trace(CamParams C, RefArgs R, int L
     }
     for (int off = 32; off > 0; off >>= 1) nrays += __shfl_down(nrays, off);
     if (lane == 0 && nrays) atomicAdd(&R.ctl[RT_REF_RAYS], nrays);
+    count_crossings(C.icount, nseg);
 }
 
 // level L + 1 ends where the slots handed out so far end
@@ -441,6 +464,7 @@ __global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double 
     double *ga = adj_lds, *gs = adj_lds + H;
     int *key = (int *)(adj_lds + 2 * H);
     const unsigned n = R.ctl[0] < R.cap ? R.ctl[0] : R.cap;
+    uint32_t nseg = 0;
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
             key[t] = -1;
@@ -453,7 +477,7 @@ __global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double 
             const float g = R.gw[i];
             const float2 xy = R.xy[i];
             if (g != 0.f && xy.x < 0.5f * RT_REF_DEAD)
-                adjoint_ray(C, xy.x, xy.y, g, key, ga, gs, g_alpha, g_src);
+                adjoint_ray(C, xy.x, xy.y, g, key, ga, gs, g_alpha, g_src, nseg);
         }
         __syncthreads();
         for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
@@ -465,6 +489,7 @@ __global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double 
         }
         __syncthreads();
     }
+    count_crossings(C.icount ? C.icount + 1 : nullptr, nseg);
 }
 
 // point-source fluxes (erg/s/cm^2/Hz/sr over one pixel) passed by value
@@ -728,6 +753,7 @@ static int setup_camera(rt_model *m, const rt_camera *cam, CamParams &C, KapArg 
     double ex[3] = {std::cos(ph), std::sin(ph), 0.0};
     double ey[3] = {n[1] * ex[2] - n[2] * ex[1], n[2] * ex[0] - n[0] * ex[2], n[0] * ex[1] - n[1] * ex[0]};
     C.g = dev_grid(m);
+    C.icount = ctx->d_icount;
     for (int a = 0; a < 3; ++a) {
         C.n[a] = (float)n[a];
         C.ex[a] = (float)ex[a];

@@ -158,11 +158,15 @@ __device__ __forceinline__ float sph_out(float b, float rr, float R2) {
 }
 
 // inner sphere; near-tangent cases (D small against b^2) are re-evaluated in
-// FP64 from the FP32 position so the hit/miss decision is consistent
+// FP64 from the FP32 position so the hit/miss decision is consistent.  A
+// discriminant below -1e-5 (r^2 + R^2), twenty times its FP32 rounding bound,
+// is a miss without that check (most rays moving inwards miss the inner
+// sphere; one lane re-evaluating holds its whole wave).
 __device__ __forceinline__ float sph_in(const Pos &p, float b, float rr, float R2, double Rd) {
     if (b >= 0.f) return RT_INF;
     float c = rr - R2;
     float D = fmaf(b, b, -c);
+    if (D < -1e-5f * (rr + R2)) return RT_INF;
     if (D < 1e-4f * b * b) {
         double bd = (double)p.x * p.dx + (double)p.y * p.dy + (double)p.z * p.dz;
         double rrd = (double)p.x * p.x + (double)p.y * p.y + (double)p.z * p.z;
@@ -200,7 +204,7 @@ __device__ __forceinline__ float cone_root(double A, double B, double C, double 
 }
 
 __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, float dy,
-                                          float dz, float cth, float sth, double cthd,
+                                          float dz, float rr, float cth, float sth, double cthd,
                                           double sthd, int req) {
     float c2 = cth * cth, s2 = sth * sth;
     float A = (dx * dx + dy * dy) * c2 - dz * dz * s2;
@@ -213,7 +217,12 @@ __device__ __forceinline__ float cone_hit(float x, float y, float z, float dx, f
     const float sreq = (float)(cth > 0.f ? req : -req);
     int want_plus = sreq > 0.f;
     float t;
-    if (D < 1e-4f * B * B) {
+    if (D < -1e-5f * rr) {
+        // no crossing, clear of rounding (|A| <= 1, |B| <= r, |C| <= r^2, so D
+        // is good to a few ulp of r^2): as the FP64 test below would find
+        if (!(C * sreq > 0.f)) return RT_INF;
+        t = 0.f;
+    } else if (D < 1e-4f * B * B) {
         double c2d = cthd * cthd, s2d = sthd * sthd;
         double Ad = ((double)dx * dx + (double)dy * dy) * c2d - (double)dz * dz * s2d;
         double Bd = ((double)x * dx + (double)y * dy) * c2d - (double)z * dz * s2d;
@@ -286,7 +295,8 @@ __device__ __forceinline__ float next_wall(const DevGrid &g, const Pos &p, int &
                 }
             }
         } else {
-            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, g.ct[j], g.st[j], g.ctd[j], g.std_[j], -1);
+            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, rr, g.ct[j], g.st[j], g.ctd[j], g.std_[j],
+                         -1);
             if (t < tb) {
                 tb = t;
                 kind = W_TLO;
@@ -311,8 +321,8 @@ __device__ __forceinline__ float next_wall(const DevGrid &g, const Pos &p, int &
                 }
             }
         } else {
-            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, g.ct[j + 1], g.st[j + 1], g.ctd[j + 1],
-                         g.std_[j + 1], +1);
+            t = cone_hit(p.x, p.y, zf, p.dx, p.dy, dzf, rr, g.ct[j + 1], g.st[j + 1],
+                         g.ctd[j + 1], g.std_[j + 1], +1);
             if (t < tb) {
                 tb = t;
                 kind = W_THI;

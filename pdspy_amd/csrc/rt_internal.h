@@ -46,6 +46,9 @@ struct rt_ctx {
     int n_cu = 0;
     int wallclock_khz = 0;
     RefinePool ref;
+    // cells crossed by camera rays since the last read: [0] image rays
+    // (sub-pixels included), [1] adjoint sweeps
+    unsigned long long *d_icount = nullptr;
     ~rt_ctx();
 };
 

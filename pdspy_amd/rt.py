@@ -306,6 +306,14 @@ class DeviceModel:
         cam._jscat = jscat
         return cam
 
+    def image_crossings(self, reset=True):
+        """(cells crossed by this context's image rays, by its adjoint sweeps)
+        since the last reset"""
+        a, b = C.c_uint64(), C.c_uint64()
+        self.ctx.check(self.ctx.L.rt_image_crossings(self.ctx.h, C.byref(a), C.byref(b),
+                                                     1 if reset else 0))
+        return a.value, b.value
+
     def refine_stats(self):
         """(rays traced by the last sub-pixelled image of this model's context,
         sub-pixels left unsplit for want of pool space since the last call)"""

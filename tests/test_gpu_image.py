@@ -631,3 +631,36 @@ def test_run_image_adds_scattering_source_unless_disabled():
     assert np.array_equal(b, c)
     assert np.all(np.isfinite(a)) and a.sum() > 0
     assert np.abs(a - b).max() > 1e-3 * np.abs(b).max()
+
+
+def test_image_crossing_counts_match_oracle_segments():
+    """rt_image_crossings: cells crossed by an image's rays (and by its
+    adjoint's reverse sweeps) equal the number of in-grid segments the oracle
+    finds along the same camera rays."""
+    print("case: image crossing counters vs oracle segments, 24x24 px", flush=True)
+    arr, T = _disk_setup(nr=32, nt=16)
+    dm = _dev(arr)
+    npix, pix, incl = 24, 5.0 * AU, 55.0
+    cam = dm.camera(npix, pix, 1300.0, incl=incl, incl_star=False)
+    Td = torch.tensor(T, dtype=torch.float32, device="cuda")
+    dm.image_crossings(reset=True)
+    dm.image(cam, Td)
+    n_img, n_adj0 = dm.image_crossings(reset=True)
+    dm.image_adjoint(cam, Td, None, torch.ones((npix, npix), dtype=torch.float32,
+                                               device="cuda"))
+    n_img1, n_adj = dm.image_crossings(reset=True)
+    assert n_adj0 == 0 and n_img1 == 0
+    i = np.radians(incl)
+    n = np.array([0.0, -np.sin(i), np.cos(i)])
+    ex = np.array([1.0, 0.0, 0.0])
+    ey = np.cross(n, ex)
+    R = arr["rw"][-1]
+    ref = 0
+    for j in range(npix):
+        for k in range(npix):
+            o = (k - npix // 2) * pix * ex + (j - npix // 2) * pix * ey + 2 * R * n
+            c, _, _ = oracle.trace_ray(arr, o, -n)
+            ref += int(np.sum(c >= 0))
+    print("crossings: image %d, adjoint %d, oracle %d" % (n_img, n_adj, ref), flush=True)
+    assert n_img == n_adj
+    assert abs(n_img - ref) <= 1e-3 * ref
