@@ -8,12 +8,11 @@ with kappa_abs = 1e3 (lam/um)^-1 (albedo 0.5, isotropic), a 4000 K 2 Rsun star,
 packets per run, seeds 0-3.  One step = one thermal run (seed = step mod 4, so
 consecutive steps cover the four seeds) plus the copy of its per-species
 temperatures to the host (what run_thermal hands back).  Steps run on
---streams (default 2) host threads, each with its own context, HIP stream and
-copy of the model, step k on thread k mod 2: a run's waves leave its launch as
-soon as its packets are all handed out, so the next run's launch takes over
-the GPU while the last long random walks of the previous one finish (each
-run's temperatures are bit-identical to a run on its own).  A warm-up step is
-a run of another seed.  No timed step is started once it would end past
+--streams (default 4) host threads, each with its own context, HIP stream and
+copy of the model, step k on thread k mod 4: the runs share the GPU, so one
+run's last long random walks (its tail, when few of its waves still hold
+packets) overlap the others' bulk (each run's temperatures are bit-identical
+to a run on its own).  A warm-up step is a run of another seed.  No timed step is started once it would end past
 --step-budget-s seconds after the process started; `steps` reports the steps
 actually timed.
 
@@ -72,7 +71,9 @@ def parse():
                         "seconds after the process started")
     p.add_argument("--side-budget-s", type=float, default=540.0,
                    help="the batched four-seed launch runs only if it ends before this")
-    p.add_argument("--streams", type=int, default=2,
+    p.add_argument("--stagger-s", type=float, default=STAGGER_S,
+                   help="least time between two thermal launches of the overlapped steps")
+    p.add_argument("--streams", type=int, default=4,
                    help="steps run on this many host threads / HIP streams, overlapping "
                         "one run's last random walks with the next run (1 = back to back)")
     p.add_argument("--acc-nphot", type=float, default=2e5,
@@ -332,10 +333,10 @@ def main():
     del src, dst
 
     # steps run on `streams` host threads, each with its own context, stream and
-    # copy of the model: step k goes to lane k mod streams.  A single-run
-    # launch's waves leave once its packets are all handed out, so the next
-    # run (queued on the other stream) fills the GPU while the last random
-    # walks of the previous one finish.
+    # copy of the model: step k goes to lane k mod streams.  The launches share
+    # the GPU (each leaves a CU per XCD free, where the other lanes' short
+    # kernels run at once), so a run's tail of long random walks overlaps the
+    # other runs' bulk.
     from pdspy_amd import _lib
     nl = max(1, int(a.streams))
     lanes = [(dm, torch.cuda.current_stream())]
@@ -348,12 +349,10 @@ def main():
     last_launch = [-1.0]
 
     def stagger():
-        """A run is launched at least STAGGER_S after the previous one, so its
-        blocks queue behind that launch's resident ones instead of being
-        dispatched alongside them: it then starts on the CUs the previous
-        run's waves leave once its packets are handed out."""
+        """A run is launched at least --stagger-s after the previous one (the
+        lanes' launches do not all start in the same instant)."""
         with launch_mu:
-            wait = last_launch[0] + STAGGER_S - time.perf_counter()
+            wait = last_launch[0] + a.stagger_s - time.perf_counter()
             if wait > 0:
                 time.sleep(wait)
             last_launch[0] = time.perf_counter()

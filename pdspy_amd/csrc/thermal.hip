@@ -139,6 +139,12 @@ struct BatchArgs {
 #ifndef RT_SNAP_POLL
 #define RT_SNAP_POLL 64u  // a busy wave looks at its run's next snapshot every this many iterations
 #endif
+#ifndef RT_FREE_CU_PER_XCD
+#define RT_FREE_CU_PER_XCD 1  // CUs of each XCD the thermal launch leaves free
+#endif
+#ifndef RT_NUM_XCD
+#define RT_NUM_XCD 8  // MI355X: 8 XCDs of 32 CUs, workgroups dealt to them in turn
+#endif
 #ifndef RT_PARK_MAXLIVE
 #define RT_PARK_MAXLIVE 8  // a wave with at most this many live packets may park them
 #endif
@@ -421,20 +427,30 @@ __device__ __forceinline__ unsigned park_get(const unsigned *w, int k) {
 
 // Adds a wave's counters for run R (one atomic per counter per wave) and
 // clears them.
-__device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uint32_t (&v)[8],
-                                             uint32_t &max_pkt, unsigned long long &ovf) {
+// The wave's event counts live in LDS, one row of 9 words per wave (8 counts
+// and the largest packet's steps): counted per lane they held 9 VGPRs, which
+// pushed the kernel past 128 into scratch -- and a launch that needs scratch
+// held other streams' kernels back until its waves retired.  In a branch the
+// first active lane adds the number of active lanes.
+#define RT_CNT_WORDS 9
+__device__ __forceinline__ void wave_count(unsigned *row, int k, int lane) {
+    const uint64_t act = __ballot(1);
+    if (lane == __ffsll((unsigned long long)act) - 1) row[k] += (unsigned)__popcll(act);
+}
+
+// adds the wave's counts to run R's counters and clears them (whole wave)
+__device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, unsigned *row,
+                                             unsigned long long &ovf) {
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        unsigned long long x = v[k];
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-        if (lane == 0 && x) atomicAdd(&R.cnt[CNT_CROSS + k], x);
-        v[k] = 0;
+        for (int k = 0; k < 8; ++k) {
+            if (row[k]) atomicAdd(&R.cnt[CNT_CROSS + k], (unsigned long long)row[k]);
+            row[k] = 0u;
+        }
+        if (row[8]) atomicMax(&R.cnt[CNT_MAXPKT], (unsigned long long)row[8]);
+        row[8] = 0u;
     }
-    unsigned long long mp = max_pkt;
-    for (int off = 32; off > 0; off >>= 1) mp = max(mp, (unsigned long long)__shfl_down(mp, off));
-    if (lane == 0 && mp) atomicMax(&R.cnt[CNT_MAXPKT], mp);
     if (__any((ovf >> 62) != 0ull) && lane == 0) atomicOr(&R.cnt[CNT_OVERFLOW], 1ull);
-    max_pkt = 0;
     ovf = 0;
 }
 
@@ -515,10 +531,11 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     for (int q = 0; q < MG; ++q) acc[q] = 0;
     // per-run wave counters: crossings, absorptions, scatterings, escapes,
     // kills, interactions, MRW steps, nudges
-    uint32_t cnt8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t &n_cross = cnt8[0], &n_abs = cnt8[1], &n_scat = cnt8[2], &n_esc = cnt8[3];
-    uint32_t &n_kill = cnt8[4], &n_ev = cnt8[5], &n_mrw = cnt8[6], &n_nudge = cnt8[7];
-    uint32_t max_pkt = 0;
+    __shared__ unsigned s_cnt[4 * RT_CNT_WORDS];
+    unsigned *const crow = s_cnt + (threadIdx.x >> 6) * RT_CNT_WORDS;
+    if (lane < RT_CNT_WORDS) crow[lane] = 0u;
+#define RT_COUNT(k) wave_count(crow, k, lane)
+#define RT_MAXPKT(v) atomicMax(&crow[8], (unsigned)(v))
     int cur_e = 0;          // wave-uniform: epoch packets are claimed from
     bool epoch_ok = false;  // wave-uniform: its snapshot has been acquired
     uint64_t n_idle = 0;
@@ -575,8 +592,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 if (alive) {
                     if (p.ir >= 0) flush_tally(P, tb, cell_index(g, p), acc, ovf, ssum);
                     flush_ssum(P, slot, ssum);
-                    max_pkt = max(max_pkt, nev + nstep);
-                    n_kill++;
+                    RT_MAXPKT(nev + nstep);
+                    RT_COUNT(4);
                     alive = false;
                 }
                 break;
@@ -600,7 +617,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 if (r >= 0) {
                     order_acquire();
                     if (BATCH && r != m) {
-                        if (m >= 0) flush_counts(P, lane, cnt8, max_pkt, ovf);
+                        if (m >= 0) flush_counts(P, lane, crow, ovf);
                         m = r;
                         Pp = BATCH ? runs + m : &P1;
                         stage_run(P, wl, lane, g, s_ka, s_ks, s_gka, s_g, s_invdr);
@@ -879,7 +896,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                         slice_left = P.slice;
                         ccell = -1;
                         if (p.ir >= g.nr) {
-                            n_esc++;
+                            RT_COUNT(3);
                             count_up(&P.slot_out[pe]);
                         } else {
                             alive = true;
@@ -943,7 +960,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             p.x = fmaf(sseg, p.dx, p.x);
             p.y = fmaf(sseg, p.dy, p.y);
             p.z = fmaf(sseg, p.dz, p.z);
-            n_ev++;
+            RT_COUNT(5);
             nev++;
             nev_cell++;
             nzero = 0;
@@ -967,7 +984,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             bool use_hg = false;
             float ghg = 0.f;
             if (scat) {
-                n_scat++;
+                RT_COUNT(2);
                 if (P.hg) {
                     float us = u[1] * asca, a = 0.f;
                     int sp = S - 1;
@@ -982,7 +999,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                     use_hg = true;
                 }
             } else {
-                n_abs++;
+                RT_COUNT(1);
                 float us = u[1] * aabs, a = 0.f;
                 int sp = S - 1;
                 for (int s = 0; s < S; ++s) {
@@ -1009,8 +1026,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
             if (P.max_events && nev >= P.max_events) {
                 flush_tally(P, tb, c, acc, ovf, ssum);
                 flush_ssum(P, slot, ssum);
-                max_pkt = max(max_pkt, nev + nstep);
-                n_kill++;
+                RT_MAXPKT(nev + nstep);
+                RT_COUNT(4);
                 alive = false;
                 if (slot < E) count_up(&P.slot_out[slot]);
                 continue;
@@ -1077,7 +1094,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                     int row = lo + (v[6] < w ? 1 : 0);
                     size_t ro = (size_t)q * P.ntemp + row;
                     ifr = sample_guided(P.cdf + ro * NL, P.guide + ro * RT_NGUIDE, v[7]);
-                    n_mrw++;
+                    RT_COUNT(6);
                 }
             }
             RT_PHASE(7);
@@ -1091,27 +1108,27 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
                 flush_tally(P, tb, c, acc, ovf, ssum);
             }
             tau -= aext * tw;
-            n_cross++;
+            RT_COUNT(0);
             nev_cell = 0;
             bool end = false;
             const float rcell = g.rw[p.ir + 1];
             if (++nstep > P.max_steps) {  // last-resort cap on crossings
-                n_kill++;
+                RT_COUNT(4);
                 end = true;
             } else if (cross_wall(g, p, tw, kind)) {
-                n_esc++;
+                RT_COUNT(3);
                 end = true;
             } else if (unstick(g, p, tw, rcell, nzero)) {
-                n_nudge++;
+                RT_COUNT(7);
                 if (p.ir >= g.nr) {
-                    n_esc++;
+                    RT_COUNT(3);
                     end = true;
                 }
             }
             RT_PHASE(5);
             if (end) {
                 flush_ssum(P, slot, ssum);
-                max_pkt = max(max_pkt, nev + nstep);
+                RT_MAXPKT(nev + nstep);
                 alive = false;
                 if (slot < E) count_up(&P.slot_out[slot]);
                 continue;
@@ -1120,7 +1137,7 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
         }
     }
     // ---- statistics of the last run served, batch counters
-    if (m >= 0) flush_counts(P, lane, cnt8, max_pkt, ovf);
+    if (m >= 0) flush_counts(P, lane, crow, ovf);
     if (lane == 0) {
         for (int r = 0; r < A.nrun; ++r)
             atomicMax(&run_at<BATCH>(runs, P1, r).snap_time[RT_MAX_EPOCH + 1],
@@ -1137,6 +1154,8 @@ k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     }
 #endif
 #undef RT_PHASE
+#undef RT_COUNT
+#undef RT_MAXPKT
 #undef P
 }
 
@@ -1454,7 +1473,19 @@ int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, 
     if (batch && !d_parked) return set_err(ctx, RT_E_ARG, "batch launch without a parked-packet counter");
     ThermParams P1{};
     if (!batch) P1 = *(const ThermParams *)h_run1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)(ctx->n_cu * bpc)), dim3(bs), lds, sm, A,
+    // one CU per XCD stays free.  The persistent launch would otherwise hold
+    // every CU until its last waves retire, and the short kernels another
+    // stream queues meanwhile (the next run's tally resets, the fold after a
+    // run, an image) waited seconds behind it, so a second run could not
+    // start while this one's long walks finished.  The dispatcher deals
+    // workgroup i to XCD i mod 8; measured, a kernel of another queue ran at
+    // once only with a whole CU free on every XCD (one free block slot per
+    // XCD, or per CU short of that, did not help), at 3 % of the slots.
+    const int nxcd = RT_NUM_XCD;
+    long nblocks = (long)ctx->n_cu * bpc;
+    if (ctx->n_cu % nxcd == 0 && ctx->n_cu >= 4 * nxcd)
+        nblocks = (long)(ctx->n_cu - nxcd * RT_FREE_CU_PER_XCD) * bpc;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bs), lds, sm, A,
                        (const ThermParams *)d_runs, P1);
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
