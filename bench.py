@@ -334,11 +334,12 @@ def main():
 
     # steps run on `streams` host threads, each with its own context, stream and
     # copy of the model: step k goes to lane k mod streams.  The launches share
-    # the GPU (each leaves a CU per XCD free, where the other lanes' short
-    # kernels run at once), so a run's tail of long random walks overlaps the
-    # other runs' bulk.
+    # the GPU (with several lanes each leaves a CU per XCD free, where the other
+    # lanes' short kernels run at once), so a run's tail of long random walks
+    # overlaps the other runs' bulk.
     from pdspy_amd import _lib
     nl = max(1, int(a.streams))
+    lane_opts = dict(opts, free_cus_per_xcd=1 if nl > 1 else 0)
     lanes = [(dm, torch.cuda.current_stream())]
     for _ in range(nl - 1):
         cx = _lib.Context(0)
@@ -375,7 +376,7 @@ def main():
                         stagger()
                         ts = time.perf_counter()
                         cv = 1 if (first_visits and k == 0) else 0
-                        st = dml.thermal(nphot, seed_of(k), count_visits=cv, **opts)
+                        st = dml.thermal(nphot, seed_of(k), count_visits=cv, **lane_opts)
                         res = dml.temperature(with_visits=bool(cv))
                         last = time.perf_counter() - ts
                         out[k] = (st, res, last)

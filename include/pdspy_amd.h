@@ -89,6 +89,13 @@ typedef struct {
                               packets whose model has none to hand out passes them
                               to another wavefront of that model); the results
                               are the same either way */
+    int free_cus_per_xcd;  /* CUs of each of the 8 XCDs the persistent launch
+                              leaves free (default 0: it takes every CU).  With 1,
+                              kernels on other streams -- another run's resets,
+                              images -- run beside it at once instead of waiting
+                              for its waves to retire, at 3 % of the slots; for
+                              runs on several streams.  Results do not depend
+                              on it */
 } rt_thermal_opts;
 
 typedef struct {

@@ -62,7 +62,8 @@ class ThermalOpts(C.Structure):
                 ("mrw_tauthres", C.c_double), ("mrw_count_trigger", C.c_int),
                 ("mrw_gamma", C.c_double), ("max_seconds", C.c_double),
                 ("block_size", C.c_int), ("blocks_per_cu", C.c_int),
-                ("slice_events", C.c_uint64), ("no_park", C.c_int)]
+                ("slice_events", C.c_uint64), ("no_park", C.c_int),
+                ("free_cus_per_xcd", C.c_int)]
 
 
 class ThermalStats(C.Structure):

@@ -139,9 +139,6 @@ struct BatchArgs {
 #ifndef RT_SNAP_POLL
 #define RT_SNAP_POLL 64u  // a busy wave looks at its run's next snapshot every this many iterations
 #endif
-#ifndef RT_FREE_CU_PER_XCD
-#define RT_FREE_CU_PER_XCD 1  // CUs of each XCD the thermal launch leaves free
-#endif
 #ifndef RT_NUM_XCD
 #define RT_NUM_XCD 8  // MI355X: 8 XCDs of 32 CUs, workgroups dealt to them in turn
 #endif
@@ -1473,18 +1470,19 @@ int thermal_enqueue_launch(rt_ctx *ctx, const void *d_runs, const void *h_run1, 
     if (batch && !d_parked) return set_err(ctx, RT_E_ARG, "batch launch without a parked-packet counter");
     ThermParams P1{};
     if (!batch) P1 = *(const ThermParams *)h_run1;
-    // one CU per XCD stays free.  The persistent launch would otherwise hold
-    // every CU until its last waves retire, and the short kernels another
+    // free_cus_per_xcd: CUs of each XCD left free.  A persistent launch holds
+    // every CU it gets until its last waves retire, and short kernels another
     // stream queues meanwhile (the next run's tally resets, the fold after a
-    // run, an image) waited seconds behind it, so a second run could not
-    // start while this one's long walks finished.  The dispatcher deals
-    // workgroup i to XCD i mod 8; measured, a kernel of another queue ran at
-    // once only with a whole CU free on every XCD (one free block slot per
-    // XCD, or per CU short of that, did not help), at 3 % of the slots.
+    // run, an image) waited seconds behind it, so a second run could not start
+    // while this one's long walks finished.  The dispatcher deals workgroup i
+    // to XCD i mod 8; measured, a kernel of another queue ran at once only
+    // with a whole CU free on every XCD (one free block slot per XCD did not
+    // help).
     const int nxcd = RT_NUM_XCD;
     long nblocks = (long)ctx->n_cu * bpc;
-    if (ctx->n_cu % nxcd == 0 && ctx->n_cu >= 4 * nxcd)
-        nblocks = (long)(ctx->n_cu - nxcd * RT_FREE_CU_PER_XCD) * bpc;
+    const int fc = opt.free_cus_per_xcd;
+    if (fc > 0 && ctx->n_cu % nxcd == 0 && ctx->n_cu - nxcd * fc >= ctx->n_cu / 2)
+        nblocks = (long)(ctx->n_cu - nxcd * fc) * bpc;
     hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(bs), lds, sm, A,
                        (const ThermParams *)d_runs, P1);
     HIPCHK(ctx, hipGetLastError());

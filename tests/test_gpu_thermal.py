@@ -100,8 +100,9 @@ def test_thermal_bitwise_reproducible_across_launch_shapes():
     arr = oracle_arrays(configs.disk_c1(nr=24, nt=24))
     dm = rt.DeviceModel(arr)
     out = []
-    for bs, bpc in [(256, 0), (256, 0), (64, 1), (128, 3)]:
-        dm.thermal(30000, 9, max_events=20000, block_size=bs, blocks_per_cu=bpc)
+    for bs, bpc, fc in [(256, 0, 0), (256, 0, 0), (64, 1, 0), (128, 3, 0), (256, 0, 1)]:
+        dm.thermal(30000, 9, max_events=20000, block_size=bs, blocks_per_cu=bpc,
+                   free_cus_per_xcd=fc)
         out.append(dm.temperature())
     for T in out[1:]:
         assert np.array_equal(T, out[0])
