@@ -65,7 +65,8 @@ static void free_model(rt_model *m) {
                     m->d_grp, m->d_gka, m->d_ginvk, m->d_invdr, m->d_cdf, m->d_guide, m->d_lnq,
                     m->d_scdf, m->d_sguide, m->d_spick, m->d_spos, m->d_kross, m->d_kplanck, m->d_mrwy,
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
-                    m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_rowI, m->d_pipe, m->d_tally2,
+                    m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_qscale, m->d_rowI, m->d_pipe,
+                    m->d_tally2,
                     m->d_csize, m->d_tabin,
                     m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
                     m->d_dens_fac, m->d_comps, m->d_vol, m->d_jtally, m->d_scum, m->d_scnt, m->d_sblk};

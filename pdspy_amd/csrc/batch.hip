@@ -356,7 +356,7 @@ static std::vector<const void *> model_buffers(const rt_batch *b) {
     for (const rt_model *m : b->models) {
         const void *p[] = {m->d_pipe, m->d_tally, m->d_tally2, m->d_xring, m->d_xrow, m->d_Tg,
                            m->d_Tout, m->d_rho, m->d_rhoL, m->d_alpha, m->d_src, m->d_galpha,
-                           m->d_gsrc, m->d_cnt, m->d_visits, m->d_vol, m->d_jtally, m->d_scum,
+                           m->d_gsrc, m->d_qscale, m->d_cnt, m->d_visits, m->d_vol, m->d_jtally, m->d_scum,
                            m->d_scnt, m->d_sblk};
         v.insert(v.end(), p, p + sizeof(p) / sizeof(p[0]));
         // the image chains capture opacities at the camera wavelength by value

@@ -36,6 +36,7 @@ struct CamParams {
     const float *src;    // [cell]
     const float *csize;  // [cell] cell size for the sub-pixeling (au)
     unsigned long long *icount;  // cells crossed: [0] image rays, [1] adjoint sweeps
+    const double *qs;  // adjoint: fixed-point units per unit of dL/dalpha, dL/dS
 };
 
 // adds the block's crossings n (summed over its threads) to *dst; every thread
@@ -168,14 +169,18 @@ __global__ void __launch_bounds__(256) k_image(CamParams C, float *img) {
     count_crossings(C.icount, nseg);
 }
 
-// The adjoint's per-cell sums are first gathered in a small open-addressing
-// table in LDS: the rays of one block are neighbours on the sky and cross
-// mostly the same cells, so each cell then costs one global FP64 atomic per
-// block instead of one per ray.  Entries: 2^hbits keys (cell index, -1 =
-// empty) and two FP64 sums; a contribution that finds no slot in 8 probes
-// goes straight to global memory.
-__device__ __forceinline__ void adj_add(int c, double va, double vs, int *key, double *ga,
-                                        double *gs, unsigned hbits, double *g_alpha,
+// The adjoint's per-cell sums are 64-bit fixed-point integers (units in
+// C.qs, set by k_adj_scale): integer adds commute, so the gradient does not
+// depend on the order in which rays and blocks add to a cell -- it is
+// bit-reproducible, as the thermal tallies are.  They are first gathered in a
+// small open-addressing table in LDS: the rays of one block are neighbours on
+// the sky and cross mostly the same cells, so each cell then costs one global
+// atomic per block instead of one per ray.  Entries: 2^hbits keys (cell index,
+// -1 = empty) and two sums; a contribution that finds no slot in 8 probes goes
+// straight to global memory.
+typedef unsigned long long u64;
+__device__ __forceinline__ void adj_add(int c, long long va, long long vs, int *key, u64 *ga,
+                                        u64 *gs, unsigned hbits, double *g_alpha,
                                         double *g_src) {
     const unsigned mask = (1u << hbits) - 1u;
     unsigned slot = ((unsigned)c * 2654435761u) >> (32u - hbits);
@@ -188,25 +193,40 @@ __device__ __forceinline__ void adj_add(int c, double va, double vs, int *key, d
             cur = exp == -1 ? c : exp;
         }
         if (cur == c) {
-            __hip_atomic_fetch_add(&ga[slot], va, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(&gs[slot], vs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&ga[slot], (u64)va, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(&gs[slot], (u64)vs, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
             return;
         }
         slot = (slot + 1u) & mask;
     }
-    unsafeAtomicAdd(&g_alpha[c], va);
-    unsafeAtomicAdd(&g_src[c], vs);
+    atomicAdd((u64 *)&g_alpha[c], (u64)va);
+    atomicAdd((u64 *)&g_src[c], (u64)vs);
+}
+
+// the block's LDS table into the global sums
+__device__ __forceinline__ void adj_flush(const int *key, const u64 *ga, const u64 *gs,
+                                          unsigned H, double *g_alpha, double *g_src) {
+    for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
+        const int c = key[i];
+        if (c >= 0) {
+            if (ga[i]) atomicAdd((u64 *)&g_alpha[c], ga[i]);
+            if (gs[i]) atomicAdd((u64 *)&g_src[c], gs[i]);
+        }
+    }
 }
 
 // gI times the gradient of the ray through sky (X, Y) w.r.t. the per-cell
 // alpha and S: the ray is integrated for I, then swept again front to back
 // recomputing tau, with R_k = I - I_<=k (nothing is stored per segment)
 __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y, float gI,
-                                            int *key, double *ga, double *gs, double *g_alpha,
+                                            int *key, u64 *ga, u64 *gs, double *g_alpha,
                                             double *g_src, uint32_t &nseg) {
     Pos p0;
     CamRay cr;
     if (!ray_start(C, X, Y, p0, cr)) return;
+    const double qa = C.qs[0], qsrc = C.qs[1];
     uint32_t nfwd = 0;
     float I = integrate_ray(C, p0, cr, nfwd);
     Pos p = p0;
@@ -226,8 +246,9 @@ __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y
             cum = fmaf(att * om, S, cum);
             float Rk = I - cum;
             float dI_ddtau = att * (1.f - om) * S - Rk;
-            adj_add(c, (double)(gI * dI_ddtau * tw), (double)(gI * att * om), key, ga, gs,
-                    C.hbits, g_alpha, g_src);
+            adj_add(c, __double2ll_rn((double)(gI * dI_ddtau * tw) * qa),
+                    __double2ll_rn((double)(gI * att * om) * qsrc), key, ga, gs, C.hbits,
+                    g_alpha, g_src);
             tau += dtau;
             if (tau > C.tau_stop) break;
         }
@@ -238,14 +259,14 @@ __device__ __forceinline__ void adjoint_ray(const CamParams &C, float X, float Y
 
 __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dimg, double *g_alpha,
                                                    double *g_src) {
-    extern __shared__ double adj_lds[];
+    extern __shared__ u64 adj_lds[];
     const unsigned H = 1u << C.hbits;
-    double *ga = adj_lds, *gs = adj_lds + H;
+    u64 *ga = adj_lds, *gs = adj_lds + H;
     int *key = (int *)(adj_lds + 2 * H);
     for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
         key[i] = -1;
-        ga[i] = 0.0;
-        gs[i] = 0.0;
+        ga[i] = 0ull;
+        gs[i] = 0ull;
     }
     __syncthreads();
     int j, k;
@@ -263,13 +284,7 @@ __global__ void __launch_bounds__(256) k_image_adj(CamParams C, const float *dim
             adjoint_ray(C, X, Y, gI, key, ga, gs, g_alpha, g_src, nseg);
         }
     __syncthreads();
-    for (unsigned i = threadIdx.x; i < H; i += blockDim.x) {
-        const int c = key[i];
-        if (c >= 0) {
-            unsafeAtomicAdd(&g_alpha[c], ga[i]);
-            unsafeAtomicAdd(&g_src[c], gs[i]);
-        }
-    }
+    adj_flush(key, ga, gs, H, g_alpha, g_src);
     count_crossings(C.icount ? C.icount + 1 : nullptr, nseg);
 }
 
@@ -459,17 +474,17 @@ __global__ void __launch_bounds__(256) k_ref_down(RefArgs R, int L) {
 // gradients in its LDS table before the global atomics, as k_image_adj does
 __global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double *g_alpha,
                                                  double *g_src) {
-    extern __shared__ double adj_lds[];
+    extern __shared__ u64 adj_lds[];
     const unsigned H = 1u << C.hbits;
-    double *ga = adj_lds, *gs = adj_lds + H;
+    u64 *ga = adj_lds, *gs = adj_lds + H;
     int *key = (int *)(adj_lds + 2 * H);
     const unsigned n = R.ctl[0] < R.cap ? R.ctl[0] : R.cap;
     uint32_t nseg = 0;
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
             key[t] = -1;
-            ga[t] = 0.0;
-            gs[t] = 0.0;
+            ga[t] = 0ull;
+            gs[t] = 0ull;
         }
         __syncthreads();
         const unsigned i = base + threadIdx.x;
@@ -480,13 +495,7 @@ __global__ void __launch_bounds__(256) k_ref_adj(CamParams C, RefArgs R, double 
                 adjoint_ray(C, xy.x, xy.y, g, key, ga, gs, g_alpha, g_src, nseg);
         }
         __syncthreads();
-        for (unsigned t = threadIdx.x; t < H; t += blockDim.x) {
-            const int c = key[t];
-            if (c >= 0) {
-                unsafeAtomicAdd(&g_alpha[c], ga[t]);
-                unsafeAtomicAdd(&g_src[c], gs[t]);
-            }
-        }
+        adj_flush(key, ga, gs, H, g_alpha, g_src);
         __syncthreads();
     }
     count_crossings(C.icount ? C.icount + 1 : nullptr, nseg);
@@ -541,8 +550,10 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
         }
         const float gI = dimg[j * C.nx + k];
         if (gI == 0.f) return;
+        const double qa = C.qs[0];
         for (int ir = lane; ir < g.nr; ir += blockDim.x)
-            unsafeAtomicAdd(&g_alpha[col + ir], (double)(-gI * Is * (g.rw[ir + 1] - g.rw[ir])));
+            atomicAdd((u64 *)&g_alpha[col + ir],
+                      (u64)__double2ll_rn((double)(-gI * Is * (g.rw[ir + 1] - g.rw[ir])) * qa));
         return;
     }
     if (lane != 0) return;
@@ -578,7 +589,9 @@ __global__ void k_stars(CamParams C, int nstar, const float *spos, StarArg sflux
         int kind;
         float tw = next_wall(g, p, kind);
         const float rcell = g.rw[p.ir + 1];
-        if (p.ir >= 0) unsafeAtomicAdd(&g_alpha[cell_index(g, p)], (double)(-gI * Is * tw));
+        if (p.ir >= 0)
+            atomicAdd((u64 *)&g_alpha[cell_index(g, p)],
+                      (u64)__double2ll_rn((double)(-gI * Is * tw) * C.qs[0]));
         if (cross_wall(g, p, tw, kind)) break;
         unstick(g, p, tw, rcell, nzero);
     }
@@ -615,21 +628,24 @@ __global__ void k_cam_prep(const float *rho, int S, int ncell, KapArg kap, float
 // chain rule to log rho_s and T.  With a scattering source (J held fixed):
 // d alpha / d log rho_s = rho_s kext_s, dS / d log rho_s = rho_s (kabs_s B +
 // ksca_s J - S kext_s) / (rho kext), dS / dT = (rho kabs / rho kext) dB/dT.
-__global__ void k_cam_post(const double *g_alpha, const double *g_src, const float *rho, int S,
-                           int ncell, KapArg kap, float lenu, const float *T, float pre,
-                           float hnu_k, const float *J, const float *src, float *g_logrho,
-                           float *g_T) {
+__global__ void k_cam_post(const double *g_alpha, const double *g_src, const double *qs,
+                           const float *rho, int S, int ncell, KapArg kap, float lenu,
+                           const float *T, float pre, float hnu_k, const float *J,
+                           const float *src, float *g_logrho, float *g_T) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
-    double ga = g_alpha[c];
+    // the fixed-point sums back to FP64
+    const double qa = qs[0], qsrc = qs[1];
+    const double ga = qa > 0.0 ? (double)((const long long *)g_alpha)[c] / qa : 0.0;
+    const double gsv = qsrc > 0.0 ? (double)((const long long *)g_src)[c] / qsrc : 0.0;
     if (!J) {
         for (int s = 0; s < S; ++s)
             g_logrho[(size_t)s * ncell + c] =
                 (float)(ga * (double)rho[(size_t)s * ncell + c] * (double)kap.k[s] * (double)lenu);
-        g_T[c] = (float)(g_src[c] * (double)dplanck_f(pre, hnu_k, T[c]));
+        g_T[c] = (float)(gsv * (double)dplanck_f(pre, hnu_k, T[c]));
         return;
     }
-    const double gs = g_src[c];
+    const double gs = gsv;
     double aa = 0.0, a = 0.0;
     for (int s = 0; s < S; ++s) {
         const double r = rho[(size_t)s * ncell + c];
@@ -650,6 +666,45 @@ __global__ void k_cam_post(const double *g_alpha, const double *g_src, const flo
 __global__ void k_zero_f64(double *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
+}
+
+// Units of the adjoint's fixed-point sums, from bounds that hold whatever rays
+// an image traces.  The cotangents of all rays sum to at most ||dimg||_1 (the
+// rays of a pixel share its dL/dI, sub-pixels a quarter of their parent's).  A
+// contribution to dL/dalpha_c is g dI/d(dtau) l with |dI/d(dtau)| <= S_max
+// (the largest source: I and every emission term lie in [0, S_max]) and
+// l <= 2 R_out; a star adds g F l with F its pixel flux.  A ray crosses a cell
+// at most 4 times.  So |sum_c| <= 8 R_out max(S_max, F) (1 + nstar) ||dimg||_1
+// for alpha and 4 ||dimg||_1 for S, and units of 2^-61 of those bounds keep
+// every partial sum below 2^61.  One block; the reductions run in a fixed
+// order, so the units (and with them the gradient) are bit-reproducible.
+__global__ void __launch_bounds__(256) k_adj_scale(const float *dimg, int npix, const float *src,
+                                                   int ncell, float fstar, int nstar,
+                                                   float rout, double *qs) {
+    __shared__ double s_l1[256];
+    __shared__ float s_max[256];
+    const int t = threadIdx.x;
+    double l1 = 0.0;
+    for (int i = t; i < npix; i += 256) l1 += fabs((double)dimg[i]);
+    float smax = fstar;
+    for (int i = t; i < ncell; i += 256) smax = fmaxf(smax, src[i]);
+    s_l1[t] = l1;
+    s_max[t] = smax;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            s_l1[t] += s_l1[t + h];
+            s_max[t] = fmaxf(s_max[t], s_max[t + h]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double two61 = 2305843009213693952.0;
+        const double ba = 8.0 * (double)rout * (double)s_max[0] * (1.0 + nstar) * s_l1[0];
+        const double bs = 4.0 * s_l1[0];
+        qs[0] = ba > 0.0 ? two61 / ba : 0.0;
+        qs[1] = bs > 0.0 ? two61 / bs : 0.0;
+    }
 }
 
 static inline unsigned nblk(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -693,6 +748,7 @@ int image_buffers(rt_model *m) {
         HIPCHK(ctx, hipMalloc(&m->d_src, nc * sizeof(float)));
         HIPCHK(ctx, hipMalloc(&m->d_galpha, nc * sizeof(double)));
         HIPCHK(ctx, hipMalloc(&m->d_gsrc, nc * sizeof(double)));
+        HIPCHK(ctx, hipMalloc(&m->d_qscale, 2 * sizeof(double)));
     }
     if (!m->d_csize) {
         std::vector<float> cs(nc);
@@ -858,7 +914,7 @@ static int refine_run(rt_model *m, const rt_camera *cam, CamParams C, float *d_i
                 hipLaunchKernelGGL(k_ref_down, dim3(gblk), dim3(256), 0, sm, R, L);
             CamParams Ca = C;
             Ca.hbits = 10;  // >= 4 entries per ray of a 256-ray chunk
-            const size_t lds = (size_t)(1u << Ca.hbits) * (2 * sizeof(double) + sizeof(int));
+            const size_t lds = (size_t)(1u << Ca.hbits) * (2 * sizeof(u64) + sizeof(int));
             hipLaunchKernelGGL(k_ref_adj, dim3(gblk), dim3(256), lds, sm, Ca, R, m->d_galpha,
                                m->d_gsrc);
         }
@@ -926,24 +982,31 @@ int image_adjoint_run(rt_model *m, const rt_camera *cam, const float *d_T, const
                        m->d_src);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, nc);
     hipLaunchKernelGGL(k_zero_f64, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_gsrc, nc);
+    const bool stars = cam->incl_star && m->nstar > 0;
+    StarArg fl;
+    star_fluxes(m, cam, fl);
+    float fmax_star = 0.f;
+    for (int s = 0; stars && s < m->nstar && s < RT_MAX_STAR; ++s) fmax_star = std::max(fmax_star, fl.f[s]);
+    hipLaunchKernelGGL(k_adj_scale, dim3(1), dim3(256), 0, sm, d_dimg, C.nx * C.ny, m->d_src,
+                       m->ncell, fmax_star, stars ? m->nstar : 0,
+                       (float)(m->rw[m->nr] / RT_LEN_UNIT), m->d_qscale);
+    C.qs = m->d_qscale;
     const unsigned ntiles = (unsigned)(C.ntx * ((C.ny + C.th - 1) / C.th));
     if (cam->nrefine > 0) {
         rc = refine_run(m, cam, C, nullptr, d_dimg);
         if (rc) return rc;
     } else {
-        const size_t adj_lds = (size_t)(1u << C.hbits) * (2 * sizeof(double) + sizeof(int));
+        const size_t adj_lds = (size_t)(1u << C.hbits) * (2 * sizeof(u64) + sizeof(int));
         hipLaunchKernelGGL(k_image_adj, dim3(ntiles), dim3(C.tw * C.th), adj_lds, sm, C, d_dimg,
                            m->d_galpha, m->d_gsrc);
     }
-    if (cam->incl_star && m->nstar > 0) {
-        StarArg fl;
-        star_fluxes(m, cam, fl);
+    if (stars)
         hipLaunchKernelGGL(k_stars, dim3(m->nstar), dim3(64), 0, sm, C, m->nstar, m->d_spos, fl,
                            (float *)nullptr, d_dimg, m->d_galpha);
-    }
     hipLaunchKernelGGL(k_cam_post, dim3(nblk(nc, 256)), dim3(256), 0, sm, m->d_galpha, m->d_gsrc,
-                       rho, m->nspec, m->ncell, kap, (float)RT_LEN_UNIT, d_T, pre, hnu_k,
-                       cam->d_jscat, m->d_src, d_g_logrho, d_g_T);
+                       (const double *)m->d_qscale, rho, m->nspec, m->ncell, kap,
+                       (float)RT_LEN_UNIT, d_T, pre, hnu_k, cam->d_jscat, m->d_src, d_g_logrho,
+                       d_g_T);
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
 }

@@ -124,7 +124,8 @@ struct rt_model {
     float *d_csize = nullptr;     // [cell] cell size for the camera's sub-pixeling (au)
     double *d_tabin = nullptr;    // inputs of the device table build
     size_t tabin_cap = 0;
-    double *d_galpha = nullptr, *d_gsrc = nullptr;
+    double *d_galpha = nullptr, *d_gsrc = nullptr;  // adjoint sums, 64-bit fixed point
+    double *d_qscale = nullptr;  // their units: [0] per unit dL/dalpha, [1] per unit dL/dS
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
     size_t rowI_cap = 0;
     // device density generator: cell centres, per-component scratch

@@ -104,9 +104,10 @@ def test_ray_tiles_do_not_change_results():
             ref = (img, glr.cpu().numpy(), gT.cpu().numpy())
             continue
         assert np.array_equal(img, ref[0]), tile
-        # adjoint accumulates with FP64 atomics: order may differ
-        assert np.allclose(gT.cpu().numpy(), ref[2], rtol=1e-5, atol=1e-6 * np.abs(ref[2]).max())
-        assert np.allclose(glr.cpu().numpy(), ref[1], rtol=1e-5, atol=1e-6 * np.abs(ref[1]).max())
+        # the adjoint's per-cell sums are fixed-point integers: the gradient
+        # does not depend on the order in which rays and blocks add to a cell
+        assert np.array_equal(gT.cpu().numpy(), ref[2]), tile
+        assert np.array_equal(glr.cpu().numpy(), ref[1]), tile
     with pytest.raises(RTError):
         dm.image(dm.camera(npix, pix, 1300.0, tile=(10, 3)), Td)
 

@@ -93,10 +93,13 @@ def test_subpixel_adjoint_matches_oracle_and_repeats():
         rel = np.linalg.norm(g - o) / np.linalg.norm(o)
         print("adjoint %s: relative L2 error %.2e" % (name, rel))
         assert rel < 5e-5
-    # the image itself is bit-reproducible
+    # the image and its adjoint are bit-reproducible (sub-pixel slots are
+    # handed out in arrival order; sums do not depend on it)
     a = dm.image(cam, Td)
     b = dm.image(cam, Td)
     assert torch.equal(a, b)
+    g_lr2, g_T2 = dm.image_adjoint(cam, Td, None, dd)
+    assert torch.equal(g_lr, g_lr2) and torch.equal(g_T, g_T2)
 
 
 def test_subpixel_pool_grows_and_cap_of_levels():
