@@ -19,6 +19,10 @@
 
 #include "rt_internal.h"
 
+// A chain's kernels are small (a 256x256 image is 256 blocks, one per CU), so
+// chains of different models fill the GPU better side by side.
+#define RT_CHAIN_STREAMS 4
+
 struct rt_batch {
     rt_ctx *ctx = nullptr;
     std::vector<rt_model *> models;
@@ -48,6 +52,10 @@ struct rt_batch {
     // timing; the step's own stream (captures need one that is not the null
     // stream) and the events that order it after / before the caller's stream
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e_in = nullptr, e_out = nullptr;
+    // the per-model image chains run on RT_CHAIN_STREAMS streams forked from
+    // the step's stream after the thermal launch and joined before its end
+    hipStream_t chain_stream[RT_CHAIN_STREAMS] = {};
+    hipEvent_t e_fork = nullptr, e_join[RT_CHAIN_STREAMS] = {};
     hipStream_t stream = nullptr;
 };
 
@@ -122,6 +130,11 @@ int batch_create(rt_ctx *ctx, rt_model *const *models, int n, rt_batch **out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_out, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_fork, hipEventDisableTiming);
+    for (int k = 0; k < RT_CHAIN_STREAMS && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&b->chain_stream[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_join[k], hipEventDisableTiming);
+    }
     if (e != hipSuccess) {
         batch_destroy(b);
         return set_err(ctx, RT_E_NOMEM, "batch allocation failed: %s", hipGetErrorString(e));
@@ -167,6 +180,11 @@ int batch_destroy(rt_batch *b) {
     if (b->e_in) (void)hipEventDestroy(b->e_in);
     if (b->e_out) (void)hipEventDestroy(b->e_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->e_fork) (void)hipEventDestroy(b->e_fork);
+    for (int k = 0; k < RT_CHAIN_STREAMS; ++k) {
+        if (b->chain_stream[k]) (void)hipStreamDestroy(b->chain_stream[k]);
+        if (b->e_join[k]) (void)hipEventDestroy(b->e_join[k]);
+    }
     delete b;
     return RT_OK;
 }
@@ -422,6 +440,37 @@ int batch_step(rt_batch *b, const uint64_t *nphot, const uint64_t *seeds,
 }
 
 // capture (once) and launch, or enqueue eagerly, on ctx->stream
+// the image chains of all models: on the chain streams, forked from sm (the
+// thermal launch before them) and joined back into it; on sm alone when an
+// item sub-pixels (the refinement's node pool is the context's, one at a time)
+static int batch_enqueue_chains(rt_batch *b, const rt_step_item *items, int n, hipStream_t sm) {
+    rt_ctx *ctx = b->ctx;
+    bool refine = false;
+    for (int i = 0; i < n; ++i) refine = refine || items[i].cam.nrefine > 0;
+    int rc = RT_OK;
+    if (refine || n < 2) {
+        for (int i = 0; i < n && rc == RT_OK; ++i) rc = batch_enqueue_image_chain(b, i, items[i], sm);
+        return rc;
+    }
+    const int ns = std::min(n, RT_CHAIN_STREAMS);
+    HIPCHK(ctx, hipEventRecord(b->e_fork, sm));
+    for (int k = 0; k < ns; ++k) HIPCHK(ctx, hipStreamWaitEvent(b->chain_stream[k], b->e_fork, 0));
+    // the chain's calls launch on the context's stream
+    for (int i = 0; i < n && rc == RT_OK; ++i) {
+        ctx->stream = b->chain_stream[i % ns];
+        rc = batch_enqueue_image_chain(b, i, items[i], ctx->stream);
+    }
+    ctx->stream = sm;
+    // joined even after an error, so a capture can end
+    for (int k = 0; k < ns; ++k) {
+        hipError_t e = hipEventRecord(b->e_join[k], b->chain_stream[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sm, b->e_join[k], 0);
+        if (e != hipSuccess && rc == RT_OK)
+            rc = set_err(ctx, RT_E_HIP, "chain join: %s", hipGetErrorString(e));
+    }
+    return rc;
+}
+
 static int batch_step_enqueue(rt_batch *b, const rt_step_item *items, int n, int use_graph) {
     rt_ctx *ctx = b->ctx;
     hipStream_t sm = ctx->stream;
@@ -431,7 +480,7 @@ static int batch_step_enqueue(rt_batch *b, const rt_step_item *items, int n, int
         batch_free_graph(b);
         HIPCHK(ctx, hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal));
         rc = batch_enqueue_thermal(b, sm, false);
-        for (int i = 0; i < n && rc == RT_OK; ++i) rc = batch_enqueue_image_chain(b, i, items[i], sm);
+        if (rc == RT_OK) rc = batch_enqueue_chains(b, items, n, sm);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(sm, &g);
         if (rc) {
@@ -453,7 +502,7 @@ static int batch_step_enqueue(rt_batch *b, const rt_step_item *items, int n, int
         HIPCHK(ctx, hipGraphLaunch(b->exec, sm));
     } else {
         rc = batch_enqueue_thermal(b, sm, false);
-        for (int i = 0; i < n && rc == RT_OK; ++i) rc = batch_enqueue_image_chain(b, i, items[i], sm);
+        if (rc == RT_OK) rc = batch_enqueue_chains(b, items, n, sm);
         if (rc) return rc;
     }
     HIPCHK(ctx, hipEventRecord(b->e2, sm));
