@@ -220,11 +220,14 @@ int rt_model_set_density(rt_model *m, const double *h_rho);
  * recaptured on its next step. */
 int rt_model_set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
                          const double *h_g);
-/* The same, with the emission, re-emission and mean-opacity tables built by a
- * kernel on the context's stream instead of on the host (FP64, the host
- * builder's operations and order: the FP32 tables agree with it to
- * rounding); the groups and the other per-species arrays are set on the host.
- * For the per-walker dust of a sampler step. */
+/* The same, stream-ordered: the groups and kmax are set on the host, then one
+ * copy from a pinned staging buffer and two kernels on the context's stream
+ * make the per-species arrays (the host's values bit for bit) and the
+ * emission, re-emission and mean-opacity tables (FP64, the host builder's
+ * operations and order: the FP32 tables agree with it to rounding).  Returns
+ * without waiting for the device: work already on the stream finishes with the
+ * old opacities, work enqueued after it sees the new ones.  For the per-walker
+ * dust of a sampler step. */
 int rt_model_set_opacity_device(rt_model *m, const double *h_kabs, const double *h_ksca,
                                 const double *h_g);
 /* Copies the model's device tables to the host (synchronises the stream):
@@ -235,6 +238,14 @@ int rt_model_set_opacity_device(rt_model *m, const double *h_kabs, const double 
 int rt_model_get_tables(rt_model *m, double *h_lnq, float *h_cdf, unsigned char *h_guide,
                         float *h_kplanck, float *h_kross);
 int rt_model_ngroups(const rt_model *m);
+/* Copies the model's per-species device opacities to the host (synchronises
+ * the stream), as the photon kernels read them: kabs, ksca, g [nspec][nlam]
+ * (float), the groups' kabs [G][nlam], 2^28 / kmax [G] (float), the per-cell
+ * tally-to-energy factors [cells][G] (double); any pointer may be NULL.  No
+ * reference counterpart (RADMC-3D keeps these in its own memory); for checks
+ * of rt_model_set_opacity_device against the host build. */
+int rt_model_get_opacity(rt_model *m, float *h_ka, float *h_ks, float *h_g, float *h_gka,
+                         float *h_ginvk, double *h_econv);
 
 /* Density components generated on the device (pdspy's model builders, which
  * run on the host there):
@@ -257,8 +268,10 @@ typedef struct {
 } rt_density_comp;
 /* Replaces the model's densities with the sum of up to 8 components computed
  * on the device (FP64; the envelope's mu0 roots by the same Brent iteration
- * as the host builder).  Synchronous; the host copy of the densities is not
- * updated (rt_get_density_device reads the result). */
+ * as the host builder).  Stream-ordered on the context's stream (the
+ * components travel as a kernel argument; h_comps may be reused on return);
+ * the host copy of the densities is not updated (rt_get_density_device reads
+ * the result). */
 int rt_model_generate_density(rt_model *m, const rt_density_comp *h_comps, int ncomp);
 /* Device copy of the model's densities as float [nspec][cells]. */
 int rt_get_density_device(rt_model *m, float *d_rho);

@@ -15,7 +15,7 @@ EXPORTS = [
     "rt_create", "rt_destroy", "rt_set_stream", "rt_last_error", "rt_device_info",
     "rt_model_create", "rt_model_destroy", "rt_model_ncells", "rt_model_set_density",
     "rt_model_set_opacity", "rt_model_set_opacity_device", "rt_model_get_tables",
-    "rt_model_ngroups",
+    "rt_model_ngroups", "rt_model_get_opacity",
     "rt_model_generate_density", "rt_get_density_device", "rt_scattering_source",
     "rt_thermal", "rt_get_temperature", "rt_thermal_timeline", "rt_get_temperature_device",
     "rt_image",
@@ -131,6 +131,7 @@ def load():
         "rt_model_set_opacity_device": ([P, P, P, P], I),
         "rt_model_get_tables": ([P, P, P, P, P, P], I),
         "rt_model_ngroups": ([P], I),
+        "rt_model_get_opacity": ([P, P, P, P, P, P, P], I),
         "rt_model_generate_density": ([P, C.POINTER(DensityComp), I], I),
         "rt_get_density_device": ([P, P], I),
         "rt_scattering_source": ([P, D, U64, U64, P, P, C.POINTER(ScatStats)], I),

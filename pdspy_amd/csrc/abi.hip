@@ -67,9 +67,10 @@ static void free_model(rt_model *m) {
                     m->d_econv, m->d_tally, m->d_xrow, m->d_Tg, m->d_visits, m->d_cnt, m->d_Tout,
                     m->d_alpha, m->d_src, m->d_galpha, m->d_gsrc, m->d_qscale, m->d_rowI, m->d_pipe,
                     m->d_tally2,
-                    m->d_csize, m->d_tabin,
+                    m->d_csize, m->d_tabin, m->d_tT, m->d_opin, m->d_dr,
                     m->d_xring, m->d_run, m->d_rc, m->d_tc, m->d_dens_w, m->d_dens_rows,
                     m->d_dens_fac, m->d_comps, m->d_vol, m->d_jtally, m->d_scum, m->d_scnt, m->d_sblk};
+    opacity_device_free(m);
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     std::free(m->h_run);
@@ -440,31 +441,57 @@ static int set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
         for (size_t i = 0; i < nk; ++i)
             if (!(h_g[i] > -1.0 && h_g[i] < 1.0)) return set_err(ctx, RT_E_ARG, "HG g must be in (-1, 1)");
     // the new tables must keep the device layout: same species grouping; the
-    // old host tables are kept for a rollback
+    // old opacities (and, for the host build, the old host tables) are kept
+    // for a rollback
     std::vector<double> old_kabs, old_ksca, old_g;
     old_kabs.swap(m->kabs);
     old_ksca.swap(m->ksca);
     old_g.swap(m->gsca);
-    const HostTables old_tab = m->tab;
     m->kabs.assign(h_kabs, h_kabs + nk);
     m->ksca.assign(h_ksca, h_ksca + nk);
     if (h_g) m->gsca.assign(h_g, h_g + nk);
     std::string err;
     int rc;
     if (on_device) {
-        // groups, frequency grid and kmax on the host; the tables by a kernel
+        // groups, frequency grid and kmax on the host (the host's emission
+        // tables are not rebuilt, so only these fields change); the rest on
+        // the device
         TableSpec sp{m->nlam, m->nspec, m->ntemp, m->nstar, m->temp0, m->temp1,
                      m->lam.data(), m->kabs.data(), m->ksca.data(), m->stars.data()};
-        rc = host_table_groups(sp, m->tab, err);
-    } else {
-        rc = build_tables(m, err);
+        HostTables g;
+        rc = host_table_groups(sp, g, err);
+        if (!rc && g.grp != m->tab.grp) {
+            rc = RT_E_ARG;
+            err = "the new opacities group the species differently (species with identical kabs "
+                  "share tables); create a new model instead";
+        }
+        if (rc) {
+            m->kabs.swap(old_kabs);
+            m->ksca.swap(old_ksca);
+            m->gsca.swap(old_g);
+            return set_err(ctx, rc, "%s", err.c_str());
+        }
+        m->tab.nu.swap(g.nu);
+        m->tab.dnu.swap(g.dnu);
+        m->tab.grep.swap(g.grep);
+        m->tab.kmax.swap(g.kmax);
+        m->tab.lnt0 = g.lnt0;
+        m->tab.dlnt = g.dlnt;
+        // stream-ordered: one staged copy and kernels on the context's stream
+        rc = opacity_device_update(m);
+        if (rc) return rc;
+        m->have_T = false;
+        m->opac_version++;
+        return RT_OK;
     }
+    const HostTables old_tab = m->tab;
+    rc = build_tables(m, err);
     if (!rc && m->tab.grp != old_tab.grp) {
         rc = RT_E_ARG;
         err = "the new opacities group the species differently (species with identical kabs "
               "share tables); create a new model instead";
     }
-    if (!rc && !on_device &&
+    if (!rc &&
         (m->tab.cdf.size() != old_tab.cdf.size() || m->tab.kross.size() != old_tab.kross.size())) {
         rc = RT_E_ARG;
         err = "table sizes changed";
@@ -476,20 +503,17 @@ static int set_opacity(rt_model *m, const double *h_kabs, const double *h_ksca,
         m->tab = old_tab;
         return set_err(ctx, rc, "%s", err.c_str());
     }
-    OpacArrays o;
-    opacity_arrays(m, o);
-    const HostTables &t = m->tab;
-    HIPCHK(ctx, hipSetDevice(ctx->device));
-    HIPCHK(ctx, hipMemcpy(m->d_ka, o.ka.data(), nk * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_ks, o.ks.data(), nk * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_g, o.gg.data(), nk * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_gka, o.gka.data(), o.gka.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_ginvk, o.ginvk.data(), o.ginvk.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(ctx, hipMemcpy(m->d_econv, o.econv.data(), o.econv.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (on_device) {
-        rc = tables_device_build(m);
-        if (rc) return rc;
-    } else {
+    {
+        OpacArrays o;
+        opacity_arrays(m, o);
+        const HostTables &t = m->tab;
+        HIPCHK(ctx, hipSetDevice(ctx->device));
+        HIPCHK(ctx, hipMemcpy(m->d_ka, o.ka.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_ks, o.ks.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_g, o.gg.data(), nk * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_gka, o.gka.data(), o.gka.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_ginvk, o.ginvk.data(), o.ginvk.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(ctx, hipMemcpy(m->d_econv, o.econv.data(), o.econv.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(ctx, hipMemcpy(m->d_cdf, t.cdf.data(), t.cdf.size() * sizeof(float), hipMemcpyHostToDevice));
         HIPCHK(ctx, hipMemcpy(m->d_guide, t.guide.data(), t.guide.size(), hipMemcpyHostToDevice));
         HIPCHK(ctx, hipMemcpy(m->d_lnq, t.lnq.data(), t.lnq.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -523,6 +547,22 @@ int rt_model_get_tables(rt_model *m, double *h_lnq, float *h_cdf, unsigned char 
     if (h_kplanck) HIPCHK(ctx, hipMemcpy(h_kplanck, m->d_kplanck, G * nT * sizeof(float), hipMemcpyDeviceToHost));
     if (h_kross)
         HIPCHK(ctx, hipMemcpy(h_kross, m->d_kross, (size_t)m->nspec * nT * sizeof(float), hipMemcpyDeviceToHost));
+    return RT_OK;
+}
+
+int rt_model_get_opacity(rt_model *m, float *h_ka, float *h_ks, float *h_g, float *h_gka,
+                         float *h_ginvk, double *h_econv) {
+    if (!m) return RT_E_ARG;
+    rt_ctx *ctx = m->ctx;
+    const size_t nk = (size_t)m->nspec * m->nlam, G = (size_t)m->tab.ngroup;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_ka) HIPCHK(ctx, hipMemcpy(h_ka, m->d_ka, nk * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_ks) HIPCHK(ctx, hipMemcpy(h_ks, m->d_ks, nk * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_g) HIPCHK(ctx, hipMemcpy(h_g, m->d_g, nk * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_gka) HIPCHK(ctx, hipMemcpy(h_gka, m->d_gka, G * m->nlam * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_ginvk) HIPCHK(ctx, hipMemcpy(h_ginvk, m->d_ginvk, G * sizeof(float), hipMemcpyDeviceToHost));
+    if (h_econv)
+        HIPCHK(ctx, hipMemcpy(h_econv, m->d_econv, (size_t)m->ncell * G * sizeof(double), hipMemcpyDeviceToHost));
     return RT_OK;
 }
 

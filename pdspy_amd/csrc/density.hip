@@ -23,6 +23,7 @@
 #pragma clang fp contract(off)
 #include "rt_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -302,17 +303,31 @@ void density_enqueue(rt_model *m, const rt_density_comp *d_comps, int ncomp, hip
                        m->d_rhopos);
 }
 
-// One model, synchronous: upload the components, generate, wait.
+struct CompArgs {
+    rt_density_comp c[RT_MAX_COMP];
+};
+
+// Stores components passed by value (kernel arguments are copied at launch, so
+// the caller's array may change as soon as the launch returns).
+__global__ void k_store_comps(CompArgs a, int ncomp, rt_density_comp *d_comps) {
+    const int k = threadIdx.x;
+    if (k < ncomp) d_comps[k] = a.c[k];
+}
+
+// One model, stream-ordered: the components go to the device as a kernel
+// argument and the generator's kernels follow on the context's stream; nothing
+// waits for the device.
 int density_generate(rt_model *m, const rt_density_comp *h_comps, int ncomp) {
     rt_ctx *ctx = m->ctx;
     int rc = density_prepare(m, h_comps, ncomp);
     if (rc) return rc;
     hipStream_t sm = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(m->d_comps, h_comps, sizeof(rt_density_comp) * ncomp,
-                               hipMemcpyHostToDevice, sm));
+    CompArgs a{};
+    std::copy(h_comps, h_comps + ncomp, a.c);
+    hipLaunchKernelGGL(k_store_comps, dim3(1), dim3(RT_MAX_COMP), 0, sm, a, ncomp,
+                       (rt_density_comp *)m->d_comps);
     density_enqueue(m, (const rt_density_comp *)m->d_comps, ncomp, sm);
     HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(sm));
     m->rho_host_stale = true;
     m->have_T = false;
     return RT_OK;

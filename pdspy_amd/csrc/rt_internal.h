@@ -122,8 +122,14 @@ struct rt_model {
     // image scratch
     float *d_alpha = nullptr, *d_src = nullptr;
     float *d_csize = nullptr;     // [cell] cell size for the camera's sub-pixeling (au)
-    double *d_tabin = nullptr;    // inputs of the device table build
-    size_t tabin_cap = 0;
+    double *d_tabin = nullptr;    // inputs of the device table build [3 + 2 G + S][nlam]
+    double *d_tT = nullptr;       // its per-temperature quantities [4 ntemp nlam + 2 ntemp]
+    // device opacity update: staged kabs, ksca, g [S][nlam] and kmax [G] (pinned
+    // host copy, the device copy, the event of the last upload from it)
+    double *h_opin = nullptr, *d_opin = nullptr;
+    hipEvent_t e_opin = nullptr;
+    bool opin_busy = false;
+    double *d_dr = nullptr;       // [nr] radial cell widths (cm)
     double *d_galpha = nullptr, *d_gsrc = nullptr;  // adjoint sums, 64-bit fixed point
     double *d_qscale = nullptr;  // their units: [0] per unit dL/dalpha, [1] per unit dL/dS
     float *d_rowI = nullptr;      // per-ray intensity for the adjoint
@@ -171,7 +177,8 @@ struct ThermPlan {
 
 // the opacity-dependent thermal-MC tables built by a kernel on the context's
 // stream from m->kabs / ksca, m->tab holding the groups (tables_dev.hip)
-int tables_device_build(rt_model *m);
+int opacity_device_update(rt_model *m);
+void opacity_device_free(rt_model *m);
 
 // the model's thermal-MC tables from its host opacities and stars
 inline int build_tables(rt_model *m, std::string &err) {

@@ -297,9 +297,11 @@ int scat_prepare(rt_model *m) {
     rt_ctx *ctx = m->ctx;
     if (m->vol.size() != (size_t)m->ncell) return set_err(ctx, RT_E_ARG, "model has no cell volumes");
     const int nc = m->ncell;
-    if (!m->d_vol) {
+    if (!m->d_vol) {  // the device opacity update may have made it already
         HIPCHK(ctx, hipMalloc((void **)&m->d_vol, sizeof(double) * nc));
         HIPCHK(ctx, hipMemcpy(m->d_vol, m->vol.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    }
+    if (!m->d_jtally) {
         HIPCHK(ctx, hipMalloc((void **)&m->d_jtally, sizeof(unsigned long long) * nc));
         HIPCHK(ctx, hipMalloc((void **)&m->d_scum, sizeof(double) * nc));
         HIPCHK(ctx, hipMalloc((void **)&m->d_scnt, sizeof(unsigned long long) * 8));

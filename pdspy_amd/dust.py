@@ -26,6 +26,15 @@ def _bracket(grid, x, name):
     return i, (x - grid[i]) / (grid[i + 1] - grid[i])
 
 
+def _bracket_many(grid, x, name):
+    bad = ~((grid[0] <= x) & (x <= grid[-1]))
+    if bad.any():
+        v = x[bad][0]
+        raise ValueError("DustGenerator: %s = %g outside the table [%g, %g]" % (name, v, grid[0], grid[-1]))
+    i = np.clip(np.searchsorted(grid, x, side="right") - 1, 0, len(grid) - 2)
+    return i, (x - grid[i]) / (grid[i + 1] - grid[i])
+
+
 class DustGenerator:
     def __init__(self, lam, amax, p, kabs, ksca):
         self.lam = np.asarray(lam, dtype=float)
@@ -61,3 +70,21 @@ class DustGenerator:
         d = Dust()
         d.set_properties(self.lam, out[0], out[1])
         return d
+
+    def draw_many(self, amax, p):
+        """kabs, ksca [n][n_lam] of the draws (amax[k], p[k]) at the tables'
+        wavelengths: the values ``self(amax[k], p[k])`` holds (the same
+        elementwise operations), for many walkers in one pass."""
+        amax = np.atleast_1d(np.asarray(amax, dtype=float))
+        p = np.atleast_1d(np.asarray(p, dtype=float))
+        if amax.shape != p.shape or amax.ndim != 1:
+            raise ValueError("DustGenerator.draw_many: amax and p must be 1-d of one length")
+        i, tp = _bracket_many(self.p, p, "p")
+        j, ta = _bracket_many(self.amax, amax, "amax")
+        tp, ta = tp[:, None], ta[:, None]
+        out = []
+        for L in self._lk:
+            v = ((1 - tp) * (1 - ta) * L[i, j] + (1 - tp) * ta * L[i, j + 1] +
+                 tp * (1 - ta) * L[i + 1, j] + tp * ta * L[i + 1, j + 1])
+            out.append(10.0 ** v)
+        return out[0], out[1]

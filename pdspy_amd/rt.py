@@ -99,6 +99,12 @@ def _loglog_interp(x_new, x, y):
     return np.interp(x_new, x, y)
 
 
+def loglog_resample(lam_um, lam_src_um, k):
+    """One opacity row k(lam_src_um) on the wavelengths lam_um, as
+    ``dust_arrays`` resamples a Dust's kabs or ksca."""
+    return _loglog_interp(np.asarray(lam_um, dtype=float), np.asarray(lam_src_um, dtype=float), k)
+
+
 def dust_arrays(dust, lam_um):
     """pdspy Dust objects (lam in cm, kabs, ksca, optional g) -> kabs, ksca, g
     [nspec][nlam] on the wavelengths lam_um (micron), log-log interpolated as
@@ -181,11 +187,10 @@ class DeviceModel:
         kabs = np.ascontiguousarray(kabs, dtype=float).reshape(self.nspec, nl)
         ksca = np.ascontiguousarray(ksca, dtype=float).reshape(self.nspec, nl)
         gg = None if g is None else np.ascontiguousarray(g, dtype=float).reshape(self.nspec, nl)
-        # synchronous copies, no stream work: safe to call from several threads
-        # for different models
+        # the device variant is ordered on torch's current stream; the host
+        # variant copies synchronously
         fn = self.ctx.L.rt_model_set_opacity_device if on_device else self.ctx.L.rt_model_set_opacity
-        if on_device:
-            self.ctx.sync_stream()
+        self.ctx.sync_stream()
         self.ctx.check(fn(self.h, kabs.ctypes.data, ksca.ctypes.data,
                           None if gg is None else gg.ctypes.data))
         self._keep.update(kabs=kabs, ksca=ksca)
@@ -217,6 +222,22 @@ class DeviceModel:
         self.ctx.check(self.ctx.L.rt_model_get_tables(
             self.h, out["lnq"].ctypes.data, out["cdf"].ctypes.data, out["guide"].ctypes.data,
             out["kplanck"].ctypes.data, out["kross"].ctypes.data))
+        return out
+
+    def opacity_arrays(self):
+        """The device's per-species opacities as the photon kernels read them:
+        ka, ks, g [nspec][nlam] (float32), gka [G][nlam], ginvk [G] (float32),
+        econv [cells][G] (float64)."""
+        G = int(self.ctx.L.rt_model_ngroups(self.h))
+        nl = len(self._keep["lam_um"])
+        out = dict(ka=np.zeros((self.nspec, nl), dtype=np.float32),
+                   ks=np.zeros((self.nspec, nl), dtype=np.float32),
+                   g=np.zeros((self.nspec, nl), dtype=np.float32),
+                   gka=np.zeros((G, nl), dtype=np.float32),
+                   ginvk=np.zeros(G, dtype=np.float32),
+                   econv=np.zeros((self.ncell, G)))
+        self.ctx.check(self.ctx.L.rt_model_get_opacity(
+            self.h, *(out[k].ctypes.data for k in ("ka", "ks", "g", "gka", "ginvk", "econv"))))
         return out
 
     def set_density(self, rho):

@@ -24,6 +24,7 @@ from .inputs import AU, arcsec, c as c_light, pc
 LAM_UM = 1300.0
 DPC = 140.0
 JY = 1e-23
+UPDATE_STREAMS = 4   # streams a new parameter draw's per-model updates run on
 
 
 class SamplerStep:
@@ -119,6 +120,25 @@ class SamplerStep:
     def _dust(self, p):
         return [self.dust_gen(p["a_max"] / 1e4, p["p"]), self.env_dust]
 
+    def _set_disk_dust(self, i, kabs_t, ksca_t):
+        """Model i's opacities from a disk dust drawn at the generator's
+        wavelengths (``DustGenerator.draw_many``) and the envelope dust: the
+        arrays ``DeviceModel.set_dust`` would make from the two Dust objects."""
+        m = self.models[i]
+        if m._keep.get("gsca") is not None:
+            raise ValueError("sampler models scatter isotropically (no g)")
+        lam = m._keep["lam_um"]
+        if not hasattr(self, "_env_rows"):
+            self._env_rows = {}
+        env = self._env_rows.get(i)
+        if env is None:
+            ek, es, _, _ = rt.dust_arrays([self.env_dust], lam)
+            env = self._env_rows[i] = (np.asarray(ek[0]), np.asarray(es[0]))
+        lam_t = self.dust_gen.lam * 1e4
+        kabs = np.stack([rt.loglog_resample(lam, lam_t, kabs_t), env[0]])
+        ksca = np.stack([rt.loglog_resample(lam, lam_t, ksca_t), env[1]])
+        m.set_opacity(kabs, ksca, on_device=True)
+
     def _arrays(self, p):
         a = rt.model_arrays(configs.batch_model(p).grid)
         if self.dust_gen is not None:
@@ -141,13 +161,28 @@ class SamplerStep:
                 raise ValueError("set_params keeps each model's grid and photon count")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for m, p in zip(self.models, params):
-            m.generate_density(configs.batch_components(p))
+        # the models' updates are small kernels (tens of waves each): they run
+        # on UPDATE_STREAMS streams, forked from and joined to the caller's
+        main = torch.cuda.current_stream()
+        if not hasattr(self, "_upd_streams"):
+            self._upd_streams = [torch.cuda.Stream() for _ in range(UPDATE_STREAMS)]
+        for st in self._upd_streams:
+            st.wait_stream(main)
         if self.dust_gen is not None:
-            # each model's new dust: interpolated on the host (2 x nlam values),
-            # the thermal tables rebuilt by a kernel on the stream
-            for m, p in zip(self.models, params):
-                m.set_dust(self._dust(p), on_device=True)
+            # the new disk dust of every model in one pass on the host (2 x nlam
+            # values per model, which the host keeps: the group check and the
+            # camera-wavelength opacities read them); the per-species arrays and
+            # thermal tables are then made by kernels on the device
+            ka, ks = self.dust_gen.draw_many([p["a_max"] / 1e4 for p in params],
+                                             [p["p"] for p in params])
+        for i, (m, p) in enumerate(zip(self.models, params)):
+            with torch.cuda.stream(self._upd_streams[i % UPDATE_STREAMS]):
+                m.generate_density(configs.batch_components(p))
+                if self.dust_gen is not None:
+                    self._set_disk_dust(i, ka[i], ks[i])
+        for st in self._upd_streams:
+            main.wait_stream(st)
+        self.models[0].ctx.sync_stream()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         self.params = params

@@ -53,3 +53,31 @@ def test_generator_closed_forms():
         gen(amax[-1] * 1.01, 3.0)
     with pytest.raises(ValueError):
         gen(1e-3, 2.4)
+
+
+def test_draw_many_equals_single_draws():
+    """The vectorized draw of many walkers (a sampler step's new parameters)
+    holds the single draws' values bit for bit, and resampled onto a model's
+    wavelengths equals dust_arrays of the Dust objects."""
+    from pdspy_amd import rt
+    lam, amax, p, kabs, ksca = _tables(2)
+    gen = DustGenerator(lam, amax, p, kabs, ksca)
+    rng = np.random.default_rng(3)
+    a = np.concatenate([10 ** rng.uniform(-4, 1, 40), [amax[0], amax[-1], amax[4]]])
+    q = np.concatenate([rng.uniform(2.5, 4.5, 40), [p[0], p[-1], 3.3]])
+    ka, ks = gen.draw_many(a, q)
+    assert ka.shape == (a.size, lam.size)
+    for k in range(a.size):
+        d = gen(a[k], q[k])
+        assert np.array_equal(d.kabs, ka[k]) and np.array_equal(d.ksca, ks[k]), k
+    # onto the tables' own wavelengths and onto a different grid (log-log
+    # interpolation with power-law ends)
+    for lam_um in (lam * 1e4, np.logspace(-1.5, 4.5, 57)):
+        for k in (0, 17, a.size - 1):
+            want = rt.dust_arrays([gen(a[k], q[k])], lam_um)
+            assert np.array_equal(rt.loglog_resample(lam_um, lam * 1e4, ka[k]), want[0][0])
+            assert np.array_equal(rt.loglog_resample(lam_um, lam * 1e4, ks[k]), want[1][0])
+    with pytest.raises(ValueError):
+        gen.draw_many([amax[-1] * 1.01], [3.0])
+    with pytest.raises(ValueError):
+        gen.draw_many([1e-3, 1e-3], [3.0])

@@ -68,6 +68,8 @@ def test_set_opacity_refuses_a_new_grouping():
     same = np.stack([base["kabs"][0], base["kabs"][0]])
     with pytest.raises(RTError):
         dm.set_opacity(same, 0.5 * same)
+    with pytest.raises(RTError):
+        dm.set_opacity(same, 0.5 * same, on_device=True)
     dm.thermal(10000, 2, max_events=100000)
     assert np.array_equal(dm.temperature(), T0)
     with pytest.raises(RTError):
@@ -142,7 +144,16 @@ def test_device_built_tables_match_host_build():
     fresh = dict(base, kabs=np.array(new[0]), ksca=np.array(new[1]))
     host = rt.DeviceModel(fresh)
     dev = rt.DeviceModel(base)
+    # two stream-ordered updates in a row (the second reuses the staging
+    # buffer while the first may still be in flight); the last one counts
+    other = rt.dust_arrays([gen(1e-2, 2.7), gen(1e-4, 3.5)], base["lam_um"])
+    dev.set_opacity(np.array(other[0]), np.array(other[1]), on_device=True)
     dev.set_opacity(np.array(new[0]), np.array(new[1]), on_device=True)
+    # the per-species arrays are the host build's bit for bit
+    oa, ob = host.opacity_arrays(), dev.opacity_arrays()
+    for k in ("ka", "ks", "g", "gka", "ginvk", "econv"):
+        assert np.array_equal(oa[k], ob[k]), k
+    assert oa["econv"].min() > 0 and oa["ginvk"].min() > 0
     a, b = host.tables(), dev.tables()
     assert np.allclose(a["lnq"], b["lnq"], rtol=1e-12, atol=0)
     d = np.abs(a["cdf"].astype(float) - b["cdf"].astype(float))

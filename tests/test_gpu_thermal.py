@@ -187,6 +187,10 @@ def test_device_density_generator_matches_host_builders():
         comps = configs.disk_envelope_components(
             mdisk=c["mdisk"], plrho=c["plrho"], plh=c["plh"], menv=c["menv"],
             rcent=c.get("rcent", 50.0))
+        # two stream-ordered generations in a row with no wait between them
+        # (the components travel by value): the second one's densities stay
+        m.generate_density(configs.disk_envelope_components(
+            mdisk=5e-2, plrho=1.0, plh=1.0, menv=1e-4, rcent=10.0))
         m.generate_density(comps)
         got = m.rho_device().cpu().numpy().astype(np.float64)
         want = rt.model_arrays(host.grid)["rho"].reshape(m.nspec, m.ncell).astype(np.float32)
