@@ -270,36 +270,50 @@ int opacity_device_update(rt_model *m) {
     const size_t nk = (size_t)S * nl, nop = 3 * nk + RT_MAX_GROUP;
     if (G > RT_MAX_GROUP) return set_err(ctx, RT_E_LIMIT, "more than %d absorption groups", RT_MAX_GROUP);
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (!m->d_opin) {
-        // first update: buffers, and the inputs that stay (frequencies, cell
-        // widths and volumes, the per-temperature quantities)
+    // first update: buffers, and the inputs that stay (frequencies, cell widths
+    // and volumes, the per-temperature quantities); each made once, so a call
+    // that failed half way is completed by the next one
+    if (!m->h_opin)
         HIPCHK(ctx, hipHostMalloc((void **)&m->h_opin, nop * sizeof(double), hipHostMallocDefault));
-        HIPCHK(ctx, hipMalloc((void **)&m->d_opin, nop * sizeof(double)));
-        HIPCHK(ctx, hipEventCreateWithFlags(&m->e_opin, hipEventDisableTiming));
+    if (!m->d_opin) HIPCHK(ctx, hipMalloc((void **)&m->d_opin, nop * sizeof(double)));
+    if (!m->e_opin) HIPCHK(ctx, hipEventCreateWithFlags(&m->e_opin, hipEventDisableTiming));
+    if (!m->d_tabin) {
         std::vector<double> in((size_t)(3 + 2 * RT_MAX_GROUP + S) * nl, 0.0);
         for (int i = 0; i < nl; ++i) {
             in[i] = t.nu[i];
             in[nl + i] = t.dnu[i];
             in[2 * nl + i] = std::log(t.dnu[i]);
         }
-        HIPCHK(ctx, hipMalloc((void **)&m->d_tabin, in.size() * sizeof(double)));
-        HIPCHK(ctx, hipMemcpy(m->d_tabin, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice));
+        double *d = nullptr;
+        HIPCHK(ctx, hipMalloc((void **)&d, in.size() * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(d, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice));
+        m->d_tabin = d;
+    }
+    if (!m->d_tT) {
         const size_t nTl = (size_t)nT * nl;
-        HIPCHK(ctx, hipMalloc((void **)&m->d_tT, (4 * nTl + 2 * (size_t)nT) * sizeof(double)));
+        double *d = nullptr;
+        HIPCHK(ctx, hipMalloc((void **)&d, (4 * nTl + 2 * (size_t)nT) * sizeof(double)));
         hipLaunchKernelGGL(k_tab_T1, dim3((unsigned)((nTl + 255) / 256)), dim3(256), 0, ctx->stream,
-                           nT, nl, t.lnt0, t.dlnt, (const double *)m->d_tabin, m->d_tT);
+                           nT, nl, t.lnt0, t.dlnt, (const double *)m->d_tabin, d);
         hipLaunchKernelGGL(k_tab_T2, dim3((nT + 63) / 64), dim3(64), 0, ctx->stream, nT, nl,
-                           (const double *)m->d_tabin, m->d_tT);
+                           (const double *)m->d_tabin, d);
         HIPCHK(ctx, hipGetLastError());
+        m->d_tT = d;
+    }
+    if (!m->d_dr) {
         std::vector<double> dr(m->nr);
         for (int i = 0; i < m->nr; ++i) dr[i] = m->rw[i + 1] - m->rw[i];
-        HIPCHK(ctx, hipMalloc((void **)&m->d_dr, m->nr * sizeof(double)));
-        HIPCHK(ctx, hipMemcpy(m->d_dr, dr.data(), m->nr * sizeof(double), hipMemcpyHostToDevice));
-        if (!m->d_vol) {
-            if (m->vol.size() != (size_t)nc) return set_err(ctx, RT_E_ARG, "model has no cell volumes");
-            HIPCHK(ctx, hipMalloc((void **)&m->d_vol, sizeof(double) * nc));
-            HIPCHK(ctx, hipMemcpy(m->d_vol, m->vol.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
-        }
+        double *d = nullptr;
+        HIPCHK(ctx, hipMalloc((void **)&d, m->nr * sizeof(double)));
+        HIPCHK(ctx, hipMemcpy(d, dr.data(), m->nr * sizeof(double), hipMemcpyHostToDevice));
+        m->d_dr = d;
+    }
+    if (!m->d_vol) {
+        if (m->vol.size() != (size_t)nc) return set_err(ctx, RT_E_ARG, "model has no cell volumes");
+        double *d = nullptr;
+        HIPCHK(ctx, hipMalloc((void **)&d, sizeof(double) * nc));
+        HIPCHK(ctx, hipMemcpy(d, m->vol.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+        m->d_vol = d;
     }
     // the staging buffer is reused once the previous upload from it is done
     if (m->opin_busy) HIPCHK(ctx, hipEventSynchronize(m->e_opin));
