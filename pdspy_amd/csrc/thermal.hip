@@ -451,6 +451,12 @@ __device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uns
     ovf = 0;
 }
 
+// waves per SIMD the register budget of the one-species variants is set for
+// (4: 127 VGPRs, no spills; 5 spills about 30 VGPRs to scratch)
+#ifndef RT_THERMAL_WAVES
+#define RT_THERMAL_WAVES 4
+#endif
+
 // One persistent launch for a batch of runs (one model each).  Each wavefront
 // serves one run at a time: its LDS slice holds that run's opacities and
 // walls; when it has no packet left and the run has none to hand out (all
@@ -461,7 +467,7 @@ __device__ __forceinline__ void flush_counts(const ThermParams &R, int lane, uns
 // species two.  BATCH: the parameter blocks are read from `runs` in
 // device memory; otherwise the single run's block P1 is a kernel argument.
 template <int MS, int MG, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 && MG <= 1 ? 4 : MS <= 4 ? 3 : 2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MS <= 2 && MG <= 1 ? RT_THERMAL_WAVES : MS <= 4 ? 3 : 2)))
 k_thermal(BatchArgs A, const ThermParams *__restrict__ runs, ThermParams P1) {
     extern __shared__ float lds[];
     const int lane = __lane_id();
