@@ -600,11 +600,24 @@ def main():
                     fn(img, dxy, u, v)
                 torch.cuda.synchronize()
                 row[name + "_ms"] = (time.perf_counter() - t0) / reps * 1e3
+            # its adjoint (the image gradient of chi^2 from the visibilities')
+            gre = torch.ones(K, dtype=torch.float32, device="cuda")
+            gim = torch.full((K,), 0.5, dtype=torch.float32, device="cuda")
+            rt.vis_dft_adjoint(gre, gim, 256, 256, dxy, u, v)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                rt.vis_dft_adjoint(gre, gim, 256, 256, dxy, u, v)
+            torch.cuda.synchronize()
+            row["dft_adj_ms"] = (time.perf_counter() - t0) / reps * 1e3
             # exact DFT of a real image: a real pixel times a complex phasor
-            # per pixel and point, 2 FMAs = 4 FP32 flops
+            # per pixel and point, 2 FMAs = 4 FP32 flops; the adjoint likewise
+            # (the real part of a complex product per pixel and point)
             fl = 4.0 * 65536 * K
             row["dft_tflops"] = fl / (row["dft_ms"] * 1e-3) / 1e12
             row["dft_frac_fp32_vector"] = row["dft_tflops"] / FP32_VECTOR_TFLOPS
+            row["dft_adj_tflops"] = fl / (row["dft_adj_ms"] * 1e-3) / 1e12
+            row["dft_adj_frac_fp32_vector"] = row["dft_adj_tflops"] / FP32_VECTOR_TFLOPS
             vis_out["K=%d" % K] = row
         out["visibilities"] = dict(vis_out, config="256x256 image above, 1.3 mm, (u,v) of "
                                    "15-2000 m baselines",

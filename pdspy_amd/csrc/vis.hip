@@ -12,9 +12,10 @@
 // column strips and read as wave-wide broadcasts; the strip's column phasors
 // sit in registers and the row phasor is one sincospi every 16 rows and a
 // rotation by one row step in between, so the inner loop is two FMAs per
-// pixel.  Adjoint: one pixel per lane in 16x16 tiles; per chunk of
-// 64 points the tile's column phasors (times the point's complex cotangent) and
-// row phasors are built once in LDS and each lane does two FMAs per point.
+// pixel.  Adjoint: one pixel per lane in 16x16 tiles, four groups of lanes per
+// tile taking alternate chunks of 64 points; per chunk the tile's column phasors
+// (times the point's complex cotangent) and row phasors are built once in LDS
+// and each lane does two FMAs per point.
 #include <cmath>
 #include <mutex>
 
@@ -90,29 +91,39 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
 
 #define ADJ_T 16
 #define ADJ_CH 64
+#define ADJ_G 4   // point groups per tile: each takes every ADJ_G-th chunk of points
+#define ADJ_BS (ADJ_T * ADJ_T * ADJ_G)
 
 // g[j][k] = Re sum_q H_q a_qk b_qj,  H_q = (gre_q - i gim_q) shift_q,
 // a_qk = exp(-2 pi i u_q x_k), b_qj = exp(-2 pi i v_q y_j)
-__global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, const float *gim, int nx,
-                                                          int ny, const double *u, const double *v,
-                                                          int K, double dxy, double dra, double ddec,
-                                                          float *gimg) {
-    __shared__ float Ha_r[ADJ_CH][ADJ_T], Ha_i[ADJ_CH][ADJ_T];
-    __shared__ float b_r[ADJ_CH][ADJ_T], b_i[ADJ_CH][ADJ_T];
-    __shared__ float H_r[ADJ_CH], H_i[ADJ_CH];
-    __shared__ float sa_c[ADJ_CH], sa_s[ADJ_CH], sb_c[ADJ_CH], sb_s[ADJ_CH];
-    const int tx = threadIdx.x % ADJ_T, ty = threadIdx.x / ADJ_T;
+// One workgroup per 16x16 tile, ADJ_G groups of 256 lanes (one pixel per lane
+// in each); group g adds the chunks g, g + ADJ_G, ... of the points in order,
+// then the groups' sums are added in group order, so the result does not
+// depend on scheduling.
+__global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const float *gim, int nx,
+                                                    int ny, const double *u, const double *v,
+                                                    int K, double dxy, double dra, double ddec,
+                                                    float *gimg) {
+    __shared__ float Ha_r[ADJ_G][ADJ_CH][ADJ_T], Ha_i[ADJ_G][ADJ_CH][ADJ_T];
+    __shared__ float b_r[ADJ_G][ADJ_CH][ADJ_T], b_i[ADJ_G][ADJ_CH][ADJ_T];
+    __shared__ float H_r[ADJ_G][ADJ_CH], H_i[ADJ_G][ADJ_CH];
+    __shared__ float sa_c[ADJ_G][ADJ_CH], sa_s[ADJ_G][ADJ_CH], sb_c[ADJ_G][ADJ_CH],
+        sb_s[ADJ_G][ADJ_CH];
+    const int g = threadIdx.x / (ADJ_T * ADJ_T), lt = threadIdx.x - g * (ADJ_T * ADJ_T);
+    const int tx = lt % ADJ_T, ty = lt / ADJ_T;
     const int k0 = blockIdx.x * ADJ_T, j0 = blockIdx.y * ADJ_T;
     const int k = k0 + tx, j = j0 + ty;
     float acc = 0.f;
     // loader: 4 threads per point, 4 consecutive columns / rows each
-    const int lq = threadIdx.x >> 2, lt = (threadIdx.x & 3) * 4;
-    for (int q0 = 0; q0 < K; q0 += ADJ_CH) {
+    const int lq = lt >> 2, ll = (lt & 3) * 4;
+    const int nch = (K + ADJ_CH - 1) / ADJ_CH;
+    for (int c0 = 0; c0 < nch; c0 += ADJ_G) {
+        const int q0 = (c0 + g) * ADJ_CH;  // past K for some groups in the last round
         __syncthreads();
         // per point: the cotangent with the phase-centre shift and the
         // one-pixel steps of its column and row phasors
-        if (threadIdx.x < ADJ_CH) {
-            const int q = q0 + threadIdx.x;
+        if (lt < ADJ_CH) {
+            const int q = q0 + lt;
             float hr = 0.f, hi = 0.f, s1 = 0.f, c1 = 1.f, s2 = 0.f, c2 = 1.f;
             if (q < K) {
                 double s, c;
@@ -124,12 +135,12 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
                 sincospif(-2.f * (float)(u[q] * dxy), &s1, &c1);
                 sincospif(-2.f * (float)(v[q] * dxy), &s2, &c2);
             }
-            H_r[threadIdx.x] = hr;
-            H_i[threadIdx.x] = hi;
-            sa_c[threadIdx.x] = c1;
-            sa_s[threadIdx.x] = s1;
-            sb_c[threadIdx.x] = c2;
-            sb_s[threadIdx.x] = s2;
+            H_r[g][lt] = hr;
+            H_i[g][lt] = hi;
+            sa_c[g][lt] = c1;
+            sa_s[g][lt] = s1;
+            sb_c[g][lt] = c2;
+            sb_s[g][lt] = s2;
         }
         __syncthreads();
         {
@@ -137,21 +148,21 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
             float hr = 0.f, hi = 0.f, ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
             float dac = 1.f, das = 0.f, dbc = 1.f, dbs = 0.f;
             if (q < K) {
-                hr = H_r[lq];
-                hi = H_i[lq];
-                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + lt - nx / 2), &ai, &ar);
-                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + lt - ny / 2), &bi, &br);
-                dac = sa_c[lq];
-                das = sa_s[lq];
-                dbc = sb_c[lq];
-                dbs = sb_s[lq];
+                hr = H_r[g][lq];
+                hi = H_i[g][lq];
+                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + ll - nx / 2), &ai, &ar);
+                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + ll - ny / 2), &bi, &br);
+                dac = sa_c[g][lq];
+                das = sa_s[g][lq];
+                dbc = sb_c[g][lq];
+                dbs = sb_s[g][lq];
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                Ha_r[lq][lt + e] = hr * ar - hi * ai;
-                Ha_i[lq][lt + e] = hr * ai + hi * ar;
-                b_r[lq][lt + e] = br;
-                b_i[lq][lt + e] = bi;
+                Ha_r[g][lq][ll + e] = hr * ar - hi * ai;
+                Ha_i[g][lq][ll + e] = hr * ai + hi * ar;
+                b_r[g][lq][ll + e] = br;
+                b_i[g][lq][ll + e] = bi;
                 // next column / row: rotate by one pixel step
                 const float nar = ar * dac - ai * das;
                 ai = fmaf(ar, das, ai * dac);
@@ -164,9 +175,19 @@ __global__ void __launch_bounds__(ADJ_T *ADJ_T) k_vis_adj(const float *gre, cons
         __syncthreads();
 #pragma unroll 8
         for (int qq = 0; qq < ADJ_CH; ++qq)
-            acc = fmaf(Ha_r[qq][tx], b_r[qq][ty], fmaf(-Ha_i[qq][tx], b_i[qq][ty], acc));
+            acc = fmaf(Ha_r[g][qq][tx], b_r[g][qq][ty], fmaf(-Ha_i[g][qq][tx], b_i[g][qq][ty], acc));
     }
-    if (k < nx && j < ny) gimg[(size_t)j * nx + k] = acc;
+    // the groups' sums, added in group order (the phasor buffers are free now)
+    __syncthreads();
+    float *red = &Ha_r[0][0][0];
+    if (g > 0) red[(g - 1) * (ADJ_T * ADJ_T) + lt] = acc;
+    __syncthreads();
+    if (g == 0 && k < nx && j < ny) {
+        float sum = acc;
+#pragma unroll
+        for (int h = 1; h < ADJ_G; ++h) sum += red[(h - 1) * (ADJ_T * ADJ_T) + lt];
+        gimg[(size_t)j * nx + k] = sum;
+    }
 }
 
 // ---- gridded sampler: galario's method (Tazzari, Beaujean & Testi 2018,
@@ -444,7 +465,7 @@ int vis_adjoint_run(rt_ctx *ctx, const float *d_gre, const float *d_gim, int nx,
                     float *d_g_img) {
     if (nx < 1 || ny < 1 || K < 0 || !(dxy > 0)) return set_err(ctx, RT_E_ARG, "vis adjoint: invalid sizes");
     dim3 grid((nx + ADJ_T - 1) / ADJ_T, (ny + ADJ_T - 1) / ADJ_T);
-    hipLaunchKernelGGL(k_vis_adj, grid, dim3(ADJ_T * ADJ_T), 0, ctx->stream, d_gre, d_gim, nx, ny,
+    hipLaunchKernelGGL(k_vis_adj, grid, dim3(ADJ_BS), 0, ctx->stream, d_gre, d_gim, nx, ny,
                        d_u, d_v, K, dxy, dra, ddec, d_g_img);
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
