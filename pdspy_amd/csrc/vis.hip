@@ -12,10 +12,10 @@
 // column strips and read as wave-wide broadcasts; the strip's column phasors
 // sit in registers and the row phasor is one sincospi every 16 rows and a
 // rotation by one row step in between, so the inner loop is two FMAs per
-// pixel.  Adjoint: one pixel per lane in 16x16 tiles, four groups of lanes per
-// tile taking alternate chunks of 64 points; per chunk the tile's column phasors
-// (times the point's complex cotangent) and row phasors are built once in LDS
-// and each lane does two FMAs per point.
+// pixel.  Adjoint: 16x16 tiles, four groups of lanes per tile taking alternate
+// chunks of 64 points; per chunk the tile's column phasors (times the point's
+// complex cotangent) and row phasors are built once in LDS, and the tile's
+// update G += B^T Ha (16 x 16, 4 points deep) runs on v_mfma_f32_16x16x4f32.
 #include <cmath>
 #include <mutex>
 
@@ -94,12 +94,15 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
 #define ADJ_G 4   // point groups per tile: each takes every ADJ_G-th chunk of points
 #define ADJ_BS (ADJ_T * ADJ_T * ADJ_G)
 
+typedef float vf4 __attribute__((ext_vector_type(4)));
+
 // g[j][k] = Re sum_q H_q a_qk b_qj,  H_q = (gre_q - i gim_q) shift_q,
 // a_qk = exp(-2 pi i u_q x_k), b_qj = exp(-2 pi i v_q y_j)
-// One workgroup per 16x16 tile, ADJ_G groups of 256 lanes (one pixel per lane
-// in each); group g adds the chunks g, g + ADJ_G, ... of the points in order,
-// then the groups' sums are added in group order, so the result does not
-// depend on scheduling.
+// One workgroup per 16x16 tile, ADJ_G groups of 256 lanes; group g takes the
+// chunks g, g + ADJ_G, ... of the points, builds the chunk's phasor tables in
+// LDS, and each of its four waves adds 16 of the chunk's points to its own copy
+// of the tile on the matrix cores.  The 16 copies are added in wave order, so
+// the result does not depend on scheduling.
 __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const float *gim, int nx,
                                                     int ny, const double *u, const double *v,
                                                     int K, double dxy, double dra, double ddec,
@@ -113,7 +116,9 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
     const int tx = lt % ADJ_T, ty = lt / ADJ_T;
     const int k0 = blockIdx.x * ADJ_T, j0 = blockIdx.y * ADJ_T;
     const int k = k0 + tx, j = j0 + ty;
-    float acc = 0.f;
+    // matrix-core layout: wave wv of the group, lane = mq * 16 + mc
+    const int wv = lt >> 6, mq = (lt >> 4) & 3, mc = lt & 15;
+    vf4 acc4 = {0.f, 0.f, 0.f, 0.f};
     // loader: 4 threads per point, 4 consecutive columns / rows each
     const int lq = lt >> 2, ll = (lt & 3) * 4;
     const int nch = (K + ADJ_CH - 1) / ADJ_CH;
@@ -173,19 +178,28 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
             }
         }
         __syncthreads();
-#pragma unroll 8
-        for (int qq = 0; qq < ADJ_CH; ++qq)
-            acc = fmaf(Ha_r[g][qq][tx], b_r[g][qq][ty], fmaf(-Ha_i[g][qq][tx], b_i[g][qq][ty], acc));
+        // the tile's update by the wave's 16 points on the matrix cores:
+        // G[j][k] += sum_q b_r[q][j] Ha_r[q][k] - b_i[q][j] Ha_i[q][k], four points
+        // per v_mfma_f32_16x16x4f32 (A[j][q] from lane q*16 + j, B[q][k] from
+        // lane q*16 + k)
+#pragma unroll
+        for (int st = 0; st < ADJ_CH / 16; ++st) {
+            const int qq = wv * 16 + st * 4 + mq;
+            acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(b_r[g][qq][mc], Ha_r[g][qq][mc], acc4, 0, 0, 0);
+            acc4 = __builtin_amdgcn_mfma_f32_16x16x4f32(-b_i[g][qq][mc], Ha_i[g][qq][mc], acc4, 0, 0, 0);
+        }
     }
-    // the groups' sums, added in group order (the phasor buffers are free now)
+    // the 16 waves' tiles (lane q*16 + k holds rows 4q..4q+3 of column k),
+    // added in wave order; the phasor buffers are free now
     __syncthreads();
     float *red = &Ha_r[0][0][0];
-    if (g > 0) red[(g - 1) * (ADJ_T * ADJ_T) + lt] = acc;
+    const int w16 = g * 4 + wv;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) red[w16 * (ADJ_T * ADJ_T) + (4 * mq + r) * ADJ_T + mc] = acc4[r];
     __syncthreads();
     if (g == 0 && k < nx && j < ny) {
-        float sum = acc;
-#pragma unroll
-        for (int h = 1; h < ADJ_G; ++h) sum += red[(h - 1) * (ADJ_T * ADJ_T) + lt];
+        float sum = red[lt];
+        for (int h = 1; h < 4 * ADJ_G; ++h) sum += red[h * (ADJ_T * ADJ_T) + lt];
         gimg[(size_t)j * nx + k] = sum;
     }
 }

@@ -340,10 +340,12 @@ def test_interpolate_model_follows_pdspys_galario_call():
     assert np.abs((c - ore) + 1j * (d - oim)).max() > 0.05 * scale
 
 
-def test_vis_adjoint_matches_oracle():
+@pytest.mark.parametrize("nx,ny,K", [(48, 48, 1500), (40, 56, 777)])
+def test_vis_adjoint_matches_oracle(nx, ny, K):
+    # square and non-square images, partial 16x16 tiles and a partial last
+    # chunk of points (the adjoint's tile update runs on the matrix cores)
+    print("case: vis adjoint vs oracle nx=%d ny=%d K=%d" % (nx, ny, K), flush=True)
     rng = np.random.default_rng(6)
-    ny = nx = 48
-    K = 1500
     dxy = 0.03 * inputs.arcsec
     u, v = configs.uv_points(K=K, lam_m=1.3e-3, seed=2)
     gr, gi = rng.normal(size=K), rng.normal(size=K)
@@ -351,7 +353,9 @@ def test_vis_adjoint_matches_oracle():
                            torch.tensor(gi, dtype=torch.float32, device="cuda"), nx, ny, dxy, u, v,
                            0.02 * inputs.arcsec, 0.01 * inputs.arcsec).double().cpu().numpy()
     o = oracle.vis_dft_adjoint(gr, gi, nx, ny, dxy, u, v, 0.02 * inputs.arcsec, 0.01 * inputs.arcsec)
+    assert g.shape == o.shape == (ny, nx)
     assert np.linalg.norm(g - o) / np.linalg.norm(o) < 1e-5
+    assert np.abs(g - o).max() <= 1e-5 * np.abs(o).max()
 
 
 def test_chisq_and_gradient():
