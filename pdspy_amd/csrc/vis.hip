@@ -8,11 +8,11 @@
 // gridded sampler further down samples it as galario does.
 // chi^2 = sum w ((Re_d - Re_m)^2 + (Im_d - Im_m)^2) (libinterferometry.pyx:619-633).
 //
-// Forward: one (u,v) point per lane; the image is staged through LDS in
-// column strips and read as wave-wide broadcasts; the strip's column phasors
-// sit in registers and the row phasor is one sincospi every 16 rows and a
-// rotation by one row step in between, so the inner loop is two FMAs per
-// pixel.  Adjoint: 16x16 tiles, four groups of lanes per tile taking alternate
+// Forward: one (u,v) point per lane in each of four row groups of a
+// workgroup; the image is staged through LDS in column strips and read as
+// wave-wide broadcasts; the strip's column phasors sit in registers and the row
+// phasor is one sincospi every 16 rows and a rotation by one row step in
+// between, so the inner loop is two FMAs per pixel.  Adjoint: 16x16 tiles, four groups of lanes per tile taking alternate
 // chunks of 64 points; per chunk the tile's column phasors (times the point's
 // complex cotangent) and row phasors are built once in LDS, and the tile's
 // update G += B^T Ha (16 x 16, 4 points deep) runs on v_mfma_f32_16x16x4f32.
@@ -24,14 +24,23 @@
 #define VIS_BS 256
 #define VIS_STRIP 32
 #define VIS_MAXROWS 1024
+#define VIS_G 4  // row groups per workgroup of the forward DFT
 
-__global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny,
-                                                const double *u, const double *v, int K,
-                                                double dxy, double dra, double ddec, float *re,
-                                                float *im) {
+// NG groups of VIS_BS lanes per workgroup share the points and take a
+// contiguous range of rows each; their sums are added in group order
+template <int NG>
+__global__ void __launch_bounds__(VIS_BS * NG) k_vis(const float *img, int nx, int ny,
+                                                     const double *u, const double *v, int K,
+                                                     double dxy, double dra, double ddec,
+                                                     float *re, float *im) {
+    constexpr int ng = NG;
     __shared__ float strip[VIS_MAXROWS * VIS_STRIP / 4];  // rows x 8 columns per pass (32 KB)
-    const int q = blockIdx.x * VIS_BS + threadIdx.x;
+    const int g = NG == 1 ? 0 : (int)threadIdx.x / VIS_BS;
+    const int lt = (int)threadIdx.x - g * VIS_BS;
+    const int q = blockIdx.x * VIS_BS + lt;
     const bool act = q < K;
+    const int rpg = NG == 1 ? ny : (ny + ng - 1) / ng;
+    const int jb = NG == 1 ? 0 : min(ny, g * rpg), je = NG == 1 ? ny : min(ny, jb + rpg);
     float ud = 0.f, vd = 0.f;
     if (act) {
         ud = (float)(u[q] * dxy);
@@ -45,7 +54,7 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
     for (int k0 = 0; k0 < nx; k0 += W) {
         int wk = min(W, nx - k0);
         __syncthreads();
-        for (int i = threadIdx.x; i < ny * W; i += VIS_BS) {
+        for (int i = threadIdx.x; i < ny * W; i += VIS_BS * ng) {
             int j = i / W, kk = i - j * W;
             strip[i] = (kk < wk) ? img[(size_t)j * nx + k0 + kk] : 0.f;
         }
@@ -58,10 +67,10 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
             ar[kk] = c;
             ai[kk] = s;
         }
-        // row phasors: exact every 16 rows, rotated by one row step in between
-        // (FP32 drift below 2e-6 over 15 rotations)
+        // row phasors: exact every 16 rows from the group's first row, rotated
+        // by one row step in between (FP32 drift below 2e-6 over 15 rotations)
         float bs = 0.f, bc = 1.f;
-        for (int j = 0; j < ny; ++j) {
+        for (int j = jb; j < je; ++j) {
             const float *row = strip + j * W;
             float rr = 0.f, ri = 0.f;
 #pragma unroll
@@ -69,7 +78,7 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
                 rr = fmaf(row[kk], ar[kk], rr);
                 ri = fmaf(row[kk], ai[kk], ri);
             }
-            if ((j & 15) == 0) {
+            if (((j - jb) & 15) == 0) {
                 sincospif(-2.f * vd * (float)(j - ny / 2), &bs, &bc);
             } else {
                 const float nc = bc * dc - bs * ds;
@@ -80,7 +89,19 @@ __global__ void __launch_bounds__(VIS_BS) k_vis(const float *img, int nx, int ny
             si += rr * bs + ri * bc;
         }
     }
-    if (act) {
+    // the groups' sums through the strip buffer (free after the last strip)
+    if (ng > 1) __syncthreads();
+    float *red = strip;  // [2][VIS_G - 1][VIS_BS]
+    if (g > 0) {
+        red[(g - 1) * VIS_BS + lt] = sr;
+        red[((VIS_G - 1) + g - 1) * VIS_BS + lt] = si;
+    }
+    if (ng > 1) __syncthreads();
+    if (g == 0 && act) {
+        for (int h = 0; h < ng - 1; ++h) {
+            sr += red[h * VIS_BS + lt];
+            si += red[((VIS_G - 1) + h) * VIS_BS + lt];
+        }
         double ph = -2.0 * (u[q] * dra + v[q] * ddec);
         double s, c;
         sincospi(ph, &s, &c);
@@ -468,8 +489,14 @@ int vis_run(rt_ctx *ctx, const float *d_img, int nx, int ny, double dxy, const d
     if (K == 0) return RT_OK;
     if (!d_img || !d_u || !d_v || !d_re || !d_im)
         return set_err(ctx, RT_E_ARG, "vis: NULL image, (u,v) or output");
-    hipLaunchKernelGGL(k_vis, dim3((K + VIS_BS - 1) / VIS_BS), dim3(VIS_BS), 0, ctx->stream, d_img, nx,
-                       ny, d_u, d_v, K, dxy, dra, ddec, d_re, d_im);
+    // row groups only where the points alone leave the GPU short of waves
+    const int nb = (K + VIS_BS - 1) / VIS_BS;
+    if (nb < 2 * ctx->n_cu)
+        hipLaunchKernelGGL(k_vis<VIS_G>, dim3(nb), dim3(VIS_BS * VIS_G), 0, ctx->stream, d_img, nx,
+                           ny, d_u, d_v, K, dxy, dra, ddec, d_re, d_im);
+    else
+        hipLaunchKernelGGL(k_vis<1>, dim3(nb), dim3(VIS_BS), 0, ctx->stream, d_img, nx, ny, d_u,
+                           d_v, K, dxy, dra, ddec, d_re, d_im);
     HIPCHK(ctx, hipGetLastError());
     return RT_OK;
 }

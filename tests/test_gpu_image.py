@@ -244,6 +244,23 @@ def test_vis_dft_matches_numpy_fft_and_oracle():
     err = max(np.abs(re2.double().cpu().numpy() - ore).max(), np.abs(im2.double().cpu().numpy() - oim).max())
     print("vis max abs err / max|V| = %.2e" % (err / scale))
     assert err < 2e-5 * scale
+    # non-square, with a row count the kernel's four row groups split unevenly
+    img3 = rng.uniform(size=(37, 45))
+    re3, im3 = rt.vis_dft(torch.tensor(img3, dtype=torch.float32, device="cuda"), dxy, uu[:700],
+                          vv[:700], 0.02 * inputs.arcsec, 0.03 * inputs.arcsec)
+    ore3, oim3 = oracle.vis_dft(img3, dxy, uu[:700], vv[:700], 0.02 * inputs.arcsec,
+                                0.03 * inputs.arcsec)
+    scale3 = np.abs(ore3 + 1j * oim3).max()
+    assert np.abs(re3.double().cpu().numpy() - ore3).max() < 2e-5 * scale3
+    assert np.abs(im3.double().cpu().numpy() - oim3).max() < 2e-5 * scale3
+    # enough points to fill the GPU without row groups (one group per workgroup)
+    u4, v4 = configs.uv_points(K=140000, lam_m=1.3e-3, seed=4)
+    img4 = rng.uniform(size=(24, 32))
+    re4, im4 = rt.vis_dft(torch.tensor(img4, dtype=torch.float32, device="cuda"), dxy, u4, v4)
+    ore4, oim4 = oracle.vis_dft(img4, dxy, u4, v4)
+    scale4 = np.abs(ore4 + 1j * oim4).max()
+    assert np.abs(re4.double().cpu().numpy() - ore4).max() < 2e-5 * scale4
+    assert np.abs(im4.double().cpu().numpy() - oim4).max() < 2e-5 * scale4
 
 
 def test_vis_grid_matches_oracle_and_dft_at_nodes():
