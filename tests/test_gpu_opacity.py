@@ -125,7 +125,9 @@ def test_sampler_step_draws_dust_per_model():
     got = _snapshot(s)
     for i, m in enumerate(s.models):
         d = gen(new[i]["a_max"] / 1e4, new[i]["p"])
-        assert np.allclose(m._keep["kabs"][0], rt.dust_arrays([d], lam_um)[0][0])
+        # the vectorized draw is the single draw's values bit for bit
+        assert np.array_equal(m._keep["kabs"][0], rt.dust_arrays([d], lam_um)[0][0])
+        assert np.array_equal(m._keep["ksca"][0], rt.dust_arrays([d], lam_um)[1][0])
         img, c2, _, _ = s.single_model(i, 2 * 1000 + i)
         assert np.array_equal(m.temperature(), got[0][i]), i
         assert _close(img.cpu().numpy(), got[2][i], 1e-6), i
@@ -134,9 +136,10 @@ def test_sampler_step_draws_dust_per_model():
 
 def test_device_built_tables_match_host_build():
     """rt_model_set_opacity_device builds the emission, re-emission and mean
-    tables with a kernel; they equal the host build of a model created with
-    the same opacities to FP32 rounding, and the thermal MC run on them is
-    that model's to within the Monte Carlo tolerance."""
+    tables with kernels; they equal the host build of a model created with the
+    same opacities (the re-emission CDFs and guides exactly, the rest to FP32
+    rounding), and the thermal MC run on them is that model's to within the
+    Monte Carlo tolerance."""
     print("case: device-built opacity tables vs host build", flush=True)
     base = _two_species(nr=24, nt=16)
     gen = _generator(base["lam_um"])
@@ -158,11 +161,13 @@ def test_device_built_tables_match_host_build():
     assert np.allclose(a["lnq"], b["lnq"], rtol=1e-12, atol=0)
     d = np.abs(a["cdf"].astype(float) - b["cdf"].astype(float))
     print("cdf max |diff| %.2e, entries differing %d of %d" % (d.max(), (d > 0).sum(), d.size))
-    assert d.max() <= 2.0 ** -22
+    # each row is summed in the host's order from the same FP64 terms: the
+    # CDFs and guides come out equal (exp and log of the device and the host
+    # may still differ in a last bit elsewhere)
+    assert d.max() == 0.0
+    assert np.array_equal(a["guide"], b["guide"])
     for k in ("kplanck", "kross"):
         assert np.allclose(a[k], b[k], rtol=3e-7, atol=0), k
-    # a guide entry can only move where a CDF value sits at its threshold
-    assert (a["guide"] != b["guide"]).mean() < 1e-4
     arr = fresh
     sh = host.thermal(50000, 3, max_events=100000, mrw=1, count_visits=1)
     Th, vis = host.temperature(with_visits=True)
