@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
     __shared__ float H_r[ADJ_G][ADJ_CH], H_i[ADJ_G][ADJ_CH];
     __shared__ float sa_c[ADJ_G][ADJ_CH], sa_s[ADJ_G][ADJ_CH], sb_c[ADJ_G][ADJ_CH],
         sb_s[ADJ_G][ADJ_CH];
+    __shared__ float s_u[ADJ_G][ADJ_CH], s_v[ADJ_G][ADJ_CH];  // u dxy, v dxy
     const int g = threadIdx.x / (ADJ_T * ADJ_T), lt = threadIdx.x - g * (ADJ_T * ADJ_T);
     const int tx = lt % ADJ_T, ty = lt / ADJ_T;
     const int k0 = blockIdx.x * ADJ_T, j0 = blockIdx.y * ADJ_T;
@@ -143,23 +144,44 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
     // loader: 4 threads per point, 4 consecutive columns / rows each
     const int lq = lt >> 2, ll = (lt & 3) * 4;
     const int nch = (K + ADJ_CH - 1) / ADJ_CH;
+    // the setup lanes' point data, loaded one round ahead so that the loads
+    // are in flight while the round before runs (the barrier waits for none)
+    double pu = 0.0, pv = 0.0;
+    float pgr = 0.f, pgi = 0.f;
+    if (lt < ADJ_CH && g * ADJ_CH + lt < K) {
+        const int q = g * ADJ_CH + lt;
+        pu = u[q];
+        pv = v[q];
+        pgr = gre[q];
+        pgi = gim[q];
+    }
     for (int c0 = 0; c0 < nch; c0 += ADJ_G) {
         const int q0 = (c0 + g) * ADJ_CH;  // past K for some groups in the last round
+        const double cu = pu, cv = pv;
+        const float gr = pgr, gi = pgi;
+        if (lt < ADJ_CH && q0 + ADJ_G * ADJ_CH + lt < K) {
+            const int qn = q0 + ADJ_G * ADJ_CH + lt;
+            pu = u[qn];
+            pv = v[qn];
+            pgr = gre[qn];
+            pgi = gim[qn];
+        }
         __syncthreads();
         // per point: the cotangent with the phase-centre shift and the
         // one-pixel steps of its column and row phasors
         if (lt < ADJ_CH) {
             const int q = q0 + lt;
-            float hr = 0.f, hi = 0.f, s1 = 0.f, c1 = 1.f, s2 = 0.f, c2 = 1.f;
+            float hr = 0.f, hi = 0.f, s1 = 0.f, c1 = 1.f, s2 = 0.f, c2 = 1.f, su = 0.f, sv = 0.f;
             if (q < K) {
                 double s, c;
-                sincospi(-2.0 * (u[q] * dra + v[q] * ddec), &s, &c);
+                sincospi(-2.0 * (cu * dra + cv * ddec), &s, &c);
                 // H = (gre - i gim) * (c + i s)
-                float gr = gre[q], gi = gim[q];
                 hr = (float)(gr * c + gi * s);
                 hi = (float)(gr * s - gi * c);
-                sincospif(-2.f * (float)(u[q] * dxy), &s1, &c1);
-                sincospif(-2.f * (float)(v[q] * dxy), &s2, &c2);
+                su = (float)(cu * dxy);
+                sv = (float)(cv * dxy);
+                sincospif(-2.f * su, &s1, &c1);
+                sincospif(-2.f * sv, &s2, &c2);
             }
             H_r[g][lt] = hr;
             H_i[g][lt] = hi;
@@ -167,6 +189,8 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
             sa_s[g][lt] = s1;
             sb_c[g][lt] = c2;
             sb_s[g][lt] = s2;
+            s_u[g][lt] = su;
+            s_v[g][lt] = sv;
         }
         __syncthreads();
         {
@@ -176,8 +200,8 @@ __global__ void __launch_bounds__(ADJ_BS) k_vis_adj(const float *gre, const floa
             if (q < K) {
                 hr = H_r[g][lq];
                 hi = H_i[g][lq];
-                sincospif(-2.f * (float)(u[q] * dxy) * (float)(k0 + ll - nx / 2), &ai, &ar);
-                sincospif(-2.f * (float)(v[q] * dxy) * (float)(j0 + ll - ny / 2), &bi, &br);
+                sincospif(-2.f * s_u[g][lq] * (float)(k0 + ll - nx / 2), &ai, &ar);
+                sincospif(-2.f * s_v[g][lq] * (float)(j0 + ll - ny / 2), &bi, &br);
                 dac = sa_c[g][lq];
                 das = sa_s[g][lq];
                 dbc = sb_c[g][lq];
