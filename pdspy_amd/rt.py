@@ -181,8 +181,9 @@ class DeviceModel:
     def set_opacity(self, kabs, ksca, g=None, on_device=False):
         """New dust opacities [nspec][nlam] (cm^2/g) on the model's wavelengths
         (rt_model_set_opacity); the species grouping must stay as it was.
-        ``on_device``: the thermal tables are built by a kernel on the stream
-        (rt_model_set_opacity_device) instead of on the host."""
+        ``on_device``: the per-species arrays and thermal tables are made by
+        kernels on torch's current stream (rt_model_set_opacity_device) instead
+        of on the host, and the call returns without waiting for them."""
         nl = len(self._keep["lam_um"])
         kabs = np.ascontiguousarray(kabs, dtype=float).reshape(self.nspec, nl)
         ksca = np.ascontiguousarray(ksca, dtype=float).reshape(self.nspec, nl)
@@ -293,7 +294,8 @@ class DeviceModel:
         """Replaces the densities with components computed on the device:
         dicts with ``kind`` ("disk" or "ulrich_ext"), ``species`` and the
         parameters of pdspy's add_disk / add_ulrichextended_envelope (Msun,
-        au, degrees).  See ``density_components``."""
+        au, degrees).  See ``density_components``.  The kernels run on torch's
+        current stream; the call returns without waiting for them."""
         arr = (_lib.DensityComp * len(components))()
         for k, c in enumerate(components):
             c = dict(c)
